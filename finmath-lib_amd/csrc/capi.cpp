@@ -1,0 +1,773 @@
+// C ABI of the LMM engine: context management, model tables, product lowering,
+// kernel orchestration and error mapping. See include/finmath_lmm.h.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fm_internal.hpp"
+#include "hip_check.hpp"
+
+namespace fm {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    void* get(size_t need) {
+        if (need == 0) need = 8;
+        if (need > bytes) {
+            if (p) { HIP_CHECK(hipFree(p)); p = nullptr; bytes = 0; }
+            HIP_CHECK(hipMalloc(&p, need));
+            bytes = need;
+        }
+        return p;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+template <class T>
+static T* upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+    T* d = (T*)b.get(v.size() * sizeof(T));
+    if (!v.empty()) HIP_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return d;
+}
+
+// TimeDiscretizationFromArray tick rounding (time/TimeDiscretizationFromArray.java:387-389).
+static double round_tick(double t) {
+    const double tick = 1.0 / (365.0 * 24.0);
+    return std::rint(t / tick) * tick;
+}
+// java.util.Arrays.binarySearch semantics (index, or -(insertion point) - 1).
+static int jsearch(const std::vector<double>& a, double key) {
+    int lo = 0, hi = (int)a.size() - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[(size_t)mid] < key) lo = mid + 1;
+        else if (a[(size_t)mid] > key) hi = mid - 1;
+        else return mid;
+    }
+    return -(lo + 1);
+}
+
+} // namespace fm
+
+using namespace fm;
+
+struct fm_lmm {
+    std::mutex mu;
+    fm_lmm_config cfg{};
+    std::vector<double> times, tenor;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    double timings[5] = {0, 0, 0, 0, 0};
+
+    // model inputs
+    std::vector<double> L0, lam, var, pd;
+    bool has_L0 = false, has_cov = false, has_pd = false;
+    bool tables_dirty = true;
+
+    // derived host tables
+    std::vector<int32_t> first_alive, tenor_ti, row_of;
+    int num_rows = 0;
+    std::vector<double> init_value;     // time-0 process value per component
+
+    // generator
+    StreamLayout lay;
+    bool states_ready = false;
+    bool dw_ready = false;
+    bool simulated = false;
+    std::vector<double> zb;             // numeraire-adjustment averages of the last simulate
+
+    DevBuf d_times, d_first, d_tenor_ti, d_lam, d_hv, d_dc1, d_dc2, d_delta, d_L0, d_sqrt_dt;
+    DevBuf d_states, d_poly, d_dw, d_row_of, d_paths, d_num;
+    DevBuf d_products, d_periods, d_wave_sum, d_wave_m2, d_contrib, d_out, d_swapmeta, d_scale;
+
+    int n() const { return cfg.num_libors; }
+    int nT() const { return cfg.num_time_steps; }
+    int F() const { return cfg.num_factors; }
+    int P() const { return cfg.num_paths; }
+    bool fp32() const { return cfg.precision == FM_PRECISION_FP32_MIXED; }
+    size_t real_size() const { return fp32() ? sizeof(float) : sizeof(double); }
+    int process_time_index(double t) const { return jsearch(times, round_tick(t)); }
+    int libor_index(double t) const { return jsearch(tenor, round_tick(t)); }
+};
+
+namespace {
+
+int fail(const std::exception& e) {
+    if (auto* fe = dynamic_cast<const FmError*>(&e)) { set_last_error(fe->what()); return fe->code; }
+    if (dynamic_cast<const std::bad_alloc*>(&e)) { set_last_error("host out of memory"); return FM_ERR_OUT_OF_MEMORY; }
+    set_last_error(e.what());
+    return FM_ERR_INTERNAL;
+}
+
+#define FM_TRY try {
+#define FM_CATCH } catch (const std::exception& e) { return fail(e); } catch (...) { set_last_error("unknown error"); return FM_ERR_INTERNAL; }
+
+void require(bool cond, int code, const std::string& msg) { if (!cond) throw FmError(code, msg); }
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) { HIP_CHECK(hipGetDevice(&prev)); if (prev != dev) HIP_CHECK(hipSetDevice(dev)); }
+    ~DeviceGuard() { if (prev >= 0) (void)hipSetDevice(prev); }
+};
+
+void validate_config(const fm_lmm_config* c) {
+    require(c != nullptr, FM_ERR_INVALID_ARGUMENT, "config is NULL");
+    require(c->num_time_steps >= 1, FM_ERR_INVALID_ARGUMENT, "number of time steps must be >= 1");
+    require(c->num_libors >= 1, FM_ERR_INVALID_ARGUMENT, "number of forward rates must be >= 1");
+    require(c->num_factors >= 1, FM_ERR_INVALID_ARGUMENT,
+            "Number of factors must be greater or equal 1 (given " + std::to_string(c->num_factors) + ").");
+    require(c->num_paths >= 1, FM_ERR_INVALID_ARGUMENT,
+            "Number of paths must be greater or equal 1 (given " + std::to_string(c->num_paths) + ").");
+    require(c->times != nullptr && c->tenor != nullptr, FM_ERR_INVALID_ARGUMENT, "times and tenor must be given");
+    for (int i = 0; i < c->num_time_steps; i++)
+        require(c->times[i + 1] > c->times[i], FM_ERR_INVALID_ARGUMENT, "simulation times must increase");
+    for (int i = 0; i < c->num_libors; i++)
+        require(c->tenor[i + 1] > c->tenor[i], FM_ERR_INVALID_ARGUMENT, "tenor dates must increase");
+    require(c->times[0] == 0.0 && c->tenor[0] == 0.0, FM_ERR_UNSUPPORTED, "simulation and tenor must start at time 0");
+    require(c->measure == FM_MEASURE_SPOT || c->measure == FM_MEASURE_TERMINAL, FM_ERR_INVALID_ARGUMENT, "unknown measure");
+    require(c->measure == FM_MEASURE_SPOT, FM_ERR_UNSUPPORTED, "only the SPOT measure is implemented on the GPU");
+    require(c->state_space == FM_STATE_SPACE_NORMAL || c->state_space == FM_STATE_SPACE_LOGNORMAL,
+            FM_ERR_INVALID_ARGUMENT, "unknown state space");
+    require(c->scheme >= FM_SCHEME_EULER && c->scheme <= FM_SCHEME_PREDICTOR_CORRECTOR_FUNCTIONAL,
+            FM_ERR_INVALID_ARGUMENT, "unknown scheme");
+    // EULER and EULER_FUNCTIONAL coincide when the inverse transform is the identity and no cap applies
+    const bool functional_equiv = c->scheme == FM_SCHEME_EULER_FUNCTIONAL ||
+        (c->scheme == FM_SCHEME_EULER && c->state_space == FM_STATE_SPACE_NORMAL && std::isinf(c->libor_cap));
+    require(functional_equiv, FM_ERR_UNSUPPORTED, "only the EULER_FUNCTIONAL scheme is implemented on the GPU");
+    require(c->precision == FM_PRECISION_FP64 || c->precision == FM_PRECISION_FP32_MIXED,
+            FM_ERR_INVALID_ARGUMENT, "unknown precision");
+    require(!std::isnan(c->libor_cap), FM_ERR_INVALID_ARGUMENT, "liborCap must not be NaN");
+    require(c->num_libors <= 80 && c->num_factors <= 10, FM_ERR_UNSUPPORTED,
+            "the fused kernel holds at most 80 forward rates and 10 factors");
+}
+
+// Rebuild derived host tables and upload the model (on the context stream).
+void prepare_model(fm_lmm* c) {
+    require(c->has_L0, FM_ERR_STATE, "initial forwards not set");
+    require(c->has_cov, FM_ERR_STATE, "covariance tables not set");
+    if (!c->tables_dirty) return;
+    const int n = c->n(), nT = c->nT(), F = c->F();
+    hipStream_t s = c->stream;
+    // first alive component at each step (LMM.getDrift:1126-1130)
+    c->first_alive.assign((size_t)nT, 0);
+    for (int i = 0; i < nT; i++) {
+        int f = jsearch(c->tenor, round_tick(c->times[(size_t)i])) + 1;
+        if (f < 0) f = -f - 1 + 1;
+        c->first_alive[(size_t)i] = std::min(f, n);
+    }
+    // process time index of each tenor date, rounded up (LMM:1055-1058)
+    c->tenor_ti.assign((size_t)n + 1, nT + 1);
+    for (int k = 0; k <= n; k++) {
+        int ti = c->process_time_index(c->tenor[(size_t)k]);
+        if (ti < 0) ti = -ti - 1;
+        c->tenor_ti[(size_t)k] = ti;
+    }
+    // time-0 process value: transform(initial state) (Euler:187-192, LMM:1081-1093,1200-1212)
+    c->init_value.assign((size_t)n, 0.0);
+    for (int j = 0; j < n; j++) {
+        double v;
+        if (c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL) v = std::exp(std::log(std::max(c->L0[(size_t)j], 0.0)));
+        else v = c->L0[(size_t)j];
+        if (!std::isinf(c->cfg.libor_cap)) v = std::min(v, c->cfg.libor_cap);
+        c->init_value[(size_t)j] = v;
+    }
+    // materialised-process row map
+    c->row_of.assign((size_t)(nT + 1) * n, -1);
+    int r = 0;
+    for (int ti = 1; ti <= nT; ti++)
+        for (int j = 0; j < n; j++) {
+            if (j >= c->first_alive[(size_t)ti - 1]) c->row_of[(size_t)ti * n + j] = r++;
+            else c->row_of[(size_t)ti * n + j] = c->row_of[(size_t)(ti - 1) * n + j];
+        }
+    c->num_rows = r;
+    std::vector<double> dc1((size_t)n), dc2((size_t)n), delta((size_t)n), sq((size_t)nT), hv((size_t)nT * n);
+    for (int j = 0; j < n; j++) {
+        const double d = c->tenor[(size_t)j + 1] - c->tenor[(size_t)j];
+        delta[(size_t)j] = d; dc1[(size_t)j] = d / d; dc2[(size_t)j] = 1.0 / d;
+    }
+    for (int i = 0; i < nT; i++) sq[(size_t)i] = std::sqrt(c->times[(size_t)i + 1] - c->times[(size_t)i]);
+    const bool lognormal = c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL;
+    for (size_t q = 0; q < hv.size(); q++) hv[q] = lognormal ? c->var[q] * (-0.5) : 0.0;
+    upload(c->d_times, c->times, s);
+    upload(c->d_first, c->first_alive, s);
+    upload(c->d_tenor_ti, c->tenor_ti, s);
+    upload(c->d_dc1, dc1, s);
+    upload(c->d_dc2, dc2, s);
+    upload(c->d_delta, delta, s);
+    upload(c->d_sqrt_dt, sq, s);
+    upload(c->d_row_of, c->row_of, s);
+    if (c->fp32()) {
+        std::vector<float> lf(c->lam.begin(), c->lam.end()), hf(hv.begin(), hv.end()), l0(c->init_value.begin(), c->init_value.end());
+        upload(c->d_lam, lf, s);
+        upload(c->d_hv, hf, s);
+        std::vector<double> l0d(l0.begin(), l0.end());   // float-rounded initial state, passed as double
+        upload(c->d_L0, l0d, s);
+    } else {
+        upload(c->d_lam, c->lam, s);
+        upload(c->d_hv, hv, s);
+        upload(c->d_L0, c->init_value, s);
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    (void)F;
+    c->tables_dirty = false;
+}
+
+void ensure_states(fm_lmm* c) {
+    if (c->states_ready) return;
+    const uint64_t total = 2ull * (uint64_t)c->P() * (uint64_t)c->nT() * (uint64_t)c->F();
+    c->lay = choose_layout(total);
+    uint32_t seedw[624];
+    mt_seed_window((int64_t)c->cfg.seed, seedw);
+    uint32_t* st = (uint32_t*)c->d_states.get((size_t)c->lay.num_sub * 624 * sizeof(uint32_t));
+    uint64_t* poly = (uint64_t*)c->d_poly.get((size_t)64 * 312 * sizeof(uint64_t));
+    launch_jump_tree(c->lay, seedw, st, poly, c->stream);
+    c->states_ready = true;
+}
+
+void ensure_brownian(fm_lmm* c) {
+    ensure_states(c);
+    if (c->dw_ready) return;
+    const size_t count = (size_t)c->P() * c->nT() * c->F();
+    void* out = c->d_dw.get(count * c->real_size());
+    launch_brownian(c->lay, c->d_states.as<uint32_t>(), c->nT(), c->F(), c->d_sqrt_dt.as<double>(), out,
+                    c->fp32() ? 1 : 0, c->stream);
+    c->dw_ready = true;
+}
+
+// Products lowered for the kernel.
+struct Lowered {
+    std::vector<KProduct> kp;
+    std::vector<KPeriod> per;
+    int num_slots = 0;
+    int num_contrib = 0;
+    std::vector<int> slot_of;                 // product q -> reduction slot
+    std::vector<int> slot_k;                  // slot -> numeraire tenor index (single-date)
+    std::vector<int32_t> swap_col0, swap_ncol, swap_slot;
+    std::vector<int> contrib_k;               // contribution column -> numeraire tenor index
+};
+
+// kcur of the kernel after the numeraire advance at time index ti
+int kcur_at(const fm_lmm* c, int ti) {
+    int k = 0;
+    while (k < c->n() && c->tenor_ti[(size_t)k] <= ti) k++;
+    return k;
+}
+
+// Process time index used by getForwardRate for `time` (LMM:1243-1252, ROUND_NEAREST).
+int forward_time_index(const fm_lmm* c, double time) {
+    int ti = c->process_time_index(time);
+    if (ti < 0) {
+        ti = -ti - 2;
+        if (ti + 1 <= c->nT() && time - c->times[(size_t)ti] > c->times[(size_t)ti + 1] - time) ti++;
+    }
+    return ti;
+}
+
+Lowered lower_products(fm_lmm* c, const fm_product* prods, int count) {
+    Lowered L;
+    require(count >= 0 && (count == 0 || prods != nullptr), FM_ERR_INVALID_ARGUMENT, "products array is NULL");
+    const int n = c->n();
+    auto numeraire_index = [&](double T) {
+        int k = c->libor_index(T);
+        require(k >= 0, FM_ERR_UNSUPPORTED, "numeraire requested off the tenor grid (interpolation not implemented)");
+        return k;
+    };
+    auto check_available = [&](int ti, int k) {
+        require(ti >= 0 && ti <= c->nT(), FM_ERR_INVALID_ARGUMENT, "product date outside the simulation horizon");
+        const int kc = kcur_at(c, ti);
+        require(k == kc || k == kc - 1, FM_ERR_UNSUPPORTED, "numeraire date not adjacent to the evaluation date");
+    };
+    L.slot_of.assign((size_t)count, -1);
+    for (int q = 0; q < count; q++) {
+        const fm_product& p = prods[q];
+        require(p.num_periods >= 1, FM_ERR_INVALID_ARGUMENT, "product must have at least one period");
+        if (p.type == FM_PRODUCT_SWAPTION) {
+            require(p.fixing_dates && p.payment_dates && p.swaprates, FM_ERR_INVALID_ARGUMENT, "swaption arrays missing");
+            KProduct k{};
+            k.type = FM_PRODUCT_SWAPTION;
+            const double Te = p.exercise_date;
+            for (int i = 0; i < p.num_periods; i++)
+                require(p.fixing_dates[i] >= Te, FM_ERR_UNSUPPORTED, "swaption periods fixing before exercise");
+            k.eval_index = forward_time_index(c, Te);
+            const int j0 = c->libor_index(p.fixing_dates[0]);
+            require(j0 >= 0, FM_ERR_UNSUPPORTED, "swaption fixing off the tenor grid");
+            for (int i = 0; i < p.num_periods; i++) {
+                const int ji = c->libor_index(p.fixing_dates[i]);
+                const int jp = c->libor_index(p.payment_dates[i]);
+                require(ji == j0 + i && jp == ji + 1 && jp <= n, FM_ERR_UNSUPPORTED,
+                        "swaption periods must be consecutive single tenor periods");
+                KPeriod pe{};
+                pe.strike = p.swaprates[i];
+                pe.period_length = p.period_lengths ? p.period_lengths[i] : p.payment_dates[i] - p.fixing_dates[i];
+                pe.tau = p.payment_dates[i] - std::max(p.fixing_dates[i], Te);
+                pe.adj = p.discounting_adjustment ? p.discounting_adjustment[i] : 1.0;
+                pe.notional = p.notional;
+                pe.j = ji; pe.pay_k = jp;
+                L.per.push_back(pe);
+            }
+            k.first_j = j0;
+            k.num_periods = p.num_periods;
+            k.notional = p.notional;
+            k.exercise = Te;
+            k.param_off = (int32_t)L.per.size() - p.num_periods;
+            k.num_k = numeraire_index(Te);
+            check_available(k.eval_index, k.num_k);
+            k.out_slot = L.num_slots++;
+            L.slot_of[(size_t)q] = k.out_slot;
+            L.slot_k.push_back(k.num_k);
+            L.kp.push_back(k);
+        } else if (p.type == FM_PRODUCT_CAPLET) {
+            require(p.period_lengths && p.swaprates, FM_ERR_INVALID_ARGUMENT, "caplet arrays missing");
+            const double Tm = p.exercise_date, dl = p.period_lengths[0];
+            const int j = c->libor_index(Tm), jp = c->libor_index(Tm + dl);
+            require(j >= 0 && jp == j + 1, FM_ERR_UNSUPPORTED, "caplet period must be a tenor period");
+            KProduct k{};
+            k.type = FM_PRODUCT_CAPLET;
+            k.eval_index = forward_time_index(c, Tm);
+            k.first_j = j; k.num_periods = 1;
+            k.is_floorlet = p.is_floorlet;
+            KPeriod pe{};
+            pe.strike = p.swaprates[0]; pe.period_length = dl; pe.tau = 0; pe.adj = 1.0;
+            pe.notional = p.notional;   // daycount fraction
+            pe.j = j; pe.pay_k = jp;
+            L.per.push_back(pe);
+            k.param_off = (int32_t)L.per.size() - 1;
+            k.num_k = numeraire_index(Tm + dl);
+            check_available(k.eval_index, k.num_k);
+            k.out_slot = L.num_slots++;
+            L.slot_of[(size_t)q] = k.out_slot;
+            L.slot_k.push_back(k.num_k);
+            L.kp.push_back(k);
+        } else if (p.type == FM_PRODUCT_SWAP) {
+            require(p.fixing_dates && p.payment_dates && p.swaprates, FM_ERR_INVALID_ARGUMENT, "swap arrays missing");
+            const int slot = L.num_slots++;
+            L.slot_of[(size_t)q] = slot;
+            L.slot_k.push_back(-1);
+            L.swap_col0.push_back(L.num_contrib);
+            L.swap_ncol.push_back(p.num_periods);
+            L.swap_slot.push_back(slot);
+            for (int i = 0; i < p.num_periods; i++) {
+                const double fd = p.fixing_dates[i], pd = p.payment_dates[i];
+                const int j = c->libor_index(fd), jp = c->libor_index(pd);
+                require(j >= 0 && jp == j + 1, FM_ERR_UNSUPPORTED, "swap periods must be tenor periods");
+                KProduct k{};
+                k.type = FM_PRODUCT_SWAP;
+                k.eval_index = forward_time_index(c, fd);
+                k.first_j = j; k.num_periods = 1;
+                k.flags = p.is_pay_fix ? 1 : 0;
+                KPeriod pe{};
+                pe.strike = p.swaprates[i]; pe.period_length = pd - fd; pe.tau = 0; pe.adj = 1.0;
+                pe.notional = p.notionals ? p.notionals[i] : 1.0;
+                pe.j = j; pe.pay_k = jp;
+                L.per.push_back(pe);
+                k.param_off = (int32_t)L.per.size() - 1;
+                k.num_k = numeraire_index(pd);
+                check_available(k.eval_index, k.num_k);
+                k.out_slot = L.num_contrib++;
+                L.contrib_k.push_back(k.num_k);
+                L.kp.push_back(k);
+            }
+        } else {
+            throw FmError(FM_ERR_INVALID_ARGUMENT, "unknown product type " + std::to_string(p.type));
+        }
+    }
+    std::stable_sort(L.kp.begin(), L.kp.end(), [](const KProduct& a, const KProduct& b) { return a.eval_index < b.eval_index; });
+    return L;
+}
+
+std::vector<double> adjustment_chain(const fm_lmm* c) {
+    const int n = c->n();
+    std::vector<double> adj((size_t)n + 1, 1.0);
+    if (!c->has_pd) return adj;
+    double a0 = c->pd[0];
+    adj[0] = a0;
+    for (int i = 0; i < n; i++) {
+        const double ts = c->tenor[(size_t)i + 1] - c->tenor[(size_t)i];
+        const double fr = (c->pd[(size_t)i] / c->pd[(size_t)i + 1] - 1.0) / ts;
+        a0 = 1.0 / (fr * (ts / a0) + 1.0 / a0);         // Scalar.discount (stochastic/Scalar.java:320-328)
+        adj[(size_t)i + 1] = a0;
+    }
+    return adj;
+}
+
+struct Timer {
+    fm_lmm* c;
+    explicit Timer(fm_lmm* cc) : c(cc) { for (int i = 0; i < 5; i++) c->timings[i] = 0.0; }
+    void mark(int e) { HIP_CHECK(hipEventRecord(c->ev[e], c->stream)); }
+    float span(int a, int b) { float ms = 0; HIP_CHECK(hipEventElapsedTime(&ms, c->ev[a], c->ev[b])); return ms; }
+};
+
+// Run generator + fused step (+ reductions). Returns per-column sums and M2 (host).
+void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums, std::vector<double>& m2s) {
+    prepare_model(c);
+    hipStream_t s = c->stream;
+    const int n = c->n(), P = c->P();
+    Timer tm(c);
+    tm.mark(0);
+    const bool regen = !c->dw_ready;
+    ensure_brownian(c);
+    tm.mark(1);
+    const int64_t num_waves = ((int64_t)P + 63) / 64;
+    const int stride = L.num_slots + n + 1;
+    double* wsum = (double*)c->d_wave_sum.get((size_t)num_waves * stride * sizeof(double));
+    double* wm2 = (double*)c->d_wave_m2.get((size_t)num_waves * stride * sizeof(double));
+    HIP_CHECK(hipMemsetAsync(wsum, 0, (size_t)num_waves * stride * sizeof(double), s));
+    HIP_CHECK(hipMemsetAsync(wm2, 0, (size_t)num_waves * stride * sizeof(double), s));
+    StepArgs a{};
+    a.nT = c->nT(); a.n = n; a.F = c->F(); a.P = P;
+    a.precision = c->cfg.precision;
+    a.state_space = c->cfg.state_space;
+    a.store_paths = store ? 1 : 0;
+    a.times = c->d_times.as<double>();
+    a.first_alive = c->d_first.as<int32_t>();
+    a.tenor_time_index = c->d_tenor_ti.as<int32_t>();
+    a.lam = c->d_lam.p; a.hv = c->d_hv.p;
+    a.dc1 = c->d_dc1.as<double>(); a.dc2 = c->d_dc2.as<double>(); a.delta = c->d_delta.as<double>();
+    a.L0 = c->d_L0.as<double>();
+    a.dw = c->d_dw.p;
+    a.cap = c->cfg.libor_cap;
+    a.scheme = c->cfg.scheme;
+    a.num_products = (int)L.kp.size();
+    a.products = upload(c->d_products, L.kp, s);
+    a.periods = upload(c->d_periods, L.per, s);
+    a.num_slots = L.num_slots;
+    a.num_contrib = L.num_contrib;
+    a.contrib = L.num_contrib ? (double*)c->d_contrib.get((size_t)L.num_contrib * P * sizeof(double)) : nullptr;
+    a.wave_sum = wsum; a.wave_m2 = wm2;
+    a.paths = store ? c->d_paths.get((size_t)std::max(c->num_rows, 1) * P * c->real_size()) : nullptr;
+    a.row_of = c->d_row_of.as<int32_t>();
+    a.numeraire = store ? (double*)c->d_num.get((size_t)(n + 1) * P * sizeof(double)) : nullptr;
+    tm.mark(2);
+    if (c->fp32()) launch_lmm_step_fp32(a, s); else launch_lmm_step_fp64(a, s);
+    tm.mark(3);
+    // numeraire-adjustment averages first (the swap contributions need them)
+    const int cols = L.num_slots + n + 1;
+    double* dout = (double*)c->d_out.get((size_t)cols * 2 * sizeof(double));
+    launch_reduce_partials(wsum, wm2, (int)num_waves, stride, L.num_slots, n + 1, P, dout, dout + cols, s);
+    sums.assign((size_t)cols, 0.0); m2s.assign((size_t)cols, 0.0);
+    HIP_CHECK(hipMemcpyAsync(sums.data(), dout, cols * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(m2s.data(), dout + cols, cols * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    // zb_k = average of N(0)/N(T_k) (LMM:872); N(T_0) = 1 and N(T_1) is deterministic
+    c->zb.assign((size_t)n + 1, 1.0);
+    for (int k = 1; k <= n; k++) c->zb[(size_t)k] = sums[(size_t)L.num_slots + k] / (double)P;
+    if (n >= 1) {
+        const double N1 = c->init_value[0] * ((c->tenor[1] - c->tenor[0]) * 1.0) + 1.0;
+        c->zb[1] = (1.0 / N1) * 1.0;
+    }
+    if (!L.swap_slot.empty()) {
+        const std::vector<double> adj = adjustment_chain(c);
+        std::vector<double> scale((size_t)L.num_contrib, 1.0);
+        for (int col = 0; col < L.num_contrib; col++) {
+            const int k = L.contrib_k[(size_t)col];
+            if (c->has_pd) scale[(size_t)col] = adj[(size_t)k] / c->zb[(size_t)k];
+        }
+        const double* dscale = upload(c->d_scale, scale, s);
+        std::vector<int32_t> meta;
+        meta.insert(meta.end(), L.swap_col0.begin(), L.swap_col0.end());
+        meta.insert(meta.end(), L.swap_ncol.begin(), L.swap_ncol.end());
+        meta.insert(meta.end(), L.swap_slot.begin(), L.swap_slot.end());
+        const int32_t* dmeta = upload(c->d_swapmeta, meta, s);
+        const int ns = (int)L.swap_slot.size();
+        launch_swap_finalize(a.contrib, P, ns, dmeta, dmeta + ns, dmeta + 2 * ns, dscale, wsum, wm2, stride, s);
+        launch_reduce_partials(wsum, wm2, (int)num_waves, stride, L.num_slots, 0, P, dout, dout + cols, s);
+        std::vector<double> s2((size_t)cols), m2b((size_t)cols);
+        HIP_CHECK(hipMemcpyAsync(s2.data(), dout, L.num_slots * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipMemcpyAsync(m2b.data(), dout + cols, L.num_slots * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        for (int ss : L.swap_slot) { sums[(size_t)ss] = s2[(size_t)ss]; m2s[(size_t)ss] = m2b[(size_t)ss]; }
+    }
+    tm.mark(4);
+    HIP_CHECK(hipEventSynchronize(c->ev[4]));
+    c->timings[0] = regen ? tm.span(0, 1) : 0.0;
+    c->timings[1] = tm.span(2, 3);
+    c->timings[2] = tm.span(3, 4);
+    c->timings[4] = tm.span(0, 4);
+}
+
+} // namespace
+
+extern "C" {
+
+const char* fm_last_error(void) { return g_last_error.c_str(); }
+
+int fm_lmm_create(const fm_lmm_config* config, fm_lmm** out) {
+    FM_TRY
+    require(out != nullptr, FM_ERR_INVALID_ARGUMENT, "output handle pointer is NULL");
+    *out = nullptr;
+    validate_config(config);
+    std::unique_ptr<fm_lmm> c(new fm_lmm());
+    c->cfg = *config;
+    c->times.assign(config->times, config->times + config->num_time_steps + 1);
+    c->tenor.assign(config->tenor, config->tenor + config->num_libors + 1);
+    c->cfg.times = nullptr; c->cfg.tenor = nullptr;
+    if (c->cfg.checkpoint_stride <= 0) c->cfg.checkpoint_stride = 4;
+    HIP_CHECK(hipGetDevice(&c->device));
+    HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) HIP_CHECK(hipEventCreate(&e));
+    *out = c.release();
+    return FM_OK;
+    FM_CATCH
+}
+
+void fm_lmm_destroy(fm_lmm* ctx) {
+    if (!ctx) return;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        for (auto& e : ctx->ev) if (e) (void)hipEventDestroy(e);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+int fm_lmm_set_initial_forwards(fm_lmm* c, const double* forwards) {
+    FM_TRY
+    require(c && forwards, FM_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->L0.assign(forwards, forwards + c->n());
+    for (double v : c->L0) require(std::isfinite(v), FM_ERR_INVALID_ARGUMENT, "initial forwards must be finite");
+    c->has_L0 = true; c->tables_dirty = true; c->simulated = false;
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_set_covariance(fm_lmm* c, const double* factor_loading, const double* variance) {
+    FM_TRY
+    require(c && factor_loading && variance, FM_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const size_t nl = (size_t)c->nT() * c->n() * c->F(), nv = (size_t)c->nT() * c->n();
+    c->lam.assign(factor_loading, factor_loading + nl);
+    c->var.assign(variance, variance + nv);
+    for (double v : c->lam) require(std::isfinite(v), FM_ERR_INVALID_ARGUMENT, "factor loadings must be finite");
+    for (double v : c->var) require(std::isfinite(v), FM_ERR_INVALID_ARGUMENT, "variances must be finite");
+    c->has_cov = true; c->tables_dirty = true; c->simulated = false;
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_set_discount_curve(fm_lmm* c, const double* df) {
+    FM_TRY
+    require(c != nullptr, FM_ERR_INVALID_ARGUMENT, "NULL context");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (df) {
+        c->pd.assign(df, df + c->n() + 1);
+        for (double v : c->pd) require(std::isfinite(v) && v > 0.0, FM_ERR_INVALID_ARGUMENT, "discount factors must be positive");
+        c->has_pd = true;
+    } else { c->pd.clear(); c->has_pd = false; }
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_simulate(fm_lmm* c) {
+    FM_TRY
+    require(c != nullptr, FM_ERR_INVALID_ARGUMENT, "NULL context");
+    std::lock_guard<std::mutex> lk(c->mu);
+    require(c->cfg.store_paths != 0, FM_ERR_STATE, "context created without store_paths");
+    DeviceGuard g(c->device);
+    Lowered none;
+    std::vector<double> sums, m2s;
+    run_pass(c, none, true, sums, m2s);
+    c->simulated = true;
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_get_process_value(fm_lmm* c, int32_t t, int32_t j, double* out) {
+    FM_TRY
+    require(c && out, FM_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    require(t >= 0 && t <= c->nT() && j >= 0 && j < c->n(), FM_ERR_INVALID_ARGUMENT, "process index out of range");
+    require(c->simulated, FM_ERR_STATE, "process not simulated (call fm_lmm_simulate)");
+    DeviceGuard g(c->device);
+    const int P = c->P();
+    const int r = c->row_of[(size_t)t * c->n() + j];
+    if (r < 0) { std::fill(out, out + P, c->init_value[(size_t)j]); return FM_OK; }
+    if (c->fp32()) {
+        std::vector<float> tmp((size_t)P);
+        HIP_CHECK(hipMemcpy(tmp.data(), (const float*)c->d_paths.p + (size_t)r * P, P * sizeof(float), hipMemcpyDeviceToHost));
+        for (int p = 0; p < P; p++) out[p] = tmp[(size_t)p];
+    } else {
+        HIP_CHECK(hipMemcpy(out, (const double*)c->d_paths.p + (size_t)r * P, P * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_process_value_device_ptr(fm_lmm* c, int32_t t, int32_t j, const void** out) {
+    FM_TRY
+    require(c && out, FM_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    require(t >= 0 && t <= c->nT() && j >= 0 && j < c->n(), FM_ERR_INVALID_ARGUMENT, "process index out of range");
+    require(c->simulated, FM_ERR_STATE, "process not simulated (call fm_lmm_simulate)");
+    const int r = c->row_of[(size_t)t * c->n() + j];
+    require(r >= 0, FM_ERR_STATE, "time-0 values are deterministic (no device realisations)");
+    *out = (const char*)c->d_paths.p + (size_t)r * c->P() * c->real_size();
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_get_numeraire(fm_lmm* c, double time, double* out) {
+    FM_TRY
+    require(c && out, FM_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    require(c->simulated, FM_ERR_STATE, "process not simulated (call fm_lmm_simulate)");
+    DeviceGuard g(c->device);
+    const int k = c->libor_index(time);
+    require(k >= 0 && k <= c->n(), FM_ERR_UNSUPPORTED, "numeraire requested off the tenor grid");
+    require(c->tenor_ti[(size_t)std::max(k - 1, 0)] <= c->nT(), FM_ERR_INVALID_ARGUMENT, "numeraire date beyond the simulation");
+    const int P = c->P();
+    if (k == 0) std::fill(out, out + P, 1.0);
+    else HIP_CHECK(hipMemcpy(out, c->d_num.as<double>() + (size_t)k * P, P * sizeof(double), hipMemcpyDeviceToHost));
+    if (c->has_pd) {
+        // N(T) * avg(N(0)/N(T)) / P^d(T) (LMM:869-874)
+        const std::vector<double> adj = adjustment_chain(c);
+        const double zbk = c->zb[(size_t)k], ak = adj[(size_t)k];
+        if (k <= 1) { for (int p = 0; p < P; p++) out[p] = (1.0 / ak) * (out[p] * zbk); }
+        else { for (int p = 0; p < P; p++) out[p] = (out[p] * zbk) / ak; }
+    }
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_get_brownian_increment(fm_lmm* c, int32_t t, int32_t k, double* out) {
+    FM_TRY
+    require(c && out, FM_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    require(t >= 0 && t < c->nT() && k >= 0 && k < c->F(), FM_ERR_INVALID_ARGUMENT, "increment index out of range");
+    DeviceGuard g(c->device);
+    prepare_model(c);
+    ensure_brownian(c);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    const int P = c->P();
+    const size_t es = c->real_size();
+    const size_t pitch = (size_t)c->nT() * c->F() * es;
+    std::vector<unsigned char> tmp((size_t)P * es);
+    HIP_CHECK(hipMemcpy2D(tmp.data(), es, (const char*)c->d_dw.p + ((size_t)t * c->F() + k) * es, pitch, es, P,
+                          hipMemcpyDeviceToHost));
+    if (c->fp32()) { const float* f = (const float*)tmp.data(); for (int p = 0; p < P; p++) out[p] = f[p]; }
+    else std::memcpy(out, tmp.data(), (size_t)P * sizeof(double));
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_value_products(fm_lmm* c, const fm_product* products, int32_t count, double* values, double* errors) {
+    FM_TRY
+    require(c != nullptr, FM_ERR_INVALID_ARGUMENT, "NULL context");
+    require(count >= 0, FM_ERR_INVALID_ARGUMENT, "negative product count");
+    require(count == 0 || values != nullptr, FM_ERR_INVALID_ARGUMENT, "values is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard g(c->device);
+    prepare_model(c);
+    Lowered L = lower_products(c, products, count);
+    std::vector<double> sums, m2s;
+    run_pass(c, L, false, sums, m2s);
+    const std::vector<double> adj = adjustment_chain(c);
+    const double P = (double)c->P();
+    for (int q = 0; q < count; q++) {
+        const int slot = L.slot_of[(size_t)q];
+        const int k = L.slot_k[(size_t)slot];
+        double scale = 1.0;
+        if (k >= 0 && c->has_pd) scale = adj[(size_t)k] / c->zb[(size_t)k];
+        const double mean = sums[(size_t)slot] / P;
+        const double var = m2s[(size_t)slot] / P;
+        values[q] = scale * mean;
+        if (errors) errors[q] = std::fabs(scale) * std::sqrt(var) / std::sqrt(P);
+        require(std::isfinite(values[q]), FM_ERR_NUMERICAL, "product value is not finite (NaN/Inf on some path)");
+    }
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_adjoint(fm_lmm* c, const fm_product*, int32_t, const double*, double*, double*, double*, double*) {
+    FM_TRY
+    require(c != nullptr, FM_ERR_INVALID_ARGUMENT, "NULL context");
+    throw FmError(FM_ERR_UNSUPPORTED, "adjoint not available in this build");
+    FM_CATCH
+}
+
+int fm_lmm_last_timings(fm_lmm* c, double* ms, int32_t count) {
+    FM_TRY
+    require(c && ms, FM_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (int i = 0; i < count && i < 5; i++) ms[i] = c->timings[i];
+    return FM_OK;
+    FM_CATCH
+}
+
+static int mt_generate(int64_t seed, uint64_t offset, uint64_t count, std::vector<uint32_t>& words) {
+    StreamLayout lay = choose_layout(offset + count + (offset + count) % 2);
+    uint32_t seedw[624];
+    mt_seed_window(seed, seedw);
+    hipStream_t s;
+    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    DevBuf st, poly, out;
+    try {
+        uint32_t* dst = (uint32_t*)st.get((size_t)lay.num_sub * 624 * sizeof(uint32_t));
+        launch_jump_tree(lay, seedw, dst, (uint64_t*)poly.get((size_t)64 * 312 * sizeof(uint64_t)), s);
+        uint32_t* dout = (uint32_t*)out.get((size_t)(offset + count) * sizeof(uint32_t));
+        launch_words(lay, dst, offset + count, dout, s);
+        words.resize((size_t)count);
+        HIP_CHECK(hipMemcpyAsync(words.data(), dout + offset, count * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) { (void)hipStreamDestroy(s); throw; }
+    HIP_CHECK(hipStreamDestroy(s));
+    return FM_OK;
+}
+
+int fm_mt19937_words(int64_t seed, uint64_t offset, uint64_t count, uint32_t* out) {
+    FM_TRY
+    require(out != nullptr || count == 0, FM_ERR_INVALID_ARGUMENT, "NULL output");
+    if (count == 0) return FM_OK;
+    std::vector<uint32_t> w;
+    mt_generate(seed, offset, count, w);
+    std::memcpy(out, w.data(), count * sizeof(uint32_t));
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_mt19937_doubles(int64_t seed, uint64_t offset, uint64_t count, double* out) {
+    FM_TRY
+    require(out != nullptr || count == 0, FM_ERR_INVALID_ARGUMENT, "NULL output");
+    if (count == 0) return FM_OK;
+    std::vector<uint32_t> w;
+    mt_generate(seed, 2 * offset, 2 * count, w);
+    for (uint64_t i = 0; i < count; i++) {
+        const uint64_t bits = ((uint64_t)(w[2 * i] >> 6) << 26) | (uint64_t)(w[2 * i + 1] >> 6);
+        out[i] = (double)bits * 0x1.0p-52;
+    }
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_device_info(char* buffer, int32_t length) {
+    FM_TRY
+    require(buffer && length > 0, FM_ERR_INVALID_ARGUMENT, "NULL buffer");
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    std::string s = std::string("finmath-lmm 0.1; device ") + std::to_string(dev) + " " + prop.name + " " +
+                    prop.gcnArchName + " CUs=" + std::to_string(prop.multiProcessorCount);
+    std::snprintf(buffer, (size_t)length, "%s", s.c_str());
+    return FM_OK;
+    FM_CATCH
+}
+
+} // extern "C"

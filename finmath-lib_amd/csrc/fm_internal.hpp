@@ -1,0 +1,120 @@
+// Internal declarations shared by the host code and kernels of the LMM engine.
+#pragma once
+
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+#include <stdexcept>
+
+#include "finmath_lmm.h"
+
+namespace fm {
+
+// Error carried to the C ABI as a status code plus message.
+struct FmError : std::runtime_error {
+    int code;
+    FmError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+// ---- MT19937 jump-ahead (mt19937_gf2.cpp) ----------------------------------
+int mt_jump_degree();
+int mt_jump_words();
+const std::vector<uint64_t>& mt_jump_poly_pow2(int e);     // x^(2^e) mod phi
+const std::vector<uint64_t>& mt_char_poly();
+void mt_jump_window_host(const uint32_t* src624, const std::vector<uint64_t>& c, uint32_t* dst624);
+
+// commons-math3 MersenneTwister(long) seeding (init_by_array with key {hi, lo}).
+void mt_seed_window(int64_t seed, uint32_t* window624);
+
+// ---- generator launch (mt19937.hip) -------------------------------------------
+struct StreamLayout {
+    uint64_t total_words = 0;    // 2 * number of doubles of the whole stream
+    int      log2_len = 0;       // sub-stream length L = 2^log2_len words
+    int64_t  num_sub = 0;        // number of sub-streams
+};
+StreamLayout choose_layout(uint64_t total_words);
+
+// Compute the start windows of all sub-streams on the device (jump tree).
+// d_states: num_sub * 624 words. d_poly: device scratch for jump polynomials.
+void launch_jump_tree(const StreamLayout& lay, const uint32_t* seed_window, uint32_t* d_states,
+                      uint64_t* d_poly, void* stream);
+
+// Generate the Brownian increments dW[D] = icdf(u_D) * sqrt_dt[(D / F) % nT] for all
+// D < total_words/2 in stream (path-major) order. out is double* or float*.
+void launch_brownian(const StreamLayout& lay, const uint32_t* d_states, int nT, int F,
+                     const double* d_sqrt_dt, void* out, int out_is_float, void* stream);
+
+// Raw tempered words [0, count) of the stream (count <= lay.total_words).
+void launch_words(const StreamLayout& lay, const uint32_t* d_states, uint64_t count,
+                  uint32_t* out, void* stream);
+
+// ---- path simulation (lmm_step.hip) ---------------------------------------------
+// Product description lowered for the kernel.
+struct KProduct {
+    int32_t type;         // FM_PRODUCT_*
+    int32_t eval_index;   // simulation time index at which it is evaluated (single-date)
+    int32_t first_j;      // first libor index of the periods
+    int32_t num_periods;
+    int32_t out_slot;     // reduction slot (single-date) or contribution column base (swap)
+    int32_t is_floorlet;
+    int32_t flags;        // bit0: pay fix (swap)
+    int32_t num_k;        // tenor index of the numeraire the payoff is divided by
+    double  notional;
+    double  exercise;
+    int32_t param_off;    // offset into the per-period parameter table
+    int32_t pad2;
+};
+
+// Per-period parameters (strike, period length, discounting tau, adjustment, notional)
+struct KPeriod {
+    double strike, period_length, tau, adj, notional;
+    int32_t j, pay_k;     // libor index and tenor index of the payment date
+};
+
+struct StepArgs {
+    int nT, n, F, P;
+    int precision;
+    int state_space;      // FM_STATE_SPACE_*
+    int store_paths;
+    const double* times;         // device nT+1
+    const int32_t* first_alive;  // device nT
+    const int32_t* tenor_time_index; // device n+1: process time index of T_k (rounded up)
+    const void* lam;             // device [nT][n][F] (double or float)
+    const void* hv;              // device [nT][n]   variance * (-0.5)
+    const double* dc1;           // device n: delta/delta
+    const double* dc2;           // device n: 1/delta
+    const double* delta;         // device n: T_{j+1}-T_j
+    const double* L0;            // device n: initial forwards in value space
+    const void* dw;              // device dW [P][nT][F]
+    double cap;
+    int32_t scheme;
+    // products
+    int num_products;
+    const KProduct* products;    // device
+    const KPeriod* periods;      // device
+    int num_slots;               // single-date product reduction slots
+    int num_contrib;             // swap contribution columns
+    double* contrib;             // device [num_contrib][P]
+    // outputs
+    double* wave_sum;            // [num_waves][num_slots + n + 1]
+    double* wave_m2;             // [num_waves][num_slots]
+    void* paths;                 // device triangle rows [rows][P] (store_paths)
+    const int32_t* row_of;       // device [(nT+1)*n] row index of (t,j), -1 = initial value
+    double* numeraire;           // device [(n+1)][P] unadjusted numeraire (store_paths)
+};
+void launch_lmm_step_fp64(const StepArgs& a, void* stream);
+void launch_lmm_step_fp32(const StepArgs& a, void* stream);
+
+// Deterministic reduction of per-wave partials: out_sum[col], out_m2[col] (Chan merge).
+void launch_reduce_partials(const double* wave_sum, const double* wave_m2, int num_waves, int stride,
+                            int num_slots, int num_sums, int P, double* out_sum, double* out_m2, void* stream);
+
+// Per-path swap values from period contributions and their scales, reduced into wave partials.
+void launch_swap_finalize(const double* contrib, int P, int num_swaps, const int32_t* col0, const int32_t* ncol,
+                          const int32_t* slot, const double* scale, double* wave_sum, double* wave_m2,
+                          int stride, void* stream);
+
+} // namespace fm

@@ -1,0 +1,197 @@
+// Fused LIBOR market model path kernel: one thread per path, the forward vector
+// in registers, the covariance tables read through the scalar cache.
+//
+// Per time step t_i -> t_{i+1} and alive component j (T_j > t_i), in the order of
+// LIBORMarketModelFromCovarianceModel.getDrift
+// (montecarlo/interestrate/models/LIBORMarketModelFromCovarianceModel.java:1144-1160,1183-1189)
+// and EulerSchemeFromProcessModel (montecarlo/process/EulerSchemeFromProcessModel.java:232-256):
+//   w    = 1 / (L_j * (d_j/d_j) + 1/d_j)       Scalar.discount (stochastic/Scalar.java:320-328)
+//   w    = w * L_j                              (log-normal state space)
+//   S_k  = S_k + w * lambda_jk                  ascending j, spot measure
+//   mu_j = sum_k S_k * lambda_jk + variance_j * (-0.5)
+//   Y    = log(L_j) + mu_j * dt + sum_k dW_k * lambda_jk   (EULER_FUNCTIONAL)
+//   L_j  = min(exp(Y), cap)
+// Products are evaluated on the fly at their exercise/fixing time index and reduced
+// per wavefront (sum and centred second moment); the numeraire is accumulated per
+// path in double precision (SPOT measure, LMM:1050-1066).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "fm_internal.hpp"
+
+namespace fm {
+
+constexpr int kStepThreads = 256;
+
+__device__ __forceinline__ double wave_reduce_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <typename real> __device__ __forceinline__ real dev_exp(real x);
+template <typename real> __device__ __forceinline__ real dev_log(real x);
+template <> __device__ __forceinline__ double dev_exp<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ double dev_log<double>(double x) { return log(x); }
+template <> __device__ __forceinline__ float dev_exp<float>(float x) { return expf(x); }
+template <> __device__ __forceinline__ float dev_log<float>(float x) { return logf(x); }
+
+// Java Math.max(x, 0.0) / Math.min(x, c): NaN propagates.
+template <typename real> __device__ __forceinline__ real jmax0(real x) { return (x != x) ? x : (x >= (real)0 ? x : (real)0); }
+template <typename real> __device__ __forceinline__ real jmin(real x, real c) { return (x > c) ? c : x; }
+
+// L[j] for a uniform runtime j without a dynamically indexed register array (which
+// the compiler would move to scratch memory): OR of the masked bit patterns.
+__device__ __forceinline__ uint64_t bits_of(double x) { return __double_as_longlong(x); }
+__device__ __forceinline__ uint32_t bits_of(float x) { return __float_as_uint(x); }
+__device__ __forceinline__ double from_bits(uint64_t b, double) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ float from_bits(uint32_t b, float) { return __uint_as_float(b); }
+
+template <int NCAP, typename real>
+__device__ __forceinline__ real pick(const real (&L)[NCAP], int j) {
+    decltype(bits_of(real(0))) acc = 0;
+#pragma unroll
+    for (int i = 0; i < NCAP; i++) acc |= (i == j) ? bits_of(L[i]) : 0;
+    return from_bits(acc, real(0));
+}
+
+// Reduce one per-path value into the wave partial of `slot`.
+__device__ __forceinline__ void reduce_slot(double x, bool active, int cnt, int lane, int64_t wave,
+                                            double* __restrict__ wave_sum, double* __restrict__ wave_m2,
+                                            int stride, int slot) {
+    const double s = wave_reduce_sum(active ? x : 0.0);
+    const double mean = (cnt > 0) ? s / (double)cnt : 0.0;
+    const double d = active ? (x - mean) : 0.0;
+    const double m2 = wave_reduce_sum(d * d);
+    if (lane == 0 && cnt > 0) {
+        wave_sum[(size_t)wave * stride + slot] = s;
+        wave_m2[(size_t)wave * stride + slot] = m2;
+    }
+}
+
+template <int NCAP, int FCAP, typename real, typename dwT>
+__global__ void __launch_bounds__(kStepThreads)
+lmm_fused_kernel(const StepArgs a,
+                 const real* __restrict__ lam, const real* __restrict__ hv,
+                 const double* __restrict__ dc1, const double* __restrict__ dc2,
+                 const double* __restrict__ delta, const double* __restrict__ times,
+                 const int32_t* __restrict__ first_alive, const int32_t* __restrict__ tenor_ti,
+                 const KProduct* __restrict__ products, const KPeriod* __restrict__ periods,
+                 const dwT* __restrict__ dw,
+                 double* __restrict__ wave_sum, double* __restrict__ wave_m2,
+                 double* __restrict__ contrib, real* __restrict__ paths,
+                 const int32_t* __restrict__ row_of, double* __restrict__ numeraire_out)
+{
+    const int64_t p = (int64_t)blockIdx.x * kStepThreads + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = p >> 6;
+    const int P = a.P, n = a.n, F = a.F, nT = a.nT;
+    const bool active = p < P;
+    const int64_t pc = active ? p : (int64_t)P - 1;
+    const int cnt = (int)((int64_t)P - wave * 64 > 64 ? 64 : ((int64_t)P - wave * 64));
+    const int stride = a.num_slots + n + 1;
+    const bool lognormal = a.state_space == FM_STATE_SPACE_LOGNORMAL;
+    const bool capped = !isinf(a.cap);
+    const real cap = (real)a.cap;
+
+    real L[NCAP];
+#pragma unroll
+    for (int j = 0; j < NCAP; j++) L[j] = (j < n) ? (real)a.L0[j] : (real)0;
+
+    double N = 1.0, Nprev = 1.0;   // unadjusted numeraire N(T_kcur) and N(T_kcur-1)
+    int kcur = 0;
+    int qp = 0;
+
+    for (int ti = 0; ti <= nT; ti++) {
+        // ---- numeraire N(T_{k+1}) = N(T_k).accrue(L_k(T_k), d_k) once L_k is final
+        while (kcur < n && tenor_ti[kcur] <= ti) {
+            const double Lk = (double)pick<NCAP, real>(L, kcur);
+            const double dk = delta[kcur];
+            double Nn;
+            if (kcur == 0)      Nn = Lk * (dk * 1.0) + 1.0;      // Scalar.accrue of Scalar 1.0
+            else if (kcur == 1) Nn = Lk * (dk * N) + N;          // Scalar.accrue of a stochastic rate
+            else                Nn = N * (1 + Lk * dk);          // RandomVariableFromDoubleArray.accrue
+            Nprev = N; N = Nn; kcur++;
+            // sum of 1/N(T_k) over paths (numeraire adjustment average)
+            const double s = wave_reduce_sum(active ? (1.0 / N) * 1.0 : 0.0);
+            if (lane == 0 && cnt > 0) wave_sum[(size_t)wave * stride + a.num_slots + kcur] = s;
+            if (numeraire_out && active) numeraire_out[(size_t)kcur * P + p] = N;
+        }
+
+        // ---- products evaluated at this time index
+        while (qp < a.num_products && products[qp].eval_index == ti) {
+            const KProduct pr = products[qp];
+            qp++;
+            const double Nsel = (pr.num_k == kcur) ? N : Nprev;
+            if (pr.type == FM_PRODUCT_SWAPTION) {
+                real V = (real)0;
+#pragma unroll
+                for (int j = NCAP - 1; j >= 0; j--) {
+                    if (j >= pr.first_j && j < pr.first_j + pr.num_periods) {
+                        const KPeriod pe = periods[pr.param_off + (j - pr.first_j)];
+                        const real Lj = L[j];
+                        const real payoff = ((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pr.notional;
+                        V = V + payoff;
+                        V = V / ((real)1.0 + Lj * (real)pe.tau);
+                        V = V * (real)pe.adj;
+                    }
+                }
+                const double x = (double)jmax0(V) / Nsel;
+                reduce_slot(x, active, cnt, lane, wave, wave_sum, wave_m2, stride, pr.out_slot);
+            } else if (pr.type == FM_PRODUCT_CAPLET) {
+                const KPeriod pe = periods[pr.param_off];
+                const real Lj = pick<NCAP, real>(L, pr.first_j);
+                real v = Lj - (real)pe.strike;
+                if (!pr.is_floorlet) v = jmax0(v) * (real)pe.notional;
+                else v = ((v != v) ? v : (v > (real)0 ? (real)0 : v)) * ((real)(-1.0) * (real)pe.notional);
+                const double x = (double)v / Nsel;
+                reduce_slot(x, active, cnt, lane, wave, wave_sum, wave_m2, stride, pr.out_slot);
+            } else {   // one period of a swap: contribution column, combined after the adjustment is known
+                const KPeriod pe = periods[pr.param_off];
+                const real Lj = pick<NCAP, real>(L, pr.first_j);
+                double payoff = (double)(((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pe.notional);
+                if (!(pr.flags & 1)) payoff = payoff * -1.0;
+                if (active) contrib[(size_t)pr.out_slot * P + p] = payoff / Nsel;
+            }
+        }
+
+        if (ti == nT) break;
+
+        // ---- Euler step ti -> ti+1
+        const int first = first_alive[ti];
+        const real dt = (real)(times[ti + 1] - times[ti]);
+        real dW[FCAP];
+#pragma unroll
+        for (int k = 0; k < FCAP; k++) dW[k] = (k < F) ? (real)dw[((size_t)pc * nT + ti) * F + k] : (real)0;
+        real S[FCAP];
+#pragma unroll
+        for (int k = 0; k < FCAP; k++) S[k] = (real)0;
+        const real* lam_i = lam + (size_t)ti * n * F;
+        const real* hv_i = hv + (size_t)ti * n;
+#pragma unroll
+        for (int j = 0; j < NCAP; j++) {
+            if (j < n && j >= first) {
+                const real Lj = L[j];
+                real w = (real)1.0 / (Lj * (real)dc1[j] + (real)dc2[j]);
+                if (lognormal) w = w * Lj;
+                const real* lj = lam_i + j * F;
+#pragma unroll
+                for (int k = 0; k < FCAP; k++) if (k < F) S[k] = S[k] + w * lj[k];
+                real mu = S[0] * lj[0];
+#pragma unroll
+                for (int k = 1; k < FCAP; k++) if (k < F) mu = mu + S[k] * lj[k];
+                mu = mu + hv_i[j];
+                real Y = lognormal ? dev_log<real>(Lj) : Lj;
+                Y = Y + mu * dt;
+#pragma unroll
+                for (int k = 0; k < FCAP; k++) if (k < F) Y = Y + dW[k] * lj[k];
+                real Ln = lognormal ? dev_exp<real>(Y) : Y;
+                if (capped) Ln = jmin(Ln, cap);
+                L[j] = Ln;
+                if (paths && active) paths[(size_t)row_of[(size_t)(ti + 1) * n + j] * P + p] = Ln;
+            }
+        }
+    }
+}
+
+} // namespace fm

@@ -1,0 +1,324 @@
+"""ctypes binding of the C ABI declared in include/finmath_lmm.h.
+
+The binding is the Python face of the drop-in boundary: plain pointers and sizes,
+status codes mapped onto the exception types the reference raises. It loads the
+in-tree shared library and fails loudly when it has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int32, c_int64, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libfinmath_lmm.so")
+
+FM_OK = 0
+FM_ERR_INVALID_ARGUMENT = 1
+FM_ERR_OUT_OF_MEMORY = 2
+FM_ERR_DEVICE = 3
+FM_ERR_NUMERICAL = 4
+FM_ERR_STATE = 5
+FM_ERR_INTERNAL = 6
+FM_ERR_UNSUPPORTED = 7
+
+MEASURE = {"SPOT": 0, "TERMINAL": 1}
+STATE_SPACE = {"NORMAL": 0, "LOGNORMAL": 1}
+SCHEME = {"EULER": 0, "PREDICTOR_CORRECTOR": 1, "EULER_FUNCTIONAL": 2, "PREDICTOR_CORRECTOR_FUNCTIONAL": 3}
+PRECISION = {"FP64": 0, "FP32_MIXED": 1}
+PRODUCT_SWAPTION, PRODUCT_SWAP, PRODUCT_CAPLET = 1, 2, 3
+
+# Every symbol include/finmath_lmm.h declares.
+EXPORTED_SYMBOLS = (
+    "fm_lmm_create", "fm_lmm_destroy", "fm_lmm_set_initial_forwards", "fm_lmm_set_covariance",
+    "fm_lmm_set_discount_curve", "fm_lmm_simulate", "fm_lmm_get_process_value",
+    "fm_lmm_process_value_device_ptr", "fm_lmm_get_numeraire", "fm_lmm_get_brownian_increment",
+    "fm_lmm_value_products", "fm_lmm_adjoint", "fm_mt19937_words", "fm_mt19937_doubles",
+    "fm_last_error", "fm_lmm_last_timings", "fm_device_info",
+)
+
+
+class CalculationException(Exception):
+    """Counterpart of net.finmath.exception.CalculationException."""
+
+
+class IllegalArgumentException(ValueError):
+    """Counterpart of java.lang.IllegalArgumentException."""
+
+
+class IllegalStateException(RuntimeError):
+    """Counterpart of java.lang.IllegalStateException."""
+
+
+class UnsupportedOperationException(NotImplementedError):
+    """Counterpart of java.lang.UnsupportedOperationException."""
+
+
+_EXC = {
+    FM_ERR_INVALID_ARGUMENT: IllegalArgumentException,
+    FM_ERR_OUT_OF_MEMORY: CalculationException,
+    FM_ERR_DEVICE: CalculationException,
+    FM_ERR_NUMERICAL: CalculationException,
+    FM_ERR_STATE: IllegalStateException,
+    FM_ERR_INTERNAL: CalculationException,
+    FM_ERR_UNSUPPORTED: UnsupportedOperationException,
+}
+
+
+class FmLmmConfig(ctypes.Structure):
+    _fields_ = [
+        ("num_time_steps", c_int32), ("times", POINTER(c_double)),
+        ("num_libors", c_int32), ("tenor", POINTER(c_double)),
+        ("num_factors", c_int32), ("num_paths", c_int32), ("seed", c_int32),
+        ("measure", c_int32), ("state_space", c_int32), ("scheme", c_int32), ("precision", c_int32),
+        ("libor_cap", c_double), ("store_paths", c_int32), ("checkpoint_stride", c_int32),
+    ]
+
+
+class FmProduct(ctypes.Structure):
+    _fields_ = [
+        ("type", c_int32), ("num_periods", c_int32), ("exercise_date", c_double),
+        ("fixing_dates", POINTER(c_double)), ("payment_dates", POINTER(c_double)),
+        ("period_lengths", POINTER(c_double)), ("swaprates", POINTER(c_double)),
+        ("notionals", POINTER(c_double)), ("discounting_adjustment", POINTER(c_double)),
+        ("notional", c_double), ("is_pay_fix", c_int32), ("is_floorlet", c_int32),
+    ]
+
+
+_lib = None
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built: run `python finmath-lib_amd/build.py` (or __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    dp = POINTER(c_double)
+    sig = {
+        "fm_lmm_create": ([POINTER(FmLmmConfig), POINTER(c_void_p)], c_int32),
+        "fm_lmm_destroy": ([c_void_p], None),
+        "fm_lmm_set_initial_forwards": ([c_void_p, dp], c_int32),
+        "fm_lmm_set_covariance": ([c_void_p, dp, dp], c_int32),
+        "fm_lmm_set_discount_curve": ([c_void_p, dp], c_int32),
+        "fm_lmm_simulate": ([c_void_p], c_int32),
+        "fm_lmm_get_process_value": ([c_void_p, c_int32, c_int32, dp], c_int32),
+        "fm_lmm_process_value_device_ptr": ([c_void_p, c_int32, c_int32, POINTER(c_void_p)], c_int32),
+        "fm_lmm_get_numeraire": ([c_void_p, c_double, dp], c_int32),
+        "fm_lmm_get_brownian_increment": ([c_void_p, c_int32, c_int32, dp], c_int32),
+        "fm_lmm_value_products": ([c_void_p, POINTER(FmProduct), c_int32, dp, dp], c_int32),
+        "fm_lmm_adjoint": ([c_void_p, POINTER(FmProduct), c_int32, dp, dp, dp, dp, dp], c_int32),
+        "fm_mt19937_words": ([c_int64, c_uint64, c_uint64, POINTER(c_uint32)], c_int32),
+        "fm_mt19937_doubles": ([c_int64, c_uint64, c_uint64, dp], c_int32),
+        "fm_last_error": ([], c_char_p),
+        "fm_lmm_last_timings": ([c_void_p, dp, c_int32], c_int32),
+        "fm_device_info": ([ctypes.c_char_p, c_int32], c_int32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != FM_OK:
+        msg = (load().fm_last_error() or b"").decode(errors="replace")
+        raise _EXC.get(status, CalculationException)(f"[status {status}] {msg}")
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(POINTER(c_double))
+
+
+def as_f64(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+class Product:
+    """Owns the arrays behind an FmProduct."""
+
+    def __init__(self, type_: int, exercise_date: float = 0.0, fixing_dates=None, payment_dates=None,
+                 period_lengths=None, swaprates=None, notionals=None, discounting_adjustment=None,
+                 notional: float = 1.0, is_pay_fix: bool = True, is_floorlet: bool = False):
+        self._keep = []
+        self.s = FmProduct()
+        self.s.type = type_
+        self.s.exercise_date = float(exercise_date)
+        self.s.notional = float(notional)
+        self.s.is_pay_fix = 1 if is_pay_fix else 0
+        self.s.is_floorlet = 1 if is_floorlet else 0
+        n = 0
+        for fname, arr in (("fixing_dates", fixing_dates), ("payment_dates", payment_dates),
+                           ("period_lengths", period_lengths), ("swaprates", swaprates),
+                           ("notionals", notionals), ("discounting_adjustment", discounting_adjustment)):
+            if arr is None:
+                setattr(self.s, fname, POINTER(c_double)())
+                continue
+            a = as_f64(arr)
+            self._keep.append(a)
+            setattr(self.s, fname, _dptr(a))
+            n = max(n, a.size)
+        self.s.num_periods = n
+
+    @staticmethod
+    def swaption(exercise_date, fixing_dates, payment_dates, swaprates, period_lengths=None, notional=1.0,
+                 discounting_adjustment=None):
+        return Product(PRODUCT_SWAPTION, exercise_date, fixing_dates, payment_dates, period_lengths, swaprates,
+                       None, discounting_adjustment, notional)
+
+    @staticmethod
+    def swap(fixing_dates, payment_dates, swaprates, notionals=None, is_pay_fix=True):
+        return Product(PRODUCT_SWAP, 0.0, fixing_dates, payment_dates, None, swaprates, notionals,
+                       is_pay_fix=is_pay_fix)
+
+    @staticmethod
+    def caplet(maturity, period_length, strike, daycount_fraction=None, is_floorlet=False):
+        dcf = period_length if daycount_fraction is None else daycount_fraction
+        return Product(PRODUCT_CAPLET, maturity, None, None, [period_length], [strike], None, None, dcf,
+                       is_floorlet=is_floorlet)
+
+
+def product_array(products):
+    arr = (FmProduct * len(products))()
+    for i, p in enumerate(products):
+        arr[i] = p.s
+    return arr
+
+
+class LmmContext:
+    """One simulation context of the engine (a model + Brownian driver + Euler process)."""
+
+    def __init__(self, times, tenor, num_factors: int, num_paths: int, seed: int, *, measure="SPOT",
+                 state_space="LOGNORMAL", scheme="EULER_FUNCTIONAL", precision="FP64", libor_cap=1e5,
+                 store_paths=False, checkpoint_stride=4):
+        self.lib = load()
+        self.times = as_f64(times)
+        self.tenor = as_f64(tenor)
+        cfg = FmLmmConfig()
+        cfg.num_time_steps = self.times.size - 1
+        cfg.times = _dptr(self.times)
+        cfg.num_libors = self.tenor.size - 1
+        cfg.tenor = _dptr(self.tenor)
+        cfg.num_factors = int(num_factors)
+        cfg.num_paths = int(num_paths)
+        cfg.seed = int(seed)
+        cfg.measure = MEASURE[measure]
+        cfg.state_space = STATE_SPACE[state_space]
+        cfg.scheme = SCHEME[scheme]
+        cfg.precision = PRECISION[precision]
+        cfg.libor_cap = float(libor_cap) if libor_cap is not None else math.inf
+        cfg.store_paths = 1 if store_paths else 0
+        cfg.checkpoint_stride = int(checkpoint_stride)
+        self.cfg = cfg
+        h = c_void_p()
+        check(self.lib.fm_lmm_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.n = cfg.num_libors
+        self.nT = cfg.num_time_steps
+        self.F = cfg.num_factors
+        self.P = cfg.num_paths
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fm_lmm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_initial_forwards(self, L0):
+        a = as_f64(L0)
+        check(self.lib.fm_lmm_set_initial_forwards(self.h, _dptr(a)))
+
+    def set_covariance(self, factor_loading, variance):
+        lam = as_f64(factor_loading).reshape(-1)
+        var = as_f64(variance).reshape(-1)
+        if lam.size != self.nT * self.n * self.F or var.size != self.nT * self.n:
+            raise IllegalArgumentException("covariance table shapes do not match the configuration")
+        check(self.lib.fm_lmm_set_covariance(self.h, _dptr(lam), _dptr(var)))
+
+    def set_discount_curve(self, df):
+        if df is None:
+            check(self.lib.fm_lmm_set_discount_curve(self.h, POINTER(c_double)()))
+        else:
+            a = as_f64(df)
+            check(self.lib.fm_lmm_set_discount_curve(self.h, _dptr(a)))
+
+    def simulate(self):
+        check(self.lib.fm_lmm_simulate(self.h))
+
+    def process_value(self, t: int, j: int) -> np.ndarray:
+        out = np.empty(self.P)
+        check(self.lib.fm_lmm_get_process_value(self.h, t, j, _dptr(out)))
+        return out
+
+    def numeraire(self, time: float) -> np.ndarray:
+        out = np.empty(self.P)
+        check(self.lib.fm_lmm_get_numeraire(self.h, float(time), _dptr(out)))
+        return out
+
+    def brownian_increment(self, t: int, k: int) -> np.ndarray:
+        out = np.empty(self.P)
+        check(self.lib.fm_lmm_get_brownian_increment(self.h, t, k, _dptr(out)))
+        return out
+
+    def value_products(self, products):
+        arr = product_array(products)
+        vals = np.empty(len(products))
+        errs = np.empty(len(products))
+        check(self.lib.fm_lmm_value_products(self.h, arr, len(products), _dptr(vals), _dptr(errs)))
+        return vals, errs
+
+    def adjoint(self, products, weights=None):
+        arr = product_array(products)
+        w = None if weights is None else as_f64(weights)
+        value = np.zeros(1)
+        dL0 = np.zeros(self.n)
+        dlam = np.zeros(self.nT * self.n * self.F)
+        dvar = np.zeros(self.nT * self.n)
+        check(self.lib.fm_lmm_adjoint(self.h, arr, len(products), POINTER(c_double)() if w is None else _dptr(w),
+                                      _dptr(value), _dptr(dL0), _dptr(dlam), _dptr(dvar)))
+        return value[0], dL0, dlam.reshape(self.nT, self.n, self.F), dvar.reshape(self.nT, self.n)
+
+    def timings(self) -> np.ndarray:
+        ms = np.zeros(5)
+        check(self.lib.fm_lmm_last_timings(self.h, _dptr(ms), 5))
+        return ms
+
+
+def mt19937_words(seed: int, offset: int, count: int) -> np.ndarray:
+    out = np.empty(count, dtype=np.uint32)
+    check(load().fm_mt19937_words(seed, offset, count, out.ctypes.data_as(POINTER(c_uint32))))
+    return out
+
+
+def mt19937_doubles(seed: int, offset: int, count: int) -> np.ndarray:
+    out = np.empty(count)
+    check(load().fm_mt19937_doubles(seed, offset, count, _dptr(out)))
+    return out
+
+
+def device_info() -> str:
+    buf = ctypes.create_string_buffer(256)
+    check(load().fm_device_info(buf, 256))
+    return buf.value.decode()

@@ -1,0 +1,219 @@
+/*
+ * finmath_lmm.h -- C ABI of the MI355X LIBOR Market Model Monte Carlo engine.
+ *
+ * The engine replaces, for one GPU, the hot path of finmath-lib's
+ *   EulerSchemeFromProcessModel over LIBORMarketModelFromCovarianceModel
+ * driven by BrownianMotionFromMersenneRandomNumbers, together with the payoff
+ * valuation of the interest-rate products that read that process.
+ *
+ * Reference paths below are relative to src/main/java/net/finmath/ of finmath-lib.
+ *
+ * Conventions
+ *  - Plain C: opaque handles, plain pointers and sizes; no C++ or torch types.
+ *  - All array arguments are host memory owned by the caller unless a function
+ *    says "device pointer". Device buffers are owned by the context.
+ *  - Every function returns an int status (FM_OK = 0). On error a message is
+ *    available from fm_last_error() (thread-local). Status codes map onto the
+ *    reference's exceptions: FM_ERR_INVALID_ARGUMENT -> IllegalArgumentException
+ *    (the reference validates with Validate.isTrue, e.g.
+ *    montecarlo/BrownianMotionFromMersenneRandomNumbers.java:77-78); the device,
+ *    memory and numerical codes -> CalculationException
+ *    (exception/CalculationException.java) or the RuntimeException the Euler
+ *    scheme throws (montecarlo/process/EulerSchemeFromProcessModel.java:212-214).
+ *  - Contexts are independent: distinct contexts may be used concurrently from
+ *    different threads (one HIP stream per context). Calls on one context are
+ *    serialised by an internal mutex.
+ */
+#ifndef FINMATH_LMM_H
+#define FINMATH_LMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------- */
+#define FM_OK                   0
+#define FM_ERR_INVALID_ARGUMENT 1
+#define FM_ERR_OUT_OF_MEMORY    2
+#define FM_ERR_DEVICE           3
+#define FM_ERR_NUMERICAL        4
+#define FM_ERR_STATE            5
+#define FM_ERR_INTERNAL         6
+#define FM_ERR_UNSUPPORTED      7
+
+/* Enumerations use the ordinals of the reference enums. */
+/* LIBORMarketModelFromCovarianceModel.Measure (models/LIBORMarketModelFromCovarianceModel.java:165) */
+#define FM_MEASURE_SPOT      0
+#define FM_MEASURE_TERMINAL  1
+/* LIBORMarketModelFromCovarianceModel.StateSpace (models/LIBORMarketModelFromCovarianceModel.java:166) */
+#define FM_STATE_SPACE_NORMAL    0
+#define FM_STATE_SPACE_LOGNORMAL 1
+/* EulerSchemeFromProcessModel.Scheme (montecarlo/process/EulerSchemeFromProcessModel.java:68-73) */
+#define FM_SCHEME_EULER                           0
+#define FM_SCHEME_PREDICTOR_CORRECTOR             1
+#define FM_SCHEME_EULER_FUNCTIONAL                2
+#define FM_SCHEME_PREDICTOR_CORRECTOR_FUNCTIONAL  3
+/* Arithmetic of the path simulation. FP64: everything in double (the plain form,
+ * RandomVariableFromDoubleArray semantics). FP32_MIXED: drift, exp/log, loadings
+ * and payoffs in float; numeraire, payoff/numeraire and all sums in double. */
+#define FM_PRECISION_FP64        0
+#define FM_PRECISION_FP32_MIXED  1
+
+typedef struct fm_lmm fm_lmm;   /* one simulation: model + Brownian driver + Euler process */
+
+/* Shape and options of a simulation.
+ * Replaces the constructor arguments of
+ *   TimeDiscretizationFromArray (time/TimeDiscretizationFromArray.java:210-217),
+ *   BrownianMotionFromMersenneRandomNumbers(td, factors, paths, seed)
+ *     (montecarlo/BrownianMotionFromMersenneRandomNumbers.java:70-104),
+ *   LIBORMarketModelFromCovarianceModel properties measure/stateSpace/liborCap
+ *     (models/LIBORMarketModelFromCovarianceModel.java:400-414) and
+ *   EulerSchemeFromProcessModel(model, driver[, scheme])
+ *     (montecarlo/process/EulerSchemeFromProcessModel.java:96-123). */
+typedef struct fm_lmm_config {
+    int32_t       num_time_steps;   /* nT >= 1 */
+    const double* times;            /* nT+1 simulation times, increasing, times[0] = 0 */
+    int32_t       num_libors;       /* n >= 1 (number of forward rates = components) */
+    const double* tenor;            /* n+1 tenor dates T_0 = 0 < T_1 < ... < T_n */
+    int32_t       num_factors;      /* F >= 1 */
+    int32_t       num_paths;        /* P >= 1 */
+    int32_t       seed;             /* Java int seed of the Mersenne Twister */
+    int32_t       measure;          /* FM_MEASURE_* */
+    int32_t       state_space;      /* FM_STATE_SPACE_* */
+    int32_t       scheme;           /* FM_SCHEME_* */
+    int32_t       precision;        /* FM_PRECISION_* */
+    double        libor_cap;        /* liborCap; +infinity disables the cap (default 1e5) */
+    int32_t       store_paths;      /* 1: fm_lmm_simulate materialises the process */
+    int32_t       checkpoint_stride;/* adjoint forward-state checkpoint stride (0 -> 4) */
+} fm_lmm_config;
+
+/* Create a context (allocates device memory, seeds the generator).
+ * Replaces the constructors listed at fm_lmm_config. */
+int fm_lmm_create(const fm_lmm_config* config, fm_lmm** out);
+void fm_lmm_destroy(fm_lmm* ctx);
+
+/* Initial forwards L_j(0), j < n, in value space.
+ * Replaces LIBORMarketModelFromCovarianceModel.getInitialState
+ * (models/LIBORMarketModelFromCovarianceModel.java:1081-1093). */
+int fm_lmm_set_initial_forwards(fm_lmm* ctx, const double* forwards);
+
+/* Covariance tables for every simulation time index i < nT and component j < n:
+ *   factor_loading[(i*n + j)*F + k] = lambda_{j,k}(t_i)
+ *   variance[i*n + j]               = instantaneous variance used by the
+ *                                     log-coordinate drift adjustment.
+ * Replaces LIBORMarketModelFromCovarianceModel.getFactorLoading
+ * (models/LIBORMarketModelFromCovarianceModel.java:1194-1197) -> covariance
+ * model getFactorLoading/getCovariance
+ * (models/covariance/AbstractLIBORCovarianceModel.java:45-76,
+ *  models/covariance/LIBORCovarianceModelFromVolatilityAndCorrelation.java:48-93). */
+int fm_lmm_set_covariance(fm_lmm* ctx, const double* factor_loading, const double* variance);
+
+/* Discount factors P^d(T_k), k <= n, of the model's discount curve, or NULL for a
+ * model without discount curve (no numeraire adjustment).
+ * Replaces the deterministic numeraire adjustment of
+ * LIBORMarketModelFromCovarianceModel.getNumeraireDefaultableZeroBondAsOfTimeZero
+ * (models/LIBORMarketModelFromCovarianceModel.java:915-944). */
+int fm_lmm_set_discount_curve(fm_lmm* ctx, const double* discount_factors);
+
+/* ---- process (materialised) --------------------------------------------------
+ * Simulate and keep the process in device memory (requires store_paths = 1).
+ * Replaces EulerSchemeFromProcessModel.doPrecalculateProcess
+ * (montecarlo/process/EulerSchemeFromProcessModel.java:170-326). */
+int fm_lmm_simulate(fm_lmm* ctx);
+
+/* P realisations of L_j(t_i). Replaces getProcessValue(timeIndex, componentIndex)
+ * (montecarlo/process/EulerSchemeFromProcessModel.java:131-146) and
+ * LIBORMarketModelFromCovarianceModel.getLIBOR (models/...CovarianceModel.java:1307-1312). */
+int fm_lmm_get_process_value(fm_lmm* ctx, int32_t time_index, int32_t component, double* out);
+
+/* Device pointer (double[P], or float[P] for FP32_MIXED) of the same values, valid
+ * until the next simulate or destroy. For a GPU-resident RandomVariable. */
+int fm_lmm_process_value_device_ptr(fm_lmm* ctx, int32_t time_index, int32_t component, const void** out);
+
+/* P realisations of the numeraire N(T) at a tenor date (adjusted by the discount
+ * curve when one is set). Replaces LIBORMarketModelFromCovarianceModel.getNumeraire
+ * (models/LIBORMarketModelFromCovarianceModel.java:860-876,1017-1074). */
+int fm_lmm_get_numeraire(fm_lmm* ctx, double time, double* out);
+
+/* P realisations of the Brownian increment dW_k(t_i).
+ * Replaces BrownianMotionFromMersenneRandomNumbers.getBrownianIncrement
+ * (montecarlo/BrownianMotionFromMersenneRandomNumbers.java:123-136). */
+int fm_lmm_get_brownian_increment(fm_lmm* ctx, int32_t time_index, int32_t factor, double* out);
+
+/* ---- products ---------------------------------------------------------------- */
+#define FM_PRODUCT_SWAPTION  1  /* interestrate/products/Swaption.java:137-199 */
+#define FM_PRODUCT_SWAP      2  /* interestrate/products/SimpleSwap.java:137-170 */
+#define FM_PRODUCT_CAPLET    3  /* interestrate/products/Caplet.java:115-140 (value unit VALUE) */
+
+/* One product. Dates are model times; arrays have num_periods entries.
+ *  SWAPTION: exercise_date, fixing/payment dates, period_lengths (NULL ->
+ *            payment - fixing), swaprates, notional, discounting_adjustment
+ *            (per period, NULL -> 1.0 as when the discount curve is the forward
+ *            curve's own, Swaption.java:165-175).
+ *  SWAP:     fixing/payment dates, swaprates, notionals (NULL -> 1.0), is_pay_fix.
+ *  CAPLET:   exercise_date = maturity, period_lengths[0] = period length,
+ *            swaprates[0] = strike, notional = daycount fraction, is_floorlet. */
+typedef struct fm_product {
+    int32_t       type;
+    int32_t       num_periods;
+    double        exercise_date;
+    const double* fixing_dates;
+    const double* payment_dates;
+    const double* period_lengths;
+    const double* swaprates;
+    const double* notionals;
+    const double* discounting_adjustment;
+    double        notional;
+    int32_t       is_pay_fix;
+    int32_t       is_floorlet;
+} fm_product;
+
+/* Value products at evaluation time 0 in one fused simulation pass:
+ * values[q] = getValue(model) (AbstractMonteCarloProduct.java:82-84),
+ * errors[q] = MC standard error of getValues (AbstractMonteCarloProduct.java:87-104). */
+int fm_lmm_value_products(fm_lmm* ctx, const fm_product* products, int32_t count,
+                          double* values, double* errors);
+
+/* ---- pathwise adjoint (Greeks) -----------------------------------------------
+ * Gradient of V = sum_q weights[q] * value_q (weights NULL -> all 1) with respect
+ * to the model inputs, by a hand-written reverse sweep of the Euler scheme with
+ * forward-state checkpoints every checkpoint_stride steps:
+ *   d_initial_forwards[j]             = dV/dL_j(0)
+ *   d_factor_loading[(i*n+j)*F + k]   = dV/dlambda_{j,k}(t_i)
+ *   d_variance[i*n + j]               = dV/dvariance_j(t_i)
+ * (any output may be NULL). The numeraire-adjustment average is held constant,
+ * as the reference's AAD treats getAverage results
+ * (montecarlo/automaticdifferentiation/backward/RandomVariableDifferentiableAAD.java:165-219).
+ * Replaces RandomVariableDifferentiable.getGradient
+ * (montecarlo/automaticdifferentiation/RandomVariableDifferentiable.java:64-75)
+ * for the leaves of LIBORMarketModelFromCovarianceModel.getModelParameters
+ * (models/LIBORMarketModelFromCovarianceModel.java:1699-1732). */
+int fm_lmm_adjoint(fm_lmm* ctx, const fm_product* products, int32_t count, const double* weights,
+                   double* value, double* d_initial_forwards, double* d_factor_loading, double* d_variance);
+
+/* ---- generator ----------------------------------------------------------------
+ * The GPU MT19937 stream of commons-math3 MersenneTwister(long seed) as used by
+ * the reference (randomnumbers/MersenneTwister.java:26-51): 32-bit output words
+ * [offset, offset+count) and nextDouble() values [offset, offset+count). */
+int fm_mt19937_words(int64_t seed, uint64_t offset, uint64_t count, uint32_t* out);
+int fm_mt19937_doubles(int64_t seed, uint64_t offset, uint64_t count, double* out);
+
+/* ---- diagnostics --------------------------------------------------------------- */
+const char* fm_last_error(void);
+
+/* Per-stage device time of the last fm_lmm_value_products / fm_lmm_simulate /
+ * fm_lmm_adjoint call, measured with HIP events on the context's stream:
+ * ms[0] generator, ms[1] path step, ms[2] reductions, ms[3] adjoint sweep,
+ * ms[4] whole call. */
+int fm_lmm_last_timings(fm_lmm* ctx, double* ms, int32_t count);
+
+/* Library version string and the device it runs on. */
+int fm_device_info(char* buffer, int32_t length);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FINMATH_LMM_H */

@@ -1,0 +1,873 @@
+// CPU oracle for the LIBOR Market Model Monte Carlo hot path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is a plain, sequential restatement of the
+// reference algorithm (finmath-lib, Java) used to check the HIP implementation.
+// Only tests/, the smoke check and the CPU-baseline leg of bench.py may load it.
+// The product path never links, loads or calls anything in this directory.
+//
+// Every function names the reference file:line it restates. Paths are relative to
+// src/main/java/net/finmath/ of the reference.
+//
+// Floating point: compiled with -O2 -ffp-contract=off (no fused multiply-add, as
+// the JVM never fuses), sequential loops in the reference's order, Kahan averages.
+//
+// Parity status of each part (what pins it) is listed in the header comment of
+// each section. Anything without a pin says "parity unpinned".
+
+#include <cstdint>
+#include <cstring>
+#include <cmath>
+#include <vector>
+#include <memory>
+#include <algorithm>
+#include <string>
+#include <stdexcept>
+#include <map>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// MT19937 with the seeding of Apache commons-math3 3.6.1 MersenneTwister(long)
+// (third-party dependency of the reference: pom.xml:22,157-160; wrapped by
+// randomnumbers/MersenneTwister.java:26-29,42-51).
+//
+// Published algorithm restated: Matsumoto & Nishimura (1998), mt19937ar.c of
+// 2002-01-26 (init_genrand, init_by_array, genrand_int32). commons-math3's
+// setSeed(long s) passes the key {(int)(s>>>32), (int)(s & 0xffffffff)} to
+// init_by_array; nextDouble() composes ((long)next(26) << 26 | next(26)) * 2^-52,
+// i.e. two 32-bit outputs per double, the first word giving the high bits.
+// Pins: numpy's MT19937 (init_by_array seeding, raw 32-bit output) for the word
+// stream; the commons seeding key and 26+26 composition are "parity unpinned"
+// (no reference test pins them; restated from the published commons source).
+// ---------------------------------------------------------------------------
+
+struct orc_mt_state { uint32_t mt[624]; int mti; };
+
+static void mt_init_genrand(orc_mt_state* s, uint32_t seed) {
+    s->mt[0] = seed;
+    for (int i = 1; i < 624; i++)
+        s->mt[i] = 1812433253u * (s->mt[i - 1] ^ (s->mt[i - 1] >> 30)) + (uint32_t)i;
+    s->mti = 624;
+}
+
+static void mt_init_by_array(orc_mt_state* s, const uint32_t* key, int len) {
+    mt_init_genrand(s, 19650218u);
+    int i = 1, j = 0;
+    for (int k = (624 > len ? 624 : len); k; k--) {
+        s->mt[i] = (s->mt[i] ^ ((s->mt[i - 1] ^ (s->mt[i - 1] >> 30)) * 1664525u)) + key[j] + (uint32_t)j;
+        i++; j++;
+        if (i >= 624) { s->mt[0] = s->mt[623]; i = 1; }
+        if (j >= len) j = 0;
+    }
+    for (int k = 623; k; k--) {
+        s->mt[i] = (s->mt[i] ^ ((s->mt[i - 1] ^ (s->mt[i - 1] >> 30)) * 1566083941u)) - (uint32_t)i;
+        i++;
+        if (i >= 624) { s->mt[0] = s->mt[623]; i = 1; }
+    }
+    s->mt[0] = 0x80000000u;
+    s->mti = 624;
+}
+
+static uint32_t mt_next(orc_mt_state* s) {
+    if (s->mti >= 624) {
+        for (int k = 0; k < 624; k++) {
+            uint32_t y = (s->mt[k] & 0x80000000u) | (s->mt[(k + 1) % 624] & 0x7fffffffu);
+            s->mt[k] = s->mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+        s->mti = 0;
+    }
+    uint32_t y = s->mt[s->mti++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+// commons-math3 MersenneTwister.setSeed(long): key {(int)(seed>>>32), (int)seed}.
+static void mt_seed_commons(orc_mt_state* s, int64_t seed) {
+    uint64_t u = (uint64_t)seed;
+    uint32_t key[2] = { (uint32_t)(u >> 32), (uint32_t)(u & 0xffffffffu) };
+    mt_init_by_array(s, key, 2);
+}
+
+// commons-math3 BitsStreamGenerator.nextDouble(): 26 high bits then 26 low bits.
+static double mt_next_double(orc_mt_state* s) {
+    uint64_t hi = (uint64_t)(mt_next(s) >> 6);
+    uint64_t lo = (uint64_t)(mt_next(s) >> 6);
+    return (double)((hi << 26) | lo) * 0x1.0p-52;
+}
+
+// Seeded state (624 words, before the first twist) for a reference seed.
+int orc_mt_seeded_state(int64_t seed, uint32_t* out624) {
+    orc_mt_state s; mt_seed_commons(&s, seed);
+    std::memcpy(out624, s.mt, sizeof(s.mt));
+    return 0;
+}
+
+// Tempered output words [offset, offset+count) of the stream for a seed.
+int orc_mt_words(int64_t seed, uint64_t offset, uint64_t count, uint32_t* out) {
+    orc_mt_state s; mt_seed_commons(&s, seed);
+    for (uint64_t i = 0; i < offset; i++) (void)mt_next(&s);
+    for (uint64_t i = 0; i < count; i++) out[i] = mt_next(&s);
+    return 0;
+}
+
+// Same generator from an explicit init_by_array key (for published known answers).
+int orc_mt_words_by_array(const uint32_t* key, int len, uint64_t count, uint32_t* out) {
+    orc_mt_state s; mt_init_by_array(&s, key, len);
+    for (uint64_t i = 0; i < count; i++) out[i] = mt_next(&s);
+    return 0;
+}
+
+// Continue the generator from an explicit 624-word state and read position (the
+// state layout of numpy's MT19937 bit generator, used to replay its test set).
+int orc_mt_words_from_state(const uint32_t* state624, int pos, uint64_t count, uint32_t* out) {
+    orc_mt_state s; std::memcpy(s.mt, state624, sizeof(s.mt)); s.mti = pos;
+    for (uint64_t i = 0; i < count; i++) out[i] = mt_next(&s);
+    return 0;
+}
+
+int orc_mt_doubles(int64_t seed, uint64_t offset_doubles, uint64_t count, double* out) {
+    orc_mt_state s; mt_seed_commons(&s, seed);
+    for (uint64_t i = 0; i < 2 * offset_doubles; i++) (void)mt_next(&s);
+    for (uint64_t i = 0; i < count; i++) out[i] = mt_next_double(&s);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Wichura AS241 (PPND16) inverse normal CDF.
+// Restates functions/NormalDistribution.java:67-162 (published algorithm:
+// M. J. Wichura, Appl. Statist. 37(3), 1988). The coefficient table is the
+// published one. Special case of the reference: p <= 0 or p >= 1 in the tail
+// branch returns 0 (NormalDistribution.java:141-143).
+// Pins: scipy.special.ndtri (relative 1e-15 over the central and tail ranges),
+// symmetry icdf(1-p) = -icdf(p), icdf(0.5) = 0, icdf(0) = 0 (reference quirk).
+// ---------------------------------------------------------------------------
+double orc_icdf(double p) {
+    const double split1 = 0.425, split2 = 5.0, const1 = 0.180625, const2 = 1.6;
+    static const double A[8] = {3.3871328727963666080e+00, 1.3314166789178437745e+02, 1.9715909503065514427e+03,
+        1.3731693765509461125e+04, 4.5921953931549871457e+04, 6.7265770927008700853e+04,
+        3.3430575583588128105e+04, 2.5090809287301226727e+03};
+    static const double B[8] = {1.0, 4.2313330701600911252e+01, 6.8718700749205790830e+02, 5.3941960214247511077e+03,
+        2.1213794301586595867e+04, 3.9307895800092710610e+04, 2.8729085735721942674e+04,
+        5.2264952788528545610e+03};
+    static const double C[8] = {1.42343711074968357734e+00, 4.63033784615654529590e+00, 5.76949722146069140550e+00,
+        3.64784832476320460504e+00, 1.27045825245236838258e+00, 2.41780725177450611770e-01,
+        2.27238449892691845833e-02, 7.74545014278341407640e-04};
+    static const double D[8] = {1.0, 2.05319162663775882187e+00, 1.67638483018380384940e+00, 6.89767334985100004550e-01,
+        1.48103976427480074590e-01, 1.51986665636164571966e-02, 5.47593808499534494600e-04,
+        1.05075007164441684324e-09};
+    static const double E[8] = {6.65790464350110377720e+00, 5.46378491116411436990e+00, 1.78482653991729133580e+00,
+        2.96560571828504891230e-01, 2.65321895265761230930e-02, 1.24266094738807843860e-03,
+        2.71155556874348757815e-05, 2.01033439929228813265e-07};
+    static const double Fc[8] = {1.0, 5.99832206555887937690e-01, 1.36929880922735805310e-01, 1.48753612908506148525e-02,
+        7.86869131145613259100e-04, 1.84631831751005468180e-05, 1.42151175831644588870e-07,
+        2.04426310338993978564e-15};
+    // Horner forms exactly as the published algorithm: ((c7 r + c6) r + ...) r + c0.
+    auto horner = [](const double* c, double r) {
+        double v = c[7];
+        for (int i = 6; i >= 0; i--) v = v * r + c[i];
+        return v;
+    };
+    double q = p - 0.5;
+    if (std::fabs(q) <= split1) {
+        double r = const1 - q * q;
+        return q * horner(A, r) / horner(B, r);
+    }
+    double r = (q < 0.0) ? p : 1.0 - p;
+    if (r <= 0.0) return 0.0;
+    r = std::sqrt(-std::log(r));
+    double v;
+    if (r <= split2) { r -= const2; v = horner(C, r) / horner(D, r); }
+    else { r -= split2; v = horner(E, r) / horner(Fc, r); }
+    if (q < 0.0) v = -v;
+    return v;
+}
+
+void orc_icdf_array(const double* p, int64_t n, double* out) {
+    for (int64_t i = 0; i < n; i++) out[i] = orc_icdf(p[i]);
+}
+
+// ---------------------------------------------------------------------------
+// TimeDiscretizationFromArray tick rounding (time/TimeDiscretizationFromArray.java:39,387-389).
+// Pins: closed form (k*0.5 survives rounding to the 1/8760 tick exactly).
+// ---------------------------------------------------------------------------
+double orc_round_time(double t) {
+    const double tick = 1.0 / (365.0 * 24.0);
+    return std::rint(t / tick) * tick;
+}
+
+// Java Arrays.binarySearch on a sorted array: index, or -(insertion point)-1.
+static int java_binary_search(const double* a, int n, double key) {
+    int lo = 0, hi = n - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        double v = a[mid];
+        if (v < key) lo = mid + 1;
+        else if (v > key) hi = mid - 1;
+        else return mid;
+    }
+    return -(lo + 1);
+}
+
+int orc_time_index(const double* times, int n, double t) {
+    return java_binary_search(times, n, orc_round_time(t));
+}
+
+// ---------------------------------------------------------------------------
+// BrownianMotionFromMersenneRandomNumbers.doGenerateBrownianMotion
+// (montecarlo/BrownianMotionFromMersenneRandomNumbers.java:141-191): loop path,
+// then time, then factor; dW = ICDF(nextDouble) * sqrt(dt_t).
+// Output layout here: [nT][F][P] as the reference's brownianIncrementsArray.
+// Pins: composition of the pinned MT words and AS241; statistical moments.
+// ---------------------------------------------------------------------------
+int orc_brownian(const double* times, int nT, int F, int P, int32_t seed, double* out) {
+    orc_mt_state s; mt_seed_commons(&s, (int64_t)seed);
+    std::vector<double> sq(nT);
+    for (int t = 0; t < nT; t++) sq[t] = std::sqrt(times[t + 1] - times[t]);
+    for (int p = 0; p < P; p++)
+        for (int t = 0; t < nT; t++)
+            for (int k = 0; k < F; k++) {
+                double u = mt_next_double(&s);
+                out[((size_t)t * F + k) * P + p] = orc_icdf(u) * sq[t];
+            }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric eigen decomposition by cyclic Jacobi rotations (textbook method,
+// Golub & Van Loan 8.5). The reference calls commons-math3 EigenDecomposition
+// (functions/LinearAlgebra.java:406); eigenvectors of distinct eigenvalues agree
+// up to sign, which the factor reduction fixes. Pins: numpy.linalg.eigh.
+// evecs: column c is the eigenvector of evals[c]; row-major [n][n].
+// ---------------------------------------------------------------------------
+int orc_eigen_sym(const double* a_in, int n, double* evals, double* evecs) {
+    std::vector<double> a(a_in, a_in + (size_t)n * n);
+    for (int i = 0; i < n; i++) for (int j = 0; j < n; j++) evecs[i * n + j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0;
+        for (int i = 0; i < n; i++) for (int j = i + 1; j < n; j++) off += a[i * n + j] * a[i * n + j];
+        if (off < 1e-300) break;
+        for (int p = 0; p < n; p++) {
+            for (int q = p + 1; q < n; q++) {
+                double apq = a[p * n + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                double app = a[p * n + p], aqq = a[q * n + q];
+                double theta = (aqq - app) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; k++) {
+                    double akp = a[k * n + p], akq = a[k * n + q];
+                    a[k * n + p] = c * akp - s * akq;
+                    a[k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    double apk = a[p * n + k], aqk = a[q * n + k];
+                    a[p * n + k] = c * apk - s * aqk;
+                    a[q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    double vkp = evecs[k * n + p], vkq = evecs[k * n + q];
+                    evecs[k * n + p] = c * vkp - s * vkq;
+                    evecs[k * n + q] = s * vkp + c * vkq;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < n; i++) evals[i] = a[i * n + i];
+    return 0;
+}
+
+// LinearAlgebra.getFactorMatrixUsingCommonsMath (functions/LinearAlgebra.java:392-445):
+// sort eigenvalues descending, sign so that v[0] > 0, scale by sqrt(max(lambda,0)/|v|^2).
+static void factor_matrix(const double* corr, int n, int m, double* out) {
+    std::vector<double> ev(n), V((size_t)n * n);
+    orc_eigen_sym(corr, n, ev.data(), V.data());
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; i++) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ev[a] > ev[b]; });
+    for (int f = 0; f < m; f++) {
+        int c = idx[f];
+        double lambda = ev[c];
+        double sign = V[0 * n + c] > 0.0 ? 1.0 : -1.0;
+        double norm2 = 0.0;
+        for (int r = 0; r < n; r++) norm2 += V[r * n + c] * V[r * n + c];
+        lambda = std::max(lambda, 0.0);
+        for (int r = 0; r < n; r++) out[r * m + f] = sign * std::sqrt(lambda / norm2) * V[r * n + c];
+    }
+}
+
+// LinearAlgebra.factorReductionUsingCommonsMath (functions/LinearAlgebra.java:454-482):
+// PCA, renormalise rows, PCA again of F F^T.
+int orc_factor_reduction(const double* corr, int n, int m, double* out) {
+    std::vector<double> f((size_t)n * m);
+    factor_matrix(corr, n, m, f.data());
+    for (int r = 0; r < n; r++) {
+        double ss = 0.0;
+        for (int k = 0; k < m; k++) ss += f[r * m + k] * f[r * m + k];
+        if (ss != 0.0) { for (int k = 0; k < m; k++) f[r * m + k] = f[r * m + k] / std::sqrt(ss); }
+        else { for (int k = 0; k < m; k++) f[r * m + k] = 1.0; }
+    }
+    std::vector<double> red((size_t)n * n);
+    for (int r = 0; r < n; r++)
+        for (int c = 0; c < n; c++) {
+            double s = 0.0;
+            for (int k = 0; k < m; k++) s += f[r * m + k] * f[c * m + k];
+            red[r * n + c] = s;
+        }
+    factor_matrix(red.data(), n, m, out);
+    return 0;
+}
+
+// LIBORCorrelationModelExponentialDecay.initialize
+// (montecarlo/interestrate/models/covariance/LIBORCorrelationModelExponentialDecay.java:102-134):
+// rho_ij = exp(-max(a,0)|T_i-T_j|), factor reduction, correlation reset to F F^T with unit diagonal.
+int orc_corr_expdecay(const double* tenor, int n, double a, int m, double* factors, double* corr_out) {
+    a = std::max(a, 0.0);
+    std::vector<double> rho((size_t)n * n);
+    for (int r = 0; r < n; r++) for (int c = 0; c < n; c++) rho[r * n + c] = std::exp(-a * std::fabs(tenor[r] - tenor[c]));
+    orc_factor_reduction(rho.data(), n, m, factors);
+    if (corr_out) {
+        for (int c1 = 0; c1 < n; c1++) {
+            for (int c2 = 0; c2 < c1; c2++) {
+                double s = 0.0;
+                for (int k = 0; k < m; k++) s += factors[c1 * m + k] * factors[c2 * m + k];
+                corr_out[c1 * n + c2] = s; corr_out[c2 * n + c1] = s;
+            }
+            corr_out[c1 * n + c1] = 1.0;
+        }
+    }
+    return 0;
+}
+
+// LIBORVolatilityModelFourParameterExponentialForm.getVolatility
+// (.../covariance/LIBORVolatilityModelFourParameterExponentialForm.java:168-195), Scalar
+// arithmetic: a.addProduct(b,tau) = b*tau + a (stochastic/Scalar.java:350-352),
+// .mult(c.mult(-tau).exp()).add(d). tau <= 0 gives 0.
+int orc_vol_fourparam(const double* times, int nT, const double* tenor, int n,
+                      double a, double b, double c, double d, double* vol) {
+    for (int i = 0; i < nT; i++)
+        for (int j = 0; j < n; j++) {
+            double tau = tenor[j] - times[i];
+            vol[i * n + j] = (tau <= 0) ? 0.0 : (b * tau + a) * std::exp(c * (-tau)) + d;
+        }
+    return 0;
+}
+
+// ForwardCurveInterpolation (LINEAR on VALUE, CONSTANT extrapolation) value at a time
+// (marketdata/model/curves/ForwardCurveInterpolation.java:298-347,
+//  interpolation/RationalFunctionInterpolation.java:184-226). At a node the node value.
+double orc_forward_curve(const double* nodeT, const double* nodeF, int nn, double t) {
+    int idx = java_binary_search(nodeT, nn, t);
+    if (idx >= 0) return nodeF[idx];
+    int iv = -idx - 2;
+    if (iv < 0) return nodeF[0];
+    if (iv > nn - 2) return nodeF[nn - 1];
+    // linear rational function: c0 + c1*(x-x_i)
+    double c0 = nodeF[iv], c1 = (nodeF[iv + 1] - nodeF[iv]) / (nodeT[iv + 1] - nodeT[iv]);
+    double x = t - nodeT[iv];
+    double num = 0.0, pw = 1.0;
+    num += c0 * pw; pw *= x;
+    num += c1 * pw;
+    return num;
+}
+
+// DiscountCurveFromForwardCurve.getDiscountFactor
+// (marketdata/model/curves/DiscountCurveFromForwardCurve.java:115-143), timeScaling 1.
+double orc_df_from_forward_curve(const double* nodeT, const double* nodeF, int nn, double paymentOffset, double maturity) {
+    double time = 0.0, df = 1.0;
+    while (time < maturity) {
+        df /= 1.0 + orc_forward_curve(nodeT, nodeF, nn, time) * std::min(paymentOffset, maturity - time) * 1.0;
+        time += paymentOffset;
+    }
+    return df;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// RandomVariable semantics: a value that is either deterministic or an array of
+// path realisations, with the type priority of Scalar (0) and
+// RandomVariableFromDoubleArray (1). Each op restates the one of
+// stochastic/Scalar.java:200-371 or montecarlo/RandomVariableFromDoubleArray.java
+// (line numbers at each op), including which operand's implementation runs.
+// ---------------------------------------------------------------------------
+namespace orc {
+
+struct RV {
+    int prio = 0;            // 0 Scalar, 1 RandomVariableFromDoubleArray
+    bool det = true;
+    double val = 0.0;
+    std::shared_ptr<const std::vector<double>> r;
+    size_t size() const { return det ? 1 : r->size(); }
+    double get(size_t i) const { return det ? val : (*r)[i]; }
+};
+
+static RV scalar(double v) { RV x; x.prio = 0; x.det = true; x.val = v; return x; }
+static RV rvdet(double v) { RV x; x.prio = 1; x.det = true; x.val = v; return x; }
+static RV rvarr(std::vector<double>&& v) {
+    RV x; x.prio = 1; x.det = false; x.r = std::make_shared<const std::vector<double>>(std::move(v)); return x;
+}
+template <class Fn> static RV map1(const RV& a, Fn fn) {
+    if (a.det) return a.prio == 0 ? scalar(fn(a.val)) : rvdet(fn(a.val));
+    std::vector<double> o(a.r->size());
+    const double* s = a.r->data();
+    for (size_t i = 0; i < o.size(); i++) o[i] = fn(s[i]);
+    return rvarr(std::move(o));
+}
+template <class Fn> static RV map2(const RV& a, const RV& b, size_t n, Fn fn) {
+    std::vector<double> o(n);
+    for (size_t i = 0; i < n; i++) o[i] = fn(a.get(i), b.get(i));
+    return rvarr(std::move(o));
+}
+static size_t nmax(const RV& a, const RV& b) { return std::max(a.size(), b.size()); }
+
+// unary / with double: Scalar.java:201-262, RandomVariableFromDoubleArray.java:743-990
+static RV add(const RV& a, double v) { return map1(a, [v](double x) { return x + v; }); }
+static RV sub(const RV& a, double v) { return map1(a, [v](double x) { return x - v; }); }
+static RV mult(const RV& a, double v) { return map1(a, [v](double x) { return x * v; }); }
+static RV divd(const RV& a, double v) { return map1(a, [v](double x) { return x / v; }); }
+static RV cap(const RV& a, double v) { return map1(a, [v](double x) { return std::min(x, v); }); }
+static RV floor_(const RV& a, double v) { return map1(a, [v](double x) { return std::max(x, v); }); }
+static RV expv(const RV& a) { return map1(a, [](double x) { return std::exp(x); }); }
+static RV logv(const RV& a) { return map1(a, [](double x) { return std::log(x); }); }
+static RV invert(const RV& a) { return map1(a, [](double x) { return 1.0 / x; }); }
+static RV powv(const RV& a, double e) { return map1(a, [e](double x) { return std::pow(x, e); }); }
+
+// binary ops
+static RV add(const RV& a, const RV& b);
+static RV mult(const RV& a, const RV& b);
+
+static RV add(const RV& a, const RV& b) {
+    if (a.prio == 0) return add(b, a.val);                          // Scalar.java:275-278
+    if (b.prio > a.prio) return add(b, a);
+    if (a.det && b.det) return rvdet(a.val + b.val);                // RandomVariableFromDoubleArray.java:1028-1058
+    if (a.det) return map2(a, b, nmax(a, b), [](double x, double y) { return x + y; });
+    if (b.det) return add(a, b.val);
+    return map2(a, b, nmax(a, b), [](double x, double y) { return x + y; });
+}
+static RV sub(const RV& a, const RV& b) {
+    if (a.prio == 0) return mult(sub(b, a.val), -1.0);              // Scalar.java:280-283
+    if (a.det && b.det) return rvdet(a.val - b.val);                // RandomVariableFromDoubleArray.java:1061-1091
+    if (a.det) return map2(a, b, nmax(a, b), [](double x, double y) { return x - y; });
+    if (b.det) return sub(a, b.val);
+    return map2(a, b, nmax(a, b), [](double x, double y) { return x - y; });
+}
+static RV mult(const RV& a, const RV& b) {
+    if (a.prio == 0) return mult(b, a.val);                         // Scalar.java:290-293
+    if (a.det && b.det) return rvdet(a.val * b.val);                // RandomVariableFromDoubleArray.java:1124-1154
+    if (b.det) return mult(a, b.val);
+    if (a.det) return map2(a, b, nmax(a, b), [](double x, double y) { return x * y; });
+    return map2(a, b, nmax(a, b), [](double x, double y) { return x * y; });
+}
+static RV div(const RV& a, const RV& b) {
+    if (a.prio == 0) return mult(invert(b), a.val);                 // Scalar.java:295-298
+    if (a.det && b.det) return rvdet(a.val / b.val);                // RandomVariableFromDoubleArray.java:1157-1184
+    return map2(a, b, nmax(a, b), [](double x, double y) { return x / y; });
+}
+// accrue: Scalar.java:315-318, RandomVariableFromDoubleArray.java:1279-1305
+static RV accrue(const RV& a, const RV& rate, double dt) {
+    if (a.prio == 0) return add(mult(rate, dt * a.val), a.val);
+    if (rate.det) return mult(a, 1.0 + rate.val * dt);
+    return map2(a, rate, nmax(a, rate), [dt](double x, double r) { return x * (1 + r * dt); });
+}
+// discount: Scalar.java:320-328, RandomVariableFromDoubleArray.java:1308-1335
+static RV discount(const RV& a, const RV& rate, double dt) {
+    if (a.prio == 0) {
+        if (a.val == 0) return mult(rate, 0.0);
+        return invert(add(mult(rate, dt / a.val), 1.0 / a.val));
+    }
+    if (rate.det) return divd(a, 1.0 + rate.val * dt);
+    return map2(a, rate, nmax(a, rate), [dt](double x, double r) { return x / (1.0 + r * dt); });
+}
+// addProduct(RV, double): Scalar.java:349-352, RandomVariableFromDoubleArray.java:1366-1391
+static RV addProduct(const RV& a, const RV& f1, double f2) {
+    if (a.prio == 0) return add(mult(f1, f2), a.val);
+    if (f1.det) return add(a, f1.val * f2);
+    return map2(a, f1, nmax(a, f1), [f2](double x, double y) { return x + y * f2; });
+}
+// addProduct(RV, RV): Scalar.java:354-357, RandomVariableFromDoubleArray.java:1395-1427
+static RV addProduct(const RV& a, const RV& f1, const RV& f2) {
+    if (a.prio == 0) return add(mult(f1, f2), a.val);
+    if (a.det && f1.det && f2.det) return rvdet(a.val + (f1.val * f2.val));
+    if (f1.det && f2.det) return add(a, f1.val * f2.val);
+    if (f2.det) return addProduct(a, f1, f2.val);
+    if (f1.det) return addProduct(a, f2, f1.val);
+    if (!a.det) {
+        std::vector<double> o(std::max(a.size(), std::max(f1.size(), f2.size())));
+        for (size_t i = 0; i < o.size(); i++) o[i] = a.get(i) + f1.get(i) * f2.get(i);
+        return rvarr(std::move(o));
+    }
+    return add(a, mult(f1, f2));
+}
+// choose: Scalar.java:330-337, RandomVariableFromDoubleArray.java:1342-1363
+static RV choose(const RV& trig, const RV& a, const RV& b) {
+    if (trig.det) return trig.val >= 0 ? a : b;
+    std::vector<double> o(trig.size());
+    for (size_t i = 0; i < o.size(); i++) o[i] = trig.get(i) >= 0.0 ? a.get(i) : b.get(i);
+    return rvarr(std::move(o));
+}
+// Kahan average: RandomVariableFromDoubleArray.java:287-307
+static double average(const RV& a) {
+    if (a.det) return a.val;
+    const auto& v = *a.r;
+    if (v.empty()) return NAN;
+    double sum = 0.0, err = 0.0;
+    for (double x : v) { double y = x - err; double t = sum + y; err = (t - sum) - y; sum = t; }
+    return sum / (double)v.size();
+}
+// getVariance / getStandardError: RandomVariableFromDoubleArray.java:333-355,419-428
+static double variance(const RV& a) {
+    if (a.det || a.size() == 1) return 0.0;
+    double avg = average(a);
+    const auto& v = *a.r;
+    double sum = 0.0, err = 0.0;
+    for (double x : v) { double y = (x - avg) * (x - avg) - err; double t = sum + y; err = (t - sum) - y; sum = t; }
+    return sum / (double)v.size();
+}
+static double stderror(const RV& a) {
+    if (a.det) return 0.0;
+    return std::sqrt(variance(a)) / std::sqrt((double)a.size());
+}
+
+// ---------------------------------------------------------------------------
+// LIBOR market model + Euler scheme.
+// LIBORMarketModelFromCovarianceModel (montecarlo/interestrate/models/
+// LIBORMarketModelFromCovarianceModel.java, "LMM" below) with
+// LIBORCovarianceModelFromVolatilityAndCorrelation, driven by
+// EulerSchemeFromProcessModel.doPrecalculateProcess (montecarlo/process/
+// EulerSchemeFromProcessModel.java:170-326, "Euler" below).
+// Pins: closed forms (zero volatility gives deterministic forwards and
+// numeraires; bonds reprice the discount curve by construction of the numeraire
+// adjustment), brute force on tiny inputs, invariants in tests/.
+// ---------------------------------------------------------------------------
+enum { SCHEME_EULER = 0, SCHEME_PC = 1, SCHEME_EULER_FUNCTIONAL = 2, SCHEME_PC_FUNCTIONAL = 3 };
+enum { MEASURE_SPOT = 0, MEASURE_TERMINAL = 1 };
+enum { SS_NORMAL = 0, SS_LOGNORMAL = 1 };
+
+struct Model {
+    int nT, n, F, P;
+    std::vector<double> times, tenor, L0, vol, fac, pd;
+    double capv;
+    int scheme, measure, ss;
+    bool has_disc;
+    std::vector<std::vector<RV>> proc;            // [nT+1][n]
+    std::vector<std::vector<RV>> dW;              // [nT][F]
+    std::map<int, RV> numeraires;                 // unadjusted at tenor index
+    std::vector<double> adjchain;                 // deterministic numeraire adjustment at tenor dates
+    std::vector<std::string> err;
+
+    int libor_period_index(double t) const { return java_binary_search(tenor.data(), n + 1, orc_round_time(t)); }
+    int process_time_index(double t) const { return java_binary_search(times.data(), nT + 1, orc_round_time(t)); }
+
+    RV lambda(int ti, int j, int k) const { return mult(scalar(vol[ti * n + j]), fac[j * F + k]); }   // FromVolAndCorr:48-57
+    RV variance_(int ti, int j) const { return mult(mult(scalar(vol[ti * n + j]), scalar(vol[ti * n + j])), 1.0); } // :83-93
+
+    RV transform(const RV& y) const {                                            // LMM:1200-1212
+        RV v = y;
+        if (ss == SS_LOGNORMAL) v = expv(v);
+        if (!std::isinf(capv)) v = cap(v, capv);
+        return v;
+    }
+    RV transform_inv(const RV& v) const { return ss == SS_LOGNORMAL ? logv(v) : v; }   // LMM:1215-1223
+
+    int first_alive(int ti) const {                                              // LMM:1126-1130
+        int f = libor_period_index(times[ti]) + 1;
+        if (f < 0) f = -f - 1 + 1;
+        return f;
+    }
+
+    std::vector<RV> drift(int ti, const std::vector<RV>& real) const {            // LMM:1125-1191
+        std::vector<RV> d(n);
+        std::vector<bool> has(n, false);
+        int first = first_alive(ti);
+        RV zero = scalar(0.0);
+        for (int j = first; j < n; j++) { d[j] = zero; has[j] = true; }
+        std::vector<RV> S(F, zero);
+        if (measure == MEASURE_SPOT) {
+            for (int j = first; j < n; j++) {
+                double dl = tenor[j + 1] - tenor[j];
+                RV w = discount(scalar(dl), real[j], dl);
+                if (ss == SS_LOGNORMAL) w = mult(w, real[j]);
+                for (int k = 0; k < F; k++) S[k] = addProduct(S[k], w, lambda(ti, j, k));
+                RV r = d[j];
+                for (int k = 0; k < F; k++) r = addProduct(r, S[k], lambda(ti, j, k));   // RandomVariable.java:659-666
+                d[j] = r;
+            }
+        } else {
+            for (int j = n - 1; j >= first; j--) {
+                double dl = tenor[j + 1] - tenor[j];
+                RV w = discount(scalar(-dl), real[j], dl);
+                if (ss == SS_LOGNORMAL) w = mult(w, real[j]);
+                RV r = d[j];
+                for (int k = 0; k < F; k++) r = addProduct(r, S[k], lambda(ti, j, k));
+                d[j] = r;
+                for (int k = 0; k < F; k++) S[k] = addProduct(S[k], w, lambda(ti, j, k));
+            }
+        }
+        if (ss == SS_LOGNORMAL)
+            for (int j = first; j < n; j++) d[j] = addProduct(d[j], variance_(ti, j), -0.5);
+        std::vector<RV> out(n);
+        for (int j = 0; j < n; j++) if (has[j]) out[j] = d[j];
+        (void)has;
+        return out;
+    }
+
+    void simulate(int32_t seed, int threads) {
+        // Brownian increments (BrownianMotionFromMersenneRandomNumbers.java:141-191)
+        std::vector<double> bm((size_t)nT * F * P);
+        orc_brownian(times.data(), nT, F, P, seed, bm.data());
+        dW.assign(nT, std::vector<RV>(F));
+        for (int t = 0; t < nT; t++)
+            for (int k = 0; k < F; k++) {
+                std::vector<double> v(bm.begin() + ((size_t)t * F + k) * P, bm.begin() + ((size_t)t * F + k + 1) * P);
+                dW[t][k] = rvarr(std::move(v));
+            }
+        proc.assign(nT + 1, std::vector<RV>(n));
+        std::vector<RV> state(n);
+        for (int j = 0; j < n; j++) {                                            // Euler:187-192, LMM:1081-1093
+            double y0 = (ss == SS_LOGNORMAL) ? std::log(std::max(L0[j], 0.0)) : L0[j];
+            state[j] = scalar(y0);
+            proc[0][j] = transform(state[j]);
+        }
+        for (int ti = 1; ti <= nT; ti++) {                                       // Euler:202-318
+            double dt = times[ti] - times[ti - 1];
+            int first = first_alive(ti - 1);
+            std::vector<RV> d = drift(ti - 1, proc[ti - 1]);
+            std::vector<RV> next(n);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic) num_threads(threads > 0 ? threads : 1)
+#endif
+            for (int j = 0; j < n; j++) {
+                if (j < first) continue;
+                RV cur = (scheme == SCHEME_EULER_FUNCTIONAL || scheme == SCHEME_PC_FUNCTIONAL) ? transform_inv(proc[ti - 1][j]) : state[j];
+                cur = addProduct(cur, d[j], dt);                                 // Euler:248
+                for (int k = 0; k < F; k++) cur = addProduct(cur, lambda(ti - 1, j, k), dW[ti - 1][k]);   // Euler:252
+                state[j] = cur;
+                next[j] = transform(cur);                                        // Euler:255
+            }
+            for (int j = 0; j < n; j++) proc[ti][j] = (j < first) ? proc[ti - 1][j] : next[j];   // Euler:279-290
+            if (scheme == SCHEME_PC || scheme == SCHEME_PC_FUNCTIONAL) {         // Euler:292-314
+                std::vector<RV> dp = drift(ti - 1, proc[ti]);
+                for (int j = first; j < n; j++) {
+                    RV adj = mult(divd(sub(dp[j], d[j]), 2.0), dt);
+                    state[j] = add(state[j], adj);
+                    proc[ti][j] = transform(state[j]);
+                }
+            }
+        }
+        // deterministic numeraire adjustment chain (LMM:915-944)
+        if (has_disc) {
+            adjchain.assign(n + 1, 0.0);
+            RV a = scalar(pd[0]);
+            adjchain[0] = a.val;
+            for (int i = 0; i < n; i++) {
+                double dfp = pd[i], dfn = pd[i + 1], ts = tenor[i + 1] - tenor[i];
+                RV fr = scalar((dfp / dfn - 1.0) / ts);
+                a = discount(a, fr, ts);
+                adjchain[i + 1] = a.val;
+            }
+        }
+    }
+
+    RV libor_at(int ti, int j) const { return proc[ti][j]; }                   // LMM:1307-1312
+
+    // LMM.getForwardRate (LMM:1237-1305), on-grid and multi-period cases.
+    RV forward_rate(double time, double start, double end) const {
+        int si = libor_period_index(start), ei = libor_period_index(end);
+        time = std::min(time, start);
+        int ti = process_time_index(time);
+        if (ti < 0) {
+            ti = -ti - 2;
+            if (time - times[ti] > times[ti + 1] - time) ti++;                   // ROUND_NEAREST
+        }
+        if (si < 0 || ei < 0) throw std::runtime_error("off-tenor forward rates are not restated");
+        if (si + 1 == ei) return libor_at(ti, si);
+        RV acc; bool have = false;
+        for (int pi = si; pi < ei; pi++) {
+            double sub_ = tenor[pi + 1] - tenor[pi];
+            RV l = libor_at(ti, pi);
+            acc = have ? accrue(acc, l, sub_) : add(mult(l, sub_), 1.0);
+            have = true;
+        }
+        return divd(sub(acc, 1.0), end - start);
+    }
+
+    // getNumerairetUnAdjustedAtLIBORIndex (LMM:1017-1074)
+    RV numeraire_unadjusted_at(int k) {
+        auto it = numeraires.find(k);
+        if (it != numeraires.end()) return it->second;
+        RV N;
+        if (measure == MEASURE_TERMINAL) {
+            int ti = process_time_index(tenor[k]);
+            if (ti < 0) ti = -ti - 1;
+            N = scalar(1.0);
+            for (int li = k; li <= n - 1; li++) N = discount(N, libor_at(ti, li), tenor[li + 1] - tenor[li]);
+        } else {
+            if (k != 0) {
+                int ti = process_time_index(tenor[k - 1]);
+                if (ti < 0) ti = -ti - 1;
+                double pl = tenor[k] - tenor[k - 1];
+                N = accrue(numeraire_unadjusted_at(k - 1), libor_at(ti, k - 1), pl);
+            } else {
+                N = scalar(1.0);
+            }
+        }
+        numeraires[k] = N;
+        return N;
+    }
+
+    // getNumeraire (LMM:860-876), tenor-grid times.
+    RV numeraire(double T) {
+        int k = libor_period_index(T);
+        if (k < 0) throw std::runtime_error("numeraire off the tenor grid is not restated");
+        RV N = numeraire_unadjusted_at(k);
+        if (has_disc) {
+            RV adj = scalar(adjchain[k]);
+            double zb = average(mult(invert(N), numeraire_unadjusted_at(0)));
+            N = div(mult(N, zb), adj);
+        }
+        return N;
+    }
+    RV weights() const { return scalar(1.0 / P); }                               // Euler:184,317
+};
+
+} // namespace orc
+
+extern "C" {
+
+typedef struct orc_lmm_spec {
+    int nT; const double* times;       // nT+1 simulation times
+    int n; const double* tenor;        // n+1 tenor dates
+    int F; int P; int32_t seed;
+    const double* L0;                  // n initial forwards
+    const double* vol;                 // [nT][n] sigma_j(t_i)
+    const double* factors;             // [n][F]
+    double cap;
+    int scheme, measure, state_space;
+    const double* pd;                  // n+1 discount-curve factors at tenor dates, or NULL
+    int threads;
+} orc_lmm_spec;
+
+static std::string g_err;
+const char* orc_last_error(void) { return g_err.c_str(); }
+
+void* orc_lmm_create(const orc_lmm_spec* s) {
+    try {
+        auto* m = new orc::Model();
+        m->nT = s->nT; m->n = s->n; m->F = s->F; m->P = s->P;
+        m->times.assign(s->times, s->times + s->nT + 1);
+        m->tenor.assign(s->tenor, s->tenor + s->n + 1);
+        m->L0.assign(s->L0, s->L0 + s->n);
+        m->vol.assign(s->vol, s->vol + (size_t)s->nT * s->n);
+        m->fac.assign(s->factors, s->factors + (size_t)s->n * s->F);
+        m->capv = s->cap; m->scheme = s->scheme; m->measure = s->measure; m->ss = s->state_space;
+        m->has_disc = s->pd != nullptr;
+        if (m->has_disc) m->pd.assign(s->pd, s->pd + s->n + 1);
+        m->simulate(s->seed, s->threads);
+        return m;
+    } catch (const std::exception& e) { g_err = e.what(); return nullptr; }
+}
+
+void orc_lmm_destroy(void* h) { delete (orc::Model*)h; }
+
+static void fill(const orc::RV& v, int P, double* out) {
+    for (int p = 0; p < P; p++) out[p] = v.get((size_t)p);
+}
+
+int orc_lmm_process_value(void* h, int t, int j, double* out) {
+    auto* m = (orc::Model*)h; fill(m->proc[t][j], m->P, out); return 0;
+}
+int orc_lmm_brownian_increment(void* h, int t, int k, double* out) {
+    auto* m = (orc::Model*)h; fill(m->dW[t][k], m->P, out); return 0;
+}
+int orc_lmm_numeraire(void* h, double T, double* out) {
+    try { auto* m = (orc::Model*)h; fill(m->numeraire(T), m->P, out); return 0; }
+    catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+int orc_lmm_numeraire_unadjusted(void* h, int k, double* out) {
+    auto* m = (orc::Model*)h; fill(m->numeraire_unadjusted_at(k), m->P, out); return 0;
+}
+double orc_lmm_numeraire_adjustment(void* h, int k) {
+    auto* m = (orc::Model*)h; return m->has_disc ? m->adjchain[k] : 1.0;
+}
+
+// Swaption.getValue (montecarlo/interestrate/products/Swaption.java:137-199), evaluationTime 0.
+// disc_adj: per-period discounting adjustment (Swaption.java:165-175), NULL = 1.0
+// (the case where the discount curve is the forward curve's own).
+int orc_lmm_swaption(void* h, double exercise, int nper, const double* fix, const double* pay,
+                     const double* periodLengths, const double* strikes, double notional,
+                     const double* disc_adj, double* value, double* error, double* per_path) {
+    try {
+        auto* m = (orc::Model*)h;
+        using namespace orc;
+        RV v = scalar(0.0);
+        for (int per = nper - 1; per >= 0; per--) {
+            double fd = fix[per], pd_ = pay[per], K = strikes[per];
+            if (pd_ <= 0.0) break;
+            double pl = periodLengths ? periodLengths[per] : pd_ - fd;
+            RV libor = m->forward_rate(exercise, fd, pd_);
+            RV payoff = mult(mult(sub(libor, K), pl), notional);
+            double dd = std::max(fd, exercise);
+            double da = disc_adj ? disc_adj[per] : 1.0;
+            v = add(v, payoff);
+            v = mult(discount(v, libor, pd_ - dd), da);
+        }
+        RV vals = floor_(v, 0.0);
+        vals = mult(div(vals, m->numeraire(exercise)), m->weights());
+        vals = div(mult(vals, m->numeraire(0.0)), m->weights());
+        *value = average(vals);
+        *error = stderror(vals);
+        if (per_path) fill(vals, m->P, per_path);
+        return 0;
+    } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// SimpleSwap.getValue (montecarlo/interestrate/products/SimpleSwap.java:137-170).
+int orc_lmm_swap(void* h, int nper, const double* fix, const double* pay, const double* strikes,
+                 const double* notionals, int isPayFix, double* value, double* error, double* per_path) {
+    try {
+        auto* m = (orc::Model*)h;
+        using namespace orc;
+        RV vals = scalar(0.0);
+        for (int per = 0; per < nper; per++) {
+            double fd = fix[per], pd_ = pay[per], K = strikes[per], pl = pd_ - fd;
+            RV fr = m->forward_rate(fd, fd, pd_);
+            RV payoff = mult(mult(sub(fr, K), pl), notionals ? notionals[per] : 1.0);
+            if (!isPayFix) payoff = mult(payoff, -1.0);
+            payoff = mult(div(payoff, m->numeraire(pd_)), m->weights());
+            vals = add(vals, payoff);
+        }
+        vals = div(mult(vals, m->numeraire(0.0)), m->weights());
+        *value = average(vals);
+        *error = stderror(vals);
+        if (per_path) fill(vals, m->P, per_path);
+        return 0;
+    } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Caplet.getValue, value unit VALUE (montecarlo/interestrate/products/Caplet.java:115-140).
+int orc_lmm_caplet(void* h, double maturity, double periodLength, double strike, double daycount,
+                   int isFloorlet, double* value, double* error) {
+    try {
+        auto* m = (orc::Model*)h;
+        using namespace orc;
+        double payDate = maturity + periodLength;
+        RV fr = m->forward_rate(maturity, maturity, payDate);
+        RV N = m->numeraire(payDate);
+        RV vals = fr;
+        if (!isFloorlet) vals = mult(floor_(sub(vals, strike), 0.0), daycount);
+        else vals = mult(cap(sub(vals, strike), 0.0), -1.0 * daycount);
+        vals = mult(div(vals, N), m->weights());
+        vals = div(mult(vals, m->numeraire(0.0)), m->weights());
+        *value = average(vals);
+        *error = stderror(vals);
+        return 0;
+    } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+} // extern "C"

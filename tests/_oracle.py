@@ -1,0 +1,260 @@
+"""ctypes access to the CPU oracle (oracle/lmm_oracle.cpp). Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+from ctypes import POINTER, c_double, c_int, c_int32, c_int64, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(REPO, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "build", "liblmm_oracle.so")
+
+_lib = None
+
+
+def build_oracle() -> str:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+    return ORACLE_LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_LIB):
+            build_oracle()
+        L = ctypes.CDLL(ORACLE_LIB)
+        dp = POINTER(c_double)
+        L.orc_mt_words.argtypes = [c_int64, c_uint64, c_uint64, POINTER(c_uint32)]
+        L.orc_mt_words_from_state.argtypes = [POINTER(c_uint32), c_int, c_uint64, POINTER(c_uint32)]
+        L.orc_mt_words_by_array.argtypes = [POINTER(c_uint32), c_int, c_uint64, POINTER(c_uint32)]
+        L.orc_mt_seeded_state.argtypes = [c_int64, POINTER(c_uint32)]
+        L.orc_mt_doubles.argtypes = [c_int64, c_uint64, c_uint64, dp]
+        L.orc_icdf.argtypes = [c_double]
+        L.orc_icdf.restype = c_double
+        L.orc_icdf_array.argtypes = [dp, c_int64, dp]
+        L.orc_round_time.argtypes = [c_double]
+        L.orc_round_time.restype = c_double
+        L.orc_brownian.argtypes = [dp, c_int, c_int, c_int, c_int32, dp]
+        L.orc_eigen_sym.argtypes = [dp, c_int, dp, dp]
+        L.orc_factor_reduction.argtypes = [dp, c_int, c_int, dp]
+        L.orc_corr_expdecay.argtypes = [dp, c_int, c_double, c_int, dp, dp]
+        L.orc_vol_fourparam.argtypes = [dp, c_int, dp, c_int, c_double, c_double, c_double, c_double, dp]
+        L.orc_forward_curve.argtypes = [dp, dp, c_int, c_double]
+        L.orc_forward_curve.restype = c_double
+        L.orc_df_from_forward_curve.argtypes = [dp, dp, c_int, c_double, c_double]
+        L.orc_df_from_forward_curve.restype = c_double
+        L.orc_lmm_create.argtypes = [c_void_p]
+        L.orc_lmm_create.restype = c_void_p
+        L.orc_lmm_destroy.argtypes = [c_void_p]
+        L.orc_lmm_process_value.argtypes = [c_void_p, c_int, c_int, dp]
+        L.orc_lmm_brownian_increment.argtypes = [c_void_p, c_int, c_int, dp]
+        L.orc_lmm_numeraire.argtypes = [c_void_p, c_double, dp]
+        L.orc_lmm_numeraire_unadjusted.argtypes = [c_void_p, c_int, dp]
+        L.orc_lmm_numeraire_adjustment.argtypes = [c_void_p, c_int]
+        L.orc_lmm_numeraire_adjustment.restype = c_double
+        L.orc_lmm_swaption.argtypes = [c_void_p, c_double, c_int, dp, dp, dp, dp, c_double, dp, dp, dp, dp]
+        L.orc_lmm_swap.argtypes = [c_void_p, c_int, dp, dp, dp, dp, c_int, dp, dp, dp]
+        L.orc_lmm_caplet.argtypes = [c_void_p, c_double, c_double, c_double, c_double, c_int, dp, dp]
+        L.orc_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(POINTER(c_double))
+
+
+def f64(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def mt_words(seed, offset, count):
+    out = np.empty(count, dtype=np.uint32)
+    lib().orc_mt_words(seed, offset, count, out.ctypes.data_as(POINTER(c_uint32)))
+    return out
+
+
+def mt_doubles(seed, offset, count):
+    out = np.empty(count)
+    lib().orc_mt_doubles(seed, offset, count, _dp(out))
+    return out
+
+
+def icdf(p):
+    p = f64(p).ravel()
+    out = np.empty_like(p)
+    lib().orc_icdf_array(_dp(p), p.size, _dp(out))
+    return out
+
+
+def brownian(times, F, P, seed):
+    times = f64(times)
+    nT = times.size - 1
+    out = np.empty((nT, F, P))
+    lib().orc_brownian(_dp(times), nT, F, P, seed, _dp(out))
+    return out
+
+
+def eigen_sym(a):
+    a = f64(a)
+    n = a.shape[0]
+    ev = np.empty(n)
+    V = np.empty((n, n))
+    lib().orc_eigen_sym(_dp(a), n, _dp(ev), _dp(V))
+    return ev, V
+
+
+def factor_reduction(corr, m):
+    corr = f64(corr)
+    n = corr.shape[0]
+    out = np.empty((n, m))
+    lib().orc_factor_reduction(_dp(corr), n, m, _dp(out))
+    return out
+
+
+def corr_expdecay(tenor_fix, a, m):
+    t = f64(tenor_fix)
+    n = t.size
+    fac = np.empty((n, m))
+    corr = np.empty((n, n))
+    lib().orc_corr_expdecay(_dp(t), n, a, m, _dp(fac), _dp(corr))
+    return fac, corr
+
+
+def vol_fourparam(times, tenor_fix, a, b, c, d):
+    times = f64(times)
+    tf = f64(tenor_fix)
+    nT = times.size - 1
+    out = np.empty((nT, tf.size))
+    lib().orc_vol_fourparam(_dp(times), nT, _dp(tf), tf.size, a, b, c, d, _dp(out))
+    return out
+
+
+def df_from_forward_curve(node_t, node_f, offset, maturity):
+    nt, nf = f64(node_t), f64(node_f)
+    return lib().orc_df_from_forward_curve(_dp(nt), _dp(nf), nt.size, offset, maturity)
+
+
+def forward_curve(node_t, node_f, t):
+    nt, nf = f64(node_t), f64(node_f)
+    return lib().orc_forward_curve(_dp(nt), _dp(nf), nt.size, t)
+
+
+class Spec(ctypes.Structure):
+    _fields_ = [
+        ("nT", c_int), ("times", POINTER(c_double)), ("n", c_int), ("tenor", POINTER(c_double)),
+        ("F", c_int), ("P", c_int), ("seed", c_int32), ("L0", POINTER(c_double)), ("vol", POINTER(c_double)),
+        ("factors", POINTER(c_double)), ("cap", c_double), ("scheme", c_int), ("measure", c_int),
+        ("state_space", c_int), ("pd", POINTER(c_double)), ("threads", c_int),
+    ]
+
+
+class ModelTables:
+    """Model inputs computed by the oracle's restatement of the reference classes."""
+
+    def __init__(self, wl, *, vol_params=None, L0=None):
+        self.wl = wl
+        self.times = f64(wl.times())
+        self.tenor = f64(wl.tenor())
+        n = wl.num_libors
+        a, b, c, d = vol_params if vol_params is not None else wl.vol_params
+        self.vol = vol_fourparam(self.times, self.tenor[:n], a, b, c, d)        # [nT][n]
+        self.factors, self.corr = corr_expdecay(self.tenor[:n], wl.corr_decay, wl.num_factors)
+        self.lam = self.vol[:, :, None] * self.factors[None, :, :]             # sigma * f (Scalar.mult)
+        self.var = (self.vol * self.vol) * 1.0
+        node_t = self.tenor[:n]
+        node_f = f64(wl.forwards)
+        self.node_t, self.node_f = node_t, node_f
+        if L0 is None:
+            self.L0 = np.array([forward_curve(node_t, node_f, self.tenor[j]) for j in range(n)])
+        else:
+            self.L0 = f64(L0)
+        self.pd = np.array([df_from_forward_curve(node_t, node_f, wl.period, self.tenor[k]) for k in range(n + 1)])
+
+
+class OracleModel:
+    def __init__(self, tables: ModelTables, num_paths=None, *, cap=1e5, scheme=2, measure=0, state_space=1,
+                 with_discount=True, threads=8, seed=None):
+        wl = tables.wl
+        self.t = tables
+        self.P = int(num_paths or wl.num_paths)
+        self.n = wl.num_libors
+        self._keep = [tables.times, tables.tenor, tables.L0, f64(tables.vol), f64(tables.factors), tables.pd]
+        s = Spec()
+        s.nT = tables.times.size - 1
+        s.times = _dp(self._keep[0])
+        s.n = self.n
+        s.tenor = _dp(self._keep[1])
+        s.F = wl.num_factors
+        s.P = self.P
+        s.seed = wl.seed if seed is None else seed
+        s.L0 = _dp(self._keep[2])
+        s.vol = _dp(self._keep[3])
+        s.factors = _dp(self._keep[4])
+        s.cap = math.inf if cap is None else cap
+        s.scheme, s.measure, s.state_space = scheme, measure, state_space
+        s.pd = _dp(self._keep[5]) if with_discount else POINTER(c_double)()
+        s.threads = threads
+        self.spec = s
+        self.h = lib().orc_lmm_create(ctypes.byref(s))
+        if not self.h:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def close(self):
+        if self.h:
+            lib().orc_lmm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def process_value(self, t, j):
+        out = np.empty(self.P)
+        lib().orc_lmm_process_value(self.h, t, j, _dp(out))
+        return out
+
+    def brownian_increment(self, t, k):
+        out = np.empty(self.P)
+        lib().orc_lmm_brownian_increment(self.h, t, k, _dp(out))
+        return out
+
+    def numeraire(self, T):
+        out = np.empty(self.P)
+        if lib().orc_lmm_numeraire(self.h, T, _dp(out)) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return out
+
+    def swaption(self, spec, per_path=False):
+        fix, pay, K = f64(spec["fixing"]), f64(spec["payment"]), f64(spec["strike"])
+        v, e = np.zeros(1), np.zeros(1)
+        pp = np.empty(self.P) if per_path else None
+        rc = lib().orc_lmm_swaption(self.h, spec["exercise"], fix.size, _dp(fix), _dp(pay), POINTER(c_double)(),
+                                    _dp(K), spec.get("notional", 1.0), POINTER(c_double)(), _dp(v), _dp(e),
+                                    _dp(pp) if per_path else POINTER(c_double)())
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return (v[0], e[0], pp) if per_path else (v[0], e[0])
+
+    def swap(self, spec):
+        fix, pay, K = f64(spec["fixing"]), f64(spec["payment"]), f64(spec["strike"])
+        v, e = np.zeros(1), np.zeros(1)
+        rc = lib().orc_lmm_swap(self.h, fix.size, _dp(fix), _dp(pay), _dp(K), POINTER(c_double)(),
+                                1 if spec.get("pay_fix", True) else 0, _dp(v), _dp(e), POINTER(c_double)())
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return v[0], e[0]
+
+    def caplet(self, maturity, period, strike, dcf=None, floorlet=False):
+        v, e = np.zeros(1), np.zeros(1)
+        rc = lib().orc_lmm_caplet(self.h, maturity, period, strike, period if dcf is None else dcf,
+                                  1 if floorlet else 0, _dp(v), _dp(e))
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return v[0], e[0]
+
+    def value(self, spec):
+        return self.swaption(spec) if spec["type"] == "swaption" else self.swap(spec)

@@ -1,0 +1,65 @@
+"""The C-ABI library: it builds, loads, exports every declared symbol and
+validates arguments before touching the device (runs without a GPU)."""
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from finmath_hip import _abi
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "finmath_lmm.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declarations_match_binding_list():
+    assert declared_functions() == sorted(_abi.EXPORTED_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    path = _abi.library_path()
+    assert os.path.exists(path), "library not built"
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [s for s in declared_functions() if s not in exported]
+    assert not missing, missing
+
+
+def test_library_loads_and_binds():
+    lib = _abi.load()
+    for s in _abi.EXPORTED_SYMBOLS:
+        assert hasattr(lib, s)
+
+
+def _ctx(**kw):
+    args = dict(times=np.arange(5) * 0.5, tenor=np.arange(5) * 0.5, num_factors=2, num_paths=16, seed=3141)
+    args.update(kw)
+    return _abi.LmmContext(**args)
+
+
+@pytest.mark.parametrize("kw,exc", [
+    (dict(num_factors=0), _abi.IllegalArgumentException),
+    (dict(num_paths=0), _abi.IllegalArgumentException),
+    (dict(times=np.array([0.0, 0.5, 0.5])), _abi.IllegalArgumentException),
+    (dict(tenor=np.array([0.0, 1.0, 0.5])), _abi.IllegalArgumentException),
+    (dict(measure="TERMINAL"), _abi.UnsupportedOperationException),
+    (dict(scheme="PREDICTOR_CORRECTOR"), _abi.UnsupportedOperationException),
+    (dict(libor_cap=math.nan), _abi.IllegalArgumentException),
+    (dict(times=np.array([0.25, 0.5, 1.0])), _abi.UnsupportedOperationException),
+])
+def test_invalid_configuration_raises_before_device_use(kw, exc):
+    with pytest.raises(exc):
+        _ctx(**kw)
+
+
+def test_reference_validation_message():
+    # BrownianMotionFromMersenneRandomNumbers.java:77-78 wording
+    with pytest.raises(_abi.IllegalArgumentException, match="Number of factors must be greater or equal 1"):
+        _ctx(num_factors=0)

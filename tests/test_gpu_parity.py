@@ -1,0 +1,148 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Bars: the MT19937 word stream and uniforms bit-exact; Brownian increments equal
+up to the last-ulp differences of the device log in the AS241 tail branch;
+process values and prices in FP64 within 1e-10 relative of the oracle
+(the plain-form tolerance), at sizes the oracle finishes in seconds.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from finmath_hip import _abi
+from finmath_hip import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_context(wl, tables, num_paths, *, precision="FP64", store_paths=False, with_discount=True, cap=1e5):
+    ctx = _abi.LmmContext(tables.times, tables.tenor, wl.num_factors, num_paths, wl.seed,
+                          precision=precision, store_paths=store_paths, libor_cap=cap)
+    ctx.set_initial_forwards(tables.L0)
+    ctx.set_covariance(tables.lam, tables.var)
+    ctx.set_discount_curve(tables.pd if with_discount else None)
+    return ctx
+
+
+def to_products(specs):
+    out = []
+    for s in specs:
+        if s["type"] == "swaption":
+            out.append(_abi.Product.swaption(s["exercise"], s["fixing"], s["payment"], s["strike"]))
+        else:
+            out.append(_abi.Product.swap(s["fixing"], s["payment"], s["strike"], is_pay_fix=s.get("pay_fix", True)))
+    return out
+
+
+@pytest.mark.parametrize("seed,offset,count", [(3141, 0, 3_000_000), (3141, 123_457, 1000), (-5, 0, 70_000),
+                                               (1352, 4095, 4099)])
+def test_mt_words_bit_exact(seed, offset, count):
+    say(f"case mt_words seed={seed} offset={offset} count={count}")
+    gpu = _abi.mt19937_words(seed, offset, count)
+    ref = O.mt_words(seed, offset, count)
+    assert np.array_equal(gpu, ref)
+
+
+def test_mt_doubles_bit_exact():
+    say("case mt_doubles seed=3141 count=200000")
+    assert np.array_equal(_abi.mt19937_doubles(3141, 0, 200_000), O.mt_doubles(3141, 0, 200_000))
+
+
+def test_brownian_increments_match_oracle():
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    P = 10_000
+    say(f"case brownian c0 P={P}")
+    ctx = make_context(wl, t, P)
+    ref = O.brownian(t.times, wl.num_factors, P, wl.seed)
+    exact = 0
+    total = 0
+    for ti in range(wl.num_time_steps):
+        for k in range(wl.num_factors):
+            g = ctx.brownian_increment(ti, k)
+            r = ref[ti, k]
+            assert np.max(np.abs(g - r) / np.maximum(np.abs(r), 1e-300)) < 8e-16
+            exact += int(np.sum(g == r))
+            total += P
+    say(f"bit-exact increments: {exact}/{total}")
+    assert exact / total > 0.999
+
+
+@pytest.mark.parametrize("name,P", [("c0", 10_000), ("c1", 4_000)])
+def test_process_values_match_oracle(name, P):
+    wl = W.ALL[name]()
+    t = O.ModelTables(wl)
+    say(f"case process {name} P={P}")
+    ctx = make_context(wl, t, P, store_paths=True)
+    ctx.simulate()
+    om = O.OracleModel(t, P)
+    worst = 0.0
+    for ti in range(wl.num_time_steps + 1):
+        for j in range(wl.num_libors):
+            g, r = ctx.process_value(ti, j), om.process_value(ti, j)
+            worst = max(worst, float(np.max(np.abs(g - r) / np.abs(r))))
+    say(f"worst relative process difference {worst:.3e}")
+    assert worst < 1e-12
+    for k in range(wl.num_libors + 1):
+        T = t.tenor[k]
+        g, r = ctx.numeraire(T), om.numeraire(T)
+        assert np.max(np.abs(g - r) / np.abs(r)) < 1e-12
+
+
+@pytest.mark.parametrize("name,P", [("c0", 10_000), ("c1", 20_000)])
+def test_prices_match_oracle_fp64(name, P):
+    wl = W.ALL[name]()
+    t = O.ModelTables(wl)
+    say(f"case prices {name} P={P}")
+    ctx = make_context(wl, t, P)
+    vals, errs = ctx.value_products(to_products(wl.products))
+    om = O.OracleModel(t, P)
+    for q, s in enumerate(wl.products):
+        rv, re_ = om.value(s)
+        say(f"  {s['type']} q={q}: gpu {vals[q]:.15e} oracle {rv:.15e} err {errs[q]:.3e}/{re_:.3e}")
+        assert abs(vals[q] - rv) <= 1e-10 * abs(rv)
+        assert abs(errs[q] - re_) <= 1e-8 * abs(re_)
+
+
+def test_prices_without_discount_curve_and_without_cap():
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    P = 5000
+    say(f"case prices c0 no-discount no-cap P={P}")
+    ctx = make_context(wl, t, P, with_discount=False, cap=None)
+    vals, _ = ctx.value_products(to_products(wl.products))
+    om = O.OracleModel(t, P, with_discount=False, cap=None)
+    for q, s in enumerate(wl.products):
+        rv, _ = om.value(s)
+        assert abs(vals[q] - rv) <= 1e-10 * abs(rv)
+
+
+def test_repeated_pricing_is_deterministic():
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    say("case determinism c0 P=30000")
+    ctx = make_context(wl, t, 30_000)
+    a, ea = ctx.value_products(to_products(wl.products))
+    b, eb = ctx.value_products(to_products(wl.products))
+    assert np.array_equal(a, b) and np.array_equal(ea, eb)
+
+
+def test_ragged_path_counts():
+    # path counts that leave partial wavefronts and partial workgroups
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    for P in (1, 63, 65, 257):
+        say(f"case ragged c0 P={P}")
+        ctx = make_context(wl, t, P)
+        vals, _ = ctx.value_products(to_products(wl.products))
+        om = O.OracleModel(t, P)
+        for q, s in enumerate(wl.products):
+            rv, _ = om.value(s)
+            assert abs(vals[q] - rv) <= 1e-10 * max(abs(rv), 1e-12)
