@@ -66,7 +66,28 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     os.replace(tmp, TARGET)
     if verbose:
         print(f"built {TARGET}")
+    host = build_host()
+    if verbose:
+        print(f"built {host}")
     return TARGET
+
+
+def build_host() -> str:
+    """Host-side mirror of the reference API (plain C++ over the C ABI) + pybind11 module."""
+    import sysconfig
+    import pybind11
+    out = os.path.join(HERE, "finmath_hip", "_finmath_host" + sysconfig.get_config_var("EXT_SUFFIX"))
+    tmp = out + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-fvisibility=hidden",
+           "-I", os.path.join(REPO, "include"), "-I", os.path.join(HERE, "host"),
+           "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
+           os.path.join(HERE, "host", "finmath_host.cpp"), os.path.join(HERE, "host", "bindings.cpp"),
+           "-L", LIB, "-lfinmath_lmm", "-Wl,-rpath,$ORIGIN/../lib", "-o", tmp]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"host build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

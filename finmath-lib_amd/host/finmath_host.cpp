@@ -1,0 +1,586 @@
+// Implementation of the host-side mirror of the finmath-lib API (see finmath_host.hpp).
+#include "finmath_host.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace finmath {
+
+void check(int status) {
+    if (status == FM_OK) return;
+    const std::string msg = fm_last_error();
+    switch (status) {
+        case FM_ERR_INVALID_ARGUMENT: throw IllegalArgumentException(msg);
+        case FM_ERR_STATE: throw IllegalStateException(msg);
+        case FM_ERR_UNSUPPORTED: throw UnsupportedOperationException(msg);
+        default: throw CalculationException(msg);
+    }
+}
+
+namespace {
+// java.util.Arrays.binarySearch semantics
+int jsearch(const std::vector<double>& a, double key) {
+    int lo = 0, hi = (int)a.size() - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[(size_t)mid] < key) lo = mid + 1;
+        else if (a[(size_t)mid] > key) hi = mid - 1;
+        else return mid;
+    }
+    return -(lo + 1);
+}
+} // namespace
+
+// ---- TimeDiscretizationFromArray -------------------------------------------------
+TimeDiscretizationFromArray::TimeDiscretizationFromArray(double initial, int numberOfTimeSteps, double deltaT) {
+    if (numberOfTimeSteps < 0) throw IllegalArgumentException("number of time steps must be non-negative");
+    t_.resize((size_t)numberOfTimeSteps + 1);
+    for (int i = 0; i <= numberOfTimeSteps; i++) t_[(size_t)i] = roundToTimeTickSize(initial + i * deltaT);
+}
+
+TimeDiscretizationFromArray::TimeDiscretizationFromArray(std::vector<double> times) {
+    for (double& x : times) x = roundToTimeTickSize(x);
+    std::sort(times.begin(), times.end());
+    times.erase(std::unique(times.begin(), times.end()), times.end());
+    t_ = std::move(times);
+}
+
+int TimeDiscretizationFromArray::getTimeIndex(double time) const { return jsearch(t_, roundToTimeTickSize(time)); }
+
+int TimeDiscretizationFromArray::getTimeIndexNearestLessOrEqual(double time) const {
+    int i = jsearch(t_, roundToTimeTickSize(time));
+    return i < 0 ? -i - 2 : i;
+}
+
+// ---- curves ---------------------------------------------------------------------------
+std::shared_ptr<ForwardCurveInterpolation> ForwardCurveInterpolation::createForwardCurveFromForwards(
+    const std::string& name, const std::vector<double>& times, const std::vector<double>& forwards,
+    double paymentOffset) {
+    if (times.size() != forwards.size() || times.empty())
+        throw IllegalArgumentException("times and forwards must have the same, non-zero length");
+    auto c = std::make_shared<ForwardCurveInterpolation>();
+    c->name_ = name;
+    std::vector<size_t> idx(times.size());
+    for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return times[a] < times[b]; });
+    for (size_t i : idx) { c->t_.push_back(times[i]); c->f_.push_back(forwards[i]); }
+    c->paymentOffset_ = paymentOffset;
+    return c;
+}
+
+double ForwardCurveInterpolation::getForward(double t) const {
+    const int i = jsearch(t_, t);
+    if (i >= 0) return f_[(size_t)i];                            // node value
+    const int iv = -i - 2;
+    if (iv < 0) return f_.front();                               // constant extrapolation
+    if (iv > (int)t_.size() - 2) return f_.back();
+    const double slope = (f_[(size_t)iv + 1] - f_[(size_t)iv]) / (t_[(size_t)iv + 1] - t_[(size_t)iv]);
+    return f_[(size_t)iv] * 1.0 + slope * (t - t_[(size_t)iv]);
+}
+
+double ForwardCurveInterpolation::getForward(double fixingTime, double paymentOffset) const {
+    double forward = getForward(fixingTime);
+    const double cpo = getPaymentOffset(fixingTime);
+    if (paymentOffset != cpo)                                    // ForwardCurveInterpolation.java:338-347
+        forward = (std::exp(std::log(1 + forward * cpo) * paymentOffset / cpo) - 1.0) / paymentOffset;
+    return forward;
+}
+
+double DiscountCurveFromForwardCurve::getDiscountFactor(double maturity) const {
+    double time = 0.0, df = 1.0;
+    while (time < maturity) {
+        const double po = fc_->getPaymentOffset(time);
+        if (po <= 0) throw CalculationException("non-positive payment offset");
+        df /= 1.0 + fc_->getForward(time) * std::min(po, maturity - time) * 1.0;
+        time += po;
+    }
+    return df;
+}
+
+double getForwardSwapRate(const TimeDiscretizationFromArray& fixTenor, const TimeDiscretizationFromArray& floatTenor,
+                          const ForwardCurveInterpolation& fc) {
+    auto fcp = std::shared_ptr<const ForwardCurveInterpolation>(&fc, [](const ForwardCurveInterpolation*) {});
+    DiscountCurveFromForwardCurve dc(fcp);
+    const double ev = fixTenor.getTime(0);
+    const double pev = dc.getDiscountFactor(ev);
+    double annuity = 0.0;
+    for (int i = 0; i < fixTenor.getNumberOfTimeSteps(); i++)
+        annuity += fixTenor.getTimeStep(i) * dc.getDiscountFactor(fixTenor.getTime(i + 1));
+    annuity /= pev;
+    double floatLeg = 0.0;
+    for (int i = 0; i < floatTenor.getNumberOfTimeSteps(); i++) {
+        const double fix = floatTenor.getTime(i), pay = floatTenor.getTime(i + 1);
+        floatLeg += fc.getForward(fix, pay - fix) * (pay - fix) * dc.getDiscountFactor(pay);
+    }
+    return (floatLeg / pev) / annuity;
+}
+
+// ---- volatility / correlation / covariance ------------------------------------------------
+LIBORVolatilityModelFourParameterExponentialForm::LIBORVolatilityModelFourParameterExponentialForm(
+    TimeDiscretizationPtr td, TimeDiscretizationPtr tenor, double a_, double b_, double c_, double d_, bool cal)
+    : LIBORVolatilityModel(std::move(td), std::move(tenor)), a(a_), b(b_), c(c_), d(d_), isCalibrateable(cal) {}
+
+double LIBORVolatilityModelFourParameterExponentialForm::getVolatility(int timeIndex, int liborIndex) const {
+    const double tau = tenor_->getTime(liborIndex) - td_->getTime(timeIndex);
+    if (tau <= 0) return 0.0;                                    // already fixed
+    return (b * tau + a) * std::exp(c * (-tau)) + d;             // Scalar arithmetic of the reference
+}
+
+std::vector<double> LIBORVolatilityModelFourParameterExponentialForm::getParameter() const {
+    if (!isCalibrateable) return {};
+    return {a, b, c, d};
+}
+
+LIBORVolatilityModelFromGivenMatrix::LIBORVolatilityModelFromGivenMatrix(TimeDiscretizationPtr td,
+                                                                         TimeDiscretizationPtr tenor,
+                                                                         std::vector<double> vol)
+    : LIBORVolatilityModel(std::move(td), std::move(tenor)), vol_(std::move(vol)) {
+    if (vol_.size() != (size_t)td_->getNumberOfTimeSteps() * tenor_->getNumberOfTimeSteps())
+        throw IllegalArgumentException("volatility matrix must be [numberOfTimeSteps][numberOfLibors]");
+}
+
+double LIBORVolatilityModelFromGivenMatrix::getVolatility(int timeIndex, int liborIndex) const {
+    return vol_[(size_t)timeIndex * tenor_->getNumberOfTimeSteps() + liborIndex];
+}
+
+namespace {
+
+// Symmetric eigen decomposition: Householder reduction to tridiagonal form, then
+// the implicit QL iteration with Wilkinson shifts (Golub & Van Loan 8.3), which is
+// also the method of commons-math3's EigenDecomposition. a is overwritten by the
+// eigenvectors (columns); d receives the eigenvalues.
+void symmetric_eigen(std::vector<double>& a, int n, std::vector<double>& d) {
+    std::vector<double> e((size_t)n, 0.0);
+    d.assign((size_t)n, 0.0);
+    auto A = [&](int i, int j) -> double& { return a[(size_t)i * n + j]; };
+    for (int i = n - 1; i > 0; i--) {
+        const int l = i - 1;
+        double h = 0.0, scale = 0.0;
+        if (l > 0) {
+            for (int k = 0; k <= l; k++) scale += std::fabs(A(i, k));
+            if (scale == 0.0) e[(size_t)i] = A(i, l);
+            else {
+                for (int k = 0; k <= l; k++) { A(i, k) /= scale; h += A(i, k) * A(i, k); }
+                double f = A(i, l);
+                double g = (f >= 0.0) ? -std::sqrt(h) : std::sqrt(h);
+                e[(size_t)i] = scale * g;
+                h -= f * g;
+                A(i, l) = f - g;
+                f = 0.0;
+                for (int j = 0; j <= l; j++) {
+                    A(j, i) = A(i, j) / h;
+                    g = 0.0;
+                    for (int k = 0; k <= j; k++) g += A(j, k) * A(i, k);
+                    for (int k = j + 1; k <= l; k++) g += A(k, j) * A(i, k);
+                    e[(size_t)j] = g / h;
+                    f += e[(size_t)j] * A(i, j);
+                }
+                const double hh = f / (h + h);
+                for (int j = 0; j <= l; j++) {
+                    f = A(i, j);
+                    e[(size_t)j] = g = e[(size_t)j] - hh * f;
+                    for (int k = 0; k <= j; k++) A(j, k) -= (f * e[(size_t)k] + g * A(i, k));
+                }
+            }
+        } else {
+            e[(size_t)i] = A(i, l);
+        }
+        d[(size_t)i] = h;
+    }
+    d[0] = 0.0;
+    e[0] = 0.0;
+    for (int i = 0; i < n; i++) {
+        const int l = i - 1;
+        if (d[(size_t)i] != 0.0) {
+            for (int j = 0; j <= l; j++) {
+                double g = 0.0;
+                for (int k = 0; k <= l; k++) g += A(i, k) * A(k, j);
+                for (int k = 0; k <= l; k++) A(k, j) -= g * A(k, i);
+            }
+        }
+        d[(size_t)i] = A(i, i);
+        A(i, i) = 1.0;
+        for (int j = 0; j <= l; j++) A(j, i) = A(i, j) = 0.0;
+    }
+    // implicit QL
+    for (int i = 1; i < n; i++) e[(size_t)i - 1] = e[(size_t)i];
+    e[(size_t)n - 1] = 0.0;
+    for (int l = 0; l < n; l++) {
+        int iter = 0, m;
+        do {
+            for (m = l; m < n - 1; m++) {
+                const double dd = std::fabs(d[(size_t)m]) + std::fabs(d[(size_t)m + 1]);
+                if (std::fabs(e[(size_t)m]) <= 1e-16 * dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 60) throw CalculationException("eigen decomposition did not converge");
+                double g = (d[(size_t)l + 1] - d[(size_t)l]) / (2.0 * e[(size_t)l]);
+                double r = std::hypot(g, 1.0);
+                g = d[(size_t)m] - d[(size_t)l] + e[(size_t)l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
+                double s = 1.0, c = 1.0, p = 0.0;
+                int i;
+                for (i = m - 1; i >= l; i--) {
+                    double f = s * e[(size_t)i];
+                    const double b = c * e[(size_t)i];
+                    e[(size_t)i + 1] = (r = std::hypot(f, g));
+                    if (r == 0.0) { d[(size_t)i + 1] -= p; e[(size_t)m] = 0.0; break; }
+                    s = f / r; c = g / r;
+                    g = d[(size_t)i + 1] - p;
+                    r = (d[(size_t)i] - g) * s + 2.0 * c * b;
+                    d[(size_t)i + 1] = g + (p = s * r);
+                    g = c * r - b;
+                    for (int k = 0; k < n; k++) {
+                        f = A(k, i + 1);
+                        A(k, i + 1) = s * A(k, i) + c * f;
+                        A(k, i) = c * A(k, i) - s * f;
+                    }
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[(size_t)l] -= p; e[(size_t)l] = g; e[(size_t)m] = 0.0;
+            }
+        } while (m != l);
+    }
+}
+
+// LinearAlgebra.getFactorMatrixUsingCommonsMath (functions/LinearAlgebra.java:392-445)
+std::vector<double> factor_matrix(const std::vector<double>& corr, int n, int m) {
+    std::vector<double> V = corr, ev;
+    symmetric_eigen(V, n, ev);
+    std::vector<int> idx((size_t)n);
+    for (int i = 0; i < n; i++) idx[(size_t)i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return ev[(size_t)x] > ev[(size_t)y]; });
+    std::vector<double> out((size_t)n * m);
+    for (int f = 0; f < m; f++) {
+        const int c = idx[(size_t)f];
+        const double sign = V[(size_t)c] > 0.0 ? 1.0 : -1.0;     // first entry positive
+        double norm2 = 0.0;
+        for (int r = 0; r < n; r++) norm2 += V[(size_t)r * n + c] * V[(size_t)r * n + c];
+        const double lambda = std::max(ev[(size_t)c], 0.0);
+        for (int r = 0; r < n; r++) out[(size_t)r * m + f] = sign * std::sqrt(lambda / norm2) * V[(size_t)r * n + c];
+    }
+    return out;
+}
+
+} // namespace
+
+std::vector<double> factorReduction(const std::vector<double>& corr, int n, int m) {
+    if (m < 1 || m > n) throw IllegalArgumentException("number of factors must be in [1, n]");
+    std::vector<double> f = factor_matrix(corr, n, m);
+    for (int r = 0; r < n; r++) {                                // renormalise rows (LinearAlgebra.java:461-475)
+        double ss = 0.0;
+        for (int k = 0; k < m; k++) ss += f[(size_t)r * m + k] * f[(size_t)r * m + k];
+        if (ss != 0.0) for (int k = 0; k < m; k++) f[(size_t)r * m + k] = f[(size_t)r * m + k] / std::sqrt(ss);
+        else for (int k = 0; k < m; k++) f[(size_t)r * m + k] = 1.0;
+    }
+    std::vector<double> red((size_t)n * n);
+    for (int r = 0; r < n; r++)
+        for (int c = 0; c < n; c++) {
+            double s = 0.0;
+            for (int k = 0; k < m; k++) s += f[(size_t)r * m + k] * f[(size_t)c * m + k];
+            red[(size_t)r * n + c] = s;
+        }
+    return factor_matrix(red, n, m);                             // orthogonalised again (:478-481)
+}
+
+LIBORCorrelationModelExponentialDecay::LIBORCorrelationModelExponentialDecay(TimeDiscretizationPtr, TimeDiscretizationPtr tenor,
+                                                                             int numberOfFactors, double a, bool)
+    : n_(tenor->getNumberOfTimeSteps()), m_(numberOfFactors), a_(a) {
+    a = std::max(a, 0.0);
+    std::vector<double> rho((size_t)n_ * n_);
+    for (int r = 0; r < n_; r++)
+        for (int c = 0; c < n_; c++) rho[(size_t)r * n_ + c] = std::exp(-a * std::fabs(tenor->getTime(r) - tenor->getTime(c)));
+    factor_ = factorReduction(rho, n_, m_);
+    corr_.assign((size_t)n_ * n_, 0.0);
+    for (int c1 = 0; c1 < n_; c1++) {
+        for (int c2 = 0; c2 < c1; c2++) {
+            double s = 0.0;
+            for (int k = 0; k < m_; k++) s += factor_[(size_t)c1 * m_ + k] * factor_[(size_t)c2 * m_ + k];
+            corr_[(size_t)c1 * n_ + c2] = corr_[(size_t)c2 * n_ + c1] = s;
+        }
+        corr_[(size_t)c1 * n_ + c1] = 1.0;
+    }
+}
+
+std::vector<double> LIBORCovarianceModelFromVolatilityAndCorrelation::getFactorLoading(int timeIndex, int component) const {
+    const int m = corr_->getNumberOfFactors();
+    std::vector<double> out((size_t)m);
+    const double v = vol_->getVolatility(timeIndex, component);
+    for (int k = 0; k < m; k++) out[(size_t)k] = v * corr_->getFactorLoading(timeIndex, k, component);
+    return out;
+}
+
+double LIBORCovarianceModelFromVolatilityAndCorrelation::getCovariance(int timeIndex, int c1, int c2) const {
+    return (vol_->getVolatility(timeIndex, c1) * vol_->getVolatility(timeIndex, c2)) * corr_->getCorrelation(timeIndex, c1, c2);
+}
+
+void LIBORCovarianceModelFromVolatilityAndCorrelation::tables(const TimeDiscretizationFromArray& sim,
+                                                              std::vector<double>& lam, std::vector<double>& var) const {
+    const int nT = sim.getNumberOfTimeSteps(), n = tenor_->getNumberOfTimeSteps(), m = getNumberOfFactors();
+    lam.assign((size_t)nT * n * m, 0.0);
+    var.assign((size_t)nT * n, 0.0);
+    for (int i = 0; i < nT; i++) {
+        // AbstractLIBORCovarianceModel.getFactorLoading(double time, ...): round-down time index (:54-60,69-76)
+        int ci = td_->getTimeIndex(sim.getTime(i));
+        if (ci < 0) ci = -ci - 2;
+        for (int j = 0; j < n; j++) {
+            const std::vector<double> f = getFactorLoading(ci, j);
+            for (int k = 0; k < m; k++) lam[((size_t)i * n + j) * m + k] = f[(size_t)k];
+            var[(size_t)i * n + j] = getCovariance(ci, j, j);
+        }
+    }
+}
+
+// ---- LIBOR market model ------------------------------------------------------------------
+std::shared_ptr<LIBORMarketModelFromCovarianceModel> LIBORMarketModelFromCovarianceModel::of(
+    TimeDiscretizationPtr tenor, ForwardCurvePtr fc, DiscountCurvePtr dc, CovarianceModelPtr cov,
+    const std::map<std::string, std::string>& props) {
+    auto m = std::make_shared<LIBORMarketModelFromCovarianceModel>();
+    m->tenor_ = std::move(tenor); m->fc_ = std::move(fc); m->dc_ = std::move(dc); m->cov_ = std::move(cov);
+    auto up = [](std::string s) { for (auto& ch : s) ch = (char)std::toupper((unsigned char)ch); return s; };
+    auto it = props.find("measure");
+    if (it != props.end()) {
+        const std::string v = up(it->second);
+        if (v == "SPOT") m->measure = FM_MEASURE_SPOT;
+        else if (v == "TERMINAL") m->measure = FM_MEASURE_TERMINAL;
+        else throw IllegalArgumentException("unknown measure " + it->second);
+    }
+    it = props.find("stateSpace");
+    if (it != props.end()) {
+        const std::string v = up(it->second);
+        if (v == "LOGNORMAL") m->stateSpace = FM_STATE_SPACE_LOGNORMAL;
+        else if (v == "NORMAL") m->stateSpace = FM_STATE_SPACE_NORMAL;
+        else throw IllegalArgumentException("unknown state space " + it->second);
+    }
+    it = props.find("liborCap");
+    if (it != props.end()) m->liborCap = std::stod(it->second);
+    return m;
+}
+
+std::vector<double> LIBORMarketModelFromCovarianceModel::getInitialForwards() const {
+    const int n = getNumberOfLibors();
+    std::vector<double> L0((size_t)n);
+    for (int j = 0; j < n; j++) L0[(size_t)j] = fc_->getForward(tenor_->getTime(j), tenor_->getTimeStep(j));
+    return L0;
+}
+
+BrownianMotionFromMersenneRandomNumbers::BrownianMotionFromMersenneRandomNumbers(TimeDiscretizationPtr td_, int F,
+                                                                                 int P, int seed_)
+    : td(std::move(td_)), numberOfFactors(F), numberOfPaths(P), seed(seed_) {
+    // Validate.isTrue (montecarlo/BrownianMotionFromMersenneRandomNumbers.java:77-78)
+    if (F <= 0) throw IllegalArgumentException("Number of factors must be greater or equal 1 (given " + std::to_string(F) + ").");
+    if (P <= 0) throw IllegalArgumentException("Number of paths must be greater or equal 1 (given " + std::to_string(P) + ").");
+}
+
+// ---- RandomVariable ---------------------------------------------------------------------
+double RandomVariable::getAverage() const {
+    if (det_) return value_;
+    if (r_.empty()) return NAN;
+    double sum = 0.0, err = 0.0;
+    for (double x : r_) { const double y = x - err; const double t = sum + y; err = (t - sum) - y; sum = t; }
+    return sum / (double)r_.size();
+}
+
+double RandomVariable::getVariance() const {
+    if (det_ || r_.size() == 1) return 0.0;
+    const double avg = getAverage();
+    double sum = 0.0, err = 0.0;
+    for (double x : r_) {
+        const double y = (x - avg) * (x - avg) - err;
+        const double t = sum + y;
+        err = (t - sum) - y;
+        sum = t;
+    }
+    return sum / (double)r_.size();
+}
+
+double RandomVariable::getStandardError() const {
+    if (det_) return 0.0;
+    return std::sqrt(getVariance()) / std::sqrt((double)r_.size());
+}
+
+// ---- Euler scheme over the engine -------------------------------------------------------
+EulerSchemeFromProcessModel::EulerSchemeFromProcessModel(LIBORMarketModelPtr model, BrownianMotionPtr driver, Scheme scheme,
+                                                         int precision)
+    : model_(std::move(model)), driver_(std::move(driver)), scheme_(scheme), precision_(precision) {
+    if (!model_ || !driver_) throw IllegalArgumentException("model and driver must be given");
+}
+
+EulerSchemeFromProcessModel::~EulerSchemeFromProcessModel() { if (ctx_) fm_lmm_destroy(ctx_); }
+
+void EulerSchemeFromProcessModel::ensureContext(bool storePaths) {
+    if (ctx_ && (!storePaths || ctxStores_)) return;
+    if (ctx_) { fm_lmm_destroy(ctx_); ctx_ = nullptr; simulated_ = false; }
+    const auto& times = driver_->getTimeDiscretization()->getAsDoubleArray();
+    const auto& tenor = model_->getLiborPeriodDiscretization()->getAsDoubleArray();
+    fm_lmm_config cfg{};
+    cfg.num_time_steps = (int32_t)times.size() - 1;
+    cfg.times = times.data();
+    cfg.num_libors = (int32_t)tenor.size() - 1;
+    cfg.tenor = tenor.data();
+    cfg.num_factors = driver_->getNumberOfFactors();
+    cfg.num_paths = driver_->getNumberOfPaths();
+    cfg.seed = driver_->getSeed();
+    cfg.measure = model_->measure;
+    cfg.state_space = model_->stateSpace;
+    cfg.scheme = (int32_t)scheme_;
+    cfg.precision = precision_;
+    cfg.libor_cap = model_->liborCap;
+    cfg.store_paths = storePaths ? 1 : 0;
+    cfg.checkpoint_stride = 4;
+    if (driver_->getNumberOfFactors() != model_->getCovarianceModel()->getNumberOfFactors())
+        throw IllegalArgumentException("number of factors of the driver and of the covariance model differ");
+    fm_lmm* c = nullptr;
+    check(fm_lmm_create(&cfg, &c));
+    try {
+        const std::vector<double> L0 = model_->getInitialForwards();
+        check(fm_lmm_set_initial_forwards(c, L0.data()));
+        std::vector<double> lam, var;
+        model_->getCovarianceModel()->tables(*driver_->getTimeDiscretization(), lam, var);
+        check(fm_lmm_set_covariance(c, lam.data(), var.data()));
+        if (model_->getDiscountCurve()) {
+            std::vector<double> pd(tenor.size());
+            for (size_t k = 0; k < tenor.size(); k++) pd[k] = model_->getDiscountCurve()->getDiscountFactor(tenor[k]);
+            check(fm_lmm_set_discount_curve(c, pd.data()));
+        }
+    } catch (...) { fm_lmm_destroy(c); throw; }
+    ctx_ = c;
+    ctxStores_ = storePaths;
+}
+
+fm_lmm* EulerSchemeFromProcessModel::handle() { ensureContext(false); return ctx_; }
+
+RandomVariable EulerSchemeFromProcessModel::getProcessValue(int timeIndex, int componentIndex) {
+    ensureContext(true);
+    if (!simulated_) { check(fm_lmm_simulate(ctx_)); simulated_ = true; }
+    std::vector<double> out((size_t)getNumberOfPaths());
+    check(fm_lmm_get_process_value(ctx_, timeIndex, componentIndex, out.data()));
+    const double t = getTime(timeIndex);
+    if (timeIndex == 0) return RandomVariable(t, out[0]);       // deterministic initial value
+    return RandomVariable(t, std::move(out));
+}
+
+RandomVariable EulerSchemeFromProcessModel::getBrownianIncrement(int timeIndex, int factor) {
+    ensureContext(false);
+    std::vector<double> out((size_t)getNumberOfPaths());
+    check(fm_lmm_get_brownian_increment(ctx_, timeIndex, factor, out.data()));
+    return RandomVariable(getTime(timeIndex + 1), std::move(out));
+}
+
+RandomVariable EulerSchemeFromProcessModel::getNumeraire(double time) {
+    ensureContext(true);
+    if (!simulated_) { check(fm_lmm_simulate(ctx_)); simulated_ = true; }
+    std::vector<double> out((size_t)getNumberOfPaths());
+    check(fm_lmm_get_numeraire(ctx_, time, out.data()));
+    const int k = model_->getLiborPeriodIndex(time);
+    if (k <= 1) return RandomVariable(time, out[0]);           // N(T_0), N(T_1) are deterministic
+    return RandomVariable(time, std::move(out));
+}
+
+void EulerSchemeFromProcessModel::valueProducts(const std::vector<ProductPtr>& products, std::vector<double>& values,
+                                                std::vector<double>& errors) {
+    ensureContext(false);
+    std::deque<std::vector<double>> arena;
+    std::vector<fm_product> lowered(products.size());
+    for (size_t q = 0; q < products.size(); q++) products[q]->lower(arena, lowered[q]);
+    values.assign(products.size(), 0.0);
+    errors.assign(products.size(), 0.0);
+    check(fm_lmm_value_products(ctx_, lowered.data(), (int32_t)lowered.size(), values.data(), errors.data()));
+}
+
+std::vector<double> EulerSchemeFromProcessModel::lastTimings() {
+    std::vector<double> ms(5, 0.0);
+    if (ctx_) check(fm_lmm_last_timings(ctx_, ms.data(), 5));
+    return ms;
+}
+
+std::vector<std::pair<double, double>> LIBORMonteCarloSimulationFromLIBORModel::getValues(const std::vector<ProductPtr>& products) {
+    std::vector<double> v, e;
+    process_->valueProducts(products, v, e);
+    std::vector<std::pair<double, double>> out;
+    for (size_t q = 0; q < v.size(); q++) out.emplace_back(v[q], e[q]);
+    return out;
+}
+
+// ---- products ----------------------------------------------------------------------------
+double Product::getValue(LIBORMonteCarloSimulationFromLIBORModel& model) const {
+    return model.getValues({shared_from_this()})[0].first;
+}
+
+std::map<std::string, double> Product::getValues(LIBORMonteCarloSimulationFromLIBORModel& model) const {
+    const auto r = model.getValues({shared_from_this()})[0];
+    return {{"value", r.first}, {"error", r.second}};
+}
+
+namespace {
+const double* keep(std::deque<std::vector<double>>& arena, const std::vector<double>& v) {
+    if (v.empty()) return nullptr;
+    arena.push_back(v);
+    return arena.back().data();
+}
+} // namespace
+
+Swaption::Swaption(double ex, std::vector<double> fix, std::vector<double> pay, std::vector<double> pl,
+                   std::vector<double> k, double notional_)
+    : exerciseDate(ex), notional(notional_), fixingDates(std::move(fix)), paymentDates(std::move(pay)),
+      periodLengths(std::move(pl)), swaprates(std::move(k)) {
+    if (fixingDates.size() != paymentDates.size() || fixingDates.size() != swaprates.size() ||
+        (!periodLengths.empty() && periodLengths.size() != fixingDates.size()))
+        throw IllegalArgumentException("swaption arrays must have the same length");
+}
+
+Swaption::Swaption(double ex, const TimeDiscretizationFromArray& swapTenor, double swaprate)
+    : exerciseDate(ex), notional(1.0) {                           // Swaption.java:110-130
+    for (int i = 0; i < swapTenor.getNumberOfTimeSteps(); i++) {
+        fixingDates.push_back(swapTenor.getTime(i));
+        paymentDates.push_back(swapTenor.getTime(i + 1));
+        swaprates.push_back(swaprate);
+    }
+}
+
+void Swaption::lower(std::deque<std::vector<double>>& arena, fm_product& o) const {
+    std::memset(&o, 0, sizeof(o));
+    o.type = FM_PRODUCT_SWAPTION;
+    o.num_periods = (int32_t)fixingDates.size();
+    o.exercise_date = exerciseDate;
+    o.fixing_dates = keep(arena, fixingDates);
+    o.payment_dates = keep(arena, paymentDates);
+    o.period_lengths = keep(arena, periodLengths);
+    o.swaprates = keep(arena, swaprates);
+    o.notional = notional;
+}
+
+SimpleSwap::SimpleSwap(std::vector<double> fix, std::vector<double> pay, std::vector<double> k, bool payFix,
+                       std::vector<double> notional_)
+    : fixingDates(std::move(fix)), paymentDates(std::move(pay)), swaprates(std::move(k)), notional(std::move(notional_)),
+      isPayFix(payFix) {
+    if (fixingDates.size() != paymentDates.size() || fixingDates.size() != swaprates.size())
+        throw IllegalArgumentException("swap arrays must have the same length");
+}
+
+void SimpleSwap::lower(std::deque<std::vector<double>>& arena, fm_product& o) const {
+    std::memset(&o, 0, sizeof(o));
+    o.type = FM_PRODUCT_SWAP;
+    o.num_periods = (int32_t)fixingDates.size();
+    o.fixing_dates = keep(arena, fixingDates);
+    o.payment_dates = keep(arena, paymentDates);
+    o.swaprates = keep(arena, swaprates);
+    o.notionals = keep(arena, notional);
+    o.is_pay_fix = isPayFix ? 1 : 0;
+    o.notional = 1.0;
+}
+
+Caplet::Caplet(double m, double pl, double k, double dcf, bool floorlet)
+    : maturity(m), periodLength(pl), strike(k), daycountFraction(dcf), isFloorlet(floorlet) {}
+
+void Caplet::lower(std::deque<std::vector<double>>& arena, fm_product& o) const {
+    std::memset(&o, 0, sizeof(o));
+    o.type = FM_PRODUCT_CAPLET;
+    o.num_periods = 1;
+    o.exercise_date = maturity;
+    o.period_lengths = keep(arena, {periodLength});
+    o.swaprates = keep(arena, {strike});
+    o.notional = daycountFraction;
+    o.is_floorlet = isFloorlet ? 1 : 0;
+}
+
+} // namespace finmath

@@ -1,0 +1,343 @@
+// Host-side mirror of the finmath-lib API for the LMM hot path, implemented over
+// the C ABI of include/finmath_lmm.h. Class and method names follow the reference
+// (paths relative to src/main/java/net/finmath/), so code written against the
+// reference reads the same against this library.
+//
+// What runs where: the model set-up (time grids, curves, volatility/correlation
+// tables, PCA) runs on the host as in the reference; path generation, the Euler
+// scheme, payoffs and their Monte Carlo reductions run on the GPU.
+#pragma once
+
+#include <cmath>
+#include <deque>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "finmath_lmm.h"
+
+namespace finmath {
+
+// Exceptions of the reference, raised from ABI status codes.
+struct CalculationException : std::runtime_error { using std::runtime_error::runtime_error; };
+struct IllegalArgumentException : std::invalid_argument { using std::invalid_argument::invalid_argument; };
+struct IllegalStateException : std::logic_error { using std::logic_error::logic_error; };
+struct UnsupportedOperationException : std::logic_error { using std::logic_error::logic_error; };
+void check(int status);
+
+// ---- time/TimeDiscretizationFromArray.java -----------------------------------
+class TimeDiscretizationFromArray {
+public:
+    static constexpr double kTickSize = 1.0 / (365.0 * 24.0);
+    TimeDiscretizationFromArray(double initial, int numberOfTimeSteps, double deltaT);   // :210-217
+    explicit TimeDiscretizationFromArray(std::vector<double> times);
+    int getNumberOfTimes() const { return (int)t_.size(); }
+    int getNumberOfTimeSteps() const { return (int)t_.size() - 1; }
+    double getTime(int i) const { return t_.at((size_t)i); }
+    double getTimeStep(int i) const { return t_.at((size_t)i + 1) - t_.at((size_t)i); }
+    int getTimeIndex(double time) const;                                                // :272-274
+    int getTimeIndexNearestLessOrEqual(double time) const;
+    const std::vector<double>& getAsDoubleArray() const { return t_; }
+    static double roundToTimeTickSize(double t) { return std::rint(t / kTickSize) * kTickSize; }
+private:
+    std::vector<double> t_;
+};
+using TimeDiscretizationPtr = std::shared_ptr<const TimeDiscretizationFromArray>;
+
+// ---- marketdata/model/curves --------------------------------------------------
+// ForwardCurveInterpolation: LINEAR interpolation of the forward VALUE with CONSTANT
+// extrapolation (marketdata/model/curves/ForwardCurveInterpolation.java:224-234,298-347).
+class ForwardCurveInterpolation {
+public:
+    static std::shared_ptr<ForwardCurveInterpolation> createForwardCurveFromForwards(
+        const std::string& name, const std::vector<double>& times, const std::vector<double>& forwards,
+        double paymentOffset);
+    double getForward(double fixingTime) const;
+    double getForward(double fixingTime, double paymentOffset) const;
+    double getPaymentOffset(double) const { return paymentOffset_; }
+    const std::string& getName() const { return name_; }
+private:
+    std::string name_;
+    std::vector<double> t_, f_;
+    double paymentOffset_ = 0.0;
+};
+using ForwardCurvePtr = std::shared_ptr<const ForwardCurveInterpolation>;
+
+// DiscountCurveFromForwardCurve (marketdata/model/curves/DiscountCurveFromForwardCurve.java:100-143).
+class DiscountCurveFromForwardCurve {
+public:
+    explicit DiscountCurveFromForwardCurve(ForwardCurvePtr forwardCurve) : fc_(std::move(forwardCurve)) {}
+    double getDiscountFactor(double maturity) const;
+    ForwardCurvePtr getForwardCurve() const { return fc_; }
+private:
+    ForwardCurvePtr fc_;
+};
+using DiscountCurvePtr = std::shared_ptr<const DiscountCurveFromForwardCurve>;
+
+// Single-curve par swap rate (marketdata/products/Swap.java:146-174).
+double getForwardSwapRate(const TimeDiscretizationFromArray& fixTenor, const TimeDiscretizationFromArray& floatTenor,
+                          const ForwardCurveInterpolation& forwardCurve);
+
+// ---- montecarlo/interestrate/models/covariance -----------------------------------
+class LIBORVolatilityModel {
+public:
+    LIBORVolatilityModel(TimeDiscretizationPtr td, TimeDiscretizationPtr tenor) : td_(std::move(td)), tenor_(std::move(tenor)) {}
+    virtual ~LIBORVolatilityModel() = default;
+    virtual double getVolatility(int timeIndex, int liborIndex) const = 0;
+    virtual std::vector<double> getParameter() const { return {}; }
+    TimeDiscretizationPtr getTimeDiscretization() const { return td_; }
+    TimeDiscretizationPtr getLiborPeriodDiscretization() const { return tenor_; }
+protected:
+    TimeDiscretizationPtr td_, tenor_;
+};
+using VolatilityModelPtr = std::shared_ptr<const LIBORVolatilityModel>;
+
+// sigma_i(t_j) = (a + b (T_i - t_j)) exp(-c (T_i - t_j)) + d
+// (covariance/LIBORVolatilityModelFourParameterExponentialForm.java:168-195).
+class LIBORVolatilityModelFourParameterExponentialForm : public LIBORVolatilityModel {
+public:
+    LIBORVolatilityModelFourParameterExponentialForm(TimeDiscretizationPtr td, TimeDiscretizationPtr tenor,
+                                                     double a, double b, double c, double d, bool isCalibrateable);
+    double getVolatility(int timeIndex, int liborIndex) const override;
+    std::vector<double> getParameter() const override;
+    double a, b, c, d;
+    bool isCalibrateable;
+};
+
+// Volatility table given by a matrix (covariance/LIBORVolatilityModelFromGivenMatrix.java).
+class LIBORVolatilityModelFromGivenMatrix : public LIBORVolatilityModel {
+public:
+    LIBORVolatilityModelFromGivenMatrix(TimeDiscretizationPtr td, TimeDiscretizationPtr tenor,
+                                        std::vector<double> volatility /* [nT][n] */);
+    double getVolatility(int timeIndex, int liborIndex) const override;
+    std::vector<double> getParameter() const override { return vol_; }
+private:
+    std::vector<double> vol_;
+};
+
+// rho_ij = exp(-a |T_i - T_j|), PCA-reduced to m factors
+// (covariance/LIBORCorrelationModelExponentialDecay.java:102-134, functions/LinearAlgebra.java:380-482).
+class LIBORCorrelationModelExponentialDecay {
+public:
+    LIBORCorrelationModelExponentialDecay(TimeDiscretizationPtr td, TimeDiscretizationPtr tenor, int numberOfFactors,
+                                          double a, bool isCalibrateable = false);
+    double getFactorLoading(int timeIndex, int factor, int component) const { return factor_[(size_t)component * m_ + factor]; }
+    double getCorrelation(int, int c1, int c2) const { return corr_[(size_t)c1 * n_ + c2]; }
+    int getNumberOfFactors() const { return m_; }
+    double getParameter() const { return a_; }
+    const std::vector<double>& factorMatrix() const { return factor_; }
+private:
+    int n_, m_;
+    double a_;
+    std::vector<double> factor_, corr_;
+};
+using CorrelationModelPtr = std::shared_ptr<const LIBORCorrelationModelExponentialDecay>;
+
+// PCA of a symmetric matrix with the reference's sign/scale conventions, returned
+// as an n x m row-major factor matrix (functions/LinearAlgebra.java:392-482).
+std::vector<double> factorReduction(const std::vector<double>& correlation, int n, int numberOfFactors);
+
+// lambda_jk(t_i) = sigma_j(t_i) f_jk; covariance sigma_j sigma_l rho_jl
+// (covariance/LIBORCovarianceModelFromVolatilityAndCorrelation.java:48-93).
+class LIBORCovarianceModelFromVolatilityAndCorrelation {
+public:
+    LIBORCovarianceModelFromVolatilityAndCorrelation(TimeDiscretizationPtr td, TimeDiscretizationPtr tenor,
+                                                     VolatilityModelPtr vol, CorrelationModelPtr corr)
+        : td_(std::move(td)), tenor_(std::move(tenor)), vol_(std::move(vol)), corr_(std::move(corr)) {}
+    std::vector<double> getFactorLoading(int timeIndex, int component) const;
+    double getCovariance(int timeIndex, int component1, int component2) const;
+    int getNumberOfFactors() const { return corr_->getNumberOfFactors(); }
+    TimeDiscretizationPtr getTimeDiscretization() const { return td_; }
+    TimeDiscretizationPtr getLiborPeriodDiscretization() const { return tenor_; }
+    VolatilityModelPtr getVolatilityModel() const { return vol_; }
+    CorrelationModelPtr getCorrelationModel() const { return corr_; }
+    // tables for the engine: factor loading [nT][n][F], variance [nT][n] on the simulation grid
+    void tables(const TimeDiscretizationFromArray& simulationTimes, std::vector<double>& factorLoading,
+                std::vector<double>& variance) const;
+private:
+    TimeDiscretizationPtr td_, tenor_;
+    VolatilityModelPtr vol_;
+    CorrelationModelPtr corr_;
+};
+using CovarianceModelPtr = std::shared_ptr<const LIBORCovarianceModelFromVolatilityAndCorrelation>;
+
+// ---- montecarlo/interestrate/models/LIBORMarketModelFromCovarianceModel.java -----
+class LIBORMarketModelFromCovarianceModel {
+public:
+    // LIBORMarketModelFromCovarianceModel.of(...) (LMM:283-322); properties: measure,
+    // stateSpace, liborCap (LMM:400-414). Calibration products are not supported.
+    static std::shared_ptr<LIBORMarketModelFromCovarianceModel> of(
+        TimeDiscretizationPtr liborPeriodDiscretization, ForwardCurvePtr forwardRateCurve,
+        DiscountCurvePtr discountCurve, CovarianceModelPtr covarianceModel,
+        const std::map<std::string, std::string>& properties = {});
+    TimeDiscretizationPtr getLiborPeriodDiscretization() const { return tenor_; }
+    ForwardCurvePtr getForwardRateCurve() const { return fc_; }
+    DiscountCurvePtr getDiscountCurve() const { return dc_; }
+    CovarianceModelPtr getCovarianceModel() const { return cov_; }
+    int getNumberOfComponents() const { return tenor_->getNumberOfTimeSteps(); }
+    int getNumberOfLibors() const { return tenor_->getNumberOfTimeSteps(); }
+    double getLiborPeriod(int i) const { return tenor_->getTime(i); }
+    int getLiborPeriodIndex(double time) const { return tenor_->getTimeIndex(time); }
+    std::vector<double> getInitialForwards() const;             // L_j(0) (LMM:1081-1093)
+    int measure = FM_MEASURE_SPOT;
+    int stateSpace = FM_STATE_SPACE_LOGNORMAL;
+    double liborCap = 1e5;
+private:
+    TimeDiscretizationPtr tenor_;
+    ForwardCurvePtr fc_;
+    DiscountCurvePtr dc_;
+    CovarianceModelPtr cov_;
+};
+using LIBORMarketModelPtr = std::shared_ptr<const LIBORMarketModelFromCovarianceModel>;
+
+// ---- montecarlo/BrownianMotionFromMersenneRandomNumbers.java ------------------------
+struct BrownianMotionFromMersenneRandomNumbers {
+    BrownianMotionFromMersenneRandomNumbers(TimeDiscretizationPtr td, int numberOfFactors, int numberOfPaths, int seed);
+    TimeDiscretizationPtr getTimeDiscretization() const { return td; }
+    int getNumberOfFactors() const { return numberOfFactors; }
+    int getNumberOfPaths() const { return numberOfPaths; }
+    int getSeed() const { return seed; }
+    TimeDiscretizationPtr td;
+    int numberOfFactors, numberOfPaths, seed;
+};
+using BrownianMotionPtr = std::shared_ptr<const BrownianMotionFromMersenneRandomNumbers>;
+
+// ---- a RandomVariable holding realisations on the host ------------------------------
+// getAverage is the Kahan mean (montecarlo/RandomVariableFromDoubleArray.java:287-307).
+class RandomVariable {
+public:
+    RandomVariable() = default;
+    RandomVariable(double time, double value) : time_(time), det_(true), value_(value) {}
+    RandomVariable(double time, std::vector<double> realizations) : time_(time), det_(false), r_(std::move(realizations)) {}
+    bool isDeterministic() const { return det_; }
+    int size() const { return det_ ? 1 : (int)r_.size(); }
+    double get(int i) const { return det_ ? value_ : r_.at((size_t)i); }
+    double getFiltrationTime() const { return time_; }
+    double getAverage() const;
+    double getVariance() const;
+    double getStandardError() const;
+    double doubleValue() const { return det_ ? value_ : getAverage(); }
+    std::vector<double> getRealizations(int numberOfPaths) const {
+        return det_ ? std::vector<double>((size_t)numberOfPaths, value_) : r_;
+    }
+private:
+    double time_ = 0.0;
+    bool det_ = true;
+    double value_ = 0.0;
+    std::vector<double> r_;
+};
+
+// ---- montecarlo/process/EulerSchemeFromProcessModel.java ------------------------------
+class Product;
+
+class EulerSchemeFromProcessModel {
+public:
+    enum Scheme { EULER = FM_SCHEME_EULER, PREDICTOR_CORRECTOR = FM_SCHEME_PREDICTOR_CORRECTOR,
+                  EULER_FUNCTIONAL = FM_SCHEME_EULER_FUNCTIONAL,
+                  PREDICTOR_CORRECTOR_FUNCTIONAL = FM_SCHEME_PREDICTOR_CORRECTOR_FUNCTIONAL };
+    // The default scheme for a model with an inverse state-space transform is
+    // EULER_FUNCTIONAL (Euler:108-123). precision selects the GPU arithmetic.
+    EulerSchemeFromProcessModel(LIBORMarketModelPtr model, BrownianMotionPtr driver, Scheme scheme = EULER_FUNCTIONAL,
+                                int precision = FM_PRECISION_FP64);
+    ~EulerSchemeFromProcessModel();
+    EulerSchemeFromProcessModel(const EulerSchemeFromProcessModel&) = delete;
+    EulerSchemeFromProcessModel& operator=(const EulerSchemeFromProcessModel&) = delete;
+
+    RandomVariable getProcessValue(int timeIndex, int componentIndex);        // Euler:131-146
+    RandomVariable getMonteCarloWeights(int) const { return RandomVariable(0.0, 1.0 / getNumberOfPaths()); }
+    RandomVariable getBrownianIncrement(int timeIndex, int factor);
+    RandomVariable getNumeraire(double time);
+    int getNumberOfPaths() const { return driver_->getNumberOfPaths(); }
+    int getNumberOfFactors() const { return driver_->getNumberOfFactors(); }
+    int getNumberOfComponents() const { return model_->getNumberOfComponents(); }
+    TimeDiscretizationPtr getTimeDiscretization() const { return driver_->getTimeDiscretization(); }
+    double getTime(int i) const { return getTimeDiscretization()->getTime(i); }
+    int getTimeIndex(double t) const { return getTimeDiscretization()->getTimeIndex(t); }
+    LIBORMarketModelPtr getModel() const { return model_; }
+    BrownianMotionPtr getStochasticDriver() const { return driver_; }
+    Scheme getScheme() const { return scheme_; }
+    int getPrecision() const { return precision_; }
+    // Engine access: value products in one fused GPU pass; adjoint sweep.
+    void valueProducts(const std::vector<std::shared_ptr<const Product>>& products, std::vector<double>& values,
+                       std::vector<double>& errors);
+    std::vector<double> lastTimings();
+    fm_lmm* handle();
+private:
+    void ensureContext(bool storePaths);
+    LIBORMarketModelPtr model_;
+    BrownianMotionPtr driver_;
+    Scheme scheme_;
+    int precision_;
+    fm_lmm* ctx_ = nullptr;
+    bool ctxStores_ = false;
+    bool simulated_ = false;
+};
+using ProcessPtr = std::shared_ptr<EulerSchemeFromProcessModel>;
+
+// ---- montecarlo/interestrate/LIBORMonteCarloSimulationFromLIBORModel.java -------------
+class LIBORMonteCarloSimulationFromLIBORModel {
+public:
+    explicit LIBORMonteCarloSimulationFromLIBORModel(ProcessPtr process) : process_(std::move(process)) {}
+    RandomVariable getLIBOR(int timeIndex, int liborIndex) { return process_->getProcessValue(timeIndex, liborIndex); }
+    RandomVariable getNumeraire(double time) { return process_->getNumeraire(time); }
+    RandomVariable getMonteCarloWeights(int timeIndex) const { return process_->getMonteCarloWeights(timeIndex); }
+    RandomVariable getBrownianIncrement(int timeIndex, int factor) { return process_->getBrownianIncrement(timeIndex, factor); }
+    int getNumberOfPaths() const { return process_->getNumberOfPaths(); }
+    int getNumberOfFactors() const { return process_->getNumberOfFactors(); }
+    TimeDiscretizationPtr getTimeDiscretization() const { return process_->getTimeDiscretization(); }
+    TimeDiscretizationPtr getLiborPeriodDiscretization() const { return process_->getModel()->getLiborPeriodDiscretization(); }
+    LIBORMarketModelPtr getModel() const { return process_->getModel(); }
+    ProcessPtr getProcess() const { return process_; }
+    // value several products in one pass: {value, error} per product
+    std::vector<std::pair<double, double>> getValues(const std::vector<std::shared_ptr<const Product>>& products);
+private:
+    ProcessPtr process_;
+};
+using SimulationPtr = std::shared_ptr<LIBORMonteCarloSimulationFromLIBORModel>;
+
+// ---- montecarlo/interestrate/products -------------------------------------------------
+class Product : public std::enable_shared_from_this<Product> {
+public:
+    virtual ~Product() = default;
+    // AbstractMonteCarloProduct.getValue(model) = getValue(0, model).getAverage() (AbstractMonteCarloProduct.java:82-84)
+    double getValue(LIBORMonteCarloSimulationFromLIBORModel& model) const;
+    // AbstractMonteCarloProduct.getValues: {"value", "error"} (AbstractMonteCarloProduct.java:87-104)
+    std::map<std::string, double> getValues(LIBORMonteCarloSimulationFromLIBORModel& model) const;
+    virtual void lower(std::deque<std::vector<double>>& arena, fm_product& out) const = 0;
+};
+using ProductPtr = std::shared_ptr<const Product>;
+
+// Swaption (montecarlo/interestrate/products/Swaption.java:58-199)
+class Swaption : public Product {
+public:
+    Swaption(double exerciseDate, std::vector<double> fixingDates, std::vector<double> paymentDates,
+             std::vector<double> periodLengths, std::vector<double> swaprates, double notional = 1.0);
+    Swaption(double exerciseDate, const TimeDiscretizationFromArray& swapTenor, double swaprate);
+    void lower(std::deque<std::vector<double>>& arena, fm_product& out) const override;
+    double exerciseDate, notional;
+    std::vector<double> fixingDates, paymentDates, periodLengths, swaprates;
+};
+
+// SimpleSwap (montecarlo/interestrate/products/SimpleSwap.java:137-170)
+class SimpleSwap : public Product {
+public:
+    SimpleSwap(std::vector<double> fixingDates, std::vector<double> paymentDates, std::vector<double> swaprates,
+               bool isPayFix = true, std::vector<double> notional = {});
+    void lower(std::deque<std::vector<double>>& arena, fm_product& out) const override;
+    std::vector<double> fixingDates, paymentDates, swaprates, notional;
+    bool isPayFix;
+};
+
+// Caplet (montecarlo/interestrate/products/Caplet.java:115-140), value unit VALUE
+class Caplet : public Product {
+public:
+    Caplet(double maturity, double periodLength, double strike, double daycountFraction, bool isFloorlet = false);
+    void lower(std::deque<std::vector<double>>& arena, fm_product& out) const override;
+    double maturity, periodLength, strike, daycountFraction;
+    bool isFloorlet;
+};
+
+} // namespace finmath

@@ -1,0 +1,98 @@
+"""Host-side model set-up of the mirror (time grids, curves, volatility,
+correlation PCA, covariance tables) against the oracle and closed forms.
+These run on the CPU: nothing here touches the device."""
+import math
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from finmath_hip import _finmath_host as H
+from finmath_hip import workloads as W
+
+
+def build_model(wl):
+    td = H.TimeDiscretizationFromArray(0.0, wl.num_time_steps, wl.period)
+    tenor = H.TimeDiscretizationFromArray(0.0, wl.num_libors, wl.period)
+    fc = H.ForwardCurveInterpolation.createForwardCurveFromForwards(
+        "forwardCurve", list(tenor.getAsDoubleArray()[:-1]), list(wl.forwards), wl.period)
+    a, b, c, d = wl.vol_params
+    vol = H.LIBORVolatilityModelFourParameterExponentialForm(td, tenor, a, b, c, d, False)
+    corr = H.LIBORCorrelationModelExponentialDecay(td, tenor, wl.num_factors, wl.corr_decay)
+    cov = H.LIBORCovarianceModelFromVolatilityAndCorrelation(td, tenor, vol, corr)
+    model = H.LIBORMarketModelFromCovarianceModel.of(tenor, None, fc, H.DiscountCurveFromForwardCurve(fc), None, cov, [],
+                                                     {"measure": "SPOT", "stateSpace": "LOGNORMAL"})
+    return td, tenor, fc, vol, corr, cov, model
+
+
+@pytest.mark.parametrize("name", ["c0", "c1", "c4"])
+def test_tables_match_oracle(name):
+    wl = W.ALL[name]()
+    td, tenor, fc, vol, corr, cov, model = build_model(wl)
+    t = O.ModelTables(wl)
+    lam, var = cov.tables(td)
+    assert np.max(np.abs(lam - t.lam)) < 1e-12          # PCA by QL vs the oracle's Jacobi
+    assert np.max(np.abs(var - t.var)) <= 1e-16
+    assert np.array_equal(model.getInitialForwards(), t.L0)
+    dc = model.getDiscountCurve()
+    pd = np.array([dc.getDiscountFactor(T) for T in tenor.getAsDoubleArray()])
+    assert np.max(np.abs(pd - t.pd)) <= 1e-16
+
+
+def test_factor_reduction_matches_numpy_pca():
+    T = np.arange(30) * 0.5
+    rho = np.exp(-0.07 * np.abs(T[:, None] - T[None, :]))
+    f = H.factorReduction(rho, 4)
+    g = O.factor_reduction(rho, 4)
+    assert np.max(np.abs(f - g)) < 1e-12
+
+
+def test_time_discretization_semantics():
+    td = H.TimeDiscretizationFromArray(0.0, 40, 0.5)
+    assert td.getNumberOfTimeSteps() == 40
+    assert td.getTime(7) == 3.5 and td.getTimeStep(3) == 0.5
+    assert td.getTimeIndex(3.5) == 7
+    assert td.getTimeIndex(3.6) == -(8 + 1)           # Java binarySearch insertion point
+    assert td.getTimeIndexNearestLessOrEqual(3.6) == 7
+    assert td.getTimeIndex(3.5 + 1e-6) == 7           # rounding to the 1/8760 tick
+
+
+def test_forward_curve_interpolation_and_offset_conversion():
+    fc = H.ForwardCurveInterpolation.createForwardCurveFromForwards("f", [0.5, 1.0, 2.0], [0.02, 0.03, 0.05], 0.5)
+    assert fc.getForward(0.0) == 0.02                   # constant extrapolation
+    assert fc.getForward(1.0) == 0.03                   # node value
+    assert abs(fc.getForward(1.5) - 0.04) < 1e-15       # linear
+    assert fc.getForward(3.0) == 0.05
+    f1 = fc.getForward(1.0, 1.0)                        # period conversion for another payment offset
+    assert abs(f1 - ((1 + 0.03 * 0.5) ** 2 - 1.0)) < 1e-15
+
+
+def test_swap_rate_flat_curve_is_the_curve_rate():
+    fc = H.ForwardCurveInterpolation.createForwardCurveFromForwards("f", [0.0, 30.0], [0.03, 0.03], 0.5)
+    tenor = H.TimeDiscretizationFromArray(5.0, 10, 0.5)
+    assert abs(H.getForwardSwapRate(tenor, tenor, fc) - 0.03) < 1e-15
+
+
+def test_volatility_four_parameter_form():
+    td = H.TimeDiscretizationFromArray(0.0, 4, 0.5)
+    tenor = H.TimeDiscretizationFromArray(0.0, 4, 0.5)
+    v = H.LIBORVolatilityModelFourParameterExponentialForm(td, tenor, 0.2, 0.1, 0.25, 0.1, True)
+    assert v.getVolatility(2, 1) == 0.0                 # already fixed
+    tau = 1.0
+    assert abs(v.getVolatility(0, 2) - ((0.1 * tau + 0.2) * math.exp(-0.25 * tau) + 0.1)) < 1e-16
+    assert list(v.getParameter()) == [0.2, 0.1, 0.25, 0.1]
+
+
+def test_brownian_motion_validation_message():
+    td = H.TimeDiscretizationFromArray(0.0, 4, 0.5)
+    with pytest.raises(ValueError, match="Number of paths must be greater or equal 1"):
+        H.BrownianMotionFromMersenneRandomNumbers(td, 2, 0, 3141)
+
+
+def test_model_rejects_calibration_and_unknown_properties():
+    wl = W.c0()
+    td, tenor, fc, vol, corr, cov, model = build_model(wl)
+    with pytest.raises(NotImplementedError):
+        H.LIBORMarketModelFromCovarianceModel.of(tenor, None, fc, None, None, cov, [object()], {})
+    with pytest.raises(ValueError):
+        H.LIBORMarketModelFromCovarianceModel.of(tenor, None, fc, None, None, cov, [], {"measure": "FOO"})
