@@ -1,0 +1,165 @@
+"""Benchmark: LIBOR market model Monte Carlo path-steps/s on one MI355X.
+
+A step is one pricing pass of the hot path over the whole workload: generate the
+MT19937 stream and the Brownian increments, run the Euler scheme for every path,
+value the product strip (30 payer swaptions: expiries 1..10y into 10y swaps at
+ATM and +-100bp) and reduce value and Monte Carlo error. Model and tables are
+set up before the timed region; nothing is cached between steps (the Brownian
+increments are regenerated every step).
+
+The default line is the plain FP64 form at 40 semiannual forwards x 5 factors x
+1,000,000 paths. Beside it: the FP32-mixed form at 4,000,000 paths with its
+per-price difference to the FP64 form in basis points, the roofline of the path
+kernel, and the CPU restatement timed on the host cores.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-fp32]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "finmath-lib_amd"))
+
+import numpy as np  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6      # MI355X published FP64 peak (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3     # MI355X published FP32 peak
+
+
+def build_simulation(wl, precision="FP64", regenerate=True):
+    from finmath_hip import _finmath_host as H
+    td = H.TimeDiscretizationFromArray(0.0, wl.num_time_steps, wl.period)
+    tenor = H.TimeDiscretizationFromArray(0.0, wl.num_libors, wl.period)
+    fc = H.ForwardCurveInterpolation.createForwardCurveFromForwards(
+        "forwardCurve", list(tenor.getAsDoubleArray()[:-1]), list(wl.forwards), wl.period)
+    a, b, c, d = wl.vol_params
+    vol = H.LIBORVolatilityModelFourParameterExponentialForm(td, tenor, a, b, c, d, False)
+    corr = H.LIBORCorrelationModelExponentialDecay(td, tenor, wl.num_factors, wl.corr_decay)
+    cov = H.LIBORCovarianceModelFromVolatilityAndCorrelation(td, tenor, vol, corr)
+    model = H.LIBORMarketModelFromCovarianceModel.of(tenor, None, fc, H.DiscountCurveFromForwardCurve(fc), None, cov,
+                                                     [], {"measure": "SPOT", "stateSpace": "LOGNORMAL"})
+    bm = H.BrownianMotionFromMersenneRandomNumbers(td, wl.num_factors, wl.num_paths, wl.seed)
+    process = H.EulerSchemeFromProcessModel(model, bm, "EULER_FUNCTIONAL", precision)
+    process.setRegenerateIncrements(regenerate)
+    sim = H.LIBORMonteCarloSimulationFromLIBORModel(process)
+    products = []
+    for s in wl.products:
+        if s["type"] == "swaption":
+            products.append(H.Swaption(s["exercise"], list(s["fixing"]), list(s["payment"]), list(s["strike"])))
+        else:
+            products.append(H.SimpleSwap(list(s["fixing"]), list(s["payment"]), list(s["strike"]), s.get("pay_fix", True)))
+    return sim, process, products
+
+
+def step_kernel_flops_per_path(wl) -> int:
+    """Algorithmic flops of the path kernel per path (FMA = 2; add/mul/div/exp/log/min = 1):
+    6F+11 per alive forward-step, 3 per numeraire date, 7 per swaption period."""
+    F, n = wl.num_factors, wl.num_libors
+    alive = n * (n - 1) // 2
+    periods = sum(len(s["fixing"]) for s in wl.products if s["type"] == "swaption")
+    return alive * (6 * F + 11) + 3 * n + 7 * periods
+
+
+def run_gpu(wl, precision, steps, warmup):
+    import torch
+    sim, process, products = build_simulation(wl, precision, regenerate=True)
+    results = None
+    for _ in range(warmup):
+        results = sim.getValues(products)
+    torch.cuda.synchronize()
+    stage = np.zeros(5)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        results = sim.getValues(products)
+        stage += np.asarray(process.lastTimings())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / steps
+    values = np.array([r["value"] for r in results])
+    errors = np.array([r["error"] for r in results])
+    return ms_per_step, stage / steps, values, errors
+
+
+def cpu_baseline(wl_gpu, paths):
+    """The oracle (C++ restatement of the reference CPU path) on a bounded sample."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import _oracle as O
+    wl = wl_gpu.with_paths(paths)
+    threads = min(16, os.cpu_count() or 1)
+    t = O.ModelTables(wl)
+    t0 = time.perf_counter()
+    om = O.OracleModel(t, paths, threads=threads)
+    for s in wl.products:
+        om.value(s)
+    dt = time.perf_counter() - t0
+    om.close()
+    return {"value": paths * wl.num_time_steps / dt, "unit": "path-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{wl.name} model, {paths} paths, simulate + {len(wl.products)} swaptions, {dt:.1f} s",
+            "seconds": round(dt, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-paths", type=int, default=20_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path axis would shard across GPUs "
+              "as described in DESIGN.md, but this build measures one MI355X.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    from finmath_hip import workloads as W
+
+    wl = W.c1()
+    ms, stage, v64, e64 = run_gpu(wl, "FP64", args.steps, args.warmup)
+    path_steps = wl.path_steps
+    value = path_steps / (ms * 1e-3)
+    flops = step_kernel_flops_per_path(wl) * wl.num_paths
+    step_ms = stage[1]
+    achieved = flops / (step_ms * 1e-3) / 1e12
+    line = {
+        "metric": "LMM path-steps/s (plain FP64, 40 fwd x 5 factors x 1M paths, 30 swaptions)",
+        "value": value, "unit": "path-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "c1: LMM spot measure, log-Euler, 40 semiannual forwards, 5 factors, 1,000,000 paths, "
+                               "seed 3141, 30 payer swaptions (1..10y into 10y, ATM and +-100bp)",
+                   "num_paths": wl.num_paths, "num_forwards": wl.num_libors, "num_factors": wl.num_factors,
+                   "time_steps": wl.num_time_steps},
+        "stage_ms": {"generator": stage[0], "path_kernel": stage[1], "reductions": stage[2], "call": stage[4]},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "lmm_fused_kernel<40,5,double>", "flops_per_launch": flops},
+    }
+    if not args.no_fp32:
+        wl2 = W.c2()
+        ms32, stage32, v32, e32 = run_gpu(wl2, "FP32_MIXED", max(2, args.steps // 2), 1)
+        ms64b, _, v64b, e64b = run_gpu(wl2, "FP64", 1, 0)
+        err_bp = np.abs(v32 - v64b) * 1e4
+        line["fp32_mixed"] = {
+            "workload": "c2: c1 model, 4,000,000 paths, FP32 path arithmetic, FP64 numeraire and sums",
+            "value": wl2.path_steps / (ms32 * 1e-3), "unit": "path-steps/s", "ms_per_step": ms32,
+            "fp64_same_paths_ms_per_step": ms64b,
+            "fp64_same_paths_value": wl2.path_steps / (ms64b * 1e-3),
+            "max_abs_err_bp_vs_fp64": float(err_bp.max()), "max_mc_stderr_bp": float(e64b.max() * 1e4),
+            "within_0.1bp": bool(err_bp.max() < 0.1),
+        }
+    if not args.no_cpu:
+        cb = cpu_baseline(wl, args.cpu_paths)
+        line["cpu_baseline"] = cb
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -421,6 +421,7 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     const int n = c->n(), P = c->P();
     Timer tm(c);
     tm.mark(0);
+    if (c->cfg.regenerate_increments) c->dw_ready = false;
     const bool regen = !c->dw_ready;
     ensure_brownian(c);
     tm.mark(1);

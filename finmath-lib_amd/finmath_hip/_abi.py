@@ -75,6 +75,7 @@ class FmLmmConfig(ctypes.Structure):
         ("num_factors", c_int32), ("num_paths", c_int32), ("seed", c_int32),
         ("measure", c_int32), ("state_space", c_int32), ("scheme", c_int32), ("precision", c_int32),
         ("libor_cap", c_double), ("store_paths", c_int32), ("checkpoint_stride", c_int32),
+        ("regenerate_increments", c_int32),
     ]
 
 
@@ -201,7 +202,7 @@ class LmmContext:
 
     def __init__(self, times, tenor, num_factors: int, num_paths: int, seed: int, *, measure="SPOT",
                  state_space="LOGNORMAL", scheme="EULER_FUNCTIONAL", precision="FP64", libor_cap=1e5,
-                 store_paths=False, checkpoint_stride=4):
+                 store_paths=False, checkpoint_stride=4, regenerate_increments=False):
         self.lib = load()
         self.times = as_f64(times)
         self.tenor = as_f64(tenor)
@@ -220,6 +221,7 @@ class LmmContext:
         cfg.libor_cap = float(libor_cap) if libor_cap is not None else math.inf
         cfg.store_paths = 1 if store_paths else 0
         cfg.checkpoint_stride = int(checkpoint_stride)
+        cfg.regenerate_increments = 1 if regenerate_increments else 0
         self.cfg = cfg
         h = c_void_p()
         check(self.lib.fm_lmm_create(ctypes.byref(cfg), ctypes.byref(h)))

@@ -213,7 +213,8 @@ PYBIND11_MODULE(_finmath_host, m) {
         .def("getNumberOfComponents", &EulerSchemeFromProcessModel::getNumberOfComponents)
         .def("getTime", &EulerSchemeFromProcessModel::getTime)
         .def("getTimeIndex", &EulerSchemeFromProcessModel::getTimeIndex)
-        .def("lastTimings", [](EulerSchemeFromProcessModel& p) { return to_array(p.lastTimings()); });
+        .def("lastTimings", [](EulerSchemeFromProcessModel& p) { return to_array(p.lastTimings()); })
+        .def("setRegenerateIncrements", &EulerSchemeFromProcessModel::setRegenerateIncrements);
 
     py::class_<LIBORMonteCarloSimulationFromLIBORModel, std::shared_ptr<LIBORMonteCarloSimulationFromLIBORModel>>(
         m, "LIBORMonteCarloSimulationFromLIBORModel")

@@ -427,6 +427,7 @@ void EulerSchemeFromProcessModel::ensureContext(bool storePaths) {
     cfg.libor_cap = model_->liborCap;
     cfg.store_paths = storePaths ? 1 : 0;
     cfg.checkpoint_stride = 4;
+    cfg.regenerate_increments = regenerate_ ? 1 : 0;
     if (driver_->getNumberOfFactors() != model_->getCovarianceModel()->getNumberOfFactors())
         throw IllegalArgumentException("number of factors of the driver and of the covariance model differ");
     fm_lmm* c = nullptr;

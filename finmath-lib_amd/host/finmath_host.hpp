@@ -260,6 +260,8 @@ public:
     BrownianMotionPtr getStochasticDriver() const { return driver_; }
     Scheme getScheme() const { return scheme_; }
     int getPrecision() const { return precision_; }
+    // 1: regenerate the Brownian increments on every GPU pass instead of keeping them
+    void setRegenerateIncrements(bool on) { regenerate_ = on; if (ctx_) { fm_lmm_destroy(ctx_); ctx_ = nullptr; simulated_ = false; } }
     // Engine access: value products in one fused GPU pass; adjoint sweep.
     void valueProducts(const std::vector<std::shared_ptr<const Product>>& products, std::vector<double>& values,
                        std::vector<double>& errors);
@@ -274,6 +276,7 @@ private:
     fm_lmm* ctx_ = nullptr;
     bool ctxStores_ = false;
     bool simulated_ = false;
+    bool regenerate_ = false;
 };
 using ProcessPtr = std::shared_ptr<EulerSchemeFromProcessModel>;
 

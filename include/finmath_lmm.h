@@ -87,6 +87,10 @@ typedef struct fm_lmm_config {
     double        libor_cap;        /* liborCap; +infinity disables the cap (default 1e5) */
     int32_t       store_paths;      /* 1: fm_lmm_simulate materialises the process */
     int32_t       checkpoint_stride;/* adjoint forward-state checkpoint stride (0 -> 4) */
+    int32_t       regenerate_increments; /* 0: the Brownian increments are generated once and
+                                            kept in device memory, as the reference caches them
+                                            (BrownianMotionFromMersenneRandomNumbers.java:123-136);
+                                            1: regenerated by every pricing/simulation call */
 } fm_lmm_config;
 
 /* Create a context (allocates device memory, seeds the generator).
