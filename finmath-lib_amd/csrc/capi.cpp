@@ -79,6 +79,8 @@ struct fm_lmm {
     // model inputs
     std::vector<double> L0, lam, var, pd;
     bool has_L0 = false, has_cov = false, has_pd = false;
+    std::vector<double> vf_sig, vf_fac, vf_c;       // volatility factorization for the adjoint
+    bool has_vf = false;
     bool tables_dirty = true;
 
     // derived host tables
@@ -96,6 +98,7 @@ struct fm_lmm {
     DevBuf d_times, d_first, d_tenor_ti, d_lam, d_hv, d_dc1, d_dc2, d_delta, d_L0, d_sqrt_dt;
     DevBuf d_states, d_poly, d_dw, d_row_of, d_paths, d_num;
     DevBuf d_products, d_periods, d_wave_sum, d_wave_m2, d_contrib, d_out, d_swapmeta, d_scale;
+    DevBuf d_ckpt, d_adj_num, d_kcur, d_fsig, d_sig, d_cdiag, d_wave_grad, d_grad_out;
 
     int n() const { return cfg.num_libors; }
     int nT() const { return cfg.num_time_steps; }
@@ -415,7 +418,8 @@ struct Timer {
 };
 
 // Run generator + fused step (+ reductions). Returns per-column sums and M2 (host).
-void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums, std::vector<double>& m2s) {
+void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums, std::vector<double>& m2s,
+              void* ckpt = nullptr, int ckpt_stride = 1, double* num_out = nullptr) {
     prepare_model(c);
     hipStream_t s = c->stream;
     const int n = c->n(), P = c->P();
@@ -454,7 +458,9 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     a.wave_sum = wsum; a.wave_m2 = wm2;
     a.paths = store ? c->d_paths.get((size_t)std::max(c->num_rows, 1) * P * c->real_size()) : nullptr;
     a.row_of = c->d_row_of.as<int32_t>();
-    a.numeraire = store ? (double*)c->d_num.get((size_t)(n + 1) * P * sizeof(double)) : nullptr;
+    a.numeraire = store ? (double*)c->d_num.get((size_t)(n + 1) * P * sizeof(double)) : num_out;
+    a.ckpt = ckpt;
+    a.ckpt_stride = ckpt ? ckpt_stride : 1;
     tm.mark(2);
     if (c->fp32()) launch_lmm_step_fp32(a, s); else launch_lmm_step_fp64(a, s);
     tm.mark(3);
@@ -697,10 +703,120 @@ int fm_lmm_value_products(fm_lmm* c, const fm_product* products, int32_t count, 
     FM_CATCH
 }
 
-int fm_lmm_adjoint(fm_lmm* c, const fm_product*, int32_t, const double*, double*, double*, double*, double*) {
+int fm_lmm_set_volatility_factorization(fm_lmm* c, const double* volatility, const double* factors,
+                                        const double* variance_factor) {
+    FM_TRY
+    require(c && volatility && factors, FM_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->vf_sig.assign(volatility, volatility + (size_t)c->nT() * c->n());
+    c->vf_fac.assign(factors, factors + (size_t)c->n() * c->F());
+    if (variance_factor) c->vf_c.assign(variance_factor, variance_factor + c->n());
+    else c->vf_c.assign((size_t)c->n(), 1.0);
+    c->has_vf = true;
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const double* weights, double* value,
+                   double* d_initial_forwards, double* d_volatility) {
     FM_TRY
     require(c != nullptr, FM_ERR_INVALID_ARGUMENT, "NULL context");
-    throw FmError(FM_ERR_UNSUPPORTED, "adjoint not available in this build");
+    require(count >= 1 && products != nullptr, FM_ERR_INVALID_ARGUMENT, "no products");
+    std::lock_guard<std::mutex> lk(c->mu);
+    require(c->has_vf, FM_ERR_STATE, "volatility factorization not set (fm_lmm_set_volatility_factorization)");
+    DeviceGuard g(c->device);
+    prepare_model(c);
+    const int n = c->n(), nT = c->nT(), P = c->P();
+    hipStream_t s = c->stream;
+    Lowered L = lower_products(c, products, count);
+    const int stride = std::max(1, c->cfg.checkpoint_stride);
+    const int ncp = nT / stride + 1;
+    void* ckpt = c->d_ckpt.get((size_t)ncp * n * P * c->real_size());
+    double* num = (double*)c->d_adj_num.get((size_t)(n + 1) * P * sizeof(double));
+    hipEvent_t e0 = c->ev[5];
+    HIP_CHECK(hipEventRecord(e0, s));
+    std::vector<double> sums, m2s;
+    run_pass(c, L, false, sums, m2s, ckpt, stride, num);
+    const double fwd_ms = c->timings[4];
+    // objective weights per lowered product: w_q * (adjustment scale) / P
+    const std::vector<double> adj = adjustment_chain(c);
+    std::vector<double> wq((size_t)count, 1.0);
+    if (weights) for (int q = 0; q < count; q++) wq[(size_t)q] = weights[q];
+    // map each lowered entry back to its product: swaptions/caplets by slot, swap periods by column
+    std::vector<int> slot_to_q((size_t)L.num_slots, -1), col_to_q((size_t)L.num_contrib, -1);
+    for (int q = 0; q < count; q++) slot_to_q[(size_t)L.slot_of[(size_t)q]] = q;
+    for (size_t i = 0; i < L.swap_slot.size(); i++) {
+        const int q = slot_to_q[(size_t)L.swap_slot[i]];
+        for (int cc = 0; cc < L.swap_ncol[i]; cc++) col_to_q[(size_t)(L.swap_col0[i] + cc)] = q;
+    }
+    std::vector<KProduct> kp = L.kp;
+    for (auto& k : kp) {
+        if (k.type == FM_PRODUCT_SWAP) {
+            const int q = col_to_q[(size_t)k.out_slot];
+            const double sc = c->has_pd ? adj[(size_t)k.num_k] / c->zb[(size_t)k.num_k] : 1.0;
+            k.weight = wq[(size_t)q] * sc / (double)P;
+        } else {
+            const int q = slot_to_q[(size_t)k.out_slot];
+            const double sc = c->has_pd ? adj[(size_t)k.num_k] / c->zb[(size_t)k.num_k] : 1.0;
+            k.weight = wq[(size_t)q] * sc / (double)P;
+        }
+    }
+    std::vector<double> kweights(kp.size());
+    for (size_t i = 0; i < kp.size(); i++) kweights[i] = kp[i].weight;
+    std::vector<int32_t> kcur((size_t)nT + 1);
+    for (int t = 0; t <= nT; t++) kcur[(size_t)t] = kcur_at(c, t);
+    StepArgs a{};
+    a.nT = nT; a.n = n; a.F = c->F(); a.P = P;
+    a.precision = c->cfg.precision; a.state_space = c->cfg.state_space;
+    a.times = c->d_times.as<double>(); a.first_alive = c->d_first.as<int32_t>();
+    a.tenor_time_index = c->d_tenor_ti.as<int32_t>();
+    a.lam = c->d_lam.p; a.hv = c->d_hv.p;
+    a.dc1 = c->d_dc1.as<double>(); a.dc2 = c->d_dc2.as<double>(); a.delta = c->d_delta.as<double>();
+    a.L0 = c->d_L0.as<double>(); a.dw = c->d_dw.p; a.cap = c->cfg.libor_cap; a.scheme = c->cfg.scheme;
+    a.num_products = (int)kp.size();
+    a.products = upload(c->d_products, kp, s);
+    a.periods = upload(c->d_periods, L.per, s);
+    a.numeraire = num; a.ckpt = ckpt; a.ckpt_stride = stride;
+    AdjArgs ga{};
+    ga.stride = stride;
+    ga.kcur_at = upload(c->d_kcur, kcur, s);
+    ga.weight = upload(c->d_scale, kweights, s);
+    ga.fsig = upload(c->d_fsig, c->vf_fac, s);
+    ga.sig = upload(c->d_sig, c->vf_sig, s);
+    ga.cdiag = upload(c->d_cdiag, c->vf_c, s);
+    const int64_t num_waves = ((int64_t)P + 63) / 64;
+    const int gstride = nT * n + n;
+    ga.wave_grad = (double*)c->d_wave_grad.get((size_t)num_waves * gstride * sizeof(double));
+    HIP_CHECK(hipMemsetAsync(ga.wave_grad, 0, (size_t)num_waves * gstride * sizeof(double), s));
+    HIP_CHECK(hipEventRecord(c->ev[0], s));
+    if (c->fp32()) launch_lmm_adjoint_fp32(a, ga, s); else launch_lmm_adjoint_fp64(a, ga, s);
+    HIP_CHECK(hipEventRecord(c->ev[1], s));
+    double* gout = (double*)c->d_grad_out.get((size_t)gstride * 2 * sizeof(double));
+    launch_reduce_partials(ga.wave_grad, ga.wave_grad, (int)num_waves, gstride, 0, gstride, P, gout, gout + gstride, s);
+    std::vector<double> grad((size_t)gstride);
+    HIP_CHECK(hipMemcpyAsync(grad.data(), gout, gstride * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipEventRecord(c->ev[2], s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    float ms_adj = 0, ms_all = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms_adj, c->ev[0], c->ev[1]));
+    HIP_CHECK(hipEventElapsedTime(&ms_all, e0, c->ev[2]));
+    c->timings[3] = ms_adj;
+    c->timings[4] = ms_all;
+    (void)fwd_ms;
+    if (value) {
+        double v = 0.0;
+        for (int q = 0; q < count; q++) {
+            const int slot = L.slot_of[(size_t)q];
+            const int k = L.slot_k[(size_t)slot];
+            const double sc = (k >= 0 && c->has_pd) ? adj[(size_t)k] / c->zb[(size_t)k] : 1.0;
+            v += wq[(size_t)q] * sc * sums[(size_t)slot] / (double)P;
+        }
+        *value = v;
+    }
+    if (d_volatility) for (size_t i = 0; i < (size_t)nT * n; i++) d_volatility[i] = grad[i];
+    if (d_initial_forwards) for (int j = 0; j < n; j++) d_initial_forwards[j] = grad[(size_t)nT * n + j];
+    for (double x : grad) require(std::isfinite(x), FM_ERR_NUMERICAL, "adjoint is not finite");
+    return FM_OK;
     FM_CATCH
 }
 

@@ -64,6 +64,7 @@ struct KProduct {
     int32_t num_k;        // tenor index of the numeraire the payoff is divided by
     double  notional;
     double  exercise;
+    double  weight;       // adjoint: objective weight of this (lowered) product per path
     int32_t param_off;    // offset into the per-period parameter table
     int32_t pad2;
 };
@@ -103,10 +104,25 @@ struct StepArgs {
     double* wave_m2;             // [num_waves][num_slots]
     void* paths;                 // device triangle rows [rows][P] (store_paths)
     const int32_t* row_of;       // device [(nT+1)*n] row index of (t,j), -1 = initial value
-    double* numeraire;           // device [(n+1)][P] unadjusted numeraire (store_paths)
+    double* numeraire;           // device [(n+1)][P] unadjusted numeraire (store_paths or adjoint)
+    void* ckpt;                  // device [ceil((nT+1)/stride)][n][P] forward states (adjoint), or NULL
+    int ckpt_stride;
 };
 void launch_lmm_step_fp64(const StepArgs& a, void* stream);
 void launch_lmm_step_fp32(const StepArgs& a, void* stream);
+
+// Reverse sweep (lmm_adjoint.hpp): adjoint arguments and launchers.
+struct AdjArgs {
+    int stride;                      // checkpoint stride
+    const int32_t* kcur_at;          // [nT+1] numeraire index after the advance at each time index
+    const double* weight;            // [num kernel products] objective weight of each lowered product
+    const double* fsig;              // [n][F] factor matrix f_jk (double)
+    const double* sig;               // [nT][n] volatility sigma_j(t_i)
+    const double* cdiag;             // [n] variance factor c_j (correlation diagonal)
+    double* wave_grad;               // [num_waves][nT*n + n] partials of dV/dsigma and dV/dL(0)
+};
+void launch_lmm_adjoint_fp64(const StepArgs& a, const AdjArgs& g, void* stream);
+void launch_lmm_adjoint_fp32(const StepArgs& a, const AdjArgs& g, void* stream);
 
 // Deterministic reduction of per-wave partials: out_sum[col], out_m2[col] (Chan merge).
 void launch_reduce_partials(const double* wave_sum, const double* wave_m2, int num_waves, int stride,

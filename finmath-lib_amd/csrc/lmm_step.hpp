@@ -69,6 +69,42 @@ __device__ __forceinline__ void reduce_slot(double x, bool active, int cnt, int 
     }
 }
 
+// One Euler step t_i -> t_{i+1} of all alive components, in place. Shared by the
+// pricing pass and the adjoint recompute so both produce identical paths.
+template <int NCAP, int FCAP, typename real>
+__device__ __forceinline__ void euler_step(real (&L)[NCAP], int n, int F, int first, real dt, const real (&dW)[FCAP],
+                                           const real* __restrict__ lam_i, const real* __restrict__ hv_i,
+                                           const double* __restrict__ dc1, const double* __restrict__ dc2,
+                                           bool lognormal, bool capped, real cap, real* __restrict__ paths,
+                                           const int32_t* __restrict__ row_next, int64_t p, int P, bool active) {
+    real S[FCAP];
+#pragma unroll
+    for (int k = 0; k < FCAP; k++) S[k] = (real)0;
+#pragma unroll
+    for (int j = 0; j < NCAP; j++) {
+        if (j < n && j >= first) {
+            const real Lj = L[j];
+            real w = (real)1.0 / (Lj * (real)dc1[j] + (real)dc2[j]);
+            if (lognormal) w = w * Lj;
+            const real* lj = lam_i + j * F;
+#pragma unroll
+            for (int k = 0; k < FCAP; k++) if (k < F) S[k] = S[k] + w * lj[k];
+            real mu = S[0] * lj[0];
+#pragma unroll
+            for (int k = 1; k < FCAP; k++) if (k < F) mu = mu + S[k] * lj[k];
+            mu = mu + hv_i[j];
+            real Y = lognormal ? dev_log<real>(Lj) : Lj;
+            Y = Y + mu * dt;
+#pragma unroll
+            for (int k = 0; k < FCAP; k++) if (k < F) Y = Y + dW[k] * lj[k];
+            real Ln = lognormal ? dev_exp<real>(Y) : Y;
+            if (capped) Ln = jmin(Ln, cap);
+            L[j] = Ln;
+            if (paths && active) paths[(size_t)row_next[j] * P + p] = Ln;
+        }
+    }
+}
+
 template <int NCAP, int FCAP, typename real, typename dwT>
 __global__ void __launch_bounds__(kStepThreads)
 lmm_fused_kernel(const StepArgs a,
@@ -80,7 +116,8 @@ lmm_fused_kernel(const StepArgs a,
                  const dwT* __restrict__ dw,
                  double* __restrict__ wave_sum, double* __restrict__ wave_m2,
                  double* __restrict__ contrib, real* __restrict__ paths,
-                 const int32_t* __restrict__ row_of, double* __restrict__ numeraire_out)
+                 const int32_t* __restrict__ row_of, double* __restrict__ numeraire_out,
+                 real* __restrict__ ckpt)
 {
     const int64_t p = (int64_t)blockIdx.x * kStepThreads + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -103,6 +140,12 @@ lmm_fused_kernel(const StepArgs a,
     int qp = 0;
 
     for (int ti = 0; ti <= nT; ti++) {
+        // ---- forward-state checkpoint for the adjoint sweep
+        if (ckpt && active && (ti % a.ckpt_stride) == 0) {
+            const size_t base = (size_t)(ti / a.ckpt_stride) * n;
+#pragma unroll
+            for (int j = 0; j < NCAP; j++) if (j < n) ckpt[(base + j) * P + p] = L[j];
+        }
         // ---- numeraire N(T_{k+1}) = N(T_k).accrue(L_k(T_k), d_k) once L_k is final
         while (kcur < n && tenor_ti[kcur] <= ti) {
             const double Lk = (double)pick<NCAP, real>(L, kcur);
@@ -158,39 +201,12 @@ lmm_fused_kernel(const StepArgs a,
         if (ti == nT) break;
 
         // ---- Euler step ti -> ti+1
-        const int first = first_alive[ti];
-        const real dt = (real)(times[ti + 1] - times[ti]);
         real dW[FCAP];
 #pragma unroll
         for (int k = 0; k < FCAP; k++) dW[k] = (k < F) ? (real)dw[((size_t)pc * nT + ti) * F + k] : (real)0;
-        real S[FCAP];
-#pragma unroll
-        for (int k = 0; k < FCAP; k++) S[k] = (real)0;
-        const real* lam_i = lam + (size_t)ti * n * F;
-        const real* hv_i = hv + (size_t)ti * n;
-#pragma unroll
-        for (int j = 0; j < NCAP; j++) {
-            if (j < n && j >= first) {
-                const real Lj = L[j];
-                real w = (real)1.0 / (Lj * (real)dc1[j] + (real)dc2[j]);
-                if (lognormal) w = w * Lj;
-                const real* lj = lam_i + j * F;
-#pragma unroll
-                for (int k = 0; k < FCAP; k++) if (k < F) S[k] = S[k] + w * lj[k];
-                real mu = S[0] * lj[0];
-#pragma unroll
-                for (int k = 1; k < FCAP; k++) if (k < F) mu = mu + S[k] * lj[k];
-                mu = mu + hv_i[j];
-                real Y = lognormal ? dev_log<real>(Lj) : Lj;
-                Y = Y + mu * dt;
-#pragma unroll
-                for (int k = 0; k < FCAP; k++) if (k < F) Y = Y + dW[k] * lj[k];
-                real Ln = lognormal ? dev_exp<real>(Y) : Y;
-                if (capped) Ln = jmin(Ln, cap);
-                L[j] = Ln;
-                if (paths && active) paths[(size_t)row_of[(size_t)(ti + 1) * n + j] * P + p] = Ln;
-            }
-        }
+        euler_step<NCAP, FCAP, real>(L, n, F, first_alive[ti], (real)(times[ti + 1] - times[ti]), dW,
+                                     lam + (size_t)ti * n * F, hv + (size_t)ti * n, dc1, dc2, lognormal, capped, cap,
+                                     paths, row_of + (size_t)(ti + 1) * n, p, P, active);
     }
 }
 

@@ -36,7 +36,8 @@ EXPORTED_SYMBOLS = (
     "fm_lmm_create", "fm_lmm_destroy", "fm_lmm_set_initial_forwards", "fm_lmm_set_covariance",
     "fm_lmm_set_discount_curve", "fm_lmm_simulate", "fm_lmm_get_process_value",
     "fm_lmm_process_value_device_ptr", "fm_lmm_get_numeraire", "fm_lmm_get_brownian_increment",
-    "fm_lmm_value_products", "fm_lmm_adjoint", "fm_mt19937_words", "fm_mt19937_doubles",
+    "fm_lmm_value_products", "fm_lmm_set_volatility_factorization", "fm_lmm_adjoint",
+    "fm_mt19937_words", "fm_mt19937_doubles",
     "fm_last_error", "fm_lmm_last_timings", "fm_device_info",
 )
 
@@ -117,7 +118,8 @@ def load() -> ctypes.CDLL:
         "fm_lmm_get_numeraire": ([c_void_p, c_double, dp], c_int32),
         "fm_lmm_get_brownian_increment": ([c_void_p, c_int32, c_int32, dp], c_int32),
         "fm_lmm_value_products": ([c_void_p, POINTER(FmProduct), c_int32, dp, dp], c_int32),
-        "fm_lmm_adjoint": ([c_void_p, POINTER(FmProduct), c_int32, dp, dp, dp, dp, dp], c_int32),
+        "fm_lmm_set_volatility_factorization": ([c_void_p, dp, dp, dp], c_int32),
+        "fm_lmm_adjoint": ([c_void_p, POINTER(FmProduct), c_int32, dp, dp, dp, dp], c_int32),
         "fm_mt19937_words": ([c_int64, c_uint64, c_uint64, POINTER(c_uint32)], c_int32),
         "fm_mt19937_doubles": ([c_int64, c_uint64, c_uint64, dp], c_int32),
         "fm_last_error": ([], c_char_p),
@@ -291,16 +293,23 @@ class LmmContext:
         check(self.lib.fm_lmm_value_products(self.h, arr, len(products), _dptr(vals), _dptr(errs)))
         return vals, errs
 
+    def set_volatility_factorization(self, volatility, factors, variance_factor=None):
+        v = as_f64(volatility).reshape(-1)
+        f = as_f64(factors).reshape(-1)
+        c = None if variance_factor is None else as_f64(variance_factor)
+        check(self.lib.fm_lmm_set_volatility_factorization(self.h, _dptr(v), _dptr(f),
+                                                           POINTER(c_double)() if c is None else _dptr(c)))
+
     def adjoint(self, products, weights=None):
+        """(value, dV/dL(0) [n], dV/dsigma [nT][n])"""
         arr = product_array(products)
         w = None if weights is None else as_f64(weights)
         value = np.zeros(1)
         dL0 = np.zeros(self.n)
-        dlam = np.zeros(self.nT * self.n * self.F)
-        dvar = np.zeros(self.nT * self.n)
+        dsig = np.zeros(self.nT * self.n)
         check(self.lib.fm_lmm_adjoint(self.h, arr, len(products), POINTER(c_double)() if w is None else _dptr(w),
-                                      _dptr(value), _dptr(dL0), _dptr(dlam), _dptr(dvar)))
-        return value[0], dL0, dlam.reshape(self.nT, self.n, self.F), dvar.reshape(self.nT, self.n)
+                                      _dptr(value), _dptr(dL0), _dptr(dsig)))
+        return value[0], dL0, dsig.reshape(self.nT, self.n)
 
     def timings(self) -> np.ndarray:
         ms = np.zeros(5)

@@ -214,7 +214,27 @@ PYBIND11_MODULE(_finmath_host, m) {
         .def("getTime", &EulerSchemeFromProcessModel::getTime)
         .def("getTimeIndex", &EulerSchemeFromProcessModel::getTimeIndex)
         .def("lastTimings", [](EulerSchemeFromProcessModel& p) { return to_array(p.lastTimings()); })
-        .def("setRegenerateIncrements", &EulerSchemeFromProcessModel::setRegenerateIncrements);
+        .def("setRegenerateIncrements", &EulerSchemeFromProcessModel::setRegenerateIncrements)
+        .def("getGradient", [](EulerSchemeFromProcessModel& p, const std::vector<std::shared_ptr<Product>>& ps,
+                               std::vector<double> weights, int stride) {
+            std::vector<ProductPtr> v(ps.begin(), ps.end());
+            Gradient g;
+            {
+                py::gil_scoped_release rel;
+                g = p.getGradient(v, weights, stride);
+            }
+            const int n = (int)g.delta.size();
+            const int nT = n ? (int)(g.volatility.size() / n) : 0;
+            py::dict d;
+            d["value"] = g.value;
+            d["delta"] = to_array(g.delta);
+            py::array_t<double> vol({nT, n});
+            std::copy(g.volatility.begin(), g.volatility.end(), vol.mutable_data());
+            d["volatility"] = vol;
+            d["forwardVega"] = to_array(g.forwardVega);
+            d["parameterVega"] = to_array(g.parameterVega);
+            return d;
+        }, py::arg("products"), py::arg("weights") = std::vector<double>{}, py::arg("checkpointStride") = 4);
 
     py::class_<LIBORMonteCarloSimulationFromLIBORModel, std::shared_ptr<LIBORMonteCarloSimulationFromLIBORModel>>(
         m, "LIBORMonteCarloSimulationFromLIBORModel")

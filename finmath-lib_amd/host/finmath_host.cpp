@@ -426,7 +426,7 @@ void EulerSchemeFromProcessModel::ensureContext(bool storePaths) {
     cfg.precision = precision_;
     cfg.libor_cap = model_->liborCap;
     cfg.store_paths = storePaths ? 1 : 0;
-    cfg.checkpoint_stride = 4;
+    cfg.checkpoint_stride = ckptStride_;
     cfg.regenerate_increments = regenerate_ ? 1 : 0;
     if (driver_->getNumberOfFactors() != model_->getCovarianceModel()->getNumberOfFactors())
         throw IllegalArgumentException("number of factors of the driver and of the covariance model differ");
@@ -443,9 +443,64 @@ void EulerSchemeFromProcessModel::ensureContext(bool storePaths) {
             for (size_t k = 0; k < tenor.size(); k++) pd[k] = model_->getDiscountCurve()->getDiscountFactor(tenor[k]);
             check(fm_lmm_set_discount_curve(c, pd.data()));
         }
+        // volatility factorization (lambda = sigma f, variance = sigma^2 rho_jj) for the adjoint
+        const auto cov = model_->getCovarianceModel();
+        const auto& sim = *driver_->getTimeDiscretization();
+        const int nT = sim.getNumberOfTimeSteps(), n = (int)tenor.size() - 1, F = cov->getNumberOfFactors();
+        std::vector<double> sig((size_t)nT * n), fac((size_t)n * F), cd((size_t)n);
+        for (int i = 0; i < nT; i++) {
+            int ci = cov->getTimeDiscretization()->getTimeIndex(sim.getTime(i));
+            if (ci < 0) ci = -ci - 2;
+            for (int j = 0; j < n; j++) sig[(size_t)i * n + j] = cov->getVolatilityModel()->getVolatility(ci, j);
+        }
+        for (int j = 0; j < n; j++) {
+            for (int k = 0; k < F; k++) fac[(size_t)j * F + k] = cov->getCorrelationModel()->getFactorLoading(0, k, j);
+            cd[(size_t)j] = cov->getCorrelationModel()->getCorrelation(0, j, j);
+        }
+        check(fm_lmm_set_volatility_factorization(c, sig.data(), fac.data(), cd.data()));
     } catch (...) { fm_lmm_destroy(c); throw; }
     ctx_ = c;
     ctxStores_ = storePaths;
+}
+
+Gradient EulerSchemeFromProcessModel::getGradient(const std::vector<ProductPtr>& products, const std::vector<double>& weights,
+                                                  int checkpointStride) {
+    if (ctx_ && checkpointStride != ckptStride_) { fm_lmm_destroy(ctx_); ctx_ = nullptr; simulated_ = false; }
+    ckptStride_ = checkpointStride;
+    ensureContext(false);
+    std::deque<std::vector<double>> arena;
+    std::vector<fm_product> lowered(products.size());
+    for (size_t q = 0; q < products.size(); q++) products[q]->lower(arena, lowered[q]);
+    if (!weights.empty() && weights.size() != products.size())
+        throw IllegalArgumentException("one weight per product expected");
+    const int n = getNumberOfComponents(), nT = getTimeDiscretization()->getNumberOfTimeSteps();
+    Gradient g;
+    g.delta.assign((size_t)n, 0.0);
+    g.volatility.assign((size_t)nT * n, 0.0);
+    check(fm_lmm_adjoint(ctx_, lowered.data(), (int32_t)lowered.size(), weights.empty() ? nullptr : weights.data(),
+                         &g.value, g.delta.data(), g.volatility.data()));
+    g.forwardVega.assign((size_t)n, 0.0);
+    for (int i = 0; i < nT; i++)
+        for (int j = 0; j < n; j++) g.forwardVega[(size_t)j] += g.volatility[(size_t)i * n + j];
+    // chain rule to (a, b, c, d) of sigma = (a + b tau) exp(-c tau) + d, tau = T_j - t_i > 0
+    auto fp = std::dynamic_pointer_cast<const LIBORVolatilityModelFourParameterExponentialForm>(
+        model_->getCovarianceModel()->getVolatilityModel());
+    if (fp) {
+        g.parameterVega.assign(4, 0.0);
+        const auto& sim = *getTimeDiscretization();
+        const auto& tenor = *model_->getLiborPeriodDiscretization();
+        for (int i = 0; i < nT; i++)
+            for (int j = 0; j < n; j++) {
+                const double tau = tenor.getTime(j) - sim.getTime(i);
+                if (tau <= 0) continue;
+                const double e = std::exp(-fp->c * tau), gs = g.volatility[(size_t)i * n + j];
+                g.parameterVega[0] += gs * e;
+                g.parameterVega[1] += gs * tau * e;
+                g.parameterVega[2] += gs * (-tau) * (fp->a + fp->b * tau) * e;
+                g.parameterVega[3] += gs;
+            }
+    }
+    return g;
 }
 
 fm_lmm* EulerSchemeFromProcessModel::handle() { ensureContext(false); return ctx_; }

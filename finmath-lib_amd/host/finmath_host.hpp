@@ -233,6 +233,16 @@ private:
 // ---- montecarlo/process/EulerSchemeFromProcessModel.java ------------------------------
 class Product;
 
+// Result of a pathwise adjoint sweep (RandomVariableDifferentiable.getGradient for the
+// model leaves FORWARD(T_j) and the volatility parameters).
+struct Gradient {
+    double value = 0.0;
+    std::vector<double> delta;            // dV/dL_j(0), j < n
+    std::vector<double> volatility;       // dV/dsigma_j(t_i), [nT][n]
+    std::vector<double> forwardVega;      // sum_i dV/dsigma_j(t_i) (parallel bump of sigma_j)
+    std::vector<double> parameterVega;    // dV/d(a,b,c,d) for the four-parameter form, else empty
+};
+
 class EulerSchemeFromProcessModel {
 public:
     enum Scheme { EULER = FM_SCHEME_EULER, PREDICTOR_CORRECTOR = FM_SCHEME_PREDICTOR_CORRECTOR,
@@ -266,6 +276,8 @@ public:
     void valueProducts(const std::vector<std::shared_ptr<const Product>>& products, std::vector<double>& values,
                        std::vector<double>& errors);
     std::vector<double> lastTimings();
+    Gradient getGradient(const std::vector<std::shared_ptr<const Product>>& products, const std::vector<double>& weights,
+                         int checkpointStride = 4);
     fm_lmm* handle();
 private:
     void ensureContext(bool storePaths);
@@ -277,6 +289,7 @@ private:
     bool ctxStores_ = false;
     bool simulated_ = false;
     bool regenerate_ = false;
+    int ckptStride_ = 4;
 };
 using ProcessPtr = std::shared_ptr<EulerSchemeFromProcessModel>;
 

@@ -181,21 +181,33 @@ int fm_lmm_value_products(fm_lmm* ctx, const fm_product* products, int32_t count
                           double* values, double* errors);
 
 /* ---- pathwise adjoint (Greeks) -----------------------------------------------
- * Gradient of V = sum_q weights[q] * value_q (weights NULL -> all 1) with respect
- * to the model inputs, by a hand-written reverse sweep of the Euler scheme with
- * forward-state checkpoints every checkpoint_stride steps:
- *   d_initial_forwards[j]             = dV/dL_j(0)
- *   d_factor_loading[(i*n+j)*F + k]   = dV/dlambda_{j,k}(t_i)
- *   d_variance[i*n + j]               = dV/dvariance_j(t_i)
- * (any output may be NULL). The numeraire-adjustment average is held constant,
- * as the reference's AAD treats getAverage results
- * (montecarlo/automaticdifferentiation/backward/RandomVariableDifferentiableAAD.java:165-219).
+ * Volatility factorization of the covariance tables, needed by the adjoint:
+ *   factor_loading[(i*n+j)*F+k] = volatility[i*n+j] * factors[j*F+k],
+ *   variance[i*n+j]              = volatility[i*n+j]^2 * variance_factor[j]
+ * (variance_factor NULL -> 1.0), i.e. the structure of
+ * LIBORCovarianceModelFromVolatilityAndCorrelation.getFactorLoading/getCovariance
+ * (models/covariance/LIBORCovarianceModelFromVolatilityAndCorrelation.java:48-93). */
+int fm_lmm_set_volatility_factorization(fm_lmm* ctx, const double* volatility, const double* factors,
+                                        const double* variance_factor);
+
+/* Gradient of V = sum_q weights[q] * value_q (weights NULL -> all 1) by a
+ * hand-written reverse sweep of the Euler scheme with forward-state checkpoints
+ * every checkpoint_stride steps (the sweep recomputes the states in between):
+ *   value                      = V
+ *   d_initial_forwards[j]      = dV/dL_j(0)              (n entries, may be NULL)
+ *   d_volatility[i*n + j]      = dV/dsigma_j(t_i)        (nT*n entries, may be NULL)
+ * Greeks of volatility parameters follow by the chain rule through sigma (the
+ * per-forward vega of forward j is sum_i d_volatility[i*n+j]). Averages over paths
+ * (the numeraire adjustment) are held constant, as the reference's AAD treats
+ * getAverage results (montecarlo/automaticdifferentiation/backward/
+ * RandomVariableDifferentiableAAD.java:165-219).
  * Replaces RandomVariableDifferentiable.getGradient
  * (montecarlo/automaticdifferentiation/RandomVariableDifferentiable.java:64-75)
- * for the leaves of LIBORMarketModelFromCovarianceModel.getModelParameters
+ * for the leaves FORWARD(T_j) and the volatility parameters of
+ * LIBORMarketModelFromCovarianceModel.getModelParameters
  * (models/LIBORMarketModelFromCovarianceModel.java:1699-1732). */
 int fm_lmm_adjoint(fm_lmm* ctx, const fm_product* products, int32_t count, const double* weights,
-                   double* value, double* d_initial_forwards, double* d_factor_loading, double* d_variance);
+                   double* value, double* d_initial_forwards, double* d_volatility);
 
 /* ---- generator ----------------------------------------------------------------
  * The GPU MT19937 stream of commons-math3 MersenneTwister(long seed) as used by
