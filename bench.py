@@ -85,6 +85,27 @@ def run_gpu(wl, precision, steps, warmup):
     return ms_per_step, stage / steps, values, errors
 
 
+def run_greeks(wl, steps, stride):
+    """Time of the adjoint call (forward pass with checkpoints + reverse sweep) over the pricing pass."""
+    import torch
+    sim, process, products = build_simulation(wl, wl.precision, regenerate=True)
+    sim.getValues(products)
+    process.getGradient(products, [], stride)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.getValues(products)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        g = process.getGradient(products, [], stride)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    price_ms = (t1 - t0) * 1e3 / steps
+    greeks_ms = (t2 - t1) * 1e3 / steps
+    return price_ms, greeks_ms, g
+
+
 def cpu_baseline(wl_gpu, paths):
     """The oracle (C++ restatement of the reference CPU path) on a bounded sample."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -108,7 +129,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-paths", type=int, default=20_000)
+    ap.add_argument("--cpu-paths", type=int, default=1_000_000)
+    ap.add_argument("--no-greeks", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
@@ -155,6 +177,21 @@ def main():
             "max_abs_err_bp_vs_fp64": float(err_bp.max()), "max_mc_stderr_bp": float(e64b.max() * 1e4),
             "within_0.1bp": bool(err_bp.max() < 0.1),
         }
+    ms_ls, stage_ls, v_ls, _ = run_gpu(wl, "FP64_LOGSTATE", args.steps, args.warmup)
+    line["fp64_logstate"] = {
+        "workload": "c1, FP64 arithmetic carrying the log-forwards between steps, fused multiply-add",
+        "value": path_steps / (ms_ls * 1e-3), "unit": "path-steps/s", "ms_per_step": ms_ls,
+        "path_kernel_ms": stage_ls[1], "max_rel_diff_vs_fp64": float(np.max(np.abs(v_ls - v64) / np.abs(v64)))}
+    if not args.no_greeks:
+        wl3 = W.c3()
+        pm, gm, g = run_greeks(wl3, max(2, args.steps // 3), 4)
+        pm1, gm1, _ = run_greeks(wl3, max(2, args.steps // 3), 1)
+        line["greeks"] = {
+            "workload": "c3: c1 strip, 1,000,000 paths, FP32 paths, FP64 gradient sums; 40 deltas, 40 forward vegas, "
+                        "4 vol-parameter vegas, dV/dsigma table",
+            "pricing_ms": pm, "adjoint_ms_stride4": gm, "ratio_stride4": gm / pm,
+            "adjoint_ms_stride1": gm1, "ratio_stride1": gm1 / pm1,
+            "sum_delta": float(np.sum(g["delta"])), "parameter_vega": [float(x) for x in g["parameterVega"]]}
     if not args.no_cpu:
         cb = cpu_baseline(wl, args.cpu_paths)
         line["cpu_baseline"] = cb

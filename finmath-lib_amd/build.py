@@ -37,6 +37,7 @@ SOURCES = {
     "mt19937.hip": NO_CONTRACT,
     "lmm_step_fp64.hip": NO_CONTRACT,
     "lmm_step_fp32.hip": [],
+    "lmm_step_fp64_logstate.hip": [],
     "lmm_reduce.hip": NO_CONTRACT,
     "capi.cpp": NO_CONTRACT,
 }

@@ -95,7 +95,7 @@ struct fm_lmm {
     bool simulated = false;
     std::vector<double> zb;             // numeraire-adjustment averages of the last simulate
 
-    DevBuf d_times, d_first, d_tenor_ti, d_lam, d_hv, d_dc1, d_dc2, d_delta, d_L0, d_sqrt_dt;
+    DevBuf d_times, d_first, d_tenor_ti, d_lam, d_hv, d_dc1, d_dc2, d_delta, d_L0, d_Y0, d_sqrt_dt;
     DevBuf d_states, d_poly, d_dw, d_row_of, d_paths, d_num;
     DevBuf d_products, d_periods, d_wave_sum, d_wave_m2, d_contrib, d_out, d_swapmeta, d_scale;
     DevBuf d_ckpt, d_adj_num, d_kcur, d_fsig, d_sig, d_cdiag, d_wave_grad, d_grad_out;
@@ -154,8 +154,8 @@ void validate_config(const fm_lmm_config* c) {
     const bool functional_equiv = c->scheme == FM_SCHEME_EULER_FUNCTIONAL ||
         (c->scheme == FM_SCHEME_EULER && c->state_space == FM_STATE_SPACE_NORMAL && std::isinf(c->libor_cap));
     require(functional_equiv, FM_ERR_UNSUPPORTED, "only the EULER_FUNCTIONAL scheme is implemented on the GPU");
-    require(c->precision == FM_PRECISION_FP64 || c->precision == FM_PRECISION_FP32_MIXED,
-            FM_ERR_INVALID_ARGUMENT, "unknown precision");
+    require(c->precision == FM_PRECISION_FP64 || c->precision == FM_PRECISION_FP32_MIXED ||
+            c->precision == FM_PRECISION_FP64_LOGSTATE, FM_ERR_INVALID_ARGUMENT, "unknown precision");
     require(!std::isnan(c->libor_cap), FM_ERR_INVALID_ARGUMENT, "liborCap must not be NaN");
     require(c->num_libors <= 80 && c->num_factors <= 10, FM_ERR_UNSUPPORTED,
             "the fused kernel holds at most 80 forward rates and 10 factors");
@@ -227,6 +227,14 @@ void prepare_model(fm_lmm* c) {
         upload(c->d_hv, hv, s);
         upload(c->d_L0, c->init_value, s);
     }
+    // initial log-state log(max(L_j(0), 0)), kept below log(cap)
+    std::vector<double> y0((size_t)n);
+    for (int j = 0; j < n; j++) {
+        double y = std::log(std::max(c->L0[(size_t)j], 0.0));
+        if (!std::isinf(c->cfg.libor_cap)) y = std::min(y, std::log(c->cfg.libor_cap));
+        y0[(size_t)j] = y;
+    }
+    upload(c->d_Y0, y0, s);
     HIP_CHECK(hipStreamSynchronize(s));
     (void)F;
     c->tables_dirty = false;
@@ -446,6 +454,8 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     a.lam = c->d_lam.p; a.hv = c->d_hv.p;
     a.dc1 = c->d_dc1.as<double>(); a.dc2 = c->d_dc2.as<double>(); a.delta = c->d_delta.as<double>();
     a.L0 = c->d_L0.as<double>();
+    a.Y0 = c->d_Y0.as<double>();
+    a.logcap = std::isinf(c->cfg.libor_cap) ? HUGE_VAL : std::log(c->cfg.libor_cap);
     a.dw = c->d_dw.p;
     a.cap = c->cfg.libor_cap;
     a.scheme = c->cfg.scheme;
@@ -462,7 +472,10 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     a.ckpt = ckpt;
     a.ckpt_stride = ckpt ? ckpt_stride : 1;
     tm.mark(2);
-    if (c->fp32()) launch_lmm_step_fp32(a, s); else launch_lmm_step_fp64(a, s);
+    if (c->fp32()) launch_lmm_step_fp32(a, s);
+    else if (c->cfg.precision == FM_PRECISION_FP64_LOGSTATE && c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL)
+        launch_lmm_step_fp64_logstate(a, s);
+    else launch_lmm_step_fp64(a, s);
     tm.mark(3);
     // numeraire-adjustment averages first (the swap contributions need them)
     const int cols = L.num_slots + n + 1;
@@ -724,6 +737,8 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     require(count >= 1 && products != nullptr, FM_ERR_INVALID_ARGUMENT, "no products");
     std::lock_guard<std::mutex> lk(c->mu);
     require(c->has_vf, FM_ERR_STATE, "volatility factorization not set (fm_lmm_set_volatility_factorization)");
+    require(c->cfg.precision != FM_PRECISION_FP64_LOGSTATE, FM_ERR_UNSUPPORTED,
+            "the adjoint runs on the FP64 and FP32_MIXED path kernels");
     DeviceGuard g(c->device);
     prepare_model(c);
     const int n = c->n(), nT = c->nT(), P = c->P();

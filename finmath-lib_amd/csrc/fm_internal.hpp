@@ -89,6 +89,8 @@ struct StepArgs {
     const double* dc2;           // device n: 1/delta
     const double* delta;         // device n: T_{j+1}-T_j
     const double* L0;            // device n: initial forwards in value space
+    const double* Y0;            // device n: initial log-forwards (log-state kernels)
+    double logcap;               // log(cap) for the log-state kernels
     const void* dw;              // device dW [P][nT][F]
     double cap;
     int32_t scheme;
@@ -110,6 +112,7 @@ struct StepArgs {
 };
 void launch_lmm_step_fp64(const StepArgs& a, void* stream);
 void launch_lmm_step_fp32(const StepArgs& a, void* stream);
+void launch_lmm_step_fp64_logstate(const StepArgs& a, void* stream);
 
 // Reverse sweep (lmm_adjoint.hpp): adjoint arguments and launchers.
 struct AdjArgs {

@@ -71,19 +71,23 @@ __device__ __forceinline__ void reduce_slot(double x, bool active, int cnt, int 
 
 // One Euler step t_i -> t_{i+1} of all alive components, in place. Shared by the
 // pricing pass and the adjoint recompute so both produce identical paths.
-template <int NCAP, int FCAP, typename real>
+// LOGSTATE: the array holds the log-forwards Y_j and the step needs one exp per
+// component (the reference re-takes log(exp(Y)) every step, EULER_FUNCTIONAL;
+// carrying Y differs from it by rounding only, the cap applied as Y <= log(cap)).
+template <int NCAP, int FCAP, typename real, bool LOGSTATE = false>
 __device__ __forceinline__ void euler_step(real (&L)[NCAP], int n, int F, int first, real dt, const real (&dW)[FCAP],
                                            const real* __restrict__ lam_i, const real* __restrict__ hv_i,
                                            const double* __restrict__ dc1, const double* __restrict__ dc2,
                                            bool lognormal, bool capped, real cap, real* __restrict__ paths,
-                                           const int32_t* __restrict__ row_next, int64_t p, int P, bool active) {
+                                           const int32_t* __restrict__ row_next, int64_t p, int P, bool active,
+                                           real logcap = (real)0) {
     real S[FCAP];
 #pragma unroll
     for (int k = 0; k < FCAP; k++) S[k] = (real)0;
 #pragma unroll
     for (int j = 0; j < NCAP; j++) {
         if (j < n && j >= first) {
-            const real Lj = L[j];
+            const real Lj = LOGSTATE ? dev_exp<real>(L[j]) : L[j];
             real w = (real)1.0 / (Lj * (real)dc1[j] + (real)dc2[j]);
             if (lognormal) w = w * Lj;
             const real* lj = lam_i + j * F;
@@ -93,19 +97,25 @@ __device__ __forceinline__ void euler_step(real (&L)[NCAP], int n, int F, int fi
 #pragma unroll
             for (int k = 1; k < FCAP; k++) if (k < F) mu = mu + S[k] * lj[k];
             mu = mu + hv_i[j];
-            real Y = lognormal ? dev_log<real>(Lj) : Lj;
+            real Y = LOGSTATE ? L[j] : (lognormal ? dev_log<real>(Lj) : Lj);
             Y = Y + mu * dt;
 #pragma unroll
             for (int k = 0; k < FCAP; k++) if (k < F) Y = Y + dW[k] * lj[k];
-            real Ln = lognormal ? dev_exp<real>(Y) : Y;
-            if (capped) Ln = jmin(Ln, cap);
-            L[j] = Ln;
-            if (paths && active) paths[(size_t)row_next[j] * P + p] = Ln;
+            if (LOGSTATE) {
+                if (capped && Y > logcap) Y = logcap;
+                L[j] = Y;
+                if (paths && active) paths[(size_t)row_next[j] * P + p] = dev_exp<real>(Y);
+            } else {
+                real Ln = lognormal ? dev_exp<real>(Y) : Y;
+                if (capped) Ln = jmin(Ln, cap);
+                L[j] = Ln;
+                if (paths && active) paths[(size_t)row_next[j] * P + p] = Ln;
+            }
         }
     }
 }
 
-template <int NCAP, int FCAP, typename real, typename dwT>
+template <int NCAP, int FCAP, typename real, typename dwT, bool LOGSTATE = false>
 __global__ void __launch_bounds__(kStepThreads)
 lmm_fused_kernel(const StepArgs a,
                  const real* __restrict__ lam, const real* __restrict__ hv,
@@ -131,9 +141,13 @@ lmm_fused_kernel(const StepArgs a,
     const bool capped = !isinf(a.cap);
     const real cap = (real)a.cap;
 
+    const real logcap = (real)a.logcap;
+    // value of component j from the state (forward rate; exp of the log-state)
+    auto val = [&](real x) -> real { return LOGSTATE ? dev_exp<real>(x) : x; };
+
     real L[NCAP];
 #pragma unroll
-    for (int j = 0; j < NCAP; j++) L[j] = (j < n) ? (real)a.L0[j] : (real)0;
+    for (int j = 0; j < NCAP; j++) L[j] = (j < n) ? (real)(LOGSTATE ? a.Y0[j] : a.L0[j]) : (real)0;
 
     double N = 1.0, Nprev = 1.0;   // unadjusted numeraire N(T_kcur) and N(T_kcur-1)
     int kcur = 0;
@@ -144,11 +158,11 @@ lmm_fused_kernel(const StepArgs a,
         if (ckpt && active && (ti % a.ckpt_stride) == 0) {
             const size_t base = (size_t)(ti / a.ckpt_stride) * n;
 #pragma unroll
-            for (int j = 0; j < NCAP; j++) if (j < n) ckpt[(base + j) * P + p] = L[j];
+            for (int j = 0; j < NCAP; j++) if (j < n) ckpt[(base + j) * P + p] = val(L[j]);
         }
         // ---- numeraire N(T_{k+1}) = N(T_k).accrue(L_k(T_k), d_k) once L_k is final
         while (kcur < n && tenor_ti[kcur] <= ti) {
-            const double Lk = (double)pick<NCAP, real>(L, kcur);
+            const double Lk = (double)val(pick<NCAP, real>(L, kcur));
             const double dk = delta[kcur];
             double Nn;
             if (kcur == 0)      Nn = Lk * (dk * 1.0) + 1.0;      // Scalar.accrue of Scalar 1.0
@@ -172,7 +186,7 @@ lmm_fused_kernel(const StepArgs a,
                 for (int j = NCAP - 1; j >= 0; j--) {
                     if (j >= pr.first_j && j < pr.first_j + pr.num_periods) {
                         const KPeriod pe = periods[pr.param_off + (j - pr.first_j)];
-                        const real Lj = L[j];
+                        const real Lj = val(L[j]);
                         const real payoff = ((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pr.notional;
                         V = V + payoff;
                         V = V / ((real)1.0 + Lj * (real)pe.tau);
@@ -183,7 +197,7 @@ lmm_fused_kernel(const StepArgs a,
                 reduce_slot(x, active, cnt, lane, wave, wave_sum, wave_m2, stride, pr.out_slot);
             } else if (pr.type == FM_PRODUCT_CAPLET) {
                 const KPeriod pe = periods[pr.param_off];
-                const real Lj = pick<NCAP, real>(L, pr.first_j);
+                const real Lj = val(pick<NCAP, real>(L, pr.first_j));
                 real v = Lj - (real)pe.strike;
                 if (!pr.is_floorlet) v = jmax0(v) * (real)pe.notional;
                 else v = ((v != v) ? v : (v > (real)0 ? (real)0 : v)) * ((real)(-1.0) * (real)pe.notional);
@@ -191,7 +205,7 @@ lmm_fused_kernel(const StepArgs a,
                 reduce_slot(x, active, cnt, lane, wave, wave_sum, wave_m2, stride, pr.out_slot);
             } else {   // one period of a swap: contribution column, combined after the adjustment is known
                 const KPeriod pe = periods[pr.param_off];
-                const real Lj = pick<NCAP, real>(L, pr.first_j);
+                const real Lj = val(pick<NCAP, real>(L, pr.first_j));
                 double payoff = (double)(((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pe.notional);
                 if (!(pr.flags & 1)) payoff = payoff * -1.0;
                 if (active) contrib[(size_t)pr.out_slot * P + p] = payoff / Nsel;
@@ -204,9 +218,9 @@ lmm_fused_kernel(const StepArgs a,
         real dW[FCAP];
 #pragma unroll
         for (int k = 0; k < FCAP; k++) dW[k] = (k < F) ? (real)dw[((size_t)pc * nT + ti) * F + k] : (real)0;
-        euler_step<NCAP, FCAP, real>(L, n, F, first_alive[ti], (real)(times[ti + 1] - times[ti]), dW,
-                                     lam + (size_t)ti * n * F, hv + (size_t)ti * n, dc1, dc2, lognormal, capped, cap,
-                                     paths, row_of + (size_t)(ti + 1) * n, p, P, active);
+        euler_step<NCAP, FCAP, real, LOGSTATE>(L, n, F, first_alive[ti], (real)(times[ti + 1] - times[ti]), dW,
+                                               lam + (size_t)ti * n * F, hv + (size_t)ti * n, dc1, dc2, lognormal,
+                                               capped, cap, paths, row_of + (size_t)(ti + 1) * n, p, P, active, logcap);
     }
 }
 

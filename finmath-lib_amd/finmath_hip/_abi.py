@@ -28,7 +28,7 @@ FM_ERR_UNSUPPORTED = 7
 MEASURE = {"SPOT": 0, "TERMINAL": 1}
 STATE_SPACE = {"NORMAL": 0, "LOGNORMAL": 1}
 SCHEME = {"EULER": 0, "PREDICTOR_CORRECTOR": 1, "EULER_FUNCTIONAL": 2, "PREDICTOR_CORRECTOR_FUNCTIONAL": 3}
-PRECISION = {"FP64": 0, "FP32_MIXED": 1}
+PRECISION = {"FP64": 0, "FP32_MIXED": 1, "FP64_LOGSTATE": 2}
 PRODUCT_SWAPTION, PRODUCT_SWAP, PRODUCT_CAPLET = 1, 2, 3
 
 # Every symbol include/finmath_lmm.h declares.

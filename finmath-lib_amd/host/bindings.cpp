@@ -20,6 +20,7 @@ py::array_t<double> to_array(const std::vector<double>& v) {
 int precision_of(const std::string& s) {
     if (s == "FP64") return FM_PRECISION_FP64;
     if (s == "FP32_MIXED") return FM_PRECISION_FP32_MIXED;
+    if (s == "FP64_LOGSTATE") return FM_PRECISION_FP64_LOGSTATE;
     throw IllegalArgumentException("unknown precision " + s);
 }
 

@@ -60,6 +60,11 @@ extern "C" {
  * and payoffs in float; numeraire, payoff/numeraire and all sums in double. */
 #define FM_PRECISION_FP64        0
 #define FM_PRECISION_FP32_MIXED  1
+/* FP64 arithmetic that carries the log-forwards between steps (one exp per alive
+ * forward-step instead of the reference's log(exp(.)) round trip) and lets the
+ * compiler fuse multiply-adds: the same values as FP64 up to a few units in the last
+ * place per step. No adjoint. */
+#define FM_PRECISION_FP64_LOGSTATE 2
 
 typedef struct fm_lmm fm_lmm;   /* one simulation: model + Brownian driver + Euler process */
 

@@ -75,12 +75,13 @@ def test_brownian_increments_match_oracle():
     assert exact / total > 0.999
 
 
+@pytest.mark.parametrize("precision", ["FP64", "FP64_LOGSTATE"])
 @pytest.mark.parametrize("name,P", [("c0", 10_000), ("c1", 4_000)])
-def test_process_values_match_oracle(name, P):
+def test_process_values_match_oracle(name, P, precision):
     wl = W.ALL[name]()
     t = O.ModelTables(wl)
-    say(f"case process {name} P={P}")
-    ctx = make_context(wl, t, P, store_paths=True)
+    say(f"case process {name} P={P} {precision}")
+    ctx = make_context(wl, t, P, store_paths=True, precision=precision)
     ctx.simulate()
     om = O.OracleModel(t, P)
     worst = 0.0
@@ -96,12 +97,13 @@ def test_process_values_match_oracle(name, P):
         assert np.max(np.abs(g - r) / np.abs(r)) < 1e-12
 
 
+@pytest.mark.parametrize("precision", ["FP64", "FP64_LOGSTATE"])
 @pytest.mark.parametrize("name,P", [("c0", 10_000), ("c1", 20_000)])
-def test_prices_match_oracle_fp64(name, P):
+def test_prices_match_oracle_fp64(name, P, precision):
     wl = W.ALL[name]()
     t = O.ModelTables(wl)
-    say(f"case prices {name} P={P}")
-    ctx = make_context(wl, t, P)
+    say(f"case prices {name} P={P} {precision}")
+    ctx = make_context(wl, t, P, precision=precision)
     vals, errs = ctx.value_products(to_products(wl.products))
     om = O.OracleModel(t, P)
     for q, s in enumerate(wl.products):
@@ -146,3 +148,38 @@ def test_ragged_path_counts():
         for q, s in enumerate(wl.products):
             rv, _ = om.value(s)
             assert abs(vals[q] - rv) <= 1e-10 * max(abs(rv), 1e-12)
+
+
+def test_fp32_mixed_prices_within_a_tenth_of_a_basis_point():
+    wl = W.c1()
+    t = O.ModelTables(wl)
+    P = 20_000
+    say(f"case fp32 mixed vs oracle c1 P={P}")
+    ctx = make_context(wl, t, P, precision="FP32_MIXED")
+    vals, errs = ctx.value_products(to_products(wl.products))
+    om = O.OracleModel(t, P)
+    worst = 0.0
+    for q, s in enumerate(wl.products):
+        rv, _ = om.value(s)
+        worst = max(worst, abs(vals[q] - rv) * 1e4)
+    say(f"worst FP32-mixed price error {worst:.4f} bp")
+    assert worst < 0.1
+
+
+def test_materialised_process_agrees_with_fused_prices():
+    # the swaption recomputed from getProcessValue/getNumeraire equals the fused pass
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    P = 5000
+    say(f"case materialised vs fused c0 P={P}")
+    ctx = make_context(wl, t, P, store_paths=True)
+    ctx.simulate()
+    s = wl.products[0]
+    e = int(round(s["exercise"] / 0.5))
+    V = np.zeros(P)
+    for j in reversed(range(e, e + len(s["fixing"]))):
+        L = ctx.process_value(e, j)
+        V = (V + (L - s["strike"][0]) * 0.5 * 1.0) / (1.0 + L * 0.5) * 1.0
+    x = np.maximum(V, 0.0) / ctx.numeraire(s["exercise"])
+    fused, _ = ctx.value_products(to_products([s]))
+    assert abs(np.mean(x) - fused[0]) < 1e-12 * abs(fused[0])
