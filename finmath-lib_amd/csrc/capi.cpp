@@ -95,7 +95,7 @@ struct fm_lmm {
     bool simulated = false;
     std::vector<double> zb;             // numeraire-adjustment averages of the last simulate
 
-    DevBuf d_times, d_first, d_tenor_ti, d_lam, d_hv, d_dc1, d_dc2, d_delta, d_L0, d_Y0, d_sqrt_dt;
+    DevBuf d_times, d_first, d_tenor_ti, d_rec, d_delta, d_L0, d_Y0, d_sqrt_dt;
     DevBuf d_states, d_poly, d_dw, d_row_of, d_paths, d_num;
     DevBuf d_products, d_periods, d_wave_sum, d_wave_m2, d_contrib, d_out, d_swapmeta, d_scale;
     DevBuf d_ckpt, d_adj_num, d_kcur, d_fsig, d_sig, d_cdiag, d_wave_grad, d_grad_out;
@@ -200,31 +200,37 @@ void prepare_model(fm_lmm* c) {
             else c->row_of[(size_t)ti * n + j] = c->row_of[(size_t)(ti - 1) * n + j];
         }
     c->num_rows = r;
-    std::vector<double> dc1((size_t)n), dc2((size_t)n), delta((size_t)n), sq((size_t)nT), hv((size_t)nT * n);
-    for (int j = 0; j < n; j++) {
-        const double d = c->tenor[(size_t)j + 1] - c->tenor[(size_t)j];
-        delta[(size_t)j] = d; dc1[(size_t)j] = d / d; dc2[(size_t)j] = 1.0 / d;
-    }
+    std::vector<double> delta((size_t)n), sq((size_t)nT);
+    for (int j = 0; j < n; j++) delta[(size_t)j] = c->tenor[(size_t)j + 1] - c->tenor[(size_t)j];
     for (int i = 0; i < nT; i++) sq[(size_t)i] = std::sqrt(c->times[(size_t)i + 1] - c->times[(size_t)i]);
     const bool lognormal = c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL;
-    for (size_t q = 0; q < hv.size(); q++) hv[q] = lognormal ? c->var[q] * (-0.5) : 0.0;
+    // records per (step i, component j): lambda_j.(t_i) zero-padded to the factor capacity,
+    // variance*(-0.5) (0 in the normal state space), d/d and 1/d
+    const int fcap = fcap_of(F), rs = rec_size_of(fcap);
+    std::vector<double> rec((size_t)nT * n * rs, 0.0);
+    for (int i = 0; i < nT; i++)
+        for (int j = 0; j < n; j++) {
+            double* r = rec.data() + ((size_t)i * n + j) * rs;
+            for (int k = 0; k < F; k++) r[k] = c->lam[((size_t)i * n + j) * F + k];
+            r[fcap] = lognormal ? c->var[(size_t)i * n + j] * (-0.5) : 0.0;
+            const double d = delta[(size_t)j];
+            r[fcap + 1] = d / d;
+            r[fcap + 2] = 1.0 / d;
+        }
     upload(c->d_times, c->times, s);
     upload(c->d_first, c->first_alive, s);
     upload(c->d_tenor_ti, c->tenor_ti, s);
-    upload(c->d_dc1, dc1, s);
-    upload(c->d_dc2, dc2, s);
     upload(c->d_delta, delta, s);
     upload(c->d_sqrt_dt, sq, s);
     upload(c->d_row_of, c->row_of, s);
     if (c->fp32()) {
-        std::vector<float> lf(c->lam.begin(), c->lam.end()), hf(hv.begin(), hv.end()), l0(c->init_value.begin(), c->init_value.end());
-        upload(c->d_lam, lf, s);
-        upload(c->d_hv, hf, s);
-        std::vector<double> l0d(l0.begin(), l0.end());   // float-rounded initial state, passed as double
+        std::vector<float> rf(rec.begin(), rec.end());
+        upload(c->d_rec, rf, s);
+        std::vector<double> l0d((size_t)n);                 // float-rounded initial state, passed as double
+        for (int j = 0; j < n; j++) l0d[(size_t)j] = (double)(float)c->init_value[(size_t)j];
         upload(c->d_L0, l0d, s);
     } else {
-        upload(c->d_lam, c->lam, s);
-        upload(c->d_hv, hv, s);
+        upload(c->d_rec, rec, s);
         upload(c->d_L0, c->init_value, s);
     }
     // initial log-state log(max(L_j(0), 0)), kept below log(cap)
@@ -236,7 +242,6 @@ void prepare_model(fm_lmm* c) {
     }
     upload(c->d_Y0, y0, s);
     HIP_CHECK(hipStreamSynchronize(s));
-    (void)F;
     c->tables_dirty = false;
 }
 
@@ -451,8 +456,10 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     a.times = c->d_times.as<double>();
     a.first_alive = c->d_first.as<int32_t>();
     a.tenor_time_index = c->d_tenor_ti.as<int32_t>();
-    a.lam = c->d_lam.p; a.hv = c->d_hv.p;
-    a.dc1 = c->d_dc1.as<double>(); a.dc2 = c->d_dc2.as<double>(); a.delta = c->d_delta.as<double>();
+    a.rec = c->d_rec.p;
+    a.lognormal = c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL ? 1 : 0;
+    a.capped = std::isinf(c->cfg.libor_cap) ? 0 : 1;
+    a.delta = c->d_delta.as<double>();
     a.L0 = c->d_L0.as<double>();
     a.Y0 = c->d_Y0.as<double>();
     a.logcap = std::isinf(c->cfg.libor_cap) ? HUGE_VAL : std::log(c->cfg.libor_cap);
@@ -785,8 +792,10 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     a.precision = c->cfg.precision; a.state_space = c->cfg.state_space;
     a.times = c->d_times.as<double>(); a.first_alive = c->d_first.as<int32_t>();
     a.tenor_time_index = c->d_tenor_ti.as<int32_t>();
-    a.lam = c->d_lam.p; a.hv = c->d_hv.p;
-    a.dc1 = c->d_dc1.as<double>(); a.dc2 = c->d_dc2.as<double>(); a.delta = c->d_delta.as<double>();
+    a.rec = c->d_rec.p;
+    a.lognormal = c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL ? 1 : 0;
+    a.capped = std::isinf(c->cfg.libor_cap) ? 0 : 1;
+    a.delta = c->d_delta.as<double>();
     a.L0 = c->d_L0.as<double>(); a.dw = c->d_dw.p; a.cap = c->cfg.libor_cap; a.scheme = c->cfg.scheme;
     a.num_products = (int)kp.size();
     a.products = upload(c->d_products, kp, s);
@@ -796,7 +805,10 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     ga.stride = stride;
     ga.kcur_at = upload(c->d_kcur, kcur, s);
     ga.weight = upload(c->d_scale, kweights, s);
-    ga.fsig = upload(c->d_fsig, c->vf_fac, s);
+    std::vector<double> fpad((size_t)n * fcap_of(c->F()), 0.0);
+    for (int j = 0; j < n; j++)
+        for (int k = 0; k < c->F(); k++) fpad[(size_t)j * fcap_of(c->F()) + k] = c->vf_fac[(size_t)j * c->F() + k];
+    ga.fsig = upload(c->d_fsig, fpad, s);
     ga.sig = upload(c->d_sig, c->vf_sig, s);
     ga.cdiag = upload(c->d_cdiag, c->vf_c, s);
     const int64_t num_waves = ((int64_t)P + 63) / 64;

@@ -52,6 +52,10 @@ void launch_words(const StreamLayout& lay, const uint32_t* d_states, uint64_t co
                   uint32_t* out, void* stream);
 
 // ---- path simulation (lmm_step.hip) ---------------------------------------------
+// Factor capacity of the compiled path kernels and the record size per (step, component).
+constexpr int fcap_of(int F) { return F <= 1 ? 1 : F <= 3 ? 3 : F <= 5 ? 5 : 10; }
+constexpr int rec_size_of(int fcap) { return ((fcap + 3 + 3) / 4) * 4; }
+
 // Product description lowered for the kernel.
 struct KProduct {
     int32_t type;         // FM_PRODUCT_*
@@ -83,10 +87,8 @@ struct StepArgs {
     const double* times;         // device nT+1
     const int32_t* first_alive;  // device nT
     const int32_t* tenor_time_index; // device n+1: process time index of T_k (rounded up)
-    const void* lam;             // device [nT][n][F] (double or float)
-    const void* hv;              // device [nT][n]   variance * (-0.5)
-    const double* dc1;           // device n: delta/delta
-    const double* dc2;           // device n: 1/delta
+    const void* rec;             // device [nT][n][rec_size_of(fcap_of(F))] records (double or float)
+    int lognormal, capped;       // state space is LOGNORMAL; the liborCap applies
     const double* delta;         // device n: T_{j+1}-T_j
     const double* L0;            // device n: initial forwards in value space
     const double* Y0;            // device n: initial log-forwards (log-state kernels)
@@ -119,7 +121,7 @@ struct AdjArgs {
     int stride;                      // checkpoint stride
     const int32_t* kcur_at;          // [nT+1] numeraire index after the advance at each time index
     const double* weight;            // [num kernel products] objective weight of each lowered product
-    const double* fsig;              // [n][F] factor matrix f_jk (double)
+    const double* fsig;              // [n][fcap_of(F)] factor matrix f_jk, zero padded (double)
     const double* sig;               // [nT][n] volatility sigma_j(t_i)
     const double* cdiag;             // [n] variance factor c_j (correlation diagonal)
     double* wave_grad;               // [num_waves][nT*n + n] partials of dV/dsigma and dV/dL(0)

@@ -23,9 +23,7 @@ namespace fm {
 
 template <int NCAP, int FCAP, typename real>
 __global__ void __launch_bounds__(kStepThreads)
-lmm_adjoint_kernel(const StepArgs a, const AdjArgs g,
-                   const real* __restrict__ lam, const real* __restrict__ hv,
-                   const double* __restrict__ dc1, const double* __restrict__ dc2,
+lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ rec,
                    const double* __restrict__ delta, const double* __restrict__ times,
                    const int32_t* __restrict__ first_alive, const int32_t* __restrict__ kcur_at,
                    const KProduct* __restrict__ products, const KPeriod* __restrict__ periods,
@@ -42,8 +40,8 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g,
     const int64_t pc = active ? p : (int64_t)P - 1;
     const int cnt = (int)((int64_t)P - wave * 64 > 64 ? 64 : ((int64_t)P - wave * 64));
     const int gstride = nT * n + n;
-    const bool lognormal = a.state_space == FM_STATE_SPACE_LOGNORMAL;
-    const bool capped = !isinf(a.cap);
+    const bool lognormal = a.lognormal != 0;
+    const bool capped = a.capped != 0;
     const real cap = (real)a.cap;
     const int stride = g.stride;
 
@@ -65,43 +63,38 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g,
             for (int j = 0; j < NCAP; j++) L[j] = Lck[j];
             for (int s = c; s < t; s++) {
                 real dWs[FCAP];
-#pragma unroll
-                for (int k = 0; k < FCAP; k++) dWs[k] = (k < F) ? dw[((size_t)pc * nT + s) * F + k] : (real)0;
-                euler_step<NCAP, FCAP, real>(L, n, F, first_alive[s], (real)(times[s + 1] - times[s]), dWs,
-                                             lam + (size_t)s * n * F, hv + (size_t)s * n, dc1, dc2, lognormal, capped,
-                                             cap, (real*)nullptr, (const int32_t*)nullptr, p, P, active);
+                load_increments<FCAP, real>(dWs, dw, pc, nT, s, F);
+                euler_step<NCAP, FCAP, real>(L, n, s, first_alive[s], (real)(times[s + 1] - times[s]), dWs, rec,
+                                             lognormal, capped, cap, (real)0, (real*)nullptr,
+                                             (const int32_t*)nullptr, p, P, active);
             }
 
             // ---- 1. reverse the Euler step t -> t+1
             if (t < nT) {
                 const int first = first_alive[t];
                 const real dt = (real)(times[t + 1] - times[t]);
-                const real* lam_t = lam + (size_t)t * n * F;
-                const real* hv_t = hv + (size_t)t * n;
                 real dW[FCAP], S[FCAP], Sb[FCAP];
+                load_increments<FCAP, real>(dW, dw, pc, nT, t, F);
 #pragma unroll
-                for (int k = 0; k < FCAP; k++) {
-                    dW[k] = (k < F) ? dw[((size_t)pc * nT + t) * F + k] : (real)0;
-                    S[k] = (real)0; Sb[k] = (real)0;
-                }
+                for (int k = 0; k < FCAP; k++) { S[k] = (real)0; Sb[k] = (real)0; }
                 // pass A (ascending): recompute the step, adjoint through exp/log/cap
 #pragma unroll
                 for (int j = 0; j < NCAP; j++) {
                     if (j < n && j >= first) {
+                        const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, t, j);
                         const real Lj = L[j];
-                        real w = (real)1.0 / (Lj * (real)dc1[j] + (real)dc2[j]);
+                        real w = (real)1.0 / (Lj * R.c1() + R.c2());
                         if (lognormal) w = w * Lj;
-                        const real* lj = lam_t + j * F;
 #pragma unroll
-                        for (int k = 0; k < FCAP; k++) if (k < F) S[k] = S[k] + w * lj[k];
-                        real mu = S[0] * lj[0];
+                        for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * R.lam(k);
+                        real mu = S[0] * R.lam(0);
 #pragma unroll
-                        for (int k = 1; k < FCAP; k++) if (k < F) mu = mu + S[k] * lj[k];
-                        mu = mu + hv_t[j];
+                        for (int k = 1; k < FCAP; k++) mu = mu + S[k] * R.lam(k);
+                        mu = mu + R.hv();
                         real Y = lognormal ? dev_log<real>(Lj) : Lj;
                         Y = Y + mu * dt;
 #pragma unroll
-                        for (int k = 0; k < FCAP; k++) if (k < F) Y = Y + dW[k] * lj[k];
+                        for (int k = 0; k < FCAP; k++) Y = Y + dW[k] * R.lam(k);
                         const real E = lognormal ? dev_exp<real>(Y) : Y;
                         real Yb = (capped && E > cap) ? (real)0 : Lb[j] * (lognormal ? E : (real)1);
                         mub[j] = Yb * dt;
@@ -112,26 +105,24 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g,
 #pragma unroll
                 for (int j = NCAP - 1; j >= 0; j--) {
                     if (j < n && j >= first) {
+                        const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, t, j);
                         const real Lj = L[j];
-                        const real r = (real)1.0 / (Lj * (real)dc1[j] + (real)dc2[j]);
+                        const real r = (real)1.0 / (Lj * R.c1() + R.c2());
                         const real w = lognormal ? r * Lj : r;
-                        const real* lj = lam_t + j * F;
                         const real mb = mub[j];
                         const real Yb = mb / dt;
                         real wb = (real)0;
                         double sgb = 0.0;               // dV/dsigma_j(t) on this path
 #pragma unroll
                         for (int k = 0; k < FCAP; k++) {
-                            if (k < F) {
-                                Sb[k] = Sb[k] + mb * lj[k];
-                                wb = wb + Sb[k] * lj[k];
-                                const real lamb = Yb * dW[k] + mb * S[k] + Sb[k] * w;
-                                sgb += (double)lamb * fsig[j * F + k];
-                                S[k] = S[k] - w * lj[k];    // S^(j) -> S^(j-1)
-                            }
+                            Sb[k] = Sb[k] + mb * R.lam(k);
+                            wb = wb + Sb[k] * R.lam(k);
+                            const real lamb = Yb * dW[k] + mb * S[k] + Sb[k] * w;
+                            sgb += (double)lamb * fsig[j * FCAP + k];
+                            S[k] = S[k] - w * R.lam(k);    // S^(j) -> S^(j-1)
                         }
                         if (lognormal) sgb += (double)(mb * (real)(-0.5)) * (2.0 * sig[(size_t)t * n + j] * cdiag[j]);
-                        const real dwdL = lognormal ? (real)dc2[j] * r * r : -(real)dc1[j] * r * r;
+                        const real dwdL = lognormal ? R.c2() * r * r : -R.c1() * r * r;
                         Lb[j] = Lb[j] + wb * dwdL;
                         const double s = wave_reduce_sum(active ? sgb : 0.0);
                         if (lane == 0 && cnt > 0) wave_grad[(size_t)wave * gstride + (size_t)t * n + j] = s;
@@ -240,7 +231,7 @@ template <int NCAP, int FCAP, typename real>
 void launch_adjoint_cap(const StepArgs& a, const AdjArgs& g, hipStream_t stream) {
     const int64_t blocks = ((int64_t)a.P + kStepThreads - 1) / kStepThreads;
     hipLaunchKernelGGL((lmm_adjoint_kernel<NCAP, FCAP, real>), dim3((unsigned)blocks), dim3(kStepThreads), 0, stream,
-                       a, g, (const real*)a.lam, (const real*)a.hv, a.dc1, a.dc2, a.delta, a.times, a.first_alive,
+                       a, g, (const real*)a.rec, a.delta, a.times, a.first_alive,
                        g.kcur_at, a.products, a.periods, (const real*)a.dw, (const real*)a.ckpt, a.numeraire,
                        g.weight, g.fsig, g.sig, g.cdiag, g.wave_grad);
     HIP_CHECK(hipGetLastError());

@@ -1,5 +1,5 @@
 // Fused LIBOR market model path kernel: one thread per path, the forward vector
-// in registers, the covariance tables read through the scalar cache.
+// in registers, the per-(step, component) model data read through the scalar cache.
 //
 // Per time step t_i -> t_{i+1} and alive component j (T_j > t_i), in the order of
 // LIBORMarketModelFromCovarianceModel.getDrift
@@ -14,6 +14,11 @@
 // Products are evaluated on the fly at their exercise/fixing time index and reduced
 // per wavefront (sum and centred second moment); the numeraire is accumulated per
 // path in double precision (SPOT measure, LMM:1050-1066).
+//
+// Model data layout: one record of rec_size_of(FCAP) reals per (time step,
+// component): the factor loadings padded with zeros to FCAP, then variance*(-0.5),
+// d/d and 1/d. A zero loading adds exact zeros, so the padding changes no value;
+// a record is one aligned scalar load.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -69,38 +74,53 @@ __device__ __forceinline__ void reduce_slot(double x, bool active, int cnt, int 
     }
 }
 
+// Record of one (time step, component).
+template <int FCAP, typename real>
+struct Rec {
+    static constexpr int size = rec_size_of(FCAP);
+    const real* __restrict__ r;
+    __device__ __forceinline__ real lam(int k) const { return r[k]; }
+    __device__ __forceinline__ real hv() const { return r[FCAP]; }
+    __device__ __forceinline__ real c1() const { return r[FCAP + 1]; }
+    __device__ __forceinline__ real c2() const { return r[FCAP + 2]; }
+};
+
+template <int FCAP, typename real>
+__device__ __forceinline__ Rec<FCAP, real> rec_at(const real* __restrict__ rec, int n, int i, int j) {
+    const real* r = rec + ((size_t)i * n + j) * Rec<FCAP, real>::size;
+    return Rec<FCAP, real>{(const real*)__builtin_assume_aligned(r, Rec<FCAP, real>::size * sizeof(real))};
+}
+
 // One Euler step t_i -> t_{i+1} of all alive components, in place. Shared by the
 // pricing pass and the adjoint recompute so both produce identical paths.
 // LOGSTATE: the array holds the log-forwards Y_j and the step needs one exp per
 // component (the reference re-takes log(exp(Y)) every step, EULER_FUNCTIONAL;
 // carrying Y differs from it by rounding only, the cap applied as Y <= log(cap)).
 template <int NCAP, int FCAP, typename real, bool LOGSTATE = false>
-__device__ __forceinline__ void euler_step(real (&L)[NCAP], int n, int F, int first, real dt, const real (&dW)[FCAP],
-                                           const real* __restrict__ lam_i, const real* __restrict__ hv_i,
-                                           const double* __restrict__ dc1, const double* __restrict__ dc2,
-                                           bool lognormal, bool capped, real cap, real* __restrict__ paths,
-                                           const int32_t* __restrict__ row_next, int64_t p, int P, bool active,
-                                           real logcap = (real)0) {
+__device__ __forceinline__ void euler_step(real (&L)[NCAP], int n, int i, int first, real dt, const real (&dW)[FCAP],
+                                           const real* __restrict__ rec, bool lognormal, bool capped, real cap,
+                                           real logcap, real* __restrict__ paths,
+                                           const int32_t* __restrict__ row_next, int64_t p, int P, bool active) {
     real S[FCAP];
 #pragma unroll
     for (int k = 0; k < FCAP; k++) S[k] = (real)0;
 #pragma unroll
     for (int j = 0; j < NCAP; j++) {
         if (j < n && j >= first) {
+            const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, i, j);
             const real Lj = LOGSTATE ? dev_exp<real>(L[j]) : L[j];
-            real w = (real)1.0 / (Lj * (real)dc1[j] + (real)dc2[j]);
+            real w = (real)1.0 / (Lj * R.c1() + R.c2());
             if (lognormal) w = w * Lj;
-            const real* lj = lam_i + j * F;
 #pragma unroll
-            for (int k = 0; k < FCAP; k++) if (k < F) S[k] = S[k] + w * lj[k];
-            real mu = S[0] * lj[0];
+            for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * R.lam(k);
+            real mu = S[0] * R.lam(0);
 #pragma unroll
-            for (int k = 1; k < FCAP; k++) if (k < F) mu = mu + S[k] * lj[k];
-            mu = mu + hv_i[j];
+            for (int k = 1; k < FCAP; k++) mu = mu + S[k] * R.lam(k);
+            mu = mu + R.hv();
             real Y = LOGSTATE ? L[j] : (lognormal ? dev_log<real>(Lj) : Lj);
             Y = Y + mu * dt;
 #pragma unroll
-            for (int k = 0; k < FCAP; k++) if (k < F) Y = Y + dW[k] * lj[k];
+            for (int k = 0; k < FCAP; k++) Y = Y + dW[k] * R.lam(k);
             if (LOGSTATE) {
                 if (capped && Y > logcap) Y = logcap;
                 L[j] = Y;
@@ -115,15 +135,19 @@ __device__ __forceinline__ void euler_step(real (&L)[NCAP], int n, int F, int fi
     }
 }
 
-template <int NCAP, int FCAP, typename real, typename dwT, bool LOGSTATE = false>
+template <int FCAP, typename real>
+__device__ __forceinline__ void load_increments(real (&dW)[FCAP], const real* __restrict__ dw, int64_t pc, int nT,
+                                                int i, int F) {
+#pragma unroll
+    for (int k = 0; k < FCAP; k++) dW[k] = (k < F) ? dw[((size_t)pc * nT + i) * F + k] : (real)0;
+}
+
+template <int NCAP, int FCAP, typename real, bool LOGSTATE = false>
 __global__ void __launch_bounds__(kStepThreads)
-lmm_fused_kernel(const StepArgs a,
-                 const real* __restrict__ lam, const real* __restrict__ hv,
-                 const double* __restrict__ dc1, const double* __restrict__ dc2,
-                 const double* __restrict__ delta, const double* __restrict__ times,
-                 const int32_t* __restrict__ first_alive, const int32_t* __restrict__ tenor_ti,
-                 const KProduct* __restrict__ products, const KPeriod* __restrict__ periods,
-                 const dwT* __restrict__ dw,
+lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* __restrict__ delta,
+                 const double* __restrict__ times, const int32_t* __restrict__ first_alive,
+                 const int32_t* __restrict__ tenor_ti, const KProduct* __restrict__ products,
+                 const KPeriod* __restrict__ periods, const real* __restrict__ dw,
                  double* __restrict__ wave_sum, double* __restrict__ wave_m2,
                  double* __restrict__ contrib, real* __restrict__ paths,
                  const int32_t* __restrict__ row_of, double* __restrict__ numeraire_out,
@@ -137,13 +161,12 @@ lmm_fused_kernel(const StepArgs a,
     const int64_t pc = active ? p : (int64_t)P - 1;
     const int cnt = (int)((int64_t)P - wave * 64 > 64 ? 64 : ((int64_t)P - wave * 64));
     const int stride = a.num_slots + n + 1;
-    const bool lognormal = a.state_space == FM_STATE_SPACE_LOGNORMAL;
-    const bool capped = !isinf(a.cap);
+    const bool lognormal = a.lognormal != 0;
+    const bool capped = a.capped != 0;
     const real cap = (real)a.cap;
-
     const real logcap = (real)a.logcap;
     // value of component j from the state (forward rate; exp of the log-state)
-    auto val = [&](real x) -> real { return LOGSTATE ? dev_exp<real>(x) : x; };
+    auto val = [](real x) -> real { return LOGSTATE ? dev_exp<real>(x) : x; };
 
     real L[NCAP];
 #pragma unroll
@@ -216,11 +239,10 @@ lmm_fused_kernel(const StepArgs a,
 
         // ---- Euler step ti -> ti+1
         real dW[FCAP];
-#pragma unroll
-        for (int k = 0; k < FCAP; k++) dW[k] = (k < F) ? (real)dw[((size_t)pc * nT + ti) * F + k] : (real)0;
-        euler_step<NCAP, FCAP, real, LOGSTATE>(L, n, F, first_alive[ti], (real)(times[ti + 1] - times[ti]), dW,
-                                               lam + (size_t)ti * n * F, hv + (size_t)ti * n, dc1, dc2, lognormal,
-                                               capped, cap, paths, row_of + (size_t)(ti + 1) * n, p, P, active, logcap);
+        load_increments<FCAP, real>(dW, dw, pc, nT, ti, F);
+        euler_step<NCAP, FCAP, real, LOGSTATE>(L, n, ti, first_alive[ti], (real)(times[ti + 1] - times[ti]), dW, rec,
+                                               lognormal, capped, cap, logcap, paths,
+                                               row_of + (size_t)(ti + 1) * n, p, P, active);
     }
 }
 

@@ -11,10 +11,10 @@ namespace fm {
 template <int NCAP, int FCAP, typename real, bool LOGSTATE>
 void launch_fused_cap(const StepArgs& a, hipStream_t stream) {
     const int64_t blocks = ((int64_t)a.P + kStepThreads - 1) / kStepThreads;
-    hipLaunchKernelGGL((lmm_fused_kernel<NCAP, FCAP, real, real, LOGSTATE>), dim3((unsigned)blocks), dim3(kStepThreads), 0, stream,
-                       a, (const real*)a.lam, (const real*)a.hv, a.dc1, a.dc2, a.delta, a.times,
-                       a.first_alive, a.tenor_time_index, a.products, a.periods, (const real*)a.dw,
-                       a.wave_sum, a.wave_m2, a.contrib, (real*)a.paths, a.row_of, a.numeraire, (real*)a.ckpt);
+    hipLaunchKernelGGL((lmm_fused_kernel<NCAP, FCAP, real, LOGSTATE>), dim3((unsigned)blocks), dim3(kStepThreads), 0,
+                       stream, a, (const real*)a.rec, a.delta, a.times, a.first_alive, a.tenor_time_index, a.products,
+                       a.periods, (const real*)a.dw, a.wave_sum, a.wave_m2, a.contrib, (real*)a.paths, a.row_of,
+                       a.numeraire, (real*)a.ckpt);
     HIP_CHECK(hipGetLastError());
 }
 
