@@ -39,6 +39,7 @@ SOURCES = {
     "lmm_step_fp32.hip": [],
     "lmm_step_fp64_logstate.hip": [],
     "lmm_reduce.hip": NO_CONTRACT,
+    "lmm_bermudan.hip": NO_CONTRACT,
     "capi.cpp": NO_CONTRACT,
 }
 

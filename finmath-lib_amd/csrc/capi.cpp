@@ -99,6 +99,7 @@ struct fm_lmm {
     DevBuf d_states, d_poly, d_dw, d_row_of, d_paths, d_num;
     DevBuf d_products, d_periods, d_wave_sum, d_wave_m2, d_contrib, d_out, d_swapmeta, d_scale;
     DevBuf d_ckpt, d_adj_num, d_kcur, d_fsig, d_sig, d_cdiag, d_wave_grad, d_grad_out;
+    DevBuf d_berm_vals, d_berm_under, d_berm_ex, d_berm_part, d_berm_coef, d_berm_col, d_berm_scale;
 
     int n() const { return cfg.num_libors; }
     int nT() const { return cfg.num_time_steps; }
@@ -267,8 +268,19 @@ void ensure_brownian(fm_lmm* c) {
     c->dw_ready = true;
 }
 
+// Backward induction plan of one Bermudan swaption.
+struct BermPlan {
+    int slot = -1;
+    int callable = 1;
+    int basis = FM_BASIS_REFERENCE;
+    int num_basis = 6;
+    std::vector<int> per_col, per_k;        // per period: payoff column and its numeraire index
+    std::vector<int> ex_period, ex_col, ex_k;   // per exercise date (ascending): period, first basis column, N index
+};
+
 // Products lowered for the kernel.
 struct Lowered {
+    std::vector<BermPlan> berm;
     std::vector<KProduct> kp;
     std::vector<KPeriod> per;
     int num_slots = 0;
@@ -400,6 +412,85 @@ Lowered lower_products(fm_lmm* c, const fm_product* prods, int count) {
                 L.contrib_k.push_back(k.num_k);
                 L.kp.push_back(k);
             }
+        } else if (p.type == FM_PRODUCT_BERMUDAN_SWAPTION) {
+            // BermudanSwaption.getValues (BermudanSwaption.java:91-157): periods become payoff
+            // columns (payoff / N(T_pay)), exercise dates become raw basis columns.
+            require(p.fixing_dates && p.payment_dates && p.period_lengths && p.swaprates && p.exercise_flags,
+                    FM_ERR_INVALID_ARGUMENT, "Bermudan swaption arrays missing");
+            require(p.basis == FM_BASIS_REFERENCE || p.basis == FM_BASIS_SWAP_RATE, FM_ERR_INVALID_ARGUMENT,
+                    "unknown regression basis " + std::to_string(p.basis));
+            const int m = p.num_periods;
+            BermPlan b;
+            b.slot = L.num_slots++;
+            L.slot_of[(size_t)q] = b.slot;
+            L.slot_k.push_back(-1);
+            b.callable = p.is_cancelable ? 0 : 1;
+            b.basis = p.basis;
+            b.num_basis = p.basis == FM_BASIS_REFERENCE ? 6 : 4;
+            const int j0 = c->libor_index(p.fixing_dates[0]);
+            for (int i = 0; i < m; i++) {
+                const double fd = p.fixing_dates[i], pd = p.payment_dates[i], pl = p.period_lengths[i];
+                const int j = c->libor_index(fd), jp = c->libor_index(pd);
+                require(j >= 0 && j == j0 + i && jp == j + 1 && c->libor_index(fd + pl) == j + 1, FM_ERR_UNSUPPORTED,
+                        "Bermudan periods must be consecutive single tenor periods");
+                KProduct k{};
+                k.type = FM_PRODUCT_SWAP;
+                k.eval_index = forward_time_index(c, fd);
+                k.first_j = j; k.num_periods = 1;
+                k.flags = 1;
+                KPeriod pe{};
+                pe.strike = p.swaprates[i]; pe.period_length = pl; pe.tau = 0; pe.adj = 1.0;
+                pe.notional = p.notionals ? p.notionals[i] : 1.0;
+                pe.j = j; pe.pay_k = jp;
+                L.per.push_back(pe);
+                k.param_off = (int32_t)L.per.size() - 1;
+                k.num_k = numeraire_index(pd);
+                check_available(k.eval_index, k.num_k);
+                k.out_slot = L.num_contrib++;
+                L.contrib_k.push_back(k.num_k);
+                L.kp.push_back(k);
+                b.per_col.push_back(k.out_slot);
+                b.per_k.push_back(k.num_k);
+            }
+            for (int i = 0; i < m; i++) {
+                if (!p.exercise_flags[i]) continue;
+                const double fd = p.fixing_dates[i];
+                KProduct k{};
+                k.type = K_BERM_BASIS;
+                k.eval_index = forward_time_index(c, fd);
+                k.first_j = j0 + i;
+                k.num_periods = m - i;
+                k.flags = p.basis;
+                k.param_off = (int32_t)L.per.size();
+                if (p.basis == FM_BASIS_REFERENCE) {
+                    // the exercise date is the fixing date of period i (getBasisFunctions :227-233)
+                    KPeriod pe{};
+                    pe.tau = p.payment_dates[i] - fd;
+                    pe.period_length = p.payment_dates[m - 1] - p.fixing_dates[i];
+                    pe.j = c->libor_index(p.payment_dates[i]);
+                    pe.pay_k = c->libor_index(p.payment_dates[m - 1]);
+                    pe.adj = 1.0;
+                    L.per.push_back(pe);
+                } else {
+                    for (int r = i; r < m; r++) {
+                        KPeriod pe{};
+                        pe.period_length = p.period_lengths[r];
+                        pe.j = j0 + r; pe.pay_k = j0 + r + 1; pe.adj = 1.0;
+                        L.per.push_back(pe);
+                    }
+                }
+                k.num_k = numeraire_index(fd);
+                check_available(k.eval_index, k.num_k);
+                k.out_slot = L.num_contrib;
+                const int ncols = p.basis == FM_BASIS_REFERENCE ? 3 : 2;
+                for (int r = 0; r < ncols; r++) L.contrib_k.push_back(-1);
+                L.num_contrib += ncols;
+                L.kp.push_back(k);
+                b.ex_period.push_back(i);
+                b.ex_col.push_back(k.out_slot);
+                b.ex_k.push_back(k.num_k);
+            }
+            L.berm.push_back(std::move(b));
         } else {
             throw FmError(FM_ERR_INVALID_ARGUMENT, "unknown product type " + std::to_string(p.type));
         }
@@ -429,6 +520,78 @@ struct Timer {
     void mark(int e) { HIP_CHECK(hipEventRecord(c->ev[e], c->stream)); }
     float span(int a, int b) { float ms = 0; HIP_CHECK(hipEventElapsedTime(&ms, c->ev[a], c->ev[b])); return ms; }
 };
+
+// Regression basis reference of exercise date e of a Bermudan plan (numeraire adjustment of
+// getNumeraire, LIBORMarketModelFromCovarianceModel.java:869-874).
+BermBasisRef basis_ref(const fm_lmm* c, const BermPlan& b, int e, const std::vector<double>& adj) {
+    BermBasisRef r{};
+    r.kind = b.basis;
+    r.col0 = b.ex_col[(size_t)e];
+    const int k = b.ex_k[(size_t)e];
+    r.nmode = !c->has_pd ? 0 : (k <= 1 ? 1 : 2);
+    r.zb = c->zb[(size_t)k];
+    r.adj = adj[(size_t)k];
+    return r;
+}
+
+// Backward induction of one Bermudan over the columns of the last path pass
+// (BermudanSwaption.java:100-146): one step launch per exercise date (latest first)
+// followed by its regression solve, then the final step into the product's slot.
+void run_bermudan(fm_lmm* c, const BermPlan& b, const std::vector<double>& adj, const double* contrib,
+                  double* wsum, double* wm2, int stride) {
+    hipStream_t s = c->stream;
+    const int P = c->P(), m = (int)b.per_col.size(), nex = (int)b.ex_period.size();
+    std::vector<int32_t> cols((size_t)m);
+    std::vector<double> scales((size_t)m);
+    for (int r = 0; r < m; r++) {            // processing order: descending period index
+        const int i = m - 1 - r, k = b.per_k[(size_t)i];
+        cols[(size_t)r] = b.per_col[(size_t)i];
+        scales[(size_t)r] = c->has_pd ? adj[(size_t)k] / c->zb[(size_t)k] : 1.0;
+    }
+    const int32_t* dcol = upload(c->d_berm_col, cols, s);
+    const double* dscale = upload(c->d_berm_scale, scales, s);
+    double* vals = (double*)c->d_berm_vals.get((size_t)P * sizeof(double));
+    double* under = (double*)c->d_berm_under.get((size_t)P * sizeof(double));
+    int32_t* ex = (int32_t*)c->d_berm_ex.get((size_t)P * sizeof(int32_t));
+    HIP_CHECK(hipMemsetAsync(ex, 0xff, (size_t)P * sizeof(int32_t), s));
+    const int nb = berm_num_blocks(P);
+    double* part = (double*)c->d_berm_part.get((size_t)nb * kBermParts * sizeof(double));
+    double* coef = (double*)c->d_berm_coef.get((size_t)std::max(nex, 1) * kBermMaxBasis * sizeof(double));
+    BermStep st{};
+    st.P = P;
+    st.callable = b.callable;
+    st.w = 1.0 / (double)P;
+    st.contrib = contrib;
+    st.values = vals; st.under = under; st.ex_index = ex;
+    st.partials = part;
+    st.wave_sum = wsum; st.wave_m2 = wm2; st.stride = stride; st.slot = b.slot;
+    st.n0 = c->has_pd ? (1.0 / adj[0]) * (1.0 * c->zb[0]) : 1.0;
+    int done = m;       // periods >= done have been added
+    int prev = -1;      // exercise date whose decision is still to be applied
+    for (int e = nex - 1; e >= -1; e--) {
+        const int lo = e >= 0 ? b.ex_period[(size_t)e] : 0;
+        BermStep t = st;
+        t.first = done == m ? 1 : 0;
+        t.apply = prev >= 0 ? 1 : 0;
+        if (prev >= 0) {
+            t.apply_basis = basis_ref(c, b, prev, adj);
+            t.apply_coef = coef + (size_t)prev * kBermMaxBasis;
+            t.apply_period = b.ex_period[(size_t)prev];
+        }
+        t.per_col = dcol + (m - done);
+        t.per_scale = dscale + (m - done);
+        t.num_per = done - lo;
+        t.regress = e >= 0 ? 1 : 0;
+        if (e >= 0) t.reg_basis = basis_ref(c, b, e, adj);
+        t.final_ = e < 0 ? 1 : 0;
+        launch_berm_step(t, s);
+        if (e >= 0) {
+            launch_berm_solve(part, nb, b.num_basis, P, coef + (size_t)e * kBermMaxBasis, s);
+            prev = e;
+        }
+        done = lo;
+    }
+}
 
 // Run generator + fused step (+ reductions). Returns per-column sums and M2 (host).
 void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums, std::vector<double>& m2s,
@@ -499,27 +662,31 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
         const double N1 = c->init_value[0] * ((c->tenor[1] - c->tenor[0]) * 1.0) + 1.0;
         c->zb[1] = (1.0 / N1) * 1.0;
     }
-    if (!L.swap_slot.empty()) {
+    if (!L.swap_slot.empty() || !L.berm.empty()) {
         const std::vector<double> adj = adjustment_chain(c);
-        std::vector<double> scale((size_t)L.num_contrib, 1.0);
-        for (int col = 0; col < L.num_contrib; col++) {
-            const int k = L.contrib_k[(size_t)col];
-            if (c->has_pd) scale[(size_t)col] = adj[(size_t)k] / c->zb[(size_t)k];
+        if (!L.swap_slot.empty()) {
+            std::vector<double> scale((size_t)L.num_contrib, 1.0);
+            for (int col = 0; col < L.num_contrib; col++) {
+                const int k = L.contrib_k[(size_t)col];
+                if (c->has_pd && k >= 0) scale[(size_t)col] = adj[(size_t)k] / c->zb[(size_t)k];
+            }
+            const double* dscale = upload(c->d_scale, scale, s);
+            std::vector<int32_t> meta;
+            meta.insert(meta.end(), L.swap_col0.begin(), L.swap_col0.end());
+            meta.insert(meta.end(), L.swap_ncol.begin(), L.swap_ncol.end());
+            meta.insert(meta.end(), L.swap_slot.begin(), L.swap_slot.end());
+            const int32_t* dmeta = upload(c->d_swapmeta, meta, s);
+            const int ns = (int)L.swap_slot.size();
+            launch_swap_finalize(a.contrib, P, ns, dmeta, dmeta + ns, dmeta + 2 * ns, dscale, wsum, wm2, stride, s);
         }
-        const double* dscale = upload(c->d_scale, scale, s);
-        std::vector<int32_t> meta;
-        meta.insert(meta.end(), L.swap_col0.begin(), L.swap_col0.end());
-        meta.insert(meta.end(), L.swap_ncol.begin(), L.swap_ncol.end());
-        meta.insert(meta.end(), L.swap_slot.begin(), L.swap_slot.end());
-        const int32_t* dmeta = upload(c->d_swapmeta, meta, s);
-        const int ns = (int)L.swap_slot.size();
-        launch_swap_finalize(a.contrib, P, ns, dmeta, dmeta + ns, dmeta + 2 * ns, dscale, wsum, wm2, stride, s);
+        for (const BermPlan& b : L.berm) run_bermudan(c, b, adj, a.contrib, wsum, wm2, stride);
         launch_reduce_partials(wsum, wm2, (int)num_waves, stride, L.num_slots, 0, P, dout, dout + cols, s);
         std::vector<double> s2((size_t)cols), m2b((size_t)cols);
         HIP_CHECK(hipMemcpyAsync(s2.data(), dout, L.num_slots * sizeof(double), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipMemcpyAsync(m2b.data(), dout + cols, L.num_slots * sizeof(double), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
         for (int ss : L.swap_slot) { sums[(size_t)ss] = s2[(size_t)ss]; m2s[(size_t)ss] = m2b[(size_t)ss]; }
+        for (const BermPlan& b : L.berm) { sums[(size_t)b.slot] = s2[(size_t)b.slot]; m2s[(size_t)b.slot] = m2b[(size_t)b.slot]; }
     }
     tm.mark(4);
     HIP_CHECK(hipEventSynchronize(c->ev[4]));
@@ -744,6 +911,9 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     require(count >= 1 && products != nullptr, FM_ERR_INVALID_ARGUMENT, "no products");
     std::lock_guard<std::mutex> lk(c->mu);
     require(c->has_vf, FM_ERR_STATE, "volatility factorization not set (fm_lmm_set_volatility_factorization)");
+    for (int q = 0; q < count; q++)
+        require(products == nullptr || products[q].type != FM_PRODUCT_BERMUDAN_SWAPTION, FM_ERR_UNSUPPORTED,
+                "the adjoint of a Bermudan swaption is not implemented");
     require(c->cfg.precision != FM_PRECISION_FP64_LOGSTATE, FM_ERR_UNSUPPORTED,
             "the adjoint runs on the FP64 and FP32_MIXED path kernels");
     DeviceGuard g(c->device);

@@ -138,4 +138,50 @@ void launch_swap_finalize(const double* contrib, int P, int num_swaps, const int
                           const int32_t* slot, const double* scale, double* wave_sum, double* wave_m2,
                           int stride, void* stream);
 
+// ---- Bermudan swaption (lmm_bermudan.hip) ----------------------------------------
+// Kernel product type of a Bermudan exercise date: the path kernel writes the raw
+// regression basis into contribution columns starting at out_slot (flags = FM_BASIS_*):
+//   FM_BASIS_REFERENCE: 1/(1+L_short dt), 1/(1+L_long dt), N(T_e) (unadjusted)
+//   FM_BASIS_SWAP_RATE: S, N(T_e) (unadjusted)
+constexpr int32_t K_BERM_BASIS = 11;
+constexpr int kBermMaxBasis = 8;
+constexpr int kBermParts = kBermMaxBasis + kBermMaxBasis * (kBermMaxBasis + 1) / 2;   // X^T y, upper X^T X
+
+struct BermBasisRef {
+    int32_t kind;          // FM_BASIS_*
+    int32_t col0;          // first raw basis column
+    int32_t nmode;         // numeraire adjustment: 0 none, 1 (1/adj)(N zb), 2 (N zb)/adj
+    int32_t pad;
+    double zb, adj;
+};
+
+struct BermStep {
+    int P;
+    int callable;                // isCallable: payoffs go to the underlying (else to the values)
+    double w;                    // Monte Carlo weight 1/P
+    const double* contrib;       // device [columns][P]
+    const int32_t* per_col;      // device: payoff columns of the periods added in this step (descending)
+    const double* per_scale;     // device: their numeraire-adjustment scales adj/zb
+    int num_per;
+    int first;                   // values and underlying start at zero
+    int apply;                   // apply the exercise decision of the later date first
+    BermBasisRef apply_basis;
+    const double* apply_coef;    // device [kBermMaxBasis]
+    int32_t apply_period;        // period index recorded in ex_index when exercised
+    int regress;                 // accumulate the regression partials of this date
+    BermBasisRef reg_basis;
+    double* partials;            // device [blocks][kBermParts]
+    int final_;                  // reduce values * N(0) / w into the product's slot
+    double n0;
+    double* values;              // device [P]
+    double* under;               // device [P]
+    int32_t* ex_index;           // device [P] exercise period (-1 never), or NULL
+    double* wave_sum;
+    double* wave_m2;
+    int stride, slot;
+};
+int berm_num_blocks(int P);
+void launch_berm_step(const BermStep& s, void* stream);
+void launch_berm_solve(const double* partials, int num_blocks, int K, int P, double* coef, void* stream);
+
 } // namespace fm

@@ -226,6 +226,51 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
                 else v = ((v != v) ? v : (v > (real)0 ? (real)0 : v)) * ((real)(-1.0) * (real)pe.notional);
                 const double x = (double)v / Nsel;
                 reduce_slot(x, active, cnt, lane, wave, wave_sum, wave_m2, stride, pr.out_slot);
+            } else if (pr.type == K_BERM_BASIS) {
+                // raw regression basis of a Bermudan exercise date T_e (BermudanSwaption.java:215-252)
+                if (pr.flags == FM_BASIS_REFERENCE) {
+                    // L_short over tenor periods [first_j, pe.j), L_long over [first_j, pe.pay_k):
+                    // a single period is the forward itself, several accrue (LMM:1290-1305)
+                    const KPeriod pe = periods[pr.param_off];
+                    real accS = (real)0, accL = (real)0, rs = (real)0, rl = (real)0;
+#pragma unroll
+                    for (int j = 0; j < NCAP; j++) {
+                        if (j >= pr.first_j && j < pe.pay_k) {
+                            const real Lj = val(L[j]);
+                            const real d = (real)delta[j];
+                            if (j == pr.first_j) { rs = Lj; rl = Lj; }
+                            accL = (j == pr.first_j) ? Lj * d + (real)1.0 : accL * ((real)1 + Lj * d);
+                            if (j < pe.j) accS = (j == pr.first_j) ? Lj * d + (real)1.0 : accS * ((real)1 + Lj * d);
+                        }
+                    }
+                    if (pe.j > pr.first_j + 1) rs = (accS - (real)1.0) / (real)pe.tau;
+                    if (pe.pay_k > pr.first_j + 1) rl = (accL - (real)1.0) / (real)pe.period_length;
+                    const real ds = (real)1.0 / (rs * (real)pe.tau + (real)1.0);
+                    const real dl = (real)1.0 / (rl * (real)pe.period_length + (real)1.0);
+                    if (active) {
+                        contrib[(size_t)pr.out_slot * P + p] = (double)ds;
+                        contrib[(size_t)(pr.out_slot + 1) * P + p] = (double)dl;
+                        contrib[(size_t)(pr.out_slot + 2) * P + p] = Nsel;
+                    }
+                } else {
+                    // par rate of the remaining periods: P = 1/(L pl + 1), A = P pl, then
+                    // P = P/(L pl + 1), A = A + P pl; S = (1 - P)/A
+                    real Pd = (real)1, A = (real)0;
+#pragma unroll
+                    for (int j = 0; j < NCAP; j++) {
+                        if (j >= pr.first_j && j < pr.first_j + pr.num_periods) {
+                            const real pl = (real)periods[pr.param_off + (j - pr.first_j)].period_length;
+                            const real Lj = val(L[j]);
+                            if (j == pr.first_j) { Pd = (real)1.0 / (Lj * pl + (real)1.0); A = Pd * pl; }
+                            else { Pd = Pd / (Lj * pl + (real)1.0); A = A + Pd * pl; }
+                        }
+                    }
+                    const real S = (Pd * (real)(-1.0) + (real)1.0) / A;
+                    if (active) {
+                        contrib[(size_t)pr.out_slot * P + p] = (double)S;
+                        contrib[(size_t)(pr.out_slot + 1) * P + p] = Nsel;
+                    }
+                }
             } else {   // one period of a swap: contribution column, combined after the adjustment is known
                 const KPeriod pe = periods[pr.param_off];
                 const real Lj = val(pick<NCAP, real>(L, pr.first_j));

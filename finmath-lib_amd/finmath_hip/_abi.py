@@ -29,7 +29,8 @@ MEASURE = {"SPOT": 0, "TERMINAL": 1}
 STATE_SPACE = {"NORMAL": 0, "LOGNORMAL": 1}
 SCHEME = {"EULER": 0, "PREDICTOR_CORRECTOR": 1, "EULER_FUNCTIONAL": 2, "PREDICTOR_CORRECTOR_FUNCTIONAL": 3}
 PRECISION = {"FP64": 0, "FP32_MIXED": 1, "FP64_LOGSTATE": 2}
-PRODUCT_SWAPTION, PRODUCT_SWAP, PRODUCT_CAPLET = 1, 2, 3
+PRODUCT_SWAPTION, PRODUCT_SWAP, PRODUCT_CAPLET, PRODUCT_BERMUDAN_SWAPTION = 1, 2, 3, 4
+BASIS_REFERENCE, BASIS_SWAP_RATE = 0, 1
 
 # Every symbol include/finmath_lmm.h declares.
 EXPORTED_SYMBOLS = (
@@ -87,6 +88,7 @@ class FmProduct(ctypes.Structure):
         ("period_lengths", POINTER(c_double)), ("swaprates", POINTER(c_double)),
         ("notionals", POINTER(c_double)), ("discounting_adjustment", POINTER(c_double)),
         ("notional", c_double), ("is_pay_fix", c_int32), ("is_floorlet", c_int32),
+        ("exercise_flags", POINTER(c_int32)), ("is_cancelable", c_int32), ("basis", c_int32),
     ]
 
 
@@ -173,6 +175,19 @@ class Product:
             setattr(self.s, fname, _dptr(a))
             n = max(n, a.size)
         self.s.num_periods = n
+        self.s.exercise_flags = POINTER(c_int32)()
+
+    @staticmethod
+    def bermudan(fixing_dates, payment_dates, period_lengths, swaprates, exercise_flags, notionals=None,
+                 is_callable=True, basis=BASIS_REFERENCE):
+        p = Product(PRODUCT_BERMUDAN_SWAPTION, 0.0, fixing_dates, payment_dates, period_lengths, swaprates,
+                    notionals)
+        flags = np.ascontiguousarray(np.asarray(exercise_flags, dtype=np.int32))
+        p._keep.append(flags)
+        p.s.exercise_flags = flags.ctypes.data_as(POINTER(c_int32))
+        p.s.is_cancelable = 0 if is_callable else 1
+        p.s.basis = int(basis)
+        return p
 
     @staticmethod
     def swaption(exercise_date, fixing_dates, payment_dates, swaprates, period_lengths=None, notional=1.0,

@@ -6,7 +6,8 @@ c1: 40 semiannual forwards, 5 factors, f_i = 0.02 + 0.001 i, 1,000,000 paths:
     payer swaptions with expiries 1..10y into 10y swaps at ATM and +-100bp.
 c2: the c1 model with 4,000,000 paths in FP32-mixed arithmetic.
 c3: the c1 model with 1,000,000 paths, adjoint Greeks of the c1 strip.
-c4: 80 forwards, 10 factors, flat 3.5 %, Bermudan swaption 10y no-call to 30y.
+c4: 80 forwards, 10 factors, flat 3.5 %, FP32-mixed, Bermudan payer swaption 10y no-call,
+    then callable annually into the swap to 30y at 3.5 %, regression basis 1, S, S^2, N.
 
 Volatility sigma_j(t) = (a + b (T_j - t)) exp(-c (T_j - t)) + d with
 (a, b, c, d) = (0.2, 0.0, 0.25, 0.1); correlation exp(-beta |T_i - T_j|).
@@ -111,6 +112,17 @@ def swap_spec(start, end, period, strike, pay_fix=True):
     return {"type": "swap", "fixing": fix, "payment": pay, "strike": np.full(fix.size, strike), "pay_fix": pay_fix}
 
 
+def bermudan_spec(first_exercise, end, period, strike, exercise_every=1.0, callable_=True, basis=0):
+    """Bermudan payer swaption: swap periods from first_exercise to end, exercisable at the
+    period starts first_exercise, first_exercise + exercise_every, ... (before the last period)."""
+    fix = round_ticks(np.arange(first_exercise, end - 1e-12, period))
+    pay = round_ticks(fix + period)
+    steps = int(round(exercise_every / period))
+    flags = np.array([1 if (i % steps == 0) else 0 for i in range(fix.size)], dtype=np.int32)
+    return {"type": "bermudan", "fixing": fix, "payment": pay, "period_lengths": pay - fix,
+            "strike": np.full(fix.size, strike), "exercise_flags": flags, "callable": callable_, "basis": basis}
+
+
 def c0() -> Workload:
     n = 20
     w = Workload("c0", n, 0.5, 3, 10_000, 0.1, np.full(n, 0.03))
@@ -149,7 +161,11 @@ def c3() -> Workload:
 
 def c4() -> Workload:
     n = 80
-    return Workload("c4", n, 0.5, 10, 1_000_000, 0.05, np.full(n, 0.035))
+    w = Workload("c4", n, 0.5, 10, 1_000_000, 0.05, np.full(n, 0.035))
+    w.precision = "FP32_MIXED"
+    # 10y no-call, then callable annually into the swap to 30y, strike 3.5 %, basis 1, S, S^2, N
+    w.products = [bermudan_spec(10.0, 30.0, 0.5, 0.035, 1.0, True, basis=1)]
+    return w
 
 
 ALL = {"c0": c0, "c1": c1, "c2": c2, "c3": c3, "c4": c4}

@@ -155,6 +155,14 @@ int fm_lmm_get_brownian_increment(fm_lmm* ctx, int32_t time_index, int32_t facto
 #define FM_PRODUCT_SWAPTION  1  /* interestrate/products/Swaption.java:137-199 */
 #define FM_PRODUCT_SWAP      2  /* interestrate/products/SimpleSwap.java:137-170 */
 #define FM_PRODUCT_CAPLET    3  /* interestrate/products/Caplet.java:115-140 (value unit VALUE) */
+#define FM_PRODUCT_BERMUDAN_SWAPTION 4  /* interestrate/products/BermudanSwaption.java:91-157 */
+
+/* Regression basis of a Bermudan swaption's exercise decision. */
+#define FM_BASIS_REFERENCE 0  /* BermudanSwaption.getBasisFunctions (BermudanSwaption.java:215-252):
+                                 1, 1/(1+L_short dt), its square, 1/(1+L_long dt), its square, 1/N(T_e) */
+#define FM_BASIS_SWAP_RATE 1  /* a RegressionBasisFunctionsProvider
+                                 (montecarlo/conditionalexpectation/RegressionBasisFunctionsProvider.java):
+                                 1, S, S*S, N(T_e), S = par rate of the remaining periods at T_e */
 
 /* One product. Dates are model times; arrays have num_periods entries.
  *  SWAPTION: exercise_date, fixing/payment dates, period_lengths (NULL ->
@@ -163,7 +171,15 @@ int fm_lmm_get_brownian_increment(fm_lmm* ctx, int32_t time_index, int32_t facto
  *            curve's own, Swaption.java:165-175).
  *  SWAP:     fixing/payment dates, swaprates, notionals (NULL -> 1.0), is_pay_fix.
  *  CAPLET:   exercise_date = maturity, period_lengths[0] = period length,
- *            swaprates[0] = strike, notional = daycount fraction, is_floorlet. */
+ *            swaprates[0] = strike, notional = daycount fraction, is_floorlet.
+ *  BERMUDAN_SWAPTION: fixing/payment dates, period_lengths, swaprates, notionals
+ *            (NULL -> 1.0), exercise_flags (isPeriodStartDateExerciseDate, nonzero =
+ *            exercisable at the period's fixing date), is_cancelable (0: the right
+ *            to enter the swap, isCallable = true; 1: the right to terminate it),
+ *            basis (FM_BASIS_*). The exercise decision regresses the continuation
+ *            value on the basis (MonteCarloConditionalExpectationRegression.java:79-150):
+ *            Gram matrix and right-hand side averaged over all paths, solved by the
+ *            pseudo-inverse of the reference's SVD solver. */
 typedef struct fm_product {
     int32_t       type;
     int32_t       num_periods;
@@ -177,6 +193,9 @@ typedef struct fm_product {
     double        notional;
     int32_t       is_pay_fix;
     int32_t       is_floorlet;
+    const int32_t* exercise_flags;
+    int32_t       is_cancelable;
+    int32_t       basis;
 } fm_product;
 
 /* Value products at evaluation time 0 in one fused simulation pass:

@@ -870,4 +870,129 @@ int orc_lmm_caplet(void* h, double maturity, double periodLength, double strike,
     } catch (const std::exception& e) { g_err = e.what(); return -1; }
 }
 
+
+// ---------------------------------------------------------------------------
+// Bermudan swaption by regression (American Monte Carlo).
+// BermudanSwaption.getValues (montecarlo/interestrate/products/BermudanSwaption.java:91-157),
+// default basis getBasisFunctions (:215-252), the conditional expectation of
+// MonteCarloConditionalExpectationRegression (montecarlo/conditionalexpectation/
+// MonteCarloConditionalExpectationRegression.java:79-150).
+//
+// The reference solves X^T X x = X^T y with commons-math3's SingularValueDecomposition
+// solver: singular values at or below tol = max(K * sigma_max * 2^-52, sqrt(2^-1022))
+// are dropped (pseudo-inverse). X^T X is symmetric positive semi-definite, so its
+// singular values are |eigenvalues| and the pseudo-inverse solve is
+// x = sum_{|lambda_i| > tol} v_i (v_i . X^T y) / lambda_i, computed here from the
+// Jacobi eigen decomposition above. Pins: numpy's pinv/lstsq on the same Gram
+// matrices; the zero-volatility closed form (rank-1 Gram matrix). The last ulp of
+// commons' SVD is "parity unpinned".
+//
+// basis 0: the reference's default basis (BermudanSwaption.java:215-252):
+//   1, 1/(1 + L_short dt), its square, 1/(1 + L_long dt), its square, 1/N(T_e).
+// basis 1: a RegressionBasisFunctionsProvider of swap rate and numeraire:
+//   1, S, S*S, N(T_e), with S the par rate of the remaining periods at T_e:
+//   P = 1/(L_e pl_e + 1), A = P pl_e; P = P/(L_p pl_p + 1), A = A + P pl_p;
+//   S = (1 - P)/A (forward rates L_p = getForwardRate(T_e, fix_p, pay_p)).
+// ---------------------------------------------------------------------------
+namespace orc {
+
+static std::vector<double> pinv_solve(const std::vector<double>& A, const std::vector<double>& b, int K) {
+    std::vector<double> ev((size_t)K), V((size_t)K * K);
+    orc_eigen_sym(A.data(), K, ev.data(), V.data());
+    double smax = 0.0;
+    for (int i = 0; i < K; i++) smax = std::max(smax, std::fabs(ev[(size_t)i]));
+    const double tol = std::max(K * smax * 0x1.0p-52, std::sqrt(0x1.0p-1022));
+    std::vector<double> x((size_t)K, 0.0);
+    for (int i = 0; i < K; i++) {
+        if (!(std::fabs(ev[(size_t)i]) > tol)) continue;
+        double d = 0.0;
+        for (int r = 0; r < K; r++) d += V[(size_t)r * K + i] * b[(size_t)r];
+        d /= ev[(size_t)i];
+        for (int r = 0; r < K; r++) x[(size_t)r] += V[(size_t)r * K + i] * d;
+    }
+    return x;
+}
+
+static std::vector<RV> bermudan_basis(Model* m, int basis, double fd, int e, int nper, const double* fix,
+                                      const double* pay, const double* pl) {
+    std::vector<RV> b;
+    b.push_back(rvdet(1.0));
+    if (basis == 0) {
+        int idx = java_binary_search(fix, nper, fd);
+        if (idx < 0) idx = -idx;
+        if (idx >= nper) idx = nper - 1;
+        RV rs = m->forward_rate(fd, fd, pay[idx]);
+        RV ds = invert(add(mult(rs, pay[idx] - fd), 1.0));
+        b.push_back(ds);
+        b.push_back(powv(ds, 2.0));
+        RV rl = m->forward_rate(fd, fix[idx], pay[nper - 1]);
+        RV dl = invert(add(mult(rl, pay[nper - 1] - fix[idx]), 1.0));
+        b.push_back(dl);
+        b.push_back(powv(dl, 2.0));
+        b.push_back(invert(m->numeraire(fd)));
+    } else {
+        RV P, A;
+        for (int p = e; p < nper; p++) {
+            RV L = m->forward_rate(fd, fix[p], pay[p]);
+            if (p == e) { P = invert(add(mult(L, pl[p]), 1.0)); A = mult(P, pl[p]); }
+            else { P = div(P, add(mult(L, pl[p]), 1.0)); A = add(A, mult(P, pl[p])); }
+        }
+        RV S = div(add(mult(P, -1.0), 1.0), A);
+        b.push_back(S);
+        b.push_back(mult(S, S));
+        b.push_back(m->numeraire(fd));
+    }
+    return b;
+}
+
+} // namespace orc
+
+// coefficients (optional): [number of exercise dates][K] regression coefficients in
+// the order the backward induction solves them (last exercise date first).
+int orc_lmm_bermudan(void* h, int nper, const int* isExercise, const double* fix, const double* periodLength,
+                     const double* pay, const double* notionals, const double* swaprates, int isCallable,
+                     int basis, double* value, double* error, double* per_path, double* exercise_time,
+                     double* coefficients) {
+    try {
+        auto* m = (orc::Model*)h;
+        using namespace orc;
+        RV values = scalar(0.0), under = scalar(0.0), ex = scalar(INFINITY);
+        int ncoef = 0;
+        for (int per = nper - 1; per >= 0; per--) {
+            const double fd = fix[per], pl = periodLength[per], pd_ = pay[per];
+            const double notional = notionals ? notionals[per] : 1.0, K = swaprates[per];
+            RV libor = m->forward_rate(fd, fd, fd + pl);
+            RV payoff = mult(mult(sub(libor, K), pl), notional);
+            payoff = mult(div(payoff, m->numeraire(pd_)), m->weights());
+            if (isCallable) under = add(under, payoff);
+            else values = add(values, payoff);
+            if (isExercise[per]) {
+                RV y = sub(values, under);
+                std::vector<RV> b = bermudan_basis(m, basis, fd, per, nper, fix, pay, periodLength);
+                const int Kb = (int)b.size();
+                std::vector<double> XTX((size_t)Kb * Kb), XTy((size_t)Kb);
+                for (int i = 0; i < Kb; i++)
+                    for (int j = i; j < Kb; j++) {
+                        XTX[(size_t)i * Kb + j] = average(mult(b[(size_t)i], b[(size_t)j]));
+                        XTX[(size_t)j * Kb + i] = XTX[(size_t)i * Kb + j];
+                    }
+                for (int i = 0; i < Kb; i++) XTy[(size_t)i] = average(mult(y, b[(size_t)i]));
+                std::vector<double> x = pinv_solve(XTX, XTy, Kb);
+                if (coefficients) for (int i = 0; i < Kb; i++) coefficients[(size_t)ncoef * Kb + i] = x[(size_t)i];
+                ncoef++;
+                RV trig = mult(b[0], x[0]);
+                for (int i = 1; i < Kb; i++) trig = addProduct(trig, b[(size_t)i], x[(size_t)i]);
+                values = choose(trig, values, under);
+                ex = choose(trig, ex, scalar(fd));
+            }
+        }
+        values = div(mult(values, m->numeraire(0.0)), m->weights());
+        *value = average(values);
+        *error = stderror(values);
+        if (per_path) fill(values, m->P, per_path);
+        if (exercise_time) fill(ex, m->P, exercise_time);
+        return 0;
+    } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
 } // extern "C"

@@ -59,6 +59,8 @@ def lib():
         L.orc_lmm_swaption.argtypes = [c_void_p, c_double, c_int, dp, dp, dp, dp, c_double, dp, dp, dp, dp]
         L.orc_lmm_swap.argtypes = [c_void_p, c_int, dp, dp, dp, dp, c_int, dp, dp, dp]
         L.orc_lmm_caplet.argtypes = [c_void_p, c_double, c_double, c_double, c_double, c_int, dp, dp]
+        L.orc_lmm_bermudan.argtypes = [c_void_p, c_int, POINTER(c_int), dp, dp, dp, dp, dp, c_int, c_int,
+                                       dp, dp, dp, dp, dp]
         L.orc_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -256,5 +258,31 @@ class OracleModel:
             raise RuntimeError(lib().orc_last_error().decode())
         return v[0], e[0]
 
+    def bermudan(self, spec, per_path=False, coefficients=False):
+        """BermudanSwaption.getValues; returns (value, error[, per-path values, exercise times][, coefficients])."""
+        fix, pay, K = f64(spec["fixing"]), f64(spec["payment"]), f64(spec["strike"])
+        pl = f64(spec.get("period_lengths", pay - fix))
+        flags = np.ascontiguousarray(np.asarray(spec["exercise_flags"], dtype=np.int32))
+        notionals = f64(spec["notionals"]) if spec.get("notionals") is not None else None
+        v, e = np.zeros(1), np.zeros(1)
+        pp, ex = np.empty(self.P), np.empty(self.P)
+        nex = int(flags.sum())
+        kb = 6 if spec.get("basis", 0) == 0 else 4
+        coef = np.zeros((max(nex, 1), kb))
+        rc = lib().orc_lmm_bermudan(self.h, fix.size, flags.ctypes.data_as(POINTER(c_int)), _dp(fix), _dp(pl),
+                                    _dp(pay), _dp(notionals) if notionals is not None else POINTER(c_double)(),
+                                    _dp(K), 1 if spec.get("callable", True) else 0, int(spec.get("basis", 0)),
+                                    _dp(v), _dp(e), _dp(pp), _dp(ex), _dp(coef))
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        out = (v[0], e[0])
+        if per_path:
+            out = out + (pp, ex)
+        if coefficients:
+            out = out + (coef[:nex],)
+        return out
+
     def value(self, spec):
+        if spec["type"] == "bermudan":
+            return self.bermudan(spec)
         return self.swaption(spec) if spec["type"] == "swaption" else self.swap(spec)

@@ -1,0 +1,120 @@
+"""GPU Bermudan swaption (regression exercise) against the oracle, through the C ABI.
+
+Bars: FP64 within 1e-9 relative of the oracle (the regression solves the same normal
+equations; different summation order moves the coefficients by ~cond * 1e-16, which
+flips no exercise decision at these path counts), the zero-volatility closed form to
+1e-12, FP32-mixed within 0.1 bp of the FP64 oracle price.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+from finmath_hip import _abi
+from finmath_hip import workloads as W
+from test_gpu_parity import make_context, say
+
+pytestmark = pytest.mark.gpu
+
+
+def berm_product(spec):
+    return _abi.Product.bermudan(spec["fixing"], spec["payment"], spec["period_lengths"], spec["strike"],
+                                 spec["exercise_flags"], is_callable=spec["callable"], basis=spec["basis"])
+
+
+@pytest.mark.parametrize("basis", [0, 1])
+@pytest.mark.parametrize("callable_", [True, False])
+def test_bermudan_fp64_matches_oracle(basis, callable_):
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    P = 20_000
+    spec = W.bermudan_spec(2.0, 10.0, 0.5, 0.03, 1.0, callable_, basis=basis)
+    say(f"case bermudan c0 P={P} basis={basis} callable={callable_}")
+    ctx = make_context(wl, t, P)
+    vals, errs = ctx.value_products([berm_product(spec)])
+    rv, re_ = O.OracleModel(t, P).bermudan(spec)
+    say(f"  gpu {vals[0]:.15e} oracle {rv:.15e} err {errs[0]:.3e}/{re_:.3e}")
+    assert abs(vals[0] - rv) <= 1e-9 * abs(rv)
+    assert abs(errs[0] - re_) <= 1e-7 * abs(re_)
+
+
+def test_bermudan_priced_with_europeans_in_one_pass():
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    P = 10_000
+    say(f"case bermudan + strip c0 P={P}")
+    spec = W.bermudan_spec(2.0, 10.0, 0.5, 0.03, 1.0, True, basis=0)
+    sw = wl.products[0]
+    ctx = make_context(wl, t, P)
+    prods = [_abi.Product.swaption(sw["exercise"], sw["fixing"], sw["payment"], sw["strike"]), berm_product(spec),
+             _abi.Product.swap(wl.products[1]["fixing"], wl.products[1]["payment"], wl.products[1]["strike"])]
+    vals, _ = ctx.value_products(prods)
+    om = O.OracleModel(t, P)
+    assert abs(vals[0] - om.swaption(sw)[0]) <= 1e-10 * abs(vals[0])
+    assert abs(vals[1] - om.bermudan(spec)[0]) <= 1e-9 * abs(vals[1])
+    assert abs(vals[2] - om.swap(wl.products[1])[0]) <= 1e-10 * abs(vals[2])
+
+
+@pytest.mark.parametrize("basis", [0, 1])
+def test_bermudan_zero_volatility_closed_form(basis):
+    wl = W.c0()
+    wl.forwards = 0.02 + 0.001 * np.arange(wl.num_libors)
+    t = O.ModelTables(wl, vol_params=(0.0, 0.0, 0.0, 0.0))
+    P = 1000
+    strike = 0.03
+    say(f"case bermudan zero-vol c0 P={P} basis={basis}")
+    ctx = make_context(wl, t, P)
+    spec = W.bermudan_spec(2.0, 10.0, 0.5, strike, 1.0, True, basis=basis)
+    vals, errs = ctx.value_products([berm_product(spec)])
+    fix, pay, pl = spec["fixing"], spec["payment"], spec["period_lengths"]
+    flows = np.array([(t.L0[int(round(f / 0.5))] - strike) * pl[p] * t.pd[int(round(pay[p] / 0.5))]
+                      for p, f in enumerate(fix)])
+    U = [flows[p:].sum() for p in range(fix.size) if spec["exercise_flags"][p]]
+    assert vals[0] == pytest.approx(max(0.0, max(U)), rel=1e-12)
+
+
+def test_bermudan_fp32_mixed_within_a_tenth_of_a_basis_point():
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    P = 20_000
+    spec = W.bermudan_spec(2.0, 10.0, 0.5, 0.03, 1.0, True, basis=1)
+    say(f"case bermudan fp32 c0 P={P}")
+    ctx = make_context(wl, t, P, precision="FP32_MIXED")
+    vals, _ = ctx.value_products([berm_product(spec)])
+    rv, _ = O.OracleModel(t, P).bermudan(spec)
+    say(f"  fp32 {vals[0]:.12e} oracle {rv:.12e} diff {(vals[0] - rv) * 1e4:.5f} bp")
+    assert abs(vals[0] - rv) * 1e4 < 0.1
+
+
+def test_bermudan_80_forwards_10_factors():
+    # the 80 x 10 model of the Bermudan workload at a path count the oracle finishes in seconds
+    wl = W.c4()
+    t = O.ModelTables(wl)
+    P = 8192
+    spec = wl.products[0]
+    say(f"case bermudan c4 model P={P} FP64 and FP32_MIXED")
+    rv, re_ = O.OracleModel(t, P).bermudan(spec)
+    v64, _ = make_context(wl, t, P).value_products([berm_product(spec)])
+    v32, _ = make_context(wl, t, P, precision="FP32_MIXED").value_products([berm_product(spec)])
+    say(f"  fp64 {v64[0]:.15e} fp32 {v32[0]:.12e} oracle {rv:.15e} +- {re_:.2e}")
+    assert abs(v64[0] - rv) <= 1e-9 * abs(rv)
+    assert abs(v32[0] - rv) * 1e4 < 0.1
+
+
+def test_bermudan_is_deterministic():
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    say("case bermudan determinism c0 P=30000")
+    ctx = make_context(wl, t, 30_000)
+    spec = W.bermudan_spec(2.0, 10.0, 0.5, 0.03, 1.0, True, basis=0)
+    a, ea = ctx.value_products([berm_product(spec)])
+    b, eb = ctx.value_products([berm_product(spec)])
+    assert np.array_equal(a, b) and np.array_equal(ea, eb)
+
+
+def test_bermudan_adjoint_is_rejected():
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    ctx = make_context(wl, t, 1000)
+    ctx.set_volatility_factorization(t.vol, t.factors)
+    with pytest.raises(_abi.UnsupportedOperationException):
+        ctx.adjoint([berm_product(W.bermudan_spec(2.0, 10.0, 0.5, 0.03))])
