@@ -51,6 +51,11 @@ def build_simulation(wl, precision="FP64", regenerate=True):
     for s in wl.products:
         if s["type"] == "swaption":
             products.append(H.Swaption(s["exercise"], list(s["fixing"]), list(s["payment"]), list(s["strike"])))
+        elif s["type"] == "bermudan":
+            provider = H.RegressionBasisFunctionsSwapRate() if s.get("basis", 0) == 1 else None
+            products.append(H.BermudanSwaption([bool(f) for f in s["exercise_flags"]], list(s["fixing"]),
+                                               list(s["period_lengths"]), list(s["payment"]), [], list(s["strike"]),
+                                               s.get("callable", True), provider))
         else:
             products.append(H.SimpleSwap(list(s["fixing"]), list(s["payment"]), list(s["strike"]), s.get("pay_fix", True)))
     return sim, process, products
@@ -106,6 +111,17 @@ def run_greeks(wl, steps, stride):
     return price_ms, greeks_ms, g
 
 
+def copy_probe(nbytes=1 << 30, reps=3):
+    """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE counter
+    readings give the scale factor from counter units to HBM bytes."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+
+
 def cpu_baseline(wl_gpu, paths):
     """The oracle (C++ restatement of the reference CPU path) on a bounded sample."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -133,7 +149,14 @@ def main():
     ap.add_argument("--no-greeks", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-logstate", action="store_true")
+    ap.add_argument("--no-bermudan", action="store_true")
+    ap.add_argument("--only-fp64", action="store_true", help="the headline leg only (for counter passes)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 1 GiB device copy (calibrates the HBM byte counters)")
     args = ap.parse_args()
+    if args.only_fp64:
+        args.no_fp32 = args.no_logstate = args.no_bermudan = args.no_greeks = args.no_cpu = True
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path axis would shard across GPUs "
               "as described in DESIGN.md, but this build measures one MI355X.", file=sys.stderr)
@@ -177,11 +200,24 @@ def main():
             "max_abs_err_bp_vs_fp64": float(err_bp.max()), "max_mc_stderr_bp": float(e64b.max() * 1e4),
             "within_0.1bp": bool(err_bp.max() < 0.1),
         }
-    ms_ls, stage_ls, v_ls, _ = run_gpu(wl, "FP64_LOGSTATE", args.steps, args.warmup)
-    line["fp64_logstate"] = {
-        "workload": "c1, FP64 arithmetic carrying the log-forwards between steps, fused multiply-add",
-        "value": path_steps / (ms_ls * 1e-3), "unit": "path-steps/s", "ms_per_step": ms_ls,
-        "path_kernel_ms": stage_ls[1], "max_rel_diff_vs_fp64": float(np.max(np.abs(v_ls - v64) / np.abs(v64)))}
+    if not args.no_logstate:
+        ms_ls, stage_ls, v_ls, _ = run_gpu(wl, "FP64_LOGSTATE", args.steps, args.warmup)
+        line["fp64_logstate"] = {
+            "workload": "c1, FP64 arithmetic carrying the log-forwards between steps, fused multiply-add",
+            "value": path_steps / (ms_ls * 1e-3), "unit": "path-steps/s", "ms_per_step": ms_ls,
+            "path_kernel_ms": stage_ls[1], "max_rel_diff_vs_fp64": float(np.max(np.abs(v_ls - v64) / np.abs(v64)))}
+    if not args.no_bermudan:
+        wl4 = W.c4()
+        ms4, st4, v4, e4 = run_gpu(wl4, "FP32_MIXED", max(2, args.steps // 2), 1)
+        ms4d, st4d, v4d, e4d = run_gpu(wl4, "FP64", 1, 1)
+        line["bermudan"] = {
+            "workload": "c4: 80 semiannual forwards, 10 factors, flat 3.5 %, 1,000,000 paths; Bermudan payer swaption "
+                        "10y no-call then callable annually to 30y at 3.5 %, regression basis 1, S, S^2, N",
+            "value": wl4.path_steps / (ms4 * 1e-3), "unit": "path-steps/s", "ms_per_step": ms4,
+            "stage_ms": {"generator": st4[0], "path_kernel": st4[1], "induction_and_reductions": st4[2]},
+            "price": float(v4[0]), "mc_stderr": float(e4[0]),
+            "fp64_ms_per_step": ms4d, "fp64_price": float(v4d[0]),
+            "err_bp_vs_fp64": float(abs(v4[0] - v4d[0]) * 1e4), "mc_stderr_bp": float(e4d[0] * 1e4)}
     if not args.no_greeks:
         wl3 = W.c3()
         pm, gm, g = run_greeks(wl3, max(2, args.steps // 3), 4)
@@ -195,6 +231,8 @@ def main():
     if not args.no_cpu:
         cb = cpu_baseline(wl, args.cpu_paths)
         line["cpu_baseline"] = cb
+    if args.copy_probe:
+        copy_probe()
     print(json.dumps(line))
 
 

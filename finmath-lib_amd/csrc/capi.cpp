@@ -584,13 +584,14 @@ void run_bermudan(fm_lmm* c, const BermPlan& b, const std::vector<double>& adj, 
         t.regress = e >= 0 ? 1 : 0;
         if (e >= 0) t.reg_basis = basis_ref(c, b, e, adj);
         t.final_ = e < 0 ? 1 : 0;
-        launch_berm_step(t, s);
+        launch_berm_step(t, b.basis, s);
         if (e >= 0) {
             launch_berm_solve(part, nb, b.num_basis, P, coef + (size_t)e * kBermMaxBasis, s);
             prev = e;
         }
         done = lo;
     }
+    HIP_CHECK(hipStreamSynchronize(s));      // cols and scales are host sources of the uploads above
 }
 
 // Run generator + fused step (+ reductions). Returns per-column sums and M2 (host).
@@ -975,12 +976,22 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     ga.stride = stride;
     ga.kcur_at = upload(c->d_kcur, kcur, s);
     ga.weight = upload(c->d_scale, kweights, s);
-    std::vector<double> fpad((size_t)n * fcap_of(c->F()), 0.0);
+    // factor matrix padded to the factor capacity and -sigma_j(t_i) c_j, in the path precision
+    std::vector<double> fpad((size_t)n * fcap_of(c->F()), 0.0), vterm((size_t)nT * n, 0.0);
     for (int j = 0; j < n; j++)
         for (int k = 0; k < c->F(); k++) fpad[(size_t)j * fcap_of(c->F()) + k] = c->vf_fac[(size_t)j * c->F() + k];
-    ga.fsig = upload(c->d_fsig, fpad, s);
-    ga.sig = upload(c->d_sig, c->vf_sig, s);
-    ga.cdiag = upload(c->d_cdiag, c->vf_c, s);
+    if (c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL)
+        for (int i = 0; i < nT; i++)
+            for (int j = 0; j < n; j++) vterm[(size_t)i * n + j] = -c->vf_sig[(size_t)i * n + j] * c->vf_c[(size_t)j];
+    // host sources stay alive until the stream synchronisation at the end of this call
+    const std::vector<float> fpad_f(fpad.begin(), fpad.end()), vterm_f(vterm.begin(), vterm.end());
+    if (c->fp32()) {
+        ga.fsig = upload(c->d_fsig, fpad_f, s);
+        ga.vterm = upload(c->d_sig, vterm_f, s);
+    } else {
+        ga.fsig = upload(c->d_fsig, fpad, s);
+        ga.vterm = upload(c->d_sig, vterm, s);
+    }
     const int64_t num_waves = ((int64_t)P + 63) / 64;
     const int gstride = nT * n + n;
     ga.wave_grad = (double*)c->d_wave_grad.get((size_t)num_waves * gstride * sizeof(double));

@@ -121,9 +121,8 @@ struct AdjArgs {
     int stride;                      // checkpoint stride
     const int32_t* kcur_at;          // [nT+1] numeraire index after the advance at each time index
     const double* weight;            // [num kernel products] objective weight of each lowered product
-    const double* fsig;              // [n][fcap_of(F)] factor matrix f_jk, zero padded (double)
-    const double* sig;               // [nT][n] volatility sigma_j(t_i)
-    const double* cdiag;             // [n] variance factor c_j (correlation diagonal)
+    const void* fsig;                // [n][fcap_of(F)] factor matrix f_jk, zero padded (path precision)
+    const void* vterm;               // [nT][n] -sigma_j(t_i) c_j, d(variance * -1/2)/dsigma (path precision)
     double* wave_grad;               // [num_waves][nT*n + n] partials of dV/dsigma and dV/dL(0)
 };
 void launch_lmm_adjoint_fp64(const StepArgs& a, const AdjArgs& g, void* stream);
@@ -181,7 +180,7 @@ struct BermStep {
     int stride, slot;
 };
 int berm_num_blocks(int P);
-void launch_berm_step(const BermStep& s, void* stream);
-void launch_berm_solve(const double* partials, int num_blocks, int K, int P, double* coef, void* stream);
+void launch_berm_step(const BermStep& s, int kind, void* stream);
+void launch_berm_solve(const double* partials, int num_rows, int K, int P, double* coef, void* stream);
 
 } // namespace fm

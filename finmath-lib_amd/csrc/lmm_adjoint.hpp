@@ -20,17 +20,37 @@
 
 namespace fm {
 
+// Wavefront sum by DPP row operations (quad_perm [1,0,3,2], [2,3,0,1], row_ror 4 and 8,
+// row_bcast 15 and 31): the total is valid in lane 63 only. Each step moves the two
+// 32-bit halves of the double; lanes without a source read 0.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v = v + dpp_move<0xb1>(v);
+    v = v + dpp_move<0x4e>(v);
+    v = v + dpp_move<0x124>(v);
+    v = v + dpp_move<0x128>(v);
+    v = v + dpp_move<0x142>(v);
+    v = v + dpp_move<0x143>(v);
+    return v;
+}
 
 template <int NCAP, int FCAP, typename real>
 __global__ void __launch_bounds__(kStepThreads)
+__attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 && NCAP <= 40 ? 3 : 1)))
 lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ rec,
                    const double* __restrict__ delta, const double* __restrict__ times,
                    const int32_t* __restrict__ first_alive, const int32_t* __restrict__ kcur_at,
                    const KProduct* __restrict__ products, const KPeriod* __restrict__ periods,
                    const real* __restrict__ dw, const real* __restrict__ ckpt,
                    const double* __restrict__ numeraire, const double* __restrict__ weight,
-                   const double* __restrict__ fsig, const double* __restrict__ sig,
-                   const double* __restrict__ cdiag, double* __restrict__ wave_grad)
+                   const real* __restrict__ fsig, const real* __restrict__ vterm, double* __restrict__ wave_grad)
 {
     const int64_t p = (int64_t)blockIdx.x * kStepThreads + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -45,9 +65,9 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
     const real cap = (real)a.cap;
     const int stride = g.stride;
 
-    real Lck[NCAP], L[NCAP], Lb[NCAP], mub[NCAP];
+    real Lck[NCAP], L[NCAP], Lb[NCAP];
 #pragma unroll
-    for (int j = 0; j < NCAP; j++) { Lb[j] = (real)0; mub[j] = (real)0; }
+    for (int j = 0; j < NCAP; j++) Lb[j] = (real)0;
     double Nb_top = 0.0, Nb_low = 0.0;      // adjoints of N(T_K) and N(T_{K-1})
     int K = kcur_at[nT];
     int qp = a.num_products - 1;            // products sorted by evaluation index
@@ -96,8 +116,7 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
 #pragma unroll
                         for (int k = 0; k < FCAP; k++) Y = Y + dW[k] * R.lam(k);
                         const real E = lognormal ? dev_exp<real>(Y) : Y;
-                        real Yb = (capped && E > cap) ? (real)0 : Lb[j] * (lognormal ? E : (real)1);
-                        mub[j] = Yb * dt;
+                        const real Yb = (capped && E > cap) ? (real)0 : Lb[j] * (lognormal ? E : (real)1);
                         Lb[j] = lognormal ? Yb / Lj : Yb;     // direct part of the adjoint of L_j(t)
                     }
                 }
@@ -109,23 +128,25 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
                         const real Lj = L[j];
                         const real r = (real)1.0 / (Lj * R.c1() + R.c2());
                         const real w = lognormal ? r * Lj : r;
-                        const real mb = mub[j];
-                        const real Yb = mb / dt;
+                        // adjoint of Y_j(t+1) from the direct part left by pass A
+                        const real Yb = lognormal ? Lb[j] * Lj : Lb[j];
+                        const real mb = Yb * dt;
                         real wb = (real)0;
-                        double sgb = 0.0;               // dV/dsigma_j(t) on this path
+                        real sgb = (real)0;             // dV/dsigma_j(t) on this path
 #pragma unroll
                         for (int k = 0; k < FCAP; k++) {
                             Sb[k] = Sb[k] + mb * R.lam(k);
                             wb = wb + Sb[k] * R.lam(k);
                             const real lamb = Yb * dW[k] + mb * S[k] + Sb[k] * w;
-                            sgb += (double)lamb * fsig[j * FCAP + k];
+                            sgb = sgb + lamb * fsig[j * FCAP + k];
                             S[k] = S[k] - w * R.lam(k);    // S^(j) -> S^(j-1)
                         }
-                        if (lognormal) sgb += (double)(mb * (real)(-0.5)) * (2.0 * sig[(size_t)t * n + j] * cdiag[j]);
+                        // variance sigma^2 c in the drift: d(-var/2)/dsigma = -sigma c
+                        if (lognormal) sgb = sgb + mb * vterm[(size_t)t * n + j];
                         const real dwdL = lognormal ? R.c2() * r * r : -R.c1() * r * r;
                         Lb[j] = Lb[j] + wb * dwdL;
-                        const double s = wave_reduce_sum(active ? sgb : 0.0);
-                        if (lane == 0 && cnt > 0) wave_grad[(size_t)wave * gstride + (size_t)t * n + j] = s;
+                        const double s = wave_sum_dpp(active ? (double)sgb : 0.0);
+                        if (lane == 63 && cnt > 0) wave_grad[(size_t)wave * gstride + (size_t)t * n + j] = s;
                     }
                 }
             }
@@ -221,8 +242,8 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
 #pragma unroll
     for (int j = 0; j < NCAP; j++) {
         if (j < n) {
-            const double s = wave_reduce_sum(active ? (double)Lb[j] : 0.0);
-            if (lane == 0 && cnt > 0) wave_grad[(size_t)wave * gstride + (size_t)nT * n + j] = s;
+            const double s = wave_sum_dpp(active ? (double)Lb[j] : 0.0);
+            if (lane == 63 && cnt > 0) wave_grad[(size_t)wave * gstride + (size_t)nT * n + j] = s;
         }
     }
 }
@@ -233,7 +254,7 @@ void launch_adjoint_cap(const StepArgs& a, const AdjArgs& g, hipStream_t stream)
     hipLaunchKernelGGL((lmm_adjoint_kernel<NCAP, FCAP, real>), dim3((unsigned)blocks), dim3(kStepThreads), 0, stream,
                        a, g, (const real*)a.rec, a.delta, a.times, a.first_alive,
                        g.kcur_at, a.products, a.periods, (const real*)a.dw, (const real*)a.ckpt, a.numeraire,
-                       g.weight, g.fsig, g.sig, g.cdiag, g.wave_grad);
+                       g.weight, (const real*)g.fsig, (const real*)g.vterm, g.wave_grad);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -37,183 +37,236 @@ __device__ __forceinline__ double adjusted_numeraire(const BermBasisRef& r, doub
     return N;
 }
 
-// Basis functions of one exercise date for path p; returns their number.
-__device__ __forceinline__ int berm_basis(const BermBasisRef& r, const double* __restrict__ contrib, int P,
-                                          int64_t p, double (&b)[kBermMaxBasis]) {
-#pragma unroll
-    for (int k = 0; k < kBermMaxBasis; k++) b[k] = 0.0;
+// Basis functions of one exercise date for path p.
+template <int KIND>
+__device__ __forceinline__ void berm_basis(const BermBasisRef& r, const double* __restrict__ contrib, int P,
+                                           int64_t p, double (&b)[KIND == FM_BASIS_REFERENCE ? 6 : 4]) {
     b[0] = 1.0;
-    if (r.kind == FM_BASIS_REFERENCE) {
+    if (KIND == FM_BASIS_REFERENCE) {
         const double ds = contrib[(size_t)r.col0 * P + p];
         const double dl = contrib[(size_t)(r.col0 + 1) * P + p];
         const double N = adjusted_numeraire(r, contrib[(size_t)(r.col0 + 2) * P + p]);
         b[1] = ds; b[2] = ds * ds; b[3] = dl; b[4] = dl * dl; b[5] = 1.0 / N;
-        return 6;
+    } else {
+        const double S = contrib[(size_t)r.col0 * P + p];
+        b[1] = S; b[2] = S * S; b[3] = adjusted_numeraire(r, contrib[(size_t)(r.col0 + 1) * P + p]);
     }
-    const double S = contrib[(size_t)r.col0 * P + p];
-    b[1] = S; b[2] = S * S; b[3] = adjusted_numeraire(r, contrib[(size_t)(r.col0 + 1) * P + p]);
-    return 4;
 }
 
+// Grid-stride over paths: every thread handles the paths p = g, g + G, ... (G threads in
+// the grid) and keeps its regression partials in registers; each workgroup writes one
+// row of partials (fixed-order wave and workgroup reductions).
+template <int KIND>
 __global__ void __launch_bounds__(kBermThreads)
 berm_step_kernel(const BermStep s)
 {
+    constexpr int K = KIND == FM_BASIS_REFERENCE ? 6 : 4;
+    constexpr int NP = K + K * (K + 1) / 2;
     const int P = s.P;
-    const int64_t p = (int64_t)blockIdx.x * kBermThreads + threadIdx.x;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int64_t wave = p >> 6;
-    const bool active = p < P;
-    const int64_t pc = active ? p : (int64_t)P - 1;
-    const int cnt = (int)((int64_t)P - wave * 64 > 64 ? 64 : ((int64_t)P - wave * 64));
-
-    double v = s.first ? 0.0 : s.values[pc];
-    double u = s.first ? 0.0 : s.under[pc];
-    if (s.apply) {
-        double b[kBermMaxBasis];
-        const int K = berm_basis(s.apply_basis, s.contrib, P, pc, b);
-        // getConditionalExpectation (MonteCarloConditionalExpectationRegression.java:79-91)
-        double t = b[0] * s.apply_coef[0];
-        for (int k = 1; k < K; k++) t = t + b[k] * s.apply_coef[k];
-        // choose (RandomVariableFromDoubleArray.java:1342-1363): trigger >= 0 keeps the values
-        if (!(t >= 0.0)) {
-            v = u;
-            if (s.ex_index && active) s.ex_index[p] = s.apply_period;
+    double acc[NP];
+#pragma unroll
+    for (int e = 0; e < NP; e++) acc[e] = 0.0;
+    const int64_t G = (int64_t)gridDim.x * kBermThreads;
+    // whole wavefronts stay together so the slot reduction sees aligned groups of 64 paths
+    for (int64_t p0 = (int64_t)blockIdx.x * kBermThreads; p0 < P; p0 += G) {
+        const int64_t p = p0 + threadIdx.x;
+        const int64_t wave = p >> 6;
+        const bool active = p < P;
+        const int64_t pc = active ? p : (int64_t)P - 1;
+        const int cnt = (int)((int64_t)P - wave * 64 > 64 ? 64 : ((int64_t)P - wave * 64));
+        double v = s.first ? 0.0 : s.values[pc];
+        double u = s.first ? 0.0 : s.under[pc];
+        if (s.apply) {
+            double b[K];
+            berm_basis<KIND>(s.apply_basis, s.contrib, P, pc, b);
+            // getConditionalExpectation (MonteCarloConditionalExpectationRegression.java:79-91)
+            double t = b[0] * s.apply_coef[0];
+#pragma unroll
+            for (int k = 1; k < K; k++) t = t + b[k] * s.apply_coef[k];
+            // choose (RandomVariableFromDoubleArray.java:1342-1363): trigger >= 0 keeps the values
+            if (!(t >= 0.0)) {
+                v = u;
+                if (s.ex_index && active) s.ex_index[p] = s.apply_period;
+            }
         }
-    }
-    for (int q = 0; q < s.num_per; q++) {
-        double x = s.contrib[(size_t)s.per_col[q] * P + pc] * s.per_scale[q];
-        x = x * s.w;
-        if (s.callable) u = u + x; else v = v + x;
+        for (int q = 0; q < s.num_per; q++) {
+            double x = s.contrib[(size_t)s.per_col[q] * P + pc] * s.per_scale[q];
+            x = x * s.w;
+            if (s.callable) u = u + x; else v = v + x;
+        }
+        if (s.regress && active) {
+            double b[K];
+            berm_basis<KIND>(s.reg_basis, s.contrib, P, pc, b);
+            const double y = v - u;
+            int e = 0;
+#pragma unroll
+            for (int i = 0; i < K; i++, e++) acc[e] += y * b[i];
+#pragma unroll
+            for (int i = 0; i < K; i++)
+#pragma unroll
+                for (int j = i; j < K; j++, e++) acc[e] += b[i] * b[j];
+        }
+        if (s.final_) {
+            const double val = (v * s.n0) / s.w;
+            reduce_slot(val, active, cnt, lane, wave, s.wave_sum, s.wave_m2, s.stride, s.slot);
+        } else if (active) {
+            s.values[p] = v;
+            s.under[p] = u;
+        }
     }
     if (s.regress) {
-        __shared__ double part[kBermThreads / 64][kBermParts];
-        double b[kBermMaxBasis];
-        const int K = berm_basis(s.reg_basis, s.contrib, P, pc, b);
-        const double y = v - u;
-        int e = 0;
-        for (int i = 0; i < K; i++, e++) {
-            const double r = wave_reduce_sum(active ? y * b[i] : 0.0);
+        __shared__ double part[kBermThreads / 64][NP];
+#pragma unroll
+        for (int e = 0; e < NP; e++) {
+            const double r = wave_reduce_sum(acc[e]);
             if (lane == 0) part[wid][e] = r;
         }
-        for (int i = 0; i < K; i++)
-            for (int j = i; j < K; j++, e++) {
-                const double r = wave_reduce_sum(active ? b[i] * b[j] : 0.0);
-                if (lane == 0) part[wid][e] = r;
-            }
         __syncthreads();
-        if ((int)threadIdx.x < e) {
-            double acc = 0.0;
-            for (int w = 0; w < kBermThreads / 64; w++) acc += part[w][threadIdx.x];
-            s.partials[(size_t)blockIdx.x * kBermParts + threadIdx.x] = acc;
+        if ((int)threadIdx.x < NP) {
+            double t = 0.0;
+            for (int w = 0; w < kBermThreads / 64; w++) t += part[w][threadIdx.x];
+            s.partials[(size_t)blockIdx.x * kBermParts + threadIdx.x] = t;
         }
-    }
-    if (s.final_) {
-        const double val = (v * s.n0) / s.w;
-        reduce_slot(val, active, cnt, lane, wave, s.wave_sum, s.wave_m2, s.stride, s.slot);
-    } else if (active) {
-        s.values[p] = v;
-        s.under[p] = u;
     }
 }
 
-// One workgroup: fixed-order reduction of the workgroup partials, then (thread 0)
-// the regression solve in FP64.
-__global__ void __launch_bounds__(kBermThreads)
-berm_solve_kernel(const double* __restrict__ partials, int num_blocks, int K, int P, double* __restrict__ coef)
-{
-    __shared__ double red[kBermThreads];
-    __shared__ double tot[kBermParts];
-    __shared__ double A[kBermMaxBasis * kBermMaxBasis], V[kBermMaxBasis * kBermMaxBasis];
-    const int nparts = K + K * (K + 1) / 2;
-    const int t = threadIdx.x;
-    for (int e = 0; e < nparts; e++) {
-        // Neumaier sum of a strided subset, then a fixed tree
-        double sum = 0.0, c = 0.0;
-        for (int b = t; b < num_blocks; b += kBermThreads) {
-            const double x = partials[(size_t)b * kBermParts + e];
-            const double tt = sum + x;
-            if (fabs(sum) >= fabs(x)) c += (sum - tt) + x; else c += (x - tt) + sum;
-            sum = tt;
-        }
-        red[t] = sum + c;
-        __syncthreads();
-        for (int m = kBermThreads / 2; m >= 1; m >>= 1) {
-            if (t < m) red[t] = red[t] + red[t + m];
-            __syncthreads();
-        }
-        if (t == 0) tot[e] = red[0];
-        __syncthreads();
-    }
-    if (t != 0) return;
-    double rhs[kBermMaxBasis], x[kBermMaxBasis], ev[kBermMaxBasis];
-    int e = 0;
-    for (int i = 0; i < K; i++, e++) rhs[i] = tot[e] / (double)P;
+// Pseudo-inverse solve of the K x K Gram system by cyclic Jacobi rotations
+// (Golub & Van Loan 8.5), all in registers (K is a compile-time constant, so every
+// index is static). Singular values at or below the SVD solver's threshold
+// max(K sigma_max 2^-52, sqrt(2^-1022)) are dropped.
+template <int K>
+__device__ __forceinline__ void jacobi_pinv_solve(const double* tot, int P, double* __restrict__ coef) {
+    double A[K][K], V[K][K], rhs[K], x[K];
+    int q = 0;
+#pragma unroll
+    for (int i = 0; i < K; i++, q++) rhs[i] = tot[q] / (double)P;
+#pragma unroll
     for (int i = 0; i < K; i++)
-        for (int j = i; j < K; j++, e++) {
-            const double g = tot[e] / (double)P;
-            A[i * K + j] = g;
-            A[j * K + i] = g;
+#pragma unroll
+        for (int j = i; j < K; j++, q++) {
+            const double g = tot[q] / (double)P;
+            A[i][j] = g;
+            A[j][i] = g;
         }
-    // cyclic Jacobi rotations (Golub & Van Loan 8.5); V accumulates the eigenvectors
+#pragma unroll
     for (int i = 0; i < K; i++)
-        for (int j = 0; j < K; j++) V[i * K + j] = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+        for (int j = 0; j < K; j++) V[i][j] = (i == j) ? 1.0 : 0.0;
     for (int sweep = 0; sweep < 100; sweep++) {
         double off = 0.0;
+#pragma unroll
         for (int i = 0; i < K; i++)
-            for (int j = i + 1; j < K; j++) off += A[i * K + j] * A[i * K + j];
+#pragma unroll
+            for (int j = i + 1; j < K; j++) off += A[i][j] * A[i][j];
         if (off < 1e-300) break;
+#pragma unroll
         for (int pp = 0; pp < K; pp++)
-            for (int q = pp + 1; q < K; q++) {
-                const double apq = A[pp * K + q];
+#pragma unroll
+            for (int qq = pp + 1; qq < K; qq++) {
+                const double apq = A[pp][qq];
                 if (fabs(apq) < 1e-300) continue;
-                const double app = A[pp * K + pp], aqq = A[q * K + q];
+                const double app = A[pp][pp], aqq = A[qq][qq];
                 const double theta = (aqq - app) / (2.0 * apq);
                 const double tn = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
                 const double cs = 1.0 / sqrt(tn * tn + 1.0), sn = tn * cs;
+#pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const double akp = A[k * K + pp], akq = A[k * K + q];
-                    A[k * K + pp] = cs * akp - sn * akq;
-                    A[k * K + q] = sn * akp + cs * akq;
+                    const double akp = A[k][pp], akq = A[k][qq];
+                    A[k][pp] = cs * akp - sn * akq;
+                    A[k][qq] = sn * akp + cs * akq;
                 }
+#pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const double apk = A[pp * K + k], aqk = A[q * K + k];
-                    A[pp * K + k] = cs * apk - sn * aqk;
-                    A[q * K + k] = sn * apk + cs * aqk;
+                    const double apk = A[pp][k], aqk = A[qq][k];
+                    A[pp][k] = cs * apk - sn * aqk;
+                    A[qq][k] = sn * apk + cs * aqk;
                 }
+#pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const double vkp = V[k * K + pp], vkq = V[k * K + q];
-                    V[k * K + pp] = cs * vkp - sn * vkq;
-                    V[k * K + q] = sn * vkp + cs * vkq;
+                    const double vkp = V[k][pp], vkq = V[k][qq];
+                    V[k][pp] = cs * vkp - sn * vkq;
+                    V[k][qq] = sn * vkp + cs * vkq;
                 }
             }
     }
     double smax = 0.0;
-    for (int i = 0; i < K; i++) { ev[i] = A[i * K + i]; smax = fmax(smax, fabs(ev[i])); x[i] = 0.0; }
-    // singular values at or below the SVD solver's threshold are dropped
+#pragma unroll
+    for (int i = 0; i < K; i++) { smax = fmax(smax, fabs(A[i][i])); x[i] = 0.0; }
     const double tol = fmax(K * smax * 0x1.0p-52, sqrt(0x1.0p-1022));
+#pragma unroll
     for (int i = 0; i < K; i++) {
-        if (!(fabs(ev[i]) > tol)) continue;
+        if (!(fabs(A[i][i]) > tol)) continue;
         double d = 0.0;
-        for (int r = 0; r < K; r++) d += V[r * K + i] * rhs[r];
-        d /= ev[i];
-        for (int r = 0; r < K; r++) x[r] += V[r * K + i] * d;
+#pragma unroll
+        for (int r = 0; r < K; r++) d += V[r][i] * rhs[r];
+        d /= A[i][i];
+#pragma unroll
+        for (int r = 0; r < K; r++) x[r] += V[r][i] * d;
     }
+#pragma unroll
     for (int i = 0; i < kBermMaxBasis; i++) coef[i] = (i < K) ? x[i] : 0.0;
+}
+
+// One workgroup: fixed-order reduction of the workgroup rows (entry e = t % nparts,
+// rows c, c + C, ... with c = t / nparts), then (thread 0) the regression solve in FP64.
+template <int K>
+__global__ void __launch_bounds__(kBermThreads)
+berm_solve_kernel(const double* __restrict__ partials, int num_rows, int P, double* __restrict__ coef)
+{
+    constexpr int nparts = K + K * (K + 1) / 2;
+    constexpr int C = kBermThreads / nparts;
+    __shared__ double red[C * nparts];
+    __shared__ double tot[nparts];
+    const int t = threadIdx.x;
+    const int e = t % nparts, ch = t / nparts;
+    if (ch < C) {
+        double sum = 0.0, c = 0.0;
+#pragma unroll 8
+        for (int r = ch; r < num_rows; r += C) {
+            const double x = partials[(size_t)r * kBermParts + e];
+            const double tt = sum + x;
+            if (fabs(sum) >= fabs(x)) c += (sum - tt) + x; else c += (x - tt) + sum;
+            sum = tt;
+        }
+        red[ch * nparts + e] = sum + c;
+    }
+    __syncthreads();
+    if (t < nparts) {
+        double sum = 0.0;
+        for (int c2 = 0; c2 < C; c2++) sum += red[c2 * nparts + t];
+        tot[t] = sum;
+    }
+    __syncthreads();
+    if (t == 0) jacobi_pinv_solve<K>(tot, P, coef);
 }
 
 } // namespace
 
-int berm_num_blocks(int P) { return (int)(((int64_t)P + kBermThreads - 1) / kBermThreads); }
+// at most 1024 workgroups (4 per CU); each thread then handles ceil(P / 262144) paths
+int berm_num_blocks(int P) {
+    const int64_t b = ((int64_t)P + kBermThreads - 1) / kBermThreads;
+    return (int)(b < 1024 ? b : 1024);
+}
 
-void launch_berm_step(const BermStep& s, void* stream) {
-    hipLaunchKernelGGL(berm_step_kernel, dim3((unsigned)berm_num_blocks(s.P)), dim3(kBermThreads), 0,
-                       (hipStream_t)stream, s);
+void launch_berm_step(const BermStep& s, int kind, void* stream) {
+    const dim3 grid((unsigned)berm_num_blocks(s.P));
+    if (kind == FM_BASIS_REFERENCE)
+        hipLaunchKernelGGL(berm_step_kernel<FM_BASIS_REFERENCE>, grid, dim3(kBermThreads), 0, (hipStream_t)stream, s);
+    else
+        hipLaunchKernelGGL(berm_step_kernel<FM_BASIS_SWAP_RATE>, grid, dim3(kBermThreads), 0, (hipStream_t)stream, s);
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_berm_solve(const double* partials, int num_blocks, int K, int P, double* coef, void* stream) {
-    hipLaunchKernelGGL(berm_solve_kernel, dim3(1), dim3(kBermThreads), 0, (hipStream_t)stream,
-                       partials, num_blocks, K, P, coef);
+void launch_berm_solve(const double* partials, int num_rows, int K, int P, double* coef, void* stream) {
+    if (K == 6)
+        hipLaunchKernelGGL(berm_solve_kernel<6>, dim3(1), dim3(kBermThreads), 0, (hipStream_t)stream,
+                           partials, num_rows, P, coef);
+    else if (K == 4)
+        hipLaunchKernelGGL(berm_solve_kernel<4>, dim3(1), dim3(kBermThreads), 0, (hipStream_t)stream,
+                           partials, num_rows, P, coef);
+    else
+        throw FmError(FM_ERR_INTERNAL, "regression basis size " + std::to_string(K) + " has no solve kernel");
     HIP_CHECK(hipGetLastError());
 }
 

@@ -3,7 +3,7 @@
 // and the Brownian increments of BrownianMotionFromMersenneRandomNumbers
 // (montecarlo/BrownianMotionFromMersenneRandomNumbers.java:141-191):
 //   u   = ((w0 >> 6) << 26 | (w1 >> 6)) * 2^-52   (nextDouble, two words)
-//   dW  = icdf_as241(u) * sqrt(dt_t)               (functions/NormalDistribution.java:67-162)
+//   dW  = AS241(u) * sqrt(dt_t)                    (functions/NormalDistribution.java:67-162)
 // written in the reference's generation order (path, then time, then factor).
 
 #include <hip/hip_runtime.h>
@@ -92,78 +92,96 @@ jump_level_kernel(uint32_t* __restrict__ states, const uint64_t* __restrict__ po
 // ping-ponged between two LDS buffers; a twist runs in three dependency phases
 // (words [0,227) need only old words, [227,454) need new [0,227), [454,624) need
 // new [227,397) and new word 0). 312 threads then each temper one word pair into a
-// uniform, apply AS241 and store one increment.
+// uniform; central-branch draws are transformed and stored at once, tail-branch draws
+// go to an LDS queue that all threads then work through, so the tail code runs in
+// about one wavefront per twist instead of in every wavefront.
 // ---------------------------------------------------------------------------
 constexpr int kGenThreads = 320;
 
-__device__ __forceinline__ double icdf_as241(double p) {
-    // Wichura AS241 (PPND16), as functions/NormalDistribution.java:67-162; the
-    // published coefficient table. p <= 0 (or >= 1) in the tail branch gives 0.
-    const double q = p - 0.5;
-    if (fabs(q) <= 0.425) {
-        const double r = 0.180625 - q * q;
-        double num = 2.5090809287301226727e+03;
-        num = num * r + 3.3430575583588128105e+04;
-        num = num * r + 6.7265770927008700853e+04;
-        num = num * r + 4.5921953931549871457e+04;
-        num = num * r + 1.3731693765509461125e+04;
-        num = num * r + 1.9715909503065514427e+03;
-        num = num * r + 1.3314166789178437745e+02;
-        num = num * r + 3.3871328727963666080e+00;
-        double den = 5.2264952788528545610e+03;
-        den = den * r + 2.8729085735721942674e+04;
-        den = den * r + 3.9307895800092710610e+04;
-        den = den * r + 2.1213794301586595867e+04;
-        den = den * r + 5.3941960214247511077e+03;
-        den = den * r + 6.8718700749205790830e+02;
-        den = den * r + 4.2313330701600911252e+01;
-        den = den * r + 1.0;
-        return q * num / den;
-    }
-    double r = (q < 0.0) ? p : 1.0 - p;
-    if (r <= 0.0) return 0.0;
-    r = sqrt(-log(r));
-    double num, den;
-    if (r <= 5.0) {
-        r -= 1.6;
-        num = 7.74545014278341407640e-04;
-        num = num * r + 2.27238449892691845833e-02;
-        num = num * r + 2.41780725177450611770e-01;
-        num = num * r + 1.27045825245236838258e+00;
-        num = num * r + 3.64784832476320460504e+00;
-        num = num * r + 5.76949722146069140550e+00;
-        num = num * r + 4.63033784615654529590e+00;
-        num = num * r + 1.42343711074968357734e+00;
-        den = 1.05075007164441684324e-09;
-        den = den * r + 5.47593808499534494600e-04;
-        den = den * r + 1.51986665636164571966e-02;
-        den = den * r + 1.48103976427480074590e-01;
-        den = den * r + 6.89767334985100004550e-01;
-        den = den * r + 1.67638483018380384940e+00;
-        den = den * r + 2.05319162663775882187e+00;
-        den = den * r + 1.0;
+// Wichura AS241 (PPND16), as functions/NormalDistribution.java:67-162; the
+// published coefficient table. Split into the central branch |q| <= 0.425 and the
+// tail branch so the generator can run the tail densely (about 15 % of the draws).
+// p <= 0 (or >= 1) in the tail branch gives 0.
+template <typename R> __device__ __forceinline__ R as241_central(R q) {
+    const R r = (R)0.180625 - q * q;
+    R num = (R)2.5090809287301226727e+03;
+    num = num * r + (R)3.3430575583588128105e+04;
+    num = num * r + (R)6.7265770927008700853e+04;
+    num = num * r + (R)4.5921953931549871457e+04;
+    num = num * r + (R)1.3731693765509461125e+04;
+    num = num * r + (R)1.9715909503065514427e+03;
+    num = num * r + (R)1.3314166789178437745e+02;
+    num = num * r + (R)3.3871328727963666080e+00;
+    R den = (R)5.2264952788528545610e+03;
+    den = den * r + (R)2.8729085735721942674e+04;
+    den = den * r + (R)3.9307895800092710610e+04;
+    den = den * r + (R)2.1213794301586595867e+04;
+    den = den * r + (R)5.3941960214247511077e+03;
+    den = den * r + (R)6.8718700749205790830e+02;
+    den = den * r + (R)4.2313330701600911252e+01;
+    den = den * r + (R)1.0;
+    return q * num / den;
+}
+
+template <typename R> __device__ __forceinline__ R dev_sqrt_neg_log(double r);
+template <> __device__ __forceinline__ double dev_sqrt_neg_log<double>(double r) { return sqrt(-log(r)); }
+template <> __device__ __forceinline__ float dev_sqrt_neg_log<float>(double r) { return sqrtf(-logf((float)r)); }
+
+// tail branch; p is the uniform, q = p - 0.5 with |q| > 0.425
+template <typename R> __device__ __forceinline__ R as241_tail(double p, double q) {
+    const double rp = (q < 0.0) ? p : 1.0 - p;
+    if (rp <= 0.0) return (R)0;
+    R r = dev_sqrt_neg_log<R>(rp);
+    R num, den;
+    if (r <= (R)5.0) {
+        r -= (R)1.6;
+        num = (R)7.74545014278341407640e-04;
+        num = num * r + (R)2.27238449892691845833e-02;
+        num = num * r + (R)2.41780725177450611770e-01;
+        num = num * r + (R)1.27045825245236838258e+00;
+        num = num * r + (R)3.64784832476320460504e+00;
+        num = num * r + (R)5.76949722146069140550e+00;
+        num = num * r + (R)4.63033784615654529590e+00;
+        num = num * r + (R)1.42343711074968357734e+00;
+        den = (R)1.05075007164441684324e-09;
+        den = den * r + (R)5.47593808499534494600e-04;
+        den = den * r + (R)1.51986665636164571966e-02;
+        den = den * r + (R)1.48103976427480074590e-01;
+        den = den * r + (R)6.89767334985100004550e-01;
+        den = den * r + (R)1.67638483018380384940e+00;
+        den = den * r + (R)2.05319162663775882187e+00;
+        den = den * r + (R)1.0;
     } else {
-        r -= 5.0;
-        num = 2.01033439929228813265e-07;
-        num = num * r + 2.71155556874348757815e-05;
-        num = num * r + 1.24266094738807843860e-03;
-        num = num * r + 2.65321895265761230930e-02;
-        num = num * r + 2.96560571828504891230e-01;
-        num = num * r + 1.78482653991729133580e+00;
-        num = num * r + 5.46378491116411436990e+00;
-        num = num * r + 6.65790464350110377720e+00;
-        den = 2.04426310338993978564e-15;
-        den = den * r + 1.42151175831644588870e-07;
-        den = den * r + 1.84631831751005468180e-05;
-        den = den * r + 7.86869131145613259100e-04;
-        den = den * r + 1.48753612908506148525e-02;
-        den = den * r + 1.36929880922735805310e-01;
-        den = den * r + 5.99832206555887937690e-01;
-        den = den * r + 1.0;
+        r -= (R)5.0;
+        num = (R)2.01033439929228813265e-07;
+        num = num * r + (R)2.71155556874348757815e-05;
+        num = num * r + (R)1.24266094738807843860e-03;
+        num = num * r + (R)2.65321895265761230930e-02;
+        num = num * r + (R)2.96560571828504891230e-01;
+        num = num * r + (R)1.78482653991729133580e+00;
+        num = num * r + (R)5.46378491116411436990e+00;
+        num = num * r + (R)6.65790464350110377720e+00;
+        den = (R)2.04426310338993978564e-15;
+        den = den * r + (R)1.42151175831644588870e-07;
+        den = den * r + (R)1.84631831751005468180e-05;
+        den = den * r + (R)7.86869131145613259100e-04;
+        den = den * r + (R)1.48753612908506148525e-02;
+        den = den * r + (R)1.36929880922735805310e-01;
+        den = den * r + (R)5.99832206555887937690e-01;
+        den = den * r + (R)1.0;
     }
-    double v = num / den;
+    const R v = num / den;
     return (q < 0.0) ? -v : v;
 }
+
+// Arithmetic of the inverse normal per output type: FP64 increments use the FP64
+// transform (bit-exact with the reference's); FP32 increments use the same formula in
+// float (about 1e-6 relative).
+template <typename OutT> struct RealOf { using type = double; };
+template <> struct RealOf<float> { using type = float; };
+template <typename OutT> using Real = typename RealOf<OutT>::type;
+template <typename OutT> __device__ __forceinline__ OutT icdf_out(double x, double sq) { return (OutT)(x * sq); }
+template <typename OutT> __device__ __forceinline__ OutT icdf_out(float x, double sq) { return x * (float)sq; }
 
 // Twist buffer `cur` (old window) into `nxt` (new window). All kGenThreads threads call.
 __device__ __forceinline__ void twist_block(const uint32_t* cur, uint32_t* nxt, int t) {
@@ -188,14 +206,22 @@ brownian_kernel(const uint32_t* __restrict__ states, uint64_t total_words, int l
                 int nT, int F, const double* __restrict__ sqrt_dt, OutT* __restrict__ out)
 {
     __shared__ uint32_t buf[2][N];
+    // tail draws of the current twist, run densely after the central ones
+    __shared__ double tail_u[N / 2];
+    __shared__ uint16_t tail_slot[N / 2], tail_ti[N / 2];
+    __shared__ int tail_n;
     const int t = threadIdx.x;
     const int64_t s = blockIdx.x;
     const uint64_t w_begin = (uint64_t)s << log2_len;
     uint64_t w_end = w_begin + ((uint64_t)1 << log2_len);
     if (w_end > total_words) w_end = total_words;
     for (int w = t; w < N; w += kGenThreads) buf[0][w] = states[(size_t)s * N + w];
+    if (t == 0) tail_n = 0;
     __syncthreads();
     const uint32_t per = (uint32_t)nT * (uint32_t)F;      // doubles per path
+    // position of this thread's double within its path, advanced by 312 per twist
+    uint32_t rem = (uint32_t)(((w_begin >> 1) + (uint64_t)t) % per);
+    const uint32_t step = (uint32_t)((N / 2) % per);
     int c = 0;
     for (uint64_t w = w_begin; w < w_end; w += N) {
         twist_block(buf[c], buf[c ^ 1], t);
@@ -207,11 +233,27 @@ brownian_kernel(const uint32_t* __restrict__ states, uint64_t total_words, int l
             const uint32_t b = temper(buf[c][2 * t + 1]);
             const uint64_t bits = ((uint64_t)(a >> 6) << 26) | (uint64_t)(b >> 6);
             const double u = (double)bits * 0x1.0p-52;
-            const uint64_t D = wd >> 1;                    // global double index
-            const uint32_t rem = (uint32_t)(D % per);
             const int ti = (int)(rem / (uint32_t)F);
-            out[D] = (OutT)(icdf_as241(u) * sqrt_dt[ti]);
+            const double q = u - 0.5;
+            if (fabs(q) <= 0.425) {
+                out[wd >> 1] = icdf_out<OutT>(as241_central<Real<OutT>>((Real<OutT>)q), sqrt_dt[ti]);
+            } else {
+                const int k = atomicAdd(&tail_n, 1);
+                tail_u[k] = u;
+                tail_slot[k] = (uint16_t)t;
+                tail_ti[k] = (uint16_t)ti;
+            }
         }
+        __syncthreads();
+        const int nt = tail_n;
+        for (int k = t; k < nt; k += kGenThreads) {
+            const double u = tail_u[k];
+            out[(w >> 1) + tail_slot[k]] = icdf_out<OutT>(as241_tail<Real<OutT>>(u, u - 0.5), sqrt_dt[tail_ti[k]]);
+        }
+        __syncthreads();
+        if (t == 0) tail_n = 0;          // ordered before the next pushes by the twist's barriers
+        rem += step;
+        if (rem >= per) rem -= per;
         // the buffer just read is overwritten only after the next twist's first barrier
     }
 }

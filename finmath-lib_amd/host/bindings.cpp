@@ -197,6 +197,20 @@ PYBIND11_MODULE(_finmath_host, m) {
             return std::make_shared<Caplet>(mat, pl, k, dcf.is_none() ? pl : dcf.cast<double>(), floorlet);
         }), py::arg("maturity"), py::arg("periodLength"), py::arg("strike"), py::arg("daycountFraction") = py::none(),
             py::arg("isFloorlet") = false);
+    py::class_<RegressionBasisFunctionsProvider, std::shared_ptr<RegressionBasisFunctionsProvider>>(
+        m, "RegressionBasisFunctionsProvider");
+    py::class_<RegressionBasisFunctionsSwapRate, RegressionBasisFunctionsProvider,
+               std::shared_ptr<RegressionBasisFunctionsSwapRate>>(m, "RegressionBasisFunctionsSwapRate")
+        .def(py::init<>());
+    py::class_<BermudanSwaption, Product, std::shared_ptr<BermudanSwaption>>(m, "BermudanSwaption")
+        .def(py::init([](std::vector<bool> ex, std::vector<double> fix, std::vector<double> pl, std::vector<double> pay,
+                         std::vector<double> notionals, std::vector<double> k, bool callable,
+                         std::shared_ptr<RegressionBasisFunctionsProvider> prov) {
+            return std::make_shared<BermudanSwaption>(ex, fix, pl, pay, notionals, k, callable, prov);
+        }), py::arg("isPeriodStartDateExerciseDate"), py::arg("fixingDates"), py::arg("periodLength"),
+            py::arg("paymentDates"), py::arg("periodNotionals"), py::arg("swaprates"), py::arg("isCallable") = true,
+            py::arg("regressionBasisFunctionsProvider") = nullptr)
+        .def("getExerciseTimes", &BermudanSwaption::getExerciseTimes);
 
     py::class_<EulerSchemeFromProcessModel, std::shared_ptr<EulerSchemeFromProcessModel>>(m, "EulerSchemeFromProcessModel")
         .def(py::init([](std::shared_ptr<LIBORMarketModelFromCovarianceModel> model,

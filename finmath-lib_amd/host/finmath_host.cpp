@@ -639,4 +639,37 @@ void Caplet::lower(std::deque<std::vector<double>>& arena, fm_product& o) const 
     o.is_floorlet = isFloorlet ? 1 : 0;
 }
 
+BermudanSwaption::BermudanSwaption(std::vector<bool> exercise, std::vector<double> fix, std::vector<double> pl,
+                                   std::vector<double> pay, std::vector<double> notionals, std::vector<double> k,
+                                   bool callable, RegressionBasisFunctionsProviderPtr prov)
+    : fixingDates(std::move(fix)), periodLengths(std::move(pl)), paymentDates(std::move(pay)),
+      periodNotionals(std::move(notionals)), swaprates(std::move(k)), isCallable(callable), provider(std::move(prov)) {
+    const size_t m = fixingDates.size();
+    if (exercise.size() != m || periodLengths.size() != m || paymentDates.size() != m || swaprates.size() != m ||
+        (!periodNotionals.empty() && periodNotionals.size() != m))
+        throw IllegalArgumentException("Bermudan swaption arrays must have the same length");
+    for (bool e : exercise) exerciseFlags_.push_back(e ? 1 : 0);
+}
+
+std::vector<double> BermudanSwaption::getExerciseTimes() const {
+    std::vector<double> t;
+    for (size_t i = 0; i < fixingDates.size(); i++) if (exerciseFlags_[i]) t.push_back(fixingDates[i]);
+    return t;
+}
+
+void BermudanSwaption::lower(std::deque<std::vector<double>>& arena, fm_product& o) const {
+    std::memset(&o, 0, sizeof(o));
+    o.type = FM_PRODUCT_BERMUDAN_SWAPTION;
+    o.num_periods = (int32_t)fixingDates.size();
+    o.fixing_dates = keep(arena, fixingDates);
+    o.payment_dates = keep(arena, paymentDates);
+    o.period_lengths = keep(arena, periodLengths);
+    o.swaprates = keep(arena, swaprates);
+    o.notionals = periodNotionals.empty() ? nullptr : keep(arena, periodNotionals);
+    o.exercise_flags = exerciseFlags_.data();
+    o.is_cancelable = isCallable ? 0 : 1;
+    o.basis = provider ? provider->basisId() : FM_BASIS_REFERENCE;
+    o.notional = 1.0;
+}
+
 } // namespace finmath

@@ -356,4 +356,39 @@ public:
     bool isFloorlet;
 };
 
+// RegressionBasisFunctionsProvider
+// (montecarlo/conditionalexpectation/RegressionBasisFunctionsProvider.java): the engine
+// evaluates the regression basis on the GPU inside the path pass, so a provider names
+// one of the engine's bases.
+class RegressionBasisFunctionsProvider {
+public:
+    virtual ~RegressionBasisFunctionsProvider() = default;
+    virtual int basisId() const = 0;
+};
+using RegressionBasisFunctionsProviderPtr = std::shared_ptr<const RegressionBasisFunctionsProvider>;
+
+// Basis 1, S, S^2, N(T_e): S the par swap rate of the remaining periods at the exercise
+// date, N the numeraire.
+class RegressionBasisFunctionsSwapRate : public RegressionBasisFunctionsProvider {
+public:
+    int basisId() const override { return FM_BASIS_SWAP_RATE; }
+};
+
+// BermudanSwaption (montecarlo/interestrate/products/BermudanSwaption.java:49-157): without a
+// provider the basis is the product's own getBasisFunctions (:215-252).
+class BermudanSwaption : public Product {
+public:
+    BermudanSwaption(std::vector<bool> isPeriodStartDateExerciseDate, std::vector<double> fixingDates,
+                     std::vector<double> periodLengths, std::vector<double> paymentDates,
+                     std::vector<double> periodNotionals, std::vector<double> swaprates, bool isCallable = true,
+                     RegressionBasisFunctionsProviderPtr regressionBasisFunctionsProvider = nullptr);
+    void lower(std::deque<std::vector<double>>& arena, fm_product& out) const override;
+    std::vector<double> getExerciseTimes() const;                               // :259-268
+    std::vector<double> fixingDates, periodLengths, paymentDates, periodNotionals, swaprates;
+    bool isCallable;
+    RegressionBasisFunctionsProviderPtr provider;
+private:
+    std::vector<int32_t> exerciseFlags_;
+};
+
 } // namespace finmath
