@@ -102,13 +102,15 @@ def run_greeks(wl, steps, stride):
         sim.getValues(products)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    stage = np.zeros(5)
     for _ in range(steps):
         g = process.getGradient(products, [], stride)
+        stage += np.asarray(process.lastTimings())
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     price_ms = (t1 - t0) * 1e3 / steps
     greeks_ms = (t2 - t1) * 1e3 / steps
-    return price_ms, greeks_ms, g
+    return price_ms, greeks_ms, g, stage / steps
 
 
 def copy_probe(nbytes=1 << 30, reps=3):
@@ -220,13 +222,15 @@ def main():
             "err_bp_vs_fp64": float(abs(v4[0] - v4d[0]) * 1e4), "mc_stderr_bp": float(e4d[0] * 1e4)}
     if not args.no_greeks:
         wl3 = W.c3()
-        pm, gm, g = run_greeks(wl3, max(2, args.steps // 3), 4)
-        pm1, gm1, _ = run_greeks(wl3, max(2, args.steps // 3), 1)
+        pm, gm, g, gst = run_greeks(wl3, max(2, args.steps // 3), 4)
+        pm1, gm1, _, _ = run_greeks(wl3, max(2, args.steps // 3), 1)
         line["greeks"] = {
             "workload": "c3: c1 strip, 1,000,000 paths, FP32 paths, FP64 gradient sums; 40 deltas, 40 forward vegas, "
                         "4 vol-parameter vegas, dV/dsigma table",
             "pricing_ms": pm, "adjoint_ms_stride4": gm, "ratio_stride4": gm / pm,
             "adjoint_ms_stride1": gm1, "ratio_stride1": gm1 / pm1,
+            "adjoint_stage_ms_stride4": {"generator": gst[0], "forward_path_kernel": gst[1], "reverse_sweep": gst[3],
+                                         "call": gst[4]},
             "sum_delta": float(np.sum(g["delta"])), "parameter_vega": [float(x) for x in g["parameterVega"]]}
     if not args.no_cpu:
         cb = cpu_baseline(wl, args.cpu_paths)

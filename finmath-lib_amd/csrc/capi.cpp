@@ -346,6 +346,7 @@ Lowered lower_products(fm_lmm* c, const fm_product* prods, int count) {
                 pe.period_length = p.period_lengths ? p.period_lengths[i] : p.payment_dates[i] - p.fixing_dates[i];
                 pe.tau = p.payment_dates[i] - std::max(p.fixing_dates[i], Te);
                 pe.adj = p.discounting_adjustment ? p.discounting_adjustment[i] : 1.0;
+                pe.inv_adj = 1.0 / pe.adj;
                 pe.notional = p.notional;
                 pe.j = ji; pe.pay_k = jp;
                 L.per.push_back(pe);

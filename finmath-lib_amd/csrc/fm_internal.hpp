@@ -76,6 +76,7 @@ struct KProduct {
 // Per-period parameters (strike, period length, discounting tau, adjustment, notional)
 struct KPeriod {
     double strike, period_length, tau, adj, notional;
+    double inv_adj;       // 1 / adj
     int32_t j, pay_k;     // libor index and tenor index of the payment date
 };
 

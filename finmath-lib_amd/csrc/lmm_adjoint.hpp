@@ -41,6 +41,34 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
     return v;
 }
 
+// Wavefront sums of four per-lane values at once (a transposed butterfly): two exchange
+// steps halve the values a lane carries, four more sum across the wavefront. Lane l < 4
+// returns the total of value 2*(l & 1) + ((l >> 1) & 1); 7 lane exchanges for 4 sums.
+__device__ __forceinline__ double wave_sum4(const double (&v)[4], int lane) {
+    const bool b0 = (lane & 1) != 0, b1 = (lane & 2) != 0;
+    double a2[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const double send = b0 ? v[i] : v[i + 2];
+        const double keep = b0 ? v[i + 2] : v[i];
+        a2[i] = keep + __shfl_xor(send, 1, 64);
+    }
+    const double send = b1 ? a2[0] : a2[1];
+    const double keep = b1 ? a2[1] : a2[0];
+    double a = keep + __shfl_xor(send, 2, 64);
+    a = a + __shfl_xor(a, 4, 64);
+    a = a + __shfl_xor(a, 8, 64);
+    a = a + __shfl_xor(a, 16, 64);
+    a = a + __shfl_xor(a, 32, 64);
+    return a;
+}
+
+// Reciprocal for the adjoint-only terms: the hardware estimate in float (about 1 ulp),
+// the correctly rounded division in double.
+template <typename real> __device__ __forceinline__ real adj_rcp(real x);
+template <> __device__ __forceinline__ float adj_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double adj_rcp<double>(double x) { return 1.0 / x; }
+
 template <int NCAP, int FCAP, typename real>
 __global__ void __launch_bounds__(kStepThreads)
 __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 && NCAP <= 40 ? 3 : 1)))
@@ -103,7 +131,7 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
                     if (j < n && j >= first) {
                         const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, t, j);
                         const real Lj = L[j];
-                        real w = (real)1.0 / (Lj * R.c1() + R.c2());
+                        real w = adj_rcp<real>(Lj * R.c1() + R.c2());
                         if (lognormal) w = w * Lj;
 #pragma unroll
                         for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * R.lam(k);
@@ -120,13 +148,19 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
                         Lb[j] = lognormal ? Yb / Lj : Yb;     // direct part of the adjoint of L_j(t)
                     }
                 }
-                // pass B (descending): drift chain S_k = sum_{l<=j} w_l lambda_lk
+                // pass B (descending): drift chain S_k = sum_{l<=j} w_l lambda_lk; the
+                // per-path dV/dsigma_j(t) of four consecutive j are summed over the wavefront together
+                double gq[4];
 #pragma unroll
                 for (int j = NCAP - 1; j >= 0; j--) {
+                    if ((j & 3) == 3 || j == NCAP - 1) {
+#pragma unroll
+                        for (int s4 = 0; s4 < 4; s4++) gq[s4] = 0.0;
+                    }
                     if (j < n && j >= first) {
                         const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, t, j);
                         const real Lj = L[j];
-                        const real r = (real)1.0 / (Lj * R.c1() + R.c2());
+                        const real r = adj_rcp<real>(Lj * R.c1() + R.c2());
                         const real w = lognormal ? r * Lj : r;
                         // adjoint of Y_j(t+1) from the direct part left by pass A
                         const real Yb = lognormal ? Lb[j] * Lj : Lb[j];
@@ -145,8 +179,13 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
                         if (lognormal) sgb = sgb + mb * vterm[(size_t)t * n + j];
                         const real dwdL = lognormal ? R.c2() * r * r : -R.c1() * r * r;
                         Lb[j] = Lb[j] + wb * dwdL;
-                        const double s = wave_sum_dpp(active ? (double)sgb : 0.0);
-                        if (lane == 63 && cnt > 0) wave_grad[(size_t)wave * gstride + (size_t)t * n + j] = s;
+                        gq[j & 3] = active ? (double)sgb : 0.0;
+                    }
+                    if ((j & 3) == 0 && j < n && j + 3 >= first) {
+                        const double s = wave_sum4(gq, lane);
+                        const int jj = j + 2 * (lane & 1) + ((lane >> 1) & 1);
+                        if (lane < 4 && cnt > 0 && jj < n)
+                            wave_grad[(size_t)wave * gstride + (size_t)t * n + jj] = s;
                     }
                 }
             }
@@ -168,10 +207,9 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
                         if (j >= j0 && j < j1) {
                             const KPeriod pe = periods[pr.param_off + (j - j0)];
                             const real Lj = L[j];
-                            V = V + ((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pr.notional;
-                            V = V / ((real)1.0 + Lj * (real)pe.tau);
-                            V = V * (real)pe.adj;
-                            Pc = Pc * ((real)pe.adj / ((real)1.0 + Lj * (real)pe.tau));
+                            const real D = (real)pe.adj * adj_rcp<real>((real)1.0 + Lj * (real)pe.tau);
+                            V = (V + ((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pr.notional) * D;
+                            Pc = Pc * D;
                         }
                     }
                     if (V > (real)0) {
@@ -186,11 +224,12 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
                                 const KPeriod pe = periods[pr.param_off + (j - j0)];
                                 const real Lj = L[j];
                                 const real one = (real)1.0 + Lj * (real)pe.tau;
+                                const real inv = adj_rcp<real>(one);
                                 const real pj = ((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pr.notional;
                                 U = U + pj * Pc;
-                                const real dV = (real)pe.period_length * (real)pr.notional * Pc - ((real)pe.tau / one) * U;
+                                const real dV = (real)pe.period_length * (real)pr.notional * Pc - ((real)pe.tau * inv) * U;
                                 Lb[j] = Lb[j] + (real)Vb * dV;
-                                Pc = Pc / ((real)pe.adj / one);
+                                Pc = Pc * (one * (real)pe.inv_adj);      // P_{j-1} = P_j / D_j
                             }
                         }
                     }

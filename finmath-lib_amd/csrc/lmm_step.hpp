@@ -23,6 +23,7 @@
 
 #include <hip/hip_runtime.h>
 #include "fm_internal.hpp"
+#include "fm_math.hpp"
 
 namespace fm {
 
@@ -37,7 +38,7 @@ __device__ __forceinline__ double wave_reduce_sum(double v) {
 template <typename real> __device__ __forceinline__ real dev_exp(real x);
 template <typename real> __device__ __forceinline__ real dev_log(real x);
 template <> __device__ __forceinline__ double dev_exp<double>(double x) { return exp(x); }
-template <> __device__ __forceinline__ double dev_log<double>(double x) { return log(x); }
+template <> __device__ __forceinline__ double dev_log<double>(double x) { return fm_log(x); }
 template <> __device__ __forceinline__ float dev_exp<float>(float x) { return expf(x); }
 template <> __device__ __forceinline__ float dev_log<float>(float x) { return logf(x); }
 
@@ -144,6 +145,7 @@ __device__ __forceinline__ void load_increments(real (&dW)[FCAP], const real* __
 
 template <int NCAP, int FCAP, typename real, bool LOGSTATE = false>
 __global__ void __launch_bounds__(kStepThreads)
+__attribute__((amdgpu_waves_per_eu((LOGSTATE && NCAP == 40) ? 3 : 1)))
 lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* __restrict__ delta,
                  const double* __restrict__ times, const int32_t* __restrict__ first_alive,
                  const int32_t* __restrict__ tenor_ti, const KProduct* __restrict__ products,

@@ -200,27 +200,32 @@ __device__ __forceinline__ void twist_block(const uint32_t* cur, uint32_t* nxt, 
     __syncthreads();
 }
 
+// Tail draws wait in an LDS queue until a full round of kGenThreads is there (or the
+// sub-stream ends), so the tail code runs in full wavefronts.
+constexpr int kTailCap = N / 2 + kGenThreads;
+
 template <typename OutT>
 __global__ void __launch_bounds__(kGenThreads)
 brownian_kernel(const uint32_t* __restrict__ states, uint64_t total_words, int log2_len,
                 int nT, int F, const double* __restrict__ sqrt_dt, OutT* __restrict__ out)
 {
     __shared__ uint32_t buf[2][N];
-    // tail draws of the current twist, run densely after the central ones
-    __shared__ double tail_u[N / 2];
-    __shared__ uint16_t tail_slot[N / 2], tail_ti[N / 2];
+    __shared__ double tail_u[kTailCap];
+    __shared__ uint32_t tail_off[kTailCap];      // double index relative to the sub-stream's first
+    __shared__ uint16_t tail_ti[kTailCap];
     __shared__ int tail_n;
     const int t = threadIdx.x;
     const int64_t s = blockIdx.x;
     const uint64_t w_begin = (uint64_t)s << log2_len;
     uint64_t w_end = w_begin + ((uint64_t)1 << log2_len);
     if (w_end > total_words) w_end = total_words;
+    const uint64_t d_begin = w_begin >> 1;
     for (int w = t; w < N; w += kGenThreads) buf[0][w] = states[(size_t)s * N + w];
     if (t == 0) tail_n = 0;
     __syncthreads();
     const uint32_t per = (uint32_t)nT * (uint32_t)F;      // doubles per path
     // position of this thread's double within its path, advanced by 312 per twist
-    uint32_t rem = (uint32_t)(((w_begin >> 1) + (uint64_t)t) % per);
+    uint32_t rem = (uint32_t)((d_begin + (uint64_t)t) % per);
     const uint32_t step = (uint32_t)((N / 2) % per);
     int c = 0;
     for (uint64_t w = w_begin; w < w_end; w += N) {
@@ -240,20 +245,23 @@ brownian_kernel(const uint32_t* __restrict__ states, uint64_t total_words, int l
             } else {
                 const int k = atomicAdd(&tail_n, 1);
                 tail_u[k] = u;
-                tail_slot[k] = (uint16_t)t;
+                tail_off[k] = (uint32_t)((wd >> 1) - d_begin);
                 tail_ti[k] = (uint16_t)ti;
             }
         }
-        __syncthreads();
-        const int nt = tail_n;
-        for (int k = t; k < nt; k += kGenThreads) {
-            const double u = tail_u[k];
-            out[(w >> 1) + tail_slot[k]] = icdf_out<OutT>(as241_tail<Real<OutT>>(u, u - 0.5), sqrt_dt[tail_ti[k]]);
-        }
-        __syncthreads();
-        if (t == 0) tail_n = 0;          // ordered before the next pushes by the twist's barriers
         rem += step;
         if (rem >= per) rem -= per;
+        __syncthreads();
+        const int nt = tail_n;                   // uniform: read after the barrier
+        if (nt >= kGenThreads || w + N >= w_end) {
+            for (int k = t; k < nt; k += kGenThreads) {
+                const double u = tail_u[k];
+                out[d_begin + tail_off[k]] =
+                    icdf_out<OutT>(as241_tail<Real<OutT>>(u, u - 0.5), sqrt_dt[tail_ti[k]]);
+            }
+            __syncthreads();
+            if (t == 0) tail_n = 0;              // ordered before the next pushes by the twist's barriers
+        }
         // the buffer just read is overwritten only after the next twist's first barrier
     }
 }
@@ -342,6 +350,9 @@ void launch_jump_tree(const StreamLayout& lay, const uint32_t* seed_window, uint
 void launch_brownian(const StreamLayout& lay, const uint32_t* d_states, int nT, int F,
                      const double* d_sqrt_dt, void* out, int out_is_float, void* stream_) {
     hipStream_t stream = (hipStream_t)stream_;
+    // the tail queue keeps 32-bit offsets within a sub-stream and 16-bit time indices
+    if (lay.log2_len > 32 || nT > 65535)
+        throw FmError(FM_ERR_UNSUPPORTED, "stream layout outside the generator's index range");
     if (out_is_float)
         hipLaunchKernelGGL(brownian_kernel<float>, dim3((unsigned)lay.num_sub), dim3(kGenThreads), 0, stream,
                            d_states, lay.total_words, lay.log2_len, nT, F, d_sqrt_dt, (float*)out);
