@@ -200,70 +200,105 @@ __device__ __forceinline__ void twist_block(const uint32_t* cur, uint32_t* nxt, 
     __syncthreads();
 }
 
-// Tail draws wait in an LDS queue until a full round of kGenThreads is there (or the
-// sub-stream ends), so the tail code runs in full wavefronts.
-constexpr int kTailCap = N / 2 + kGenThreads;
+// ---------------------------------------------------------------------------
+// Brownian generator, one wavefront per sub-stream (four per workgroup, no workgroup
+// barriers). The wavefront twists its 624-word state in place in rounds of 64 words,
+// in increasing word order as the sequential algorithm does: every word of a round
+// reads the old k and k+1 and either the old k+397 or the new k-227 (written in an
+// earlier round), and all lanes read before any lane writes. It then tempers the new
+// block into 312 uniforms in five rounds. Central-branch draws are transformed and
+// stored at once; tail-branch draws are appended (ballot prefix) to the wavefront's
+// LDS queue and transformed 64 at a time.
+// ---------------------------------------------------------------------------
+constexpr int kGenWaves = 4;
+constexpr int kTailQ = 128;
+
+// order this wavefront's LDS accesses (the lanes of a wavefront execute together)
+__device__ __forceinline__ void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
 
 template <typename OutT>
-__global__ void __launch_bounds__(kGenThreads)
-brownian_kernel(const uint32_t* __restrict__ states, uint64_t total_words, int log2_len,
+__global__ void __launch_bounds__(64 * kGenWaves)
+brownian_kernel(const uint32_t* __restrict__ states, int64_t num_sub, uint64_t total_words, int log2_len,
                 int nT, int F, const double* __restrict__ sqrt_dt, OutT* __restrict__ out)
 {
-    __shared__ uint32_t buf[2][N];
-    __shared__ double tail_u[kTailCap];
-    __shared__ uint32_t tail_off[kTailCap];      // double index relative to the sub-stream's first
-    __shared__ uint16_t tail_ti[kTailCap];
-    __shared__ int tail_n;
-    const int t = threadIdx.x;
-    const int64_t s = blockIdx.x;
+    __shared__ uint32_t st[kGenWaves][N];
+    __shared__ double tq_u[kGenWaves][kTailQ];
+    __shared__ uint32_t tq_off[kGenWaves][kTailQ];     // double index relative to the sub-stream's first
+    __shared__ uint16_t tq_ti[kGenWaves][kTailQ];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t s = (int64_t)blockIdx.x * kGenWaves + wv;
+    if (s >= num_sub) return;                          // the whole wavefront leaves
+    uint32_t* mt = st[wv];
     const uint64_t w_begin = (uint64_t)s << log2_len;
     uint64_t w_end = w_begin + ((uint64_t)1 << log2_len);
     if (w_end > total_words) w_end = total_words;
     const uint64_t d_begin = w_begin >> 1;
-    for (int w = t; w < N; w += kGenThreads) buf[0][w] = states[(size_t)s * N + w];
-    if (t == 0) tail_n = 0;
-    __syncthreads();
-    const uint32_t per = (uint32_t)nT * (uint32_t)F;      // doubles per path
-    // position of this thread's double within its path, advanced by 312 per twist
-    uint32_t rem = (uint32_t)((d_begin + (uint64_t)t) % per);
-    const uint32_t step = (uint32_t)((N / 2) % per);
-    int c = 0;
+    for (int w = lane; w < N; w += 64) mt[w] = states[(size_t)s * N + w];
+    wave_lds_order();
+    const uint32_t per = (uint32_t)nT * (uint32_t)F;   // doubles per path
+    // position within its path of this lane's double in the current round
+    uint32_t rem = (uint32_t)((d_begin + (uint64_t)lane) % per);
+    const uint32_t step64 = 64u % per, step56 = (uint32_t)(N / 2 - 4 * 64) % per;
+    int count = 0;                                     // queued tail draws (uniform)
+    auto drain = [&](int nq) {
+        wave_lds_order();
+        for (int k = lane; k < nq; k += 64) {
+            const double u = tq_u[wv][k];
+            out[d_begin + tq_off[wv][k]] = icdf_out<OutT>(as241_tail<Real<OutT>>(u, u - 0.5), sqrt_dt[tq_ti[wv][k]]);
+        }
+        wave_lds_order();
+    };
     for (uint64_t w = w_begin; w < w_end; w += N) {
-        twist_block(buf[c], buf[c ^ 1], t);
-        c ^= 1;
-        // doubles d in [0,312) of this twist: words 2d, 2d+1
-        const uint64_t wd = w + 2 * (uint64_t)t;
-        if (t < N / 2 && wd < w_end) {
-            const uint32_t a = temper(buf[c][2 * t]);
-            const uint32_t b = temper(buf[c][2 * t + 1]);
-            const uint64_t bits = ((uint64_t)(a >> 6) << 26) | (uint64_t)(b >> 6);
-            const double u = (double)bits * 0x1.0p-52;
+        // in-place twist
+#pragma unroll 1
+        for (int k0 = 0; k0 < N; k0 += 64) {
+            const int k = k0 + lane;
+            uint32_t a = 0, b = 0, c = 0;
+            if (k < N) {
+                a = mt[k];
+                b = mt[(k + 1 < N) ? k + 1 : 0];
+                c = mt[(k + M < N) ? k + M : k + M - N];
+            }
+            wave_lds_order();
+            if (k < N) mt[k] = twist_word(a, b, c);
+            wave_lds_order();
+        }
+        // 312 uniforms: doubles d = 64 r + lane, words 2d and 2d + 1
+#pragma unroll 1
+        for (int r = 0; r < 5; r++) {
+            const int d = r * 64 + lane;
+            const uint64_t wd = w + 2 * (uint64_t)d;
+            bool tail = false;
+            double u = 0.0;
             const int ti = (int)(rem / (uint32_t)F);
-            const double q = u - 0.5;
-            if (fabs(q) <= 0.425) {
-                out[wd >> 1] = icdf_out<OutT>(as241_central<Real<OutT>>((Real<OutT>)q), sqrt_dt[ti]);
-            } else {
-                const int k = atomicAdd(&tail_n, 1);
-                tail_u[k] = u;
-                tail_off[k] = (uint32_t)((wd >> 1) - d_begin);
-                tail_ti[k] = (uint16_t)ti;
+            if (d < N / 2 && wd < w_end) {
+                const uint32_t a = temper(mt[2 * d]);
+                const uint32_t b = temper(mt[2 * d + 1]);
+                const uint64_t bits = ((uint64_t)(a >> 6) << 26) | (uint64_t)(b >> 6);
+                u = (double)bits * 0x1.0p-52;
+                const double q = u - 0.5;
+                if (fabs(q) <= 0.425)
+                    out[wd >> 1] = icdf_out<OutT>(as241_central<Real<OutT>>((Real<OutT>)q), sqrt_dt[ti]);
+                else
+                    tail = true;
             }
-        }
-        rem += step;
-        if (rem >= per) rem -= per;
-        __syncthreads();
-        const int nt = tail_n;                   // uniform: read after the barrier
-        if (nt >= kGenThreads || w + N >= w_end) {
-            for (int k = t; k < nt; k += kGenThreads) {
-                const double u = tail_u[k];
-                out[d_begin + tail_off[k]] =
-                    icdf_out<OutT>(as241_tail<Real<OutT>>(u, u - 0.5), sqrt_dt[tail_ti[k]]);
+            const uint64_t m = __ballot(tail);
+            if (tail) {
+                const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+                tq_u[wv][pos] = u;
+                tq_off[wv][pos] = (uint32_t)((wd >> 1) - d_begin);
+                tq_ti[wv][pos] = (uint16_t)ti;
             }
-            __syncthreads();
-            if (t == 0) tail_n = 0;              // ordered before the next pushes by the twist's barriers
+            count += __popcll(m);
+            if (count >= 64) { drain(count); count = 0; }
+            rem += (r < 4) ? step64 : step56;
+            if (rem >= per) rem -= per;
         }
-        // the buffer just read is overwritten only after the next twist's first barrier
     }
+    if (count > 0) drain(count);
 }
 
 __global__ void __launch_bounds__(kGenThreads)
@@ -312,9 +347,9 @@ void mt_seed_window(int64_t seed, uint32_t* mt) {
 StreamLayout choose_layout(uint64_t total_words) {
     StreamLayout l;
     l.total_words = total_words;
-    // about 2048 sub-streams, each at least 2^12 words (7 twists)
+    // up to 8192 sub-streams (one wavefront each), each at least 2^12 words (7 twists)
     int k = 12;
-    while (k < 40 && (total_words >> k) > 2048) k++;
+    while (k < 32 && (total_words >> k) > 8192) k++;
     l.log2_len = k;
     l.num_sub = (int64_t)((total_words + ((uint64_t)1 << k) - 1) >> k);
     if (l.num_sub < 1) l.num_sub = 1;
@@ -353,12 +388,13 @@ void launch_brownian(const StreamLayout& lay, const uint32_t* d_states, int nT, 
     // the tail queue keeps 32-bit offsets within a sub-stream and 16-bit time indices
     if (lay.log2_len > 32 || nT > 65535)
         throw FmError(FM_ERR_UNSUPPORTED, "stream layout outside the generator's index range");
+    const dim3 grid((unsigned)((lay.num_sub + kGenWaves - 1) / kGenWaves));
     if (out_is_float)
-        hipLaunchKernelGGL(brownian_kernel<float>, dim3((unsigned)lay.num_sub), dim3(kGenThreads), 0, stream,
-                           d_states, lay.total_words, lay.log2_len, nT, F, d_sqrt_dt, (float*)out);
+        hipLaunchKernelGGL(brownian_kernel<float>, grid, dim3(64 * kGenWaves), 0, stream,
+                           d_states, lay.num_sub, lay.total_words, lay.log2_len, nT, F, d_sqrt_dt, (float*)out);
     else
-        hipLaunchKernelGGL(brownian_kernel<double>, dim3((unsigned)lay.num_sub), dim3(kGenThreads), 0, stream,
-                           d_states, lay.total_words, lay.log2_len, nT, F, d_sqrt_dt, (double*)out);
+        hipLaunchKernelGGL(brownian_kernel<double>, grid, dim3(64 * kGenWaves), 0, stream,
+                           d_states, lay.num_sub, lay.total_words, lay.log2_len, nT, F, d_sqrt_dt, (double*)out);
     HIP_CHECK(hipGetLastError());
 }
 
