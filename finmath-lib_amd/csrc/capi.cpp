@@ -94,6 +94,7 @@ struct fm_lmm {
     bool dw_ready = false;
     bool simulated = false;
     std::vector<double> zb;             // numeraire-adjustment averages of the last simulate
+    bool uniform_dt = false;            // all simulation steps have the same length
 
     DevBuf d_times, d_first, d_tenor_ti, d_rec, d_delta, d_L0, d_Y0, d_sqrt_dt;
     DevBuf d_states, d_poly, d_dw, d_row_of, d_paths, d_num;
@@ -204,6 +205,7 @@ void prepare_model(fm_lmm* c) {
     std::vector<double> delta((size_t)n), sq((size_t)nT);
     for (int j = 0; j < n; j++) delta[(size_t)j] = c->tenor[(size_t)j + 1] - c->tenor[(size_t)j];
     for (int i = 0; i < nT; i++) sq[(size_t)i] = std::sqrt(c->times[(size_t)i + 1] - c->times[(size_t)i]);
+    c->uniform_dt = std::all_of(sq.begin(), sq.end(), [&](double v) { return v == sq[0]; });
     const bool lognormal = c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL;
     // records per (step i, component j): lambda_j.(t_i) zero-padded to the factor capacity,
     // variance*(-0.5) (0 in the normal state space), d/d and 1/d
@@ -263,8 +265,8 @@ void ensure_brownian(fm_lmm* c) {
     if (c->dw_ready) return;
     const size_t count = (size_t)c->P() * c->nT() * c->F();
     void* out = c->d_dw.get(count * c->real_size());
-    launch_brownian(c->lay, c->d_states.as<uint32_t>(), c->nT(), c->F(), c->d_sqrt_dt.as<double>(), out,
-                    c->fp32() ? 1 : 0, c->stream);
+    launch_brownian(c->lay, c->d_states.as<uint32_t>(), c->nT(), c->F(), c->d_sqrt_dt.as<double>(),
+                    c->uniform_dt ? 1 : 0, out, c->fp32() ? 1 : 0, c->stream);
     c->dw_ready = true;
 }
 

@@ -44,8 +44,9 @@ void launch_jump_tree(const StreamLayout& lay, const uint32_t* seed_window, uint
 
 // Generate the Brownian increments dW[D] = icdf(u_D) * sqrt_dt[(D / F) % nT] for all
 // D < total_words/2 in stream (path-major) order. out is double* or float*.
+// uniform_dt: every sqrt_dt entry is the same (one scale for all steps).
 void launch_brownian(const StreamLayout& lay, const uint32_t* d_states, int nT, int F,
-                     const double* d_sqrt_dt, void* out, int out_is_float, void* stream);
+                     const double* d_sqrt_dt, int uniform_dt, void* out, int out_is_float, void* stream);
 
 // Raw tempered words [0, count) of the stream (count <= lay.total_words).
 void launch_words(const StreamLayout& lay, const uint32_t* d_states, uint64_t count,

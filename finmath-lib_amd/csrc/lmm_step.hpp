@@ -39,8 +39,13 @@ template <typename real> __device__ __forceinline__ real dev_exp(real x);
 template <typename real> __device__ __forceinline__ real dev_log(real x);
 template <> __device__ __forceinline__ double dev_exp<double>(double x) { return exp(x); }
 template <> __device__ __forceinline__ double dev_log<double>(double x) { return fm_log(x); }
-template <> __device__ __forceinline__ float dev_exp<float>(float x) { return expf(x); }
-template <> __device__ __forceinline__ float dev_log<float>(float x) { return logf(x); }
+// FP32 form: the hardware base-2 exponential and logarithm (about 1 ulp each) and the
+// hardware reciprocal; the FP64 form keeps correctly rounded division.
+template <> __device__ __forceinline__ float dev_exp<float>(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+template <> __device__ __forceinline__ float dev_log<float>(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+template <typename real> __device__ __forceinline__ real dev_div(real a, real b);
+template <> __device__ __forceinline__ double dev_div<double>(double a, double b) { return a / b; }
+template <> __device__ __forceinline__ float dev_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // Java Math.max(x, 0.0) / Math.min(x, c): NaN propagates.
 template <typename real> __device__ __forceinline__ real jmax0(real x) { return (x != x) ? x : (x >= (real)0 ? x : (real)0); }
@@ -110,7 +115,7 @@ __device__ __forceinline__ void euler_step(real (&L)[NCAP], int n, int i, int fi
         if (j < n && j >= first) {
             const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, i, j);
             const real Lj = LOGSTATE ? dev_exp<real>(L[j]) : L[j];
-            real w = (real)1.0 / (Lj * R.c1() + R.c2());
+            real w = dev_div<real>((real)1.0, Lj * R.c1() + R.c2());
             if (lognormal) w = w * Lj;
 #pragma unroll
             for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * R.lam(k);
@@ -214,7 +219,7 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
                         const real Lj = val(L[j]);
                         const real payoff = ((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pr.notional;
                         V = V + payoff;
-                        V = V / ((real)1.0 + Lj * (real)pe.tau);
+                        V = dev_div<real>(V, (real)1.0 + Lj * (real)pe.tau);
                         V = V * (real)pe.adj;
                     }
                 }

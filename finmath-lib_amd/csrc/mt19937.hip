@@ -123,6 +123,30 @@ template <typename R> __device__ __forceinline__ R as241_central(R q) {
     return q * num / den;
 }
 
+// Central branch in float for FP32 increments: the same rational function with fused
+// multiply-adds and the hardware reciprocal (about 1e-6 relative; the FP64 path above
+// keeps the reference's operation order).
+__device__ __forceinline__ float as241_central_f32(float q) {
+    const float r = 0.180625f - q * q;
+    float num = 2.5090809287301226727e+03f;
+    num = fmaf(num, r, 3.3430575583588128105e+04f);
+    num = fmaf(num, r, 6.7265770927008700853e+04f);
+    num = fmaf(num, r, 4.5921953931549871457e+04f);
+    num = fmaf(num, r, 1.3731693765509461125e+04f);
+    num = fmaf(num, r, 1.9715909503065514427e+03f);
+    num = fmaf(num, r, 1.3314166789178437745e+02f);
+    num = fmaf(num, r, 3.3871328727963666080e+00f);
+    float den = 5.2264952788528545610e+03f;
+    den = fmaf(den, r, 2.8729085735721942674e+04f);
+    den = fmaf(den, r, 3.9307895800092710610e+04f);
+    den = fmaf(den, r, 2.1213794301586595867e+04f);
+    den = fmaf(den, r, 5.3941960214247511077e+03f);
+    den = fmaf(den, r, 6.8718700749205790830e+02f);
+    den = fmaf(den, r, 4.2313330701600911252e+01f);
+    den = fmaf(den, r, 1.0f);
+    return q * num * __builtin_amdgcn_rcpf(den);
+}
+
 template <typename R> __device__ __forceinline__ R dev_sqrt_neg_log(double r);
 template <> __device__ __forceinline__ double dev_sqrt_neg_log<double>(double r) { return sqrt(-log(r)); }
 template <> __device__ __forceinline__ float dev_sqrt_neg_log<float>(double r) { return sqrtf(-logf((float)r)); }
@@ -222,10 +246,11 @@ __device__ __forceinline__ void wave_lds_order() {
 template <typename OutT>
 __global__ void __launch_bounds__(64 * kGenWaves)
 brownian_kernel(const uint32_t* __restrict__ states, int64_t num_sub, uint64_t total_words, int log2_len,
-                int nT, int F, const double* __restrict__ sqrt_dt, OutT* __restrict__ out)
+                int nT, int F, const double* __restrict__ sqrt_dt, int uniform_dt, OutT* __restrict__ out)
 {
+    constexpr bool kF32 = sizeof(OutT) == 4;
     __shared__ uint32_t st[kGenWaves][N];
-    __shared__ double tq_u[kGenWaves][kTailQ];
+    __shared__ uint64_t tq_bits[kGenWaves][kTailQ];    // 52 uniform bits of a tail draw
     __shared__ uint32_t tq_off[kGenWaves][kTailQ];     // double index relative to the sub-stream's first
     __shared__ uint16_t tq_ti[kGenWaves][kTailQ];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -238,16 +263,18 @@ brownian_kernel(const uint32_t* __restrict__ states, int64_t num_sub, uint64_t t
     const uint64_t d_begin = w_begin >> 1;
     for (int w = lane; w < N; w += 64) mt[w] = states[(size_t)s * N + w];
     wave_lds_order();
+    const double sq0 = sqrt_dt[0];                     // the scale of every step on a uniform grid
     const uint32_t per = (uint32_t)nT * (uint32_t)F;   // doubles per path
-    // position within its path of this lane's double in the current round
-    uint32_t rem = (uint32_t)((d_begin + (uint64_t)lane) % per);
+    // position within its path of this lane's double in the current round (non-uniform grids)
+    uint32_t rem = uniform_dt ? 0u : (uint32_t)((d_begin + (uint64_t)lane) % per);
     const uint32_t step64 = 64u % per, step56 = (uint32_t)(N / 2 - 4 * 64) % per;
     int count = 0;                                     // queued tail draws (uniform)
     auto drain = [&](int nq) {
         wave_lds_order();
         for (int k = lane; k < nq; k += 64) {
-            const double u = tq_u[wv][k];
-            out[d_begin + tq_off[wv][k]] = icdf_out<OutT>(as241_tail<Real<OutT>>(u, u - 0.5), sqrt_dt[tq_ti[wv][k]]);
+            const double u = (double)tq_bits[wv][k] * 0x1.0p-52;
+            const double sq = uniform_dt ? sq0 : sqrt_dt[tq_ti[wv][k]];
+            out[d_begin + tq_off[wv][k]] = icdf_out<OutT>(as241_tail<Real<OutT>>(u, u - 0.5), sq);
         }
         wave_lds_order();
     };
@@ -272,30 +299,38 @@ brownian_kernel(const uint32_t* __restrict__ states, int64_t num_sub, uint64_t t
             const int d = r * 64 + lane;
             const uint64_t wd = w + 2 * (uint64_t)d;
             bool tail = false;
-            double u = 0.0;
-            const int ti = (int)(rem / (uint32_t)F);
+            uint64_t bits = 0;
+            const int ti = uniform_dt ? 0 : (int)(rem / (uint32_t)F);
             if (d < N / 2 && wd < w_end) {
                 const uint32_t a = temper(mt[2 * d]);
                 const uint32_t b = temper(mt[2 * d + 1]);
-                const uint64_t bits = ((uint64_t)(a >> 6) << 26) | (uint64_t)(b >> 6);
-                u = (double)bits * 0x1.0p-52;
-                const double q = u - 0.5;
-                if (fabs(q) <= 0.425)
-                    out[wd >> 1] = icdf_out<OutT>(as241_central<Real<OutT>>((Real<OutT>)q), sqrt_dt[ti]);
-                else
-                    tail = true;
+                bits = ((uint64_t)(a >> 6) << 26) | (uint64_t)(b >> 6);
+                const double sq = uniform_dt ? sq0 : sqrt_dt[ti];
+                if (kF32) {
+                    // q = u - 1/2 from the top 32 of the 52 bits (float keeps 24)
+                    const float q = (float)(int32_t)((uint32_t)(bits >> 20) ^ 0x80000000u) * 0x1.0p-32f;
+                    if (fabsf(q) <= 0.425f) out[wd >> 1] = (OutT)(as241_central_f32(q) * (float)sq);
+                    else tail = true;
+                } else {
+                    const double u = (double)bits * 0x1.0p-52;
+                    const double q = u - 0.5;
+                    if (fabs(q) <= 0.425) out[wd >> 1] = icdf_out<OutT>(as241_central<double>(q), sq);
+                    else tail = true;
+                }
             }
             const uint64_t m = __ballot(tail);
             if (tail) {
                 const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
-                tq_u[wv][pos] = u;
+                tq_bits[wv][pos] = bits;
                 tq_off[wv][pos] = (uint32_t)((wd >> 1) - d_begin);
                 tq_ti[wv][pos] = (uint16_t)ti;
             }
             count += __popcll(m);
             if (count >= 64) { drain(count); count = 0; }
-            rem += (r < 4) ? step64 : step56;
-            if (rem >= per) rem -= per;
+            if (!uniform_dt) {
+                rem += (r < 4) ? step64 : step56;
+                if (rem >= per) rem -= per;
+            }
         }
     }
     if (count > 0) drain(count);
@@ -383,7 +418,7 @@ void launch_jump_tree(const StreamLayout& lay, const uint32_t* seed_window, uint
 }
 
 void launch_brownian(const StreamLayout& lay, const uint32_t* d_states, int nT, int F,
-                     const double* d_sqrt_dt, void* out, int out_is_float, void* stream_) {
+                     const double* d_sqrt_dt, int uniform_dt, void* out, int out_is_float, void* stream_) {
     hipStream_t stream = (hipStream_t)stream_;
     // the tail queue keeps 32-bit offsets within a sub-stream and 16-bit time indices
     if (lay.log2_len > 32 || nT > 65535)
@@ -391,10 +426,12 @@ void launch_brownian(const StreamLayout& lay, const uint32_t* d_states, int nT, 
     const dim3 grid((unsigned)((lay.num_sub + kGenWaves - 1) / kGenWaves));
     if (out_is_float)
         hipLaunchKernelGGL(brownian_kernel<float>, grid, dim3(64 * kGenWaves), 0, stream,
-                           d_states, lay.num_sub, lay.total_words, lay.log2_len, nT, F, d_sqrt_dt, (float*)out);
+                           d_states, lay.num_sub, lay.total_words, lay.log2_len, nT, F, d_sqrt_dt, uniform_dt,
+                           (float*)out);
     else
         hipLaunchKernelGGL(brownian_kernel<double>, grid, dim3(64 * kGenWaves), 0, stream,
-                           d_states, lay.num_sub, lay.total_words, lay.log2_len, nT, F, d_sqrt_dt, (double*)out);
+                           d_states, lay.num_sub, lay.total_words, lay.log2_len, nT, F, d_sqrt_dt, uniform_dt,
+                           (double*)out);
     HIP_CHECK(hipGetLastError());
 }
 

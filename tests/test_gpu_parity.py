@@ -75,6 +75,25 @@ def test_brownian_increments_match_oracle():
     assert exact / total > 0.999
 
 
+@pytest.mark.parametrize("precision", ["FP64", "FP32_MIXED"])
+def test_brownian_increments_on_a_non_uniform_grid(precision):
+    # steps of different lengths: every increment takes the sqrt(dt) of its own step
+    times = W.round_ticks(np.array([0.0, 0.1, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0]))
+    tenor = W.round_ticks(np.array([0.0, 0.5, 1.0, 1.5, 2.0]))
+    F, P, n, nT = 2, 3001, 4, 7
+    say(f"case brownian non-uniform grid {precision} P={P}")
+    ctx = _abi.LmmContext(times, tenor, F, P, 3141, precision=precision)
+    ctx.set_initial_forwards(np.full(n, 0.03))
+    ctx.set_covariance(np.full((nT, n, F), 0.1), np.full((nT, n), 0.02))
+    ref = O.brownian(times, F, P, 3141)
+    tol = 8e-16 if precision == "FP64" else 3e-6
+    for ti in range(nT):
+        for k in range(F):
+            g = ctx.brownian_increment(ti, k)
+            r = ref[ti, k]
+            assert np.max(np.abs(g - r) / np.maximum(np.abs(r), 1e-3)) < tol
+
+
 @pytest.mark.parametrize("precision", ["FP64", "FP64_LOGSTATE"])
 @pytest.mark.parametrize("name,P", [("c0", 10_000), ("c1", 4_000)])
 def test_process_values_match_oracle(name, P, precision):
