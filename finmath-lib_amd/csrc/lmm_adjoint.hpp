@@ -112,7 +112,7 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
             for (int s = c; s < t; s++) {
                 real dWs[FCAP];
                 load_increments<FCAP, real>(dWs, dw, pc, nT, s, F);
-                euler_step<NCAP, FCAP, real>(L, n, s, first_alive[s], (real)(times[s + 1] - times[s]), dWs, rec,
+                euler_step<NCAP, FCAP, real, false, 4>(L, n, s, first_alive[s], (real)(times[s + 1] - times[s]), dWs, rec,
                                              lognormal, capped, cap, (real)0, (real*)nullptr,
                                              (const int32_t*)nullptr, p, P, active);
             }
@@ -125,14 +125,21 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
                 load_increments<FCAP, real>(dW, dw, pc, nT, t, F);
 #pragma unroll
                 for (int k = 0; k < FCAP; k++) { S[k] = (real)0; Sb[k] = (real)0; }
-                // pass A (ascending): recompute the step, adjoint through exp/log/cap
+                // pass A (ascending): recompute the step, adjoint through exp/log/cap. Components
+                // in groups of four without inner branches (a component outside [first, n) of a
+                // partly alive group is computed on a valid record and discarded).
 #pragma unroll
-                for (int j = 0; j < NCAP; j++) {
-                    if (j < n && j >= first) {
-                        const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, t, j);
-                        const real Lj = L[j];
+                for (int j0 = 0; j0 < NCAP; j0 += 4) {
+                    if (j0 + 3 < first || j0 >= n) continue;
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++) {
+                        const int j = j0 + jj;
+                        const bool alive = j >= first && j < n;
+                        const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, t, j < n ? j : n - 1);
+                        const real Lj = alive ? L[j] : (real)1;
                         real w = adj_rcp<real>(Lj * R.c1() + R.c2());
                         if (lognormal) w = w * Lj;
+                        if (!alive) w = (real)0;
 #pragma unroll
                         for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * R.lam(k);
                         real mu = S[0] * R.lam(0);
@@ -145,25 +152,26 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
                         for (int k = 0; k < FCAP; k++) Y = Y + dW[k] * R.lam(k);
                         const real E = lognormal ? dev_exp<real>(Y) : Y;
                         const real Yb = (capped && E > cap) ? (real)0 : Lb[j] * (lognormal ? E : (real)1);
-                        Lb[j] = lognormal ? Yb / Lj : Yb;     // direct part of the adjoint of L_j(t)
+                        // direct part of the adjoint of L_j(t)
+                        if (alive) Lb[j] = lognormal ? Yb * adj_rcp<real>(Lj) : Yb;
                     }
                 }
-                // pass B (descending): drift chain S_k = sum_{l<=j} w_l lambda_lk; the
-                // per-path dV/dsigma_j(t) of four consecutive j are summed over the wavefront together
-                double gq[4];
+                // pass B (descending): drift chain S_k = sum_{l<=j} w_l lambda_lk; the per-path
+                // dV/dsigma_j(t) of a group of four j are summed over the wavefront together
 #pragma unroll
-                for (int j = NCAP - 1; j >= 0; j--) {
-                    if ((j & 3) == 3 || j == NCAP - 1) {
+                for (int j0 = NCAP - 4; j0 >= 0; j0 -= 4) {
+                    if (j0 + 3 < first || j0 >= n) continue;
+                    double gq[4];
 #pragma unroll
-                        for (int s4 = 0; s4 < 4; s4++) gq[s4] = 0.0;
-                    }
-                    if (j < n && j >= first) {
-                        const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, t, j);
-                        const real Lj = L[j];
+                    for (int jj = 3; jj >= 0; jj--) {
+                        const int j = j0 + jj;
+                        const bool alive = j >= first && j < n;
+                        const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, t, j < n ? j : n - 1);
+                        const real Lj = alive ? L[j] : (real)1;
                         const real r = adj_rcp<real>(Lj * R.c1() + R.c2());
-                        const real w = lognormal ? r * Lj : r;
+                        const real w = alive ? (lognormal ? r * Lj : r) : (real)0;
                         // adjoint of Y_j(t+1) from the direct part left by pass A
-                        const real Yb = lognormal ? Lb[j] * Lj : Lb[j];
+                        const real Yb = alive ? (lognormal ? Lb[j] * Lj : Lb[j]) : (real)0;
                         const real mb = Yb * dt;
                         real wb = (real)0;
                         real sgb = (real)0;             // dV/dsigma_j(t) on this path
@@ -172,21 +180,19 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
                             Sb[k] = Sb[k] + mb * R.lam(k);
                             wb = wb + Sb[k] * R.lam(k);
                             const real lamb = Yb * dW[k] + mb * S[k] + Sb[k] * w;
-                            sgb = sgb + lamb * fsig[j * FCAP + k];
+                            sgb = sgb + lamb * fsig[(j < n ? j : n - 1) * FCAP + k];
                             S[k] = S[k] - w * R.lam(k);    // S^(j) -> S^(j-1)
                         }
                         // variance sigma^2 c in the drift: d(-var/2)/dsigma = -sigma c
-                        if (lognormal) sgb = sgb + mb * vterm[(size_t)t * n + j];
+                        if (lognormal) sgb = sgb + mb * vterm[(size_t)t * n + (j < n ? j : n - 1)];
                         const real dwdL = lognormal ? R.c2() * r * r : -R.c1() * r * r;
-                        Lb[j] = Lb[j] + wb * dwdL;
-                        gq[j & 3] = active ? (double)sgb : 0.0;
+                        if (alive) Lb[j] = Lb[j] + wb * dwdL;
+                        gq[jj] = (active && alive) ? (double)sgb : 0.0;
                     }
-                    if ((j & 3) == 0 && j < n && j + 3 >= first) {
-                        const double s = wave_sum4(gq, lane);
-                        const int jj = j + 2 * (lane & 1) + ((lane >> 1) & 1);
-                        if (lane < 4 && cnt > 0 && jj < n)
-                            wave_grad[(size_t)wave * gstride + (size_t)t * n + jj] = s;
-                    }
+                    const double s = wave_sum4(gq, lane);
+                    const int jw = j0 + 2 * (lane & 1) + ((lane >> 1) & 1);
+                    if (lane < 4 && cnt > 0 && jw < n)
+                        wave_grad[(size_t)wave * gstride + (size_t)t * n + jw] = s;
                 }
             }
 

@@ -102,41 +102,69 @@ __device__ __forceinline__ Rec<FCAP, real> rec_at(const real* __restrict__ rec, 
 // LOGSTATE: the array holds the log-forwards Y_j and the step needs one exp per
 // component (the reference re-takes log(exp(Y)) every step, EULER_FUNCTIONAL;
 // carrying Y differs from it by rounding only, the cap applied as Y <= log(cap)).
-template <int NCAP, int FCAP, typename real, bool LOGSTATE = false>
+// One component's update inside euler_step (alive: the component moves, else it is
+// computed on a valid record and discarded).
+template <int NCAP, int FCAP, typename real, bool LOGSTATE>
+__device__ __forceinline__ void euler_component(real (&L)[NCAP], real (&S)[FCAP], int j, bool alive, int jr, int n,
+                                                int i, real dt, const real (&dW)[FCAP], const real* __restrict__ rec,
+                                                bool lognormal, bool capped, real cap, real logcap,
+                                                real* __restrict__ paths, const int32_t* __restrict__ row_next,
+                                                int64_t p, int P, bool active) {
+    const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, i, jr);
+    const real Lj = LOGSTATE ? dev_exp<real>(L[j]) : L[j];
+    real w = dev_div<real>((real)1.0, Lj * R.c1() + R.c2());
+    if (lognormal) w = w * Lj;
+    if (!alive) w = (real)0;
+#pragma unroll
+    for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * R.lam(k);
+    real mu = S[0] * R.lam(0);
+#pragma unroll
+    for (int k = 1; k < FCAP; k++) mu = mu + S[k] * R.lam(k);
+    mu = mu + R.hv();
+    real Y = LOGSTATE ? L[j] : (lognormal ? dev_log<real>(Lj) : Lj);
+    Y = Y + mu * dt;
+#pragma unroll
+    for (int k = 0; k < FCAP; k++) Y = Y + dW[k] * R.lam(k);
+    if (LOGSTATE) {
+        if (capped && Y > logcap) Y = logcap;
+        if (alive) L[j] = Y;
+        if (alive && paths && active) paths[(size_t)row_next[j] * P + p] = dev_exp<real>(Y);
+    } else {
+        real Ln = lognormal ? dev_exp<real>(Y) : Y;
+        if (capped) Ln = jmin(Ln, cap);
+        if (alive) L[j] = Ln;
+        if (alive && paths && active) paths[(size_t)row_next[j] * P + p] = Ln;
+    }
+}
+
+// One Euler step t_i -> t_{i+1} of all alive components, in place. Shared by the
+// pricing pass and the adjoint recompute; both group sizes give bit-identical states.
+// GROUP 1: a uniform branch per component. GROUP 4: components in groups of four
+// without branches inside a group, so the loads and transcendentals of four
+// components overlap; a component outside [first, n) of a partly alive group is
+// computed and discarded (w = 0 adds exact zeros to S).
+// LOGSTATE: the array holds the log-forwards Y_j and the step needs one exp per
+// component (the reference re-takes log(exp(Y)) every step, EULER_FUNCTIONAL;
+// carrying Y differs from it by rounding only, the cap applied as Y <= log(cap)).
+template <int NCAP, int FCAP, typename real, bool LOGSTATE = false, int GROUP = 1>
 __device__ __forceinline__ void euler_step(real (&L)[NCAP], int n, int i, int first, real dt, const real (&dW)[FCAP],
                                            const real* __restrict__ rec, bool lognormal, bool capped, real cap,
                                            real logcap, real* __restrict__ paths,
                                            const int32_t* __restrict__ row_next, int64_t p, int P, bool active) {
+    static_assert(NCAP % GROUP == 0, "component capacity must be a multiple of the group size");
     real S[FCAP];
 #pragma unroll
     for (int k = 0; k < FCAP; k++) S[k] = (real)0;
 #pragma unroll
-    for (int j = 0; j < NCAP; j++) {
-        if (j < n && j >= first) {
-            const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, i, j);
-            const real Lj = LOGSTATE ? dev_exp<real>(L[j]) : L[j];
-            real w = dev_div<real>((real)1.0, Lj * R.c1() + R.c2());
-            if (lognormal) w = w * Lj;
+    for (int j0 = 0; j0 < NCAP; j0 += GROUP) {
+        if (j0 + GROUP - 1 < first || j0 >= n) continue;  // uniform: the whole group is frozen
 #pragma unroll
-            for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * R.lam(k);
-            real mu = S[0] * R.lam(0);
-#pragma unroll
-            for (int k = 1; k < FCAP; k++) mu = mu + S[k] * R.lam(k);
-            mu = mu + R.hv();
-            real Y = LOGSTATE ? L[j] : (lognormal ? dev_log<real>(Lj) : Lj);
-            Y = Y + mu * dt;
-#pragma unroll
-            for (int k = 0; k < FCAP; k++) Y = Y + dW[k] * R.lam(k);
-            if (LOGSTATE) {
-                if (capped && Y > logcap) Y = logcap;
-                L[j] = Y;
-                if (paths && active) paths[(size_t)row_next[j] * P + p] = dev_exp<real>(Y);
-            } else {
-                real Ln = lognormal ? dev_exp<real>(Y) : Y;
-                if (capped) Ln = jmin(Ln, cap);
-                L[j] = Ln;
-                if (paths && active) paths[(size_t)row_next[j] * P + p] = Ln;
-            }
+        for (int jj = 0; jj < GROUP; jj++) {
+            const int j = j0 + jj;
+            const bool alive = j >= first && j < n;        // uniform
+            euler_component<NCAP, FCAP, real, LOGSTATE>(L, S, j, alive, j < n ? j : n - 1, n, i, dt, dW, rec,
+                                                       lognormal, capped, cap, logcap, paths, row_next, p, P,
+                                                       active);
         }
     }
 }
