@@ -95,6 +95,7 @@ struct fm_lmm {
     bool simulated = false;
     std::vector<double> zb;             // numeraire-adjustment averages of the last simulate
     bool uniform_dt = false;            // all simulation steps have the same length
+    std::vector<std::vector<double>> berm_fixing;   // fixing dates of the Bermudans of the last pass
 
     DevBuf d_times, d_first, d_tenor_ti, d_rec, d_delta, d_L0, d_Y0, d_sqrt_dt;
     DevBuf d_states, d_poly, d_dw, d_row_of, d_paths, d_num;
@@ -278,6 +279,7 @@ struct BermPlan {
     int num_basis = 6;
     std::vector<int> per_col, per_k;        // per period: payoff column and its numeraire index
     std::vector<int> ex_period, ex_col, ex_k;   // per exercise date (ascending): period, first basis column, N index
+    std::vector<double> fixing;                 // fixing dates of the periods
 };
 
 // Products lowered for the kernel.
@@ -430,6 +432,7 @@ Lowered lower_products(fm_lmm* c, const fm_product* prods, int count) {
             b.callable = p.is_cancelable ? 0 : 1;
             b.basis = p.basis;
             b.num_basis = p.basis == FM_BASIS_REFERENCE ? 6 : 4;
+            b.fixing.assign(p.fixing_dates, p.fixing_dates + p.num_periods);
             const int j0 = c->libor_index(p.fixing_dates[0]);
             for (int i = 0; i < m; i++) {
                 const double fd = p.fixing_dates[i], pd = p.payment_dates[i], pl = p.period_lengths[i];
@@ -540,7 +543,7 @@ BermBasisRef basis_ref(const fm_lmm* c, const BermPlan& b, int e, const std::vec
 // Backward induction of one Bermudan over the columns of the last path pass
 // (BermudanSwaption.java:100-146): one step launch per exercise date (latest first)
 // followed by its regression solve, then the final step into the product's slot.
-void run_bermudan(fm_lmm* c, const BermPlan& b, const std::vector<double>& adj, const double* contrib,
+void run_bermudan(fm_lmm* c, const BermPlan& b, int bi, const std::vector<double>& adj, const double* contrib,
                   double* wsum, double* wm2, int stride) {
     hipStream_t s = c->stream;
     const int P = c->P(), m = (int)b.per_col.size(), nex = (int)b.ex_period.size();
@@ -555,7 +558,9 @@ void run_bermudan(fm_lmm* c, const BermPlan& b, const std::vector<double>& adj, 
     const double* dscale = upload(c->d_berm_scale, scales, s);
     double* vals = (double*)c->d_berm_vals.get((size_t)P * sizeof(double));
     double* under = (double*)c->d_berm_under.get((size_t)P * sizeof(double));
-    int32_t* ex = (int32_t*)c->d_berm_ex.get((size_t)P * sizeof(int32_t));
+    // exercise period per path of Bermudan bi (kept for the adjoint); the buffer is sized
+    // for all Bermudans of the call by run_pass
+    int32_t* ex = c->d_berm_ex.as<int32_t>() + (size_t)bi * P;
     HIP_CHECK(hipMemsetAsync(ex, 0xff, (size_t)P * sizeof(int32_t), s));
     const int nb = berm_num_blocks(P);
     double* part = (double*)c->d_berm_part.get((size_t)nb * kBermParts * sizeof(double));
@@ -683,7 +688,11 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
             const int ns = (int)L.swap_slot.size();
             launch_swap_finalize(a.contrib, P, ns, dmeta, dmeta + ns, dmeta + 2 * ns, dscale, wsum, wm2, stride, s);
         }
-        for (const BermPlan& b : L.berm) run_bermudan(c, b, adj, a.contrib, wsum, wm2, stride);
+        if (!L.berm.empty()) c->d_berm_ex.get(L.berm.size() * (size_t)P * sizeof(int32_t));
+        c->berm_fixing.clear();
+        for (const BermPlan& b : L.berm) c->berm_fixing.push_back(b.fixing);
+        for (size_t bi = 0; bi < L.berm.size(); bi++)
+            run_bermudan(c, L.berm[bi], (int)bi, adj, a.contrib, wsum, wm2, stride);
         launch_reduce_partials(wsum, wm2, (int)num_waves, stride, L.num_slots, 0, P, dout, dout + cols, s);
         std::vector<double> s2((size_t)cols), m2b((size_t)cols);
         HIP_CHECK(hipMemcpyAsync(s2.data(), dout, L.num_slots * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -915,9 +924,6 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     require(count >= 1 && products != nullptr, FM_ERR_INVALID_ARGUMENT, "no products");
     std::lock_guard<std::mutex> lk(c->mu);
     require(c->has_vf, FM_ERR_STATE, "volatility factorization not set (fm_lmm_set_volatility_factorization)");
-    for (int q = 0; q < count; q++)
-        require(products == nullptr || products[q].type != FM_PRODUCT_BERMUDAN_SWAPTION, FM_ERR_UNSUPPORTED,
-                "the adjoint of a Bermudan swaption is not implemented");
     require(c->cfg.precision != FM_PRECISION_FP64_LOGSTATE, FM_ERR_UNSUPPORTED,
             "the adjoint runs on the FP64 and FP32_MIXED path kernels");
     DeviceGuard g(c->device);
@@ -945,12 +951,31 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
         const int q = slot_to_q[(size_t)L.swap_slot[i]];
         for (int cc = 0; cc < L.swap_ncol[i]; cc++) col_to_q[(size_t)(L.swap_col0[i] + cc)] = q;
     }
+    // Bermudan periods: swap periods gated per path by the exercise period of the pass above
+    std::vector<int> col_berm((size_t)L.num_contrib, -1), col_period((size_t)L.num_contrib, -1);
+    for (size_t bi = 0; bi < L.berm.size(); bi++) {
+        const BermPlan& b = L.berm[bi];
+        for (size_t i = 0; i < b.per_col.size(); i++) {
+            col_to_q[(size_t)b.per_col[i]] = slot_to_q[(size_t)b.slot];
+            col_berm[(size_t)b.per_col[i]] = (int)bi;
+            col_period[(size_t)b.per_col[i]] = (int)i;
+        }
+    }
     std::vector<KProduct> kp = L.kp;
     for (auto& k : kp) {
-        if (k.type == FM_PRODUCT_SWAP) {
+        if (k.type == K_BERM_BASIS) {
+            k.weight = 0.0;
+        } else if (k.type == FM_PRODUCT_SWAP) {
             const int q = col_to_q[(size_t)k.out_slot];
             const double sc = c->has_pd ? adj[(size_t)k.num_k] / c->zb[(size_t)k.num_k] : 1.0;
             k.weight = wq[(size_t)q] * sc / (double)P;
+            const int bi = col_berm[(size_t)k.out_slot];
+            if (bi >= 0) {
+                k.type = K_BERM_PERIOD;
+                k.is_floorlet = col_period[(size_t)k.out_slot];
+                k.pad2 = bi;
+                k.flags = 1 | (L.berm[(size_t)bi].callable ? 2 : 0);
+            }
         } else {
             const int q = slot_to_q[(size_t)k.out_slot];
             const double sc = c->has_pd ? adj[(size_t)k.num_k] / c->zb[(size_t)k.num_k] : 1.0;
@@ -979,6 +1004,7 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     ga.stride = stride;
     ga.kcur_at = upload(c->d_kcur, kcur, s);
     ga.weight = upload(c->d_scale, kweights, s);
+    ga.berm_ex = L.berm.empty() ? nullptr : c->d_berm_ex.as<int32_t>();
     // factor matrix padded to the factor capacity and -sigma_j(t_i) c_j, in the path precision
     std::vector<double> fpad((size_t)n * fcap_of(c->F()), 0.0), vterm((size_t)nT * n, 0.0);
     for (int j = 0; j < n; j++)
@@ -1027,6 +1053,23 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     if (d_volatility) for (size_t i = 0; i < (size_t)nT * n; i++) d_volatility[i] = grad[i];
     if (d_initial_forwards) for (int j = 0; j < n; j++) d_initial_forwards[j] = grad[(size_t)nT * n + j];
     for (double x : grad) require(std::isfinite(x), FM_ERR_NUMERICAL, "adjoint is not finite");
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_bermudan_exercise_times(fm_lmm* c, int32_t bermudan, double* out) {
+    FM_TRY
+    require(c && out, FM_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    require(bermudan >= 0 && bermudan < (int)c->berm_fixing.size(), FM_ERR_STATE,
+            "no Bermudan swaption with this index in the last valuation");
+    DeviceGuard g(c->device);
+    const int P = c->P();
+    std::vector<int32_t> ex((size_t)P);
+    HIP_CHECK(hipMemcpy(ex.data(), c->d_berm_ex.as<int32_t>() + (size_t)bermudan * P, (size_t)P * sizeof(int32_t),
+                        hipMemcpyDeviceToHost));
+    const std::vector<double>& fix = c->berm_fixing[(size_t)bermudan];
+    for (int p = 0; p < P; p++) out[p] = ex[(size_t)p] >= 0 ? fix[(size_t)ex[(size_t)p]] : HUGE_VAL;
     return FM_OK;
     FM_CATCH
 }

@@ -126,6 +126,7 @@ struct AdjArgs {
     const void* fsig;                // [n][fcap_of(F)] factor matrix f_jk, zero padded (path precision)
     const void* vterm;               // [nT][n] -sigma_j(t_i) c_j, d(variance * -1/2)/dsigma (path precision)
     double* wave_grad;               // [num_waves][nT*n + n] partials of dV/dsigma and dV/dL(0)
+    const int32_t* berm_ex;          // [num Bermudans][P] exercise period per path (-1: never), or NULL
 };
 void launch_lmm_adjoint_fp64(const StepArgs& a, const AdjArgs& g, void* stream);
 void launch_lmm_adjoint_fp32(const StepArgs& a, const AdjArgs& g, void* stream);
@@ -145,6 +146,10 @@ void launch_swap_finalize(const double* contrib, int P, int num_swaps, const int
 //   FM_BASIS_REFERENCE: 1/(1+L_short dt), 1/(1+L_long dt), N(T_e) (unadjusted)
 //   FM_BASIS_SWAP_RATE: S, N(T_e) (unadjusted)
 constexpr int32_t K_BERM_BASIS = 11;
+// Adjoint only: one period of a Bermudan, a swap period gated per path by the exercise
+// period of the pricing pass (first_j = libor index, is_floorlet = period index,
+// pad2 = Bermudan index, flags bit1 = callable).
+constexpr int32_t K_BERM_PERIOD = 12;
 constexpr int kBermMaxBasis = 8;
 constexpr int kBermParts = kBermMaxBasis + kBermMaxBasis * (kBermMaxBasis + 1) / 2;   // X^T y, upper X^T X
 

@@ -253,7 +253,18 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
 #pragma unroll
                         for (int j = 0; j < NCAP; j++) Lb[j] = Lb[j] + ((j == pr.first_j) ? (real)dLj : (real)0);
                     }
+                } else if (pr.type == K_BERM_BASIS) {
+                    // regression basis: the exercise decisions are held fixed (the derivative of
+                    // choose in its trigger is zero, RandomVariableDifferentiableAAD.java:230-452)
                 } else {
+                    if (pr.type == K_BERM_PERIOD) {
+                        // the period pays on this path if the exercise (callable: entered at or
+                        // before it; cancelable: terminated after it) keeps it in the value
+                        const int ex = g.berm_ex[(size_t)pr.pad2 * P + pc];
+                        const bool pays = (pr.flags & 2) ? (ex >= 0 && pr.is_floorlet >= ex)
+                                                         : (ex < 0 || pr.is_floorlet < ex);
+                        if (!pays) continue;
+                    }
                     const KPeriod pe = periods[pr.param_off];
                     const real Lj = pick<NCAP, real>(L, pr.first_j);
                     const double sgn = (pr.flags & 1) ? 1.0 : -1.0;

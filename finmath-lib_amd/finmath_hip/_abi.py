@@ -38,6 +38,7 @@ EXPORTED_SYMBOLS = (
     "fm_lmm_set_discount_curve", "fm_lmm_simulate", "fm_lmm_get_process_value",
     "fm_lmm_process_value_device_ptr", "fm_lmm_get_numeraire", "fm_lmm_get_brownian_increment",
     "fm_lmm_value_products", "fm_lmm_set_volatility_factorization", "fm_lmm_adjoint",
+    "fm_lmm_bermudan_exercise_times",
     "fm_mt19937_words", "fm_mt19937_doubles",
     "fm_last_error", "fm_lmm_last_timings", "fm_device_info",
 )
@@ -122,6 +123,7 @@ def load() -> ctypes.CDLL:
         "fm_lmm_value_products": ([c_void_p, POINTER(FmProduct), c_int32, dp, dp], c_int32),
         "fm_lmm_set_volatility_factorization": ([c_void_p, dp, dp, dp], c_int32),
         "fm_lmm_adjoint": ([c_void_p, POINTER(FmProduct), c_int32, dp, dp, dp, dp], c_int32),
+        "fm_lmm_bermudan_exercise_times": ([c_void_p, c_int32, dp], c_int32),
         "fm_mt19937_words": ([c_int64, c_uint64, c_uint64, POINTER(c_uint32)], c_int32),
         "fm_mt19937_doubles": ([c_int64, c_uint64, c_uint64, dp], c_int32),
         "fm_last_error": ([], c_char_p),
@@ -314,6 +316,12 @@ class LmmContext:
         c = None if variance_factor is None else as_f64(variance_factor)
         check(self.lib.fm_lmm_set_volatility_factorization(self.h, _dptr(v), _dptr(f),
                                                            POINTER(c_double)() if c is None else _dptr(c)))
+
+    def bermudan_exercise_times(self, bermudan: int = 0) -> np.ndarray:
+        """exerciseTime of BermudanSwaption.getValues per path (+inf: never exercised)."""
+        out = np.empty(self.P)
+        check(self.lib.fm_lmm_bermudan_exercise_times(self.h, bermudan, _dptr(out)))
+        return out
 
     def adjoint(self, products, weights=None):
         """(value, dV/dL(0) [n], dV/dsigma [nT][n])"""

@@ -204,6 +204,13 @@ typedef struct fm_product {
 int fm_lmm_value_products(fm_lmm* ctx, const fm_product* products, int32_t count,
                           double* values, double* errors);
 
+/* Exercise time of every path of Bermudan swaption number `bermudan` (counting only the
+ * Bermudan products, in order) of the last fm_lmm_value_products / fm_lmm_adjoint call:
+ * the fixing date of the period at which the backward induction exercised, +infinity if
+ * it never did. Replaces the "exerciseTime" entry of BermudanSwaption.getValues
+ * (montecarlo/interestrate/products/BermudanSwaption.java:140-156). */
+int fm_lmm_bermudan_exercise_times(fm_lmm* ctx, int32_t bermudan, double* out);
+
 /* ---- pathwise adjoint (Greeks) -----------------------------------------------
  * Volatility factorization of the covariance tables, needed by the adjoint:
  *   factor_loading[(i*n+j)*F+k] = volatility[i*n+j] * factors[j*F+k],
