@@ -7,12 +7,20 @@ ATM and +-100bp) and reduce value and Monte Carlo error. Model and tables are
 set up before the timed region; nothing is cached between steps (the Brownian
 increments are regenerated every step).
 
-The default line is the plain FP64 form at 40 semiannual forwards x 5 factors x
-1,000,000 paths. Beside it: the FP32-mixed form at 4,000,000 paths with its
-per-price difference to the FP64 form in basis points, the roofline of the path
-kernel, and the CPU restatement timed on the host cores.
+The headline line is the plain FP64 form at 40 semiannual forwards x 5 factors x
+1,000,000 paths. Beside it:
+  fp32_mixed     the FP32-mixed form at 4,000,000 paths and its per-price difference to
+                 the FP64 form on the same paths, in basis points
+  fp64_logstate  FP64 carrying the log-forwards between steps
+  bermudan       the Bermudan swaption by regression at 80 forwards x 10 factors x
+                 1,000,000 paths (FP32-mixed), its FP64 price, and its adjoint Greeks
+  greeks         adjoint Greeks of the strip (1,000,000 paths, FP32 paths) over one
+                 pricing pass, checkpoints every 4 steps and every step
+  roofline       the FP64 path kernel against the FP64 peak, with its measured HBM traffic
+  cpu_baseline   the CPU restatement of the reference timed on the host cores
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-fp32]
+                    [--no-logstate] [--no-bermudan] [--no-greeks] [--only-fp64] [--copy-probe]
 """
 from __future__ import annotations
 
@@ -28,6 +36,10 @@ sys.path.insert(0, os.path.join(REPO, "finmath-lib_amd"))
 import numpy as np  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X published FP64 peak (vector and matrix)
+# HBM bytes per launch of lmm_fused_kernel<40,5,double> on the 40 x 5 x 1,000,000-path strip:
+# FETCH_SIZE and WRITE_SIZE counter passes scaled by a plain 1 GiB copy
+# (profiles/lmm_fused_fp64_c1_hbm_traffic.md)
+PATH_KERNEL_TRAFFIC_BYTES = 1.981e9
 FP32_PEAK_TFLOPS = 157.3     # MI355X published FP32 peak
 
 
@@ -186,7 +198,9 @@ def main():
                    "time_steps": wl.num_time_steps},
         "stage_ms": {"generator": stage[0], "path_kernel": stage[1], "reductions": stage[2], "call": stage[4]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": PATH_KERNEL_TRAFFIC_BYTES,
+                     "alu": "FP64 vector (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+                     "hbm_gbps_at_traffic": PATH_KERNEL_TRAFFIC_BYTES / (step_ms * 1e-3) / 1e9,
                      "kernel": "lmm_fused_kernel<40,5,double>", "flops_per_launch": flops},
     }
     if not args.no_fp32:
@@ -220,6 +234,14 @@ def main():
             "price": float(v4[0]), "mc_stderr": float(e4[0]),
             "fp64_ms_per_step": ms4d, "fp64_price": float(v4d[0]),
             "err_bp_vs_fp64": float(abs(v4[0] - v4d[0]) * 1e4), "mc_stderr_bp": float(e4d[0] * 1e4)}
+        if not args.no_greeks:
+            pm4, gm4, g4, gst4 = run_greeks(wl4, 2, 4)
+            line["bermudan"]["greeks"] = {
+                "outputs": "80 deltas, 80 forward vegas, 4 vol-parameter vegas, dV/dsigma table; exercise decisions "
+                           "held fixed (pathwise)",
+                "pricing_ms": pm4, "adjoint_ms_stride4": gm4, "ratio_stride4": gm4 / pm4,
+                "adjoint_stage_ms": {"generator": gst4[0], "forward_path_kernel": gst4[1], "reverse_sweep": gst4[3]},
+                "sum_delta": float(np.sum(g4["delta"])), "parameter_vega": [float(x) for x in g4["parameterVega"]]}
     if not args.no_greeks:
         wl3 = W.c3()
         pm, gm, g, gst = run_greeks(wl3, max(2, args.steps // 3), 4)
