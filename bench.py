@@ -39,7 +39,7 @@ FP64_PEAK_TFLOPS = 78.6      # MI355X published FP64 peak (vector and matrix)
 # HBM bytes per launch of lmm_fused_kernel<40,5,double> on the 40 x 5 x 1,000,000-path strip:
 # FETCH_SIZE and WRITE_SIZE counter passes scaled by a plain 1 GiB copy
 # (profiles/lmm_fused_fp64_c1_hbm_traffic.md)
-PATH_KERNEL_TRAFFIC_BYTES = 1.981e9
+PATH_KERNEL_TRAFFIC_BYTES = 1.967e9
 FP32_PEAK_TFLOPS = 157.3     # MI355X published FP32 peak
 
 

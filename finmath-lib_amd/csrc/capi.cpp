@@ -1028,8 +1028,8 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     HIP_CHECK(hipEventRecord(c->ev[0], s));
     if (c->fp32()) launch_lmm_adjoint_fp32(a, ga, s); else launch_lmm_adjoint_fp64(a, ga, s);
     HIP_CHECK(hipEventRecord(c->ev[1], s));
-    double* gout = (double*)c->d_grad_out.get((size_t)gstride * 2 * sizeof(double));
-    launch_reduce_partials(ga.wave_grad, ga.wave_grad, (int)num_waves, gstride, 0, gstride, P, gout, gout + gstride, s);
+    double* gout = (double*)c->d_grad_out.get((size_t)(gstride + reduce_columns_scratch(gstride)) * sizeof(double));
+    launch_reduce_columns(ga.wave_grad, (int)num_waves, gstride, gstride, gout + gstride, gout, s);
     std::vector<double> grad((size_t)gstride);
     HIP_CHECK(hipMemcpyAsync(grad.data(), gout, gstride * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipEventRecord(c->ev[2], s));

@@ -135,6 +135,12 @@ void launch_lmm_adjoint_fp32(const StepArgs& a, const AdjArgs& g, void* stream);
 void launch_reduce_partials(const double* wave_sum, const double* wave_m2, int num_waves, int stride,
                             int num_slots, int num_sums, int P, double* out_sum, double* out_m2, void* stream);
 
+// Column sums of a row-major table [num_rows][stride], first `cols` columns, in a fixed
+// order; scratch holds reduce_columns_scratch(cols) doubles.
+void launch_reduce_columns(const double* rows, int num_rows, int stride, int cols, double* scratch, double* out,
+                           void* stream);
+int reduce_columns_scratch(int cols);
+
 // Per-path swap values from period contributions and their scales, reduced into wave partials.
 void launch_swap_finalize(const double* contrib, int P, int num_swaps, const int32_t* col0, const int32_t* ncol,
                           const int32_t* slot, const double* scale, double* wave_sum, double* wave_m2,

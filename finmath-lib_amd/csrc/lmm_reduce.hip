@@ -87,7 +87,63 @@ swap_finalize_kernel(const double* __restrict__ contrib, int P, int num_swaps,
     }
 }
 
+// Column sums of a row-major table [num_rows][stride] (the per-wavefront gradient
+// partials): workgroup (x, y) sums 64 adjacent columns over row chunk y, each wavefront
+// taking every fourth row (coalesced 512-byte reads), Neumaier-compensated; a second
+// kernel adds the chunk partials in chunk order. Deterministic.
+constexpr int kColChunks = 16;
+
+__global__ void __launch_bounds__(256)
+reduce_columns_kernel(const double* __restrict__ rows, int num_rows, int stride, int cols,
+                      double* __restrict__ partial)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int col = blockIdx.x * 64 + lane;
+    const int chunk = (num_rows + (int)gridDim.y - 1) / (int)gridDim.y;
+    const int r0 = (int)blockIdx.y * chunk;
+    const int r1 = min(num_rows, r0 + chunk);
+    double sum = 0.0, c = 0.0;
+    if (col < cols) {
+#pragma unroll 4
+        for (int r = r0 + w; r < r1; r += 4) {
+            const double x = rows[(size_t)r * stride + col];
+            const double t = sum + x;
+            if (fabs(sum) >= fabs(x)) c += (sum - t) + x; else c += (x - t) + sum;
+            sum = t;
+        }
+    }
+    __shared__ double red[4][64];
+    red[w][lane] = sum + c;
+    __syncthreads();
+    if (w == 0 && col < cols)
+        partial[(size_t)blockIdx.y * cols + col] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
+
+__global__ void __launch_bounds__(256)
+sum_chunks_kernel(const double* __restrict__ partial, int chunks, int cols, double* __restrict__ out)
+{
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    if (col >= cols) return;
+    double s = 0.0;
+    for (int y = 0; y < chunks; y++) s += partial[(size_t)y * cols + col];
+    out[col] = s;
+}
+
 } // namespace
+
+void launch_reduce_columns(const double* rows, int num_rows, int stride, int cols, double* scratch, double* out,
+                           void* stream) {
+    if (cols <= 0) return;
+    const dim3 grid((unsigned)((cols + 63) / 64), (unsigned)kColChunks);
+    hipLaunchKernelGGL(reduce_columns_kernel, grid, dim3(256), 0, (hipStream_t)stream, rows, num_rows, stride, cols,
+                       scratch);
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(sum_chunks_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       scratch, kColChunks, cols, out);
+    HIP_CHECK(hipGetLastError());
+}
+
+int reduce_columns_scratch(int cols) { return kColChunks * cols; }
 
 void launch_reduce_partials(const double* wave_sum, const double* wave_m2, int num_waves, int stride,
                             int num_slots, int num_sums, int P, double* out_sum, double* out_m2, void* stream) {
