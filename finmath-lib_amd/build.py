@@ -38,6 +38,10 @@ SOURCES = {
     "lmm_step_fp64.hip": NO_CONTRACT,
     "lmm_step_fp32.hip": [],
     "lmm_step_fp64_logstate.hip": [],
+    "lmm_step_general_fp64.hip": NO_CONTRACT,
+    "lmm_step_pc_fp64.hip": NO_CONTRACT,
+    "lmm_step_general_fp32.hip": [],
+    "lmm_step_pc_fp32.hip": [],
     "lmm_reduce.hip": NO_CONTRACT,
     "lmm_bermudan.hip": NO_CONTRACT,
     "capi.cpp": NO_CONTRACT,

@@ -87,6 +87,7 @@ struct fm_lmm {
     std::vector<int32_t> first_alive, tenor_ti, row_of;
     int num_rows = 0;
     std::vector<double> init_value;     // time-0 process value per component
+    double n0_terminal = 1.0;           // N(T_0) under the terminal measure (deterministic)
 
     // generator
     StreamLayout lay;
@@ -97,7 +98,7 @@ struct fm_lmm {
     bool uniform_dt = false;            // all simulation steps have the same length
     std::vector<std::vector<double>> berm_fixing;   // fixing dates of the Bermudans of the last pass
 
-    DevBuf d_times, d_first, d_tenor_ti, d_rec, d_delta, d_L0, d_Y0, d_sqrt_dt;
+    DevBuf d_times, d_first, d_tenor_ti, d_rec, d_delta, d_L0, d_Y0, d_Y0cap, d_sqrt_dt;
     DevBuf d_states, d_poly, d_dw, d_row_of, d_paths, d_num;
     DevBuf d_products, d_periods, d_wave_sum, d_wave_m2, d_contrib, d_out, d_swapmeta, d_scale;
     DevBuf d_ckpt, d_adj_num, d_kcur, d_fsig, d_sig, d_cdiag, d_wave_grad, d_grad_out;
@@ -108,6 +109,18 @@ struct fm_lmm {
     int F() const { return cfg.num_factors; }
     int P() const { return cfg.num_paths; }
     bool fp32() const { return cfg.precision == FM_PRECISION_FP32_MIXED; }
+    bool terminal() const { return cfg.measure == FM_MEASURE_TERMINAL; }
+    bool pc() const { return cfg.scheme == FM_SCHEME_PREDICTOR_CORRECTOR || cfg.scheme == FM_SCHEME_PREDICTOR_CORRECTOR_FUNCTIONAL; }
+    // the state is carried between steps (EULER, PREDICTOR_CORRECTOR); the functional
+    // schemes re-derive it from the value (EulerSchemeFromProcessModel.java:234-237)
+    bool carry() const { return cfg.scheme == FM_SCHEME_EULER || cfg.scheme == FM_SCHEME_PREDICTOR_CORRECTOR; }
+    // carrying the state is the functional scheme when the inverse transform is the
+    // identity and no cap applies
+    bool carry_matters() const {
+        return carry() && (cfg.state_space == FM_STATE_SPACE_LOGNORMAL || !std::isinf(cfg.libor_cap));
+    }
+    // spot measure, functional (or equivalent) Euler: the kernel the adjoint differentiates
+    bool plain_euler() const { return !terminal() && !pc() && !carry_matters(); }
     size_t real_size() const { return fp32() ? sizeof(float) : sizeof(double); }
     int process_time_index(double t) const { return jsearch(times, round_tick(t)); }
     int libor_index(double t) const { return jsearch(tenor, round_tick(t)); }
@@ -148,15 +161,10 @@ void validate_config(const fm_lmm_config* c) {
         require(c->tenor[i + 1] > c->tenor[i], FM_ERR_INVALID_ARGUMENT, "tenor dates must increase");
     require(c->times[0] == 0.0 && c->tenor[0] == 0.0, FM_ERR_UNSUPPORTED, "simulation and tenor must start at time 0");
     require(c->measure == FM_MEASURE_SPOT || c->measure == FM_MEASURE_TERMINAL, FM_ERR_INVALID_ARGUMENT, "unknown measure");
-    require(c->measure == FM_MEASURE_SPOT, FM_ERR_UNSUPPORTED, "only the SPOT measure is implemented on the GPU");
     require(c->state_space == FM_STATE_SPACE_NORMAL || c->state_space == FM_STATE_SPACE_LOGNORMAL,
             FM_ERR_INVALID_ARGUMENT, "unknown state space");
     require(c->scheme >= FM_SCHEME_EULER && c->scheme <= FM_SCHEME_PREDICTOR_CORRECTOR_FUNCTIONAL,
             FM_ERR_INVALID_ARGUMENT, "unknown scheme");
-    // EULER and EULER_FUNCTIONAL coincide when the inverse transform is the identity and no cap applies
-    const bool functional_equiv = c->scheme == FM_SCHEME_EULER_FUNCTIONAL ||
-        (c->scheme == FM_SCHEME_EULER && c->state_space == FM_STATE_SPACE_NORMAL && std::isinf(c->libor_cap));
-    require(functional_equiv, FM_ERR_UNSUPPORTED, "only the EULER_FUNCTIONAL scheme is implemented on the GPU");
     require(c->precision == FM_PRECISION_FP64 || c->precision == FM_PRECISION_FP32_MIXED ||
             c->precision == FM_PRECISION_FP64_LOGSTATE, FM_ERR_INVALID_ARGUMENT, "unknown precision");
     require(!std::isnan(c->libor_cap), FM_ERR_INVALID_ARGUMENT, "liborCap must not be NaN");
@@ -217,9 +225,10 @@ void prepare_model(fm_lmm* c) {
             double* r = rec.data() + ((size_t)i * n + j) * rs;
             for (int k = 0; k < F; k++) r[k] = c->lam[((size_t)i * n + j) * F + k];
             r[fcap] = lognormal ? c->var[(size_t)i * n + j] * (-0.5) : 0.0;
-            const double d = delta[(size_t)j];
-            r[fcap + 1] = d / d;
-            r[fcap + 2] = 1.0 / d;
+            // Scalar.of(+-d).discount(L, d) = 1/(L (d/(+-d)) + 1/(+-d)) (LMM:1145,1167)
+            const double d = delta[(size_t)j], m = c->terminal() ? -d : d;
+            r[fcap + 1] = d / m;
+            r[fcap + 2] = 1.0 / m;
         }
     upload(c->d_times, c->times, s);
     upload(c->d_first, c->first_alive, s);
@@ -237,14 +246,23 @@ void prepare_model(fm_lmm* c) {
         upload(c->d_rec, rec, s);
         upload(c->d_L0, c->init_value, s);
     }
-    // initial log-state log(max(L_j(0), 0)), kept below log(cap)
+    // initial state (LMM.getInitialState:1081-1093): log(max(L_j(0), 0)) or L_j(0), uncapped
     std::vector<double> y0((size_t)n);
-    for (int j = 0; j < n; j++) {
-        double y = std::log(std::max(c->L0[(size_t)j], 0.0));
-        if (!std::isinf(c->cfg.libor_cap)) y = std::min(y, std::log(c->cfg.libor_cap));
-        y0[(size_t)j] = y;
-    }
+    for (int j = 0; j < n; j++)
+        y0[(size_t)j] = lognormal ? std::log(std::max(c->L0[(size_t)j], 0.0)) : c->L0[(size_t)j];
     upload(c->d_Y0, y0, s);
+    // the log-forward kernel caps its log-states (Y <= log(cap))
+    std::vector<double> y0c(y0);
+    if (!std::isinf(c->cfg.libor_cap))
+        for (double& y : y0c) y = std::min(y, std::log(c->cfg.libor_cap));
+    upload(c->d_Y0cap, y0c, s);
+    // terminal measure N(T_0): the Scalar chain of the time-0 values (LMM:1028-1042,
+    // stochastic/Scalar.java:320-328)
+    c->n0_terminal = 1.0;
+    for (int j = 0; j < n; j++) {
+        const double v = c->n0_terminal;
+        c->n0_terminal = 1.0 / (c->init_value[(size_t)j] * (delta[(size_t)j] / v) + 1.0 / v);
+    }
     HIP_CHECK(hipStreamSynchronize(s));
     c->tables_dirty = false;
 }
@@ -323,8 +341,26 @@ Lowered lower_products(fm_lmm* c, const fm_product* prods, int count) {
     };
     auto check_available = [&](int ti, int k) {
         require(ti >= 0 && ti <= c->nT(), FM_ERR_INVALID_ARGUMENT, "product date outside the simulation horizon");
+        if (c->terminal()) {
+            // the kernel holds N(T_k) from the time index of T_k on (and the one before it)
+            int kn = 0;
+            while (kn <= n && c->tenor_ti[(size_t)kn] <= ti) kn++;
+            require(k == kn - 1, FM_ERR_UNSUPPORTED, "numeraire date not on the simulation time grid");
+            return;
+        }
         const int kc = kcur_at(c, ti);
         require(k == kc || k == kc - 1, FM_ERR_UNSUPPORTED, "numeraire date not adjacent to the evaluation date");
+    };
+    // Terminal measure: N(T_pay) is known only at the time index of T_pay, so a payoff of
+    // a forward fixed earlier is evaluated there; the forward is frozen in between (its
+    // process value aliases the fixing value, EulerSchemeFromProcessModel.java:279-290).
+    auto defer_to_payment = [&](KProduct& k, int j) {
+        if (!c->terminal()) return;
+        const int te = c->tenor_ti[(size_t)k.num_k];
+        if (te <= k.eval_index || te > c->nT()) return;
+        for (int ti = k.eval_index; ti < te; ti++)
+            require(c->first_alive[(size_t)ti] > j, FM_ERR_UNSUPPORTED, "forward not fixed before its payment date");
+        k.eval_index = te;
     };
     L.slot_of.assign((size_t)count, -1);
     for (int q = 0; q < count; q++) {
@@ -383,6 +419,7 @@ Lowered lower_products(fm_lmm* c, const fm_product* prods, int count) {
             L.per.push_back(pe);
             k.param_off = (int32_t)L.per.size() - 1;
             k.num_k = numeraire_index(Tm + dl);
+            defer_to_payment(k, j);
             check_available(k.eval_index, k.num_k);
             k.out_slot = L.num_slots++;
             L.slot_of[(size_t)q] = k.out_slot;
@@ -412,6 +449,7 @@ Lowered lower_products(fm_lmm* c, const fm_product* prods, int count) {
                 L.per.push_back(pe);
                 k.param_off = (int32_t)L.per.size() - 1;
                 k.num_k = numeraire_index(pd);
+                defer_to_payment(k, j);
                 check_available(k.eval_index, k.num_k);
                 k.out_slot = L.num_contrib++;
                 L.contrib_k.push_back(k.num_k);
@@ -451,6 +489,7 @@ Lowered lower_products(fm_lmm* c, const fm_product* prods, int count) {
                 L.per.push_back(pe);
                 k.param_off = (int32_t)L.per.size() - 1;
                 k.num_k = numeraire_index(pd);
+                defer_to_payment(k, j);
                 check_available(k.eval_index, k.num_k);
                 k.out_slot = L.num_contrib++;
                 L.contrib_k.push_back(k.num_k);
@@ -505,6 +544,22 @@ Lowered lower_products(fm_lmm* c, const fm_product* prods, int count) {
     return L;
 }
 
+// Adjusted numeraire at time 0 by which the products multiply (Swaption.java:192-196):
+// N(0) avg(N(0)/N(0)) / P^d(0), a Scalar (Scalar.div: (1/adj) (N zb), stochastic/Scalar.java:296-298).
+double numeraire0(const fm_lmm* c, const std::vector<double>& adj) {
+    const double N0 = c->terminal() ? c->n0_terminal : 1.0;
+    return c->has_pd ? (1.0 / adj[0]) * (N0 * c->zb[0]) : N0;
+}
+
+// Scale turning mean(payoff / N(T_k)) into the product value: the numeraire adjustment
+// N(T) avg(N(0)/N(T)) / P^d(T) of T_k (LMM:869-874) and, under the terminal measure, the
+// time-0 numeraire (1 under the spot measure without discount curve).
+double value_scale(const fm_lmm* c, int k, const std::vector<double>& adj) {
+    double sc = (k >= 0 && c->has_pd) ? adj[(size_t)k] / c->zb[(size_t)k] : 1.0;
+    if (c->terminal()) sc *= numeraire0(c, adj);
+    return sc;
+}
+
 std::vector<double> adjustment_chain(const fm_lmm* c) {
     const int n = c->n();
     std::vector<double> adj((size_t)n + 1, 1.0);
@@ -534,7 +589,8 @@ BermBasisRef basis_ref(const fm_lmm* c, const BermPlan& b, int e, const std::vec
     r.kind = b.basis;
     r.col0 = b.ex_col[(size_t)e];
     const int k = b.ex_k[(size_t)e];
-    r.nmode = !c->has_pd ? 0 : (k <= 1 ? 1 : 2);
+    const bool scalar_num = c->terminal() ? (k == 0 || k == c->n()) : k <= 1;
+    r.nmode = !c->has_pd ? 0 : (scalar_num ? 1 : 2);
     r.zb = c->zb[(size_t)k];
     r.adj = adj[(size_t)k];
     return r;
@@ -573,7 +629,7 @@ void run_bermudan(fm_lmm* c, const BermPlan& b, int bi, const std::vector<double
     st.values = vals; st.under = under; st.ex_index = ex;
     st.partials = part;
     st.wave_sum = wsum; st.wave_m2 = wm2; st.stride = stride; st.slot = b.slot;
-    st.n0 = c->has_pd ? (1.0 / adj[0]) * (1.0 * c->zb[0]) : 1.0;
+    st.n0 = numeraire0(c, adj);
     int done = m;       // periods >= done have been added
     int prev = -1;      // exercise date whose decision is still to be applied
     for (int e = nex - 1; e >= -1; e--) {
@@ -635,6 +691,9 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     a.L0 = c->d_L0.as<double>();
     a.Y0 = c->d_Y0.as<double>();
     a.logcap = std::isinf(c->cfg.libor_cap) ? HUGE_VAL : std::log(c->cfg.libor_cap);
+    a.terminal = c->terminal() ? 1 : 0;
+    a.carry = c->carry() ? 1 : 0;
+    a.pc = c->pc() ? 1 : 0;
     a.dw = c->d_dw.p;
     a.cap = c->cfg.libor_cap;
     a.scheme = c->cfg.scheme;
@@ -651,10 +710,27 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     a.ckpt = ckpt;
     a.ckpt_stride = ckpt ? ckpt_stride : 1;
     tm.mark(2);
-    if (c->fp32()) launch_lmm_step_fp32(a, s);
-    else if (c->cfg.precision == FM_PRECISION_FP64_LOGSTATE && c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL)
+    // kernel variant: the spot-measure Euler kernels, else the general kernel (terminal
+    // measure, carried state, predictor-corrector)
+    // the log-forward kernel caps the log-state, as EULER_FUNCTIONAL does up to rounding;
+    // EULER with a finite cap caps only the value and runs on the general kernel
+    const bool logstate_kernel = c->cfg.precision == FM_PRECISION_FP64_LOGSTATE &&
+        c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL && !c->terminal() && !c->pc() &&
+        (!c->carry() || std::isinf(c->cfg.libor_cap));
+    if (logstate_kernel) {
+        a.Y0 = c->d_Y0cap.as<double>();
         launch_lmm_step_fp64_logstate(a, s);
-    else launch_lmm_step_fp64(a, s);
+    }
+    else if (c->plain_euler()) {
+        if (c->fp32()) launch_lmm_step_fp32(a, s);
+        else launch_lmm_step_fp64(a, s);
+    } else if (c->pc()) {
+        if (c->fp32()) launch_lmm_step_pc_fp32(a, s);
+        else launch_lmm_step_pc_fp64(a, s);
+    } else {
+        if (c->fp32()) launch_lmm_step_general_fp32(a, s);
+        else launch_lmm_step_general_fp64(a, s);
+    }
     tm.mark(3);
     // numeraire-adjustment averages first (the swap contributions need them)
     const int cols = L.num_slots + n + 1;
@@ -664,12 +740,21 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     HIP_CHECK(hipMemcpyAsync(sums.data(), dout, cols * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(m2s.data(), dout + cols, cols * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    // zb_k = average of N(0)/N(T_k) (LMM:872); N(T_0) = 1 and N(T_1) is deterministic
+    // zb_k = average of N(0)/N(T_k) (LMM:872)
     c->zb.assign((size_t)n + 1, 1.0);
-    for (int k = 1; k <= n; k++) c->zb[(size_t)k] = sums[(size_t)L.num_slots + k] / (double)P;
-    if (n >= 1) {
-        const double N1 = c->init_value[0] * ((c->tenor[1] - c->tenor[0]) * 1.0) + 1.0;
-        c->zb[1] = (1.0 / N1) * 1.0;
+    if (c->terminal()) {
+        // N(T_0) and N(T_n) = 1 are Scalars: (1/N0) N0 and (1/1) N0
+        const double N0 = c->n0_terminal;
+        for (int k = 1; k < n; k++) c->zb[(size_t)k] = sums[(size_t)L.num_slots + k] / (double)P;
+        c->zb[0] = (1.0 / N0) * N0;
+        c->zb[(size_t)n] = N0;
+    } else {
+        // N(T_0) = 1 and N(T_1) is deterministic
+        for (int k = 1; k <= n; k++) c->zb[(size_t)k] = sums[(size_t)L.num_slots + k] / (double)P;
+        if (n >= 1) {
+            const double N1 = c->init_value[0] * ((c->tenor[1] - c->tenor[0]) * 1.0) + 1.0;
+            c->zb[1] = (1.0 / N1) * 1.0;
+        }
     }
     if (!L.swap_slot.empty() || !L.berm.empty()) {
         const std::vector<double> adj = adjustment_chain(c);
@@ -677,7 +762,7 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
             std::vector<double> scale((size_t)L.num_contrib, 1.0);
             for (int col = 0; col < L.num_contrib; col++) {
                 const int k = L.contrib_k[(size_t)col];
-                if (c->has_pd && k >= 0) scale[(size_t)col] = adj[(size_t)k] / c->zb[(size_t)k];
+                if (k >= 0) scale[(size_t)col] = value_scale(c, k, adj);
             }
             const double* dscale = upload(c->d_scale, scale, s);
             std::vector<int32_t> meta;
@@ -839,15 +924,21 @@ int fm_lmm_get_numeraire(fm_lmm* c, double time, double* out) {
     DeviceGuard g(c->device);
     const int k = c->libor_index(time);
     require(k >= 0 && k <= c->n(), FM_ERR_UNSUPPORTED, "numeraire requested off the tenor grid");
-    require(c->tenor_ti[(size_t)std::max(k - 1, 0)] <= c->nT(), FM_ERR_INVALID_ARGUMENT, "numeraire date beyond the simulation");
+    // spot: N(T_k) is known at the time index of T_{k-1}; terminal: at that of T_k
+    const int known = c->terminal() ? k : std::max(k - 1, 0);
+    require(c->tenor_ti[(size_t)known] <= c->nT(), FM_ERR_INVALID_ARGUMENT, "numeraire date beyond the simulation");
     const int P = c->P();
-    if (k == 0) std::fill(out, out + P, 1.0);
+    // deterministic numeraires are Scalars: N(T_0) = 1 (spot), N(T_0) and N(T_n) = 1 (terminal)
+    const bool scalar_num = c->terminal() ? (k == 0 || k == c->n()) : k == 0;
+    if (scalar_num) std::fill(out, out + P, c->terminal() && k == 0 ? c->n0_terminal : 1.0);
     else HIP_CHECK(hipMemcpy(out, c->d_num.as<double>() + (size_t)k * P, P * sizeof(double), hipMemcpyDeviceToHost));
     if (c->has_pd) {
-        // N(T) * avg(N(0)/N(T)) / P^d(T) (LMM:869-874)
+        // N(T) * avg(N(0)/N(T)) / P^d(T) (LMM:869-874); Scalar numeraires divide as
+        // (1/adj) (N zb) (stochastic/Scalar.java:296-298)
         const std::vector<double> adj = adjustment_chain(c);
         const double zbk = c->zb[(size_t)k], ak = adj[(size_t)k];
-        if (k <= 1) { for (int p = 0; p < P; p++) out[p] = (1.0 / ak) * (out[p] * zbk); }
+        const bool scalar_adj = c->terminal() ? scalar_num : k <= 1;
+        if (scalar_adj) { for (int p = 0; p < P; p++) out[p] = (1.0 / ak) * (out[p] * zbk); }
         else { for (int p = 0; p < P; p++) out[p] = (out[p] * zbk) / ak; }
     }
     return FM_OK;
@@ -891,8 +982,7 @@ int fm_lmm_value_products(fm_lmm* c, const fm_product* products, int32_t count, 
     for (int q = 0; q < count; q++) {
         const int slot = L.slot_of[(size_t)q];
         const int k = L.slot_k[(size_t)slot];
-        double scale = 1.0;
-        if (k >= 0 && c->has_pd) scale = adj[(size_t)k] / c->zb[(size_t)k];
+        const double scale = k >= 0 ? value_scale(c, k, adj) : 1.0;
         const double mean = sums[(size_t)slot] / P;
         const double var = m2s[(size_t)slot] / P;
         values[q] = scale * mean;
@@ -926,6 +1016,8 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     require(c->has_vf, FM_ERR_STATE, "volatility factorization not set (fm_lmm_set_volatility_factorization)");
     require(c->cfg.precision != FM_PRECISION_FP64_LOGSTATE, FM_ERR_UNSUPPORTED,
             "the adjoint runs on the FP64 and FP32_MIXED path kernels");
+    require(c->plain_euler(), FM_ERR_UNSUPPORTED,
+            "the adjoint differentiates the spot-measure Euler scheme (EULER_FUNCTIONAL)");
     DeviceGuard g(c->device);
     prepare_model(c);
     const int n = c->n(), nT = c->nT(), P = c->P();

@@ -93,8 +93,12 @@ struct StepArgs {
     int lognormal, capped;       // state space is LOGNORMAL; the liborCap applies
     const double* delta;         // device n: T_{j+1}-T_j
     const double* L0;            // device n: initial forwards in value space
-    const double* Y0;            // device n: initial log-forwards (log-state kernels)
-    double logcap;               // log(cap) for the log-state kernels
+    const double* Y0;            // device n: initial states (log(max(L0, 0)) or L0) of the carried schemes;
+                                 // for the log-forward kernel kept below log(cap)
+    double logcap;               // log(cap) for the log-forward kernel
+    int terminal;                // TERMINAL measure (general kernel)
+    int carry;                   // EULER / PREDICTOR_CORRECTOR: the state is carried (general kernel)
+    int pc;                      // predictor-corrector (general kernel)
     const void* dw;              // device dW [P][nT][F]
     double cap;
     int32_t scheme;
@@ -117,6 +121,11 @@ struct StepArgs {
 void launch_lmm_step_fp64(const StepArgs& a, void* stream);
 void launch_lmm_step_fp32(const StepArgs& a, void* stream);
 void launch_lmm_step_fp64_logstate(const StepArgs& a, void* stream);
+// General variant (terminal measure, carried state, predictor-corrector), see lmm_step.hpp.
+void launch_lmm_step_general_fp64(const StepArgs& a, void* stream);
+void launch_lmm_step_general_fp32(const StepArgs& a, void* stream);
+void launch_lmm_step_pc_fp64(const StepArgs& a, void* stream);
+void launch_lmm_step_pc_fp32(const StepArgs& a, void* stream);
 
 // Reverse sweep (lmm_adjoint.hpp): adjoint arguments and launchers.
 struct AdjArgs {

@@ -97,13 +97,20 @@ __device__ __forceinline__ Rec<FCAP, real> rec_at(const real* __restrict__ rec, 
     return Rec<FCAP, real>{(const real*)__builtin_assume_aligned(r, Rec<FCAP, real>::size * sizeof(real))};
 }
 
-// One Euler step t_i -> t_{i+1} of all alive components, in place. Shared by the
-// pricing pass and the adjoint recompute so both produce identical paths.
-// LOGSTATE: the array holds the log-forwards Y_j and the step needs one exp per
-// component (the reference re-takes log(exp(Y)) every step, EULER_FUNCTIONAL;
-// carrying Y differs from it by rounding only, the cap applied as Y <= log(cap)).
+// Value of a state: applyStateSpaceTransform (LMM:1200-1212), exp in the log-normal
+// state space, then the cap.
+template <typename real>
+__device__ __forceinline__ real to_value(real y, bool lognormal, bool capped, real cap) {
+    const real v = lognormal ? dev_exp<real>(y) : y;
+    return capped ? jmin(v, cap) : v;
+}
+
 // One component's update inside euler_step (alive: the component moves, else it is
 // computed on a valid record and discarded).
+// LOGSTATE (log-normal state space only): the array holds the log-forward Y_j between
+// steps and the step needs one exp per component instead of the log(exp(.)) round trip
+// of EULER_FUNCTIONAL; the cap is applied to the log-state (Y <= log(cap)), which is what
+// EULER_FUNCTIONAL's log(min(exp(Y), cap)) gives up to rounding.
 template <int NCAP, int FCAP, typename real, bool LOGSTATE>
 __device__ __forceinline__ void euler_component(real (&L)[NCAP], real (&S)[FCAP], int j, bool alive, int jr, int n,
                                                 int i, real dt, const real (&dW)[FCAP], const real* __restrict__ rec,
@@ -143,9 +150,7 @@ __device__ __forceinline__ void euler_component(real (&L)[NCAP], real (&S)[FCAP]
 // without branches inside a group, so the loads and transcendentals of four
 // components overlap; a component outside [first, n) of a partly alive group is
 // computed and discarded (w = 0 adds exact zeros to S).
-// LOGSTATE: the array holds the log-forwards Y_j and the step needs one exp per
-// component (the reference re-takes log(exp(Y)) every step, EULER_FUNCTIONAL;
-// carrying Y differs from it by rounding only, the cap applied as Y <= log(cap)).
+// LOGSTATE: the array holds the log-forwards (see euler_component).
 template <int NCAP, int FCAP, typename real, bool LOGSTATE = false, int GROUP = 1>
 __device__ __forceinline__ void euler_step(real (&L)[NCAP], int n, int i, int first, real dt, const real (&dW)[FCAP],
                                            const real* __restrict__ rec, bool lognormal, bool capped, real cap,
@@ -169,6 +174,92 @@ __device__ __forceinline__ void euler_step(real (&L)[NCAP], int n, int i, int fi
     }
 }
 
+// ---- general variant: terminal measure, carried state, predictor-corrector -------------
+// Run-time choices of the general path kernel (uniform over the launch).
+template <typename real>
+struct Variant {
+    bool lognormal, capped;
+    bool carry;        // EULER / PREDICTOR_CORRECTOR: the state is carried between steps
+                       // (else the *_FUNCTIONAL schemes re-derive it from the value, Euler:234-237)
+    real cap;
+};
+
+// Drift of one component and the update of the factor sums S, in the measure's order
+// (LMM:1132-1181): SPOT adds w lambda_j to S and then takes mu_j = S . lambda_j;
+// TERMINAL takes mu_j = S . lambda_j of the later components first and then adds
+// w lambda_j. The record's d/d and 1/d hold -1 and -1/d under TERMINAL
+// (Scalar.of(-d).discount(L, d), stochastic/Scalar.java:320-328), so w carries the sign.
+template <int FCAP, typename real, bool TERMINAL>
+__device__ __forceinline__ real drift_component(real (&S)[FCAP], const Rec<FCAP, real>& R, real Lj, bool lognormal) {
+    real w = dev_div<real>((real)1.0, Lj * R.c1() + R.c2());
+    if (lognormal) w = w * Lj;
+    real mu;
+    if (TERMINAL) {
+        mu = S[0] * R.lam(0);
+#pragma unroll
+        for (int k = 1; k < FCAP; k++) mu = mu + S[k] * R.lam(k);
+#pragma unroll
+        for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * R.lam(k);
+    } else {
+#pragma unroll
+        for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * R.lam(k);
+        mu = S[0] * R.lam(0);
+#pragma unroll
+        for (int k = 1; k < FCAP; k++) mu = mu + S[k] * R.lam(k);
+    }
+    return mu + R.hv();
+}
+
+// One step of the general variant. X holds the value (functional schemes) or the state
+// (carried schemes) of each component. The predictor pass follows Euler:232-276; with PC
+// the corrector re-evaluates the drift on the predicted values and adds
+// ((mu_pred - mu) / 2) dt to the state (Euler:292-314). MU keeps the predictor's drift.
+template <int NCAP, int FCAP, typename real, bool TERMINAL, bool PC>
+__device__ __forceinline__ void general_step(real (&X)[NCAP], real (&MU)[NCAP], int n, int i, int first, real dt,
+                                             const real (&dW)[FCAP], const real* __restrict__ rec,
+                                             const Variant<real>& v, real* __restrict__ paths,
+                                             const int32_t* __restrict__ row_next, int64_t p, int P, bool active) {
+    real S[FCAP];
+#pragma unroll
+    for (int k = 0; k < FCAP; k++) S[k] = (real)0;
+#pragma unroll
+    for (int jj = 0; jj < NCAP; jj++) {
+        const int j = TERMINAL ? NCAP - 1 - jj : jj;
+        if (j < first || j >= n) continue;                  // uniform
+        const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, i, j);
+        const real Lj = v.carry ? to_value<real>(X[j], v.lognormal, v.capped, v.cap) : X[j];
+        const real mu = drift_component<FCAP, real, TERMINAL>(S, R, Lj, v.lognormal);
+        real Y = v.carry ? X[j] : (v.lognormal ? dev_log<real>(Lj) : Lj);
+        Y = Y + mu * dt;
+#pragma unroll
+        for (int k = 0; k < FCAP; k++) Y = Y + dW[k] * R.lam(k);
+        if (PC) {
+            MU[j] = mu;
+            X[j] = Y;
+        } else {
+            const real Ln = to_value<real>(Y, v.lognormal, v.capped, v.cap);
+            X[j] = v.carry ? Y : Ln;
+            if (paths && active) paths[(size_t)row_next[j] * P + p] = Ln;
+        }
+    }
+    if (PC) {
+#pragma unroll
+        for (int k = 0; k < FCAP; k++) S[k] = (real)0;
+#pragma unroll
+        for (int jj = 0; jj < NCAP; jj++) {
+            const int j = TERMINAL ? NCAP - 1 - jj : jj;
+            if (j < first || j >= n) continue;
+            const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, i, j);
+            const real Lp = to_value<real>(X[j], v.lognormal, v.capped, v.cap);    // predicted value
+            const real mup = drift_component<FCAP, real, TERMINAL>(S, R, Lp, v.lognormal);
+            const real Y = X[j] + ((mup - MU[j]) * (real)0.5) * dt;
+            const real Ln = to_value<real>(Y, v.lognormal, v.capped, v.cap);
+            X[j] = v.carry ? Y : Ln;
+            if (paths && active) paths[(size_t)row_next[j] * P + p] = Ln;
+        }
+    }
+}
+
 template <int FCAP, typename real>
 __device__ __forceinline__ void load_increments(real (&dW)[FCAP], const real* __restrict__ dw, int64_t pc, int nT,
                                                 int i, int F) {
@@ -176,9 +267,17 @@ __device__ __forceinline__ void load_increments(real (&dW)[FCAP], const real* __
     for (int k = 0; k < FCAP; k++) dW[k] = (k < F) ? dw[((size_t)pc * nT + i) * F + k] : (real)0;
 }
 
-template <int NCAP, int FCAP, typename real, bool LOGSTATE = false>
+// Variants of the path kernel (template argument VAR):
+//   kVarFunctional  spot measure, EULER_FUNCTIONAL (the array holds the values)
+//   kVarLogState    spot measure, log-normal, the log-forwards carried between steps and
+//                   capped as log-states, contraction allowed
+//   kVarGeneral     spot or terminal measure, any state space, functional or carried state
+//   kVarGeneralPC   kVarGeneral with the predictor-corrector
+constexpr int kVarFunctional = 0, kVarLogState = 1, kVarGeneral = 2, kVarGeneralPC = 3;
+
+template <int NCAP, int FCAP, typename real, int VAR = kVarFunctional>
 __global__ void __launch_bounds__(kStepThreads)
-__attribute__((amdgpu_waves_per_eu((LOGSTATE && NCAP == 40) ? 3 : 1)))
+__attribute__((amdgpu_waves_per_eu((VAR == kVarLogState && NCAP == 40) ? 3 : 1)))
 lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* __restrict__ delta,
                  const double* __restrict__ times, const int32_t* __restrict__ first_alive,
                  const int32_t* __restrict__ tenor_ti, const KProduct* __restrict__ products,
@@ -188,6 +287,9 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
                  const int32_t* __restrict__ row_of, double* __restrict__ numeraire_out,
                  real* __restrict__ ckpt)
 {
+    constexpr bool LOGSTATE = VAR == kVarLogState;
+    constexpr bool GENERAL = VAR >= kVarGeneral;
+    constexpr bool PC = VAR == kVarGeneralPC;
     const int64_t p = (int64_t)blockIdx.x * kStepThreads + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int64_t wave = p >> 6;
@@ -199,16 +301,22 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
     const bool lognormal = a.lognormal != 0;
     const bool capped = a.capped != 0;
     const real cap = (real)a.cap;
+    const Variant<real> var{lognormal, capped, GENERAL && a.carry != 0, cap};
+    const bool terminal = GENERAL && a.terminal != 0;
+    // the array holds states (carried schemes) or values; val gives the value
     const real logcap = (real)a.logcap;
-    // value of component j from the state (forward rate; exp of the log-state)
-    auto val = [](real x) -> real { return LOGSTATE ? dev_exp<real>(x) : x; };
+    auto val = [&](real x) -> real {
+        return LOGSTATE ? dev_exp<real>(x) : (var.carry ? to_value<real>(x, lognormal, capped, cap) : x);
+    };
 
     real L[NCAP];
 #pragma unroll
-    for (int j = 0; j < NCAP; j++) L[j] = (j < n) ? (real)(LOGSTATE ? a.Y0[j] : a.L0[j]) : (real)0;
+    for (int j = 0; j < NCAP; j++) L[j] = (j < n) ? (real)((LOGSTATE || var.carry) ? a.Y0[j] : a.L0[j]) : (real)0;
+    real MU[NCAP];      // predictor drift (PC only)
 
-    double N = 1.0, Nprev = 1.0;   // unadjusted numeraire N(T_kcur) and N(T_kcur-1)
-    int kcur = 0;
+    double N = 1.0, Nprev = 1.0;   // unadjusted numeraire N(T_knum) and the one before it
+    double N0 = 1.0;               // N(T_0) (terminal measure)
+    int kcur = 0, knum = 0;
     int qp = 0;
 
     for (int ti = 0; ti <= nT; ti++) {
@@ -218,15 +326,37 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
 #pragma unroll
             for (int j = 0; j < NCAP; j++) if (j < n) ckpt[(base + j) * P + p] = val(L[j]);
         }
-        // ---- numeraire N(T_{k+1}) = N(T_k).accrue(L_k(T_k), d_k) once L_k is final
-        while (kcur < n && tenor_ti[kcur] <= ti) {
+        // ---- terminal measure: N(T_k) = prod_{l >= k} 1/(1 + L_l(T_k) d_l) at the time index
+        // of T_k (LMM:1020-1042); the time-0 forwards are Scalars, whose discount is
+        // 1/(L (d/v) + 1/v) (stochastic/Scalar.java:320-328); later ones are random
+        // variables: 1/(L d + 1), then v / (1 + L d) (RandomVariableFromDoubleArray.java:1308-1333)
+        while (terminal && kcur <= n && tenor_ti[kcur] <= ti) {
+            double Nn = 1.0;
+#pragma unroll
+            for (int l = 0; l < NCAP; l++) {
+                if (l < kcur || l >= n) continue;
+                const double Ll = (double)val(L[l]);
+                const double dl = delta[l];
+                if (ti == 0) Nn = 1.0 / (Ll * (dl / Nn) + 1.0 / Nn);
+                else if (l == kcur) Nn = 1.0 / (Ll * dl + 1.0);
+                else Nn = Nn / (1.0 + Ll * dl);
+            }
+            if (kcur == 0) N0 = Nn;
+            Nprev = N; N = Nn; knum = kcur; kcur++;
+            // sum of N(T_0)/N(T_k) over paths (numeraire adjustment average)
+            const double s = wave_reduce_sum(active ? (1.0 / N) * N0 : 0.0);
+            if (lane == 0 && cnt > 0) wave_sum[(size_t)wave * stride + a.num_slots + knum] = s;
+            if (numeraire_out && active) numeraire_out[(size_t)knum * P + p] = N;
+        }
+        // ---- spot measure: N(T_{k+1}) = N(T_k).accrue(L_k(T_k), d_k) once L_k is final
+        while (!terminal && kcur < n && tenor_ti[kcur] <= ti) {
             const double Lk = (double)val(pick<NCAP, real>(L, kcur));
             const double dk = delta[kcur];
             double Nn;
             if (kcur == 0)      Nn = Lk * (dk * 1.0) + 1.0;      // Scalar.accrue of Scalar 1.0
             else if (kcur == 1) Nn = Lk * (dk * N) + N;          // Scalar.accrue of a stochastic rate
             else                Nn = N * (1 + Lk * dk);          // RandomVariableFromDoubleArray.accrue
-            Nprev = N; N = Nn; kcur++;
+            Nprev = N; N = Nn; kcur++; knum = kcur;
             // sum of 1/N(T_k) over paths (numeraire adjustment average)
             const double s = wave_reduce_sum(active ? (1.0 / N) * 1.0 : 0.0);
             if (lane == 0 && cnt > 0) wave_sum[(size_t)wave * stride + a.num_slots + kcur] = s;
@@ -237,7 +367,7 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
         while (qp < a.num_products && products[qp].eval_index == ti) {
             const KProduct pr = products[qp];
             qp++;
-            const double Nsel = (pr.num_k == kcur) ? N : Nprev;
+            const double Nsel = (pr.num_k == knum) ? N : Nprev;
             if (pr.type == FM_PRODUCT_SWAPTION) {
                 real V = (real)0;
 #pragma unroll
@@ -320,9 +450,17 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
         // ---- Euler step ti -> ti+1
         real dW[FCAP];
         load_increments<FCAP, real>(dW, dw, pc, nT, ti, F);
-        euler_step<NCAP, FCAP, real, LOGSTATE>(L, n, ti, first_alive[ti], (real)(times[ti + 1] - times[ti]), dW, rec,
-                                               lognormal, capped, cap, logcap, paths,
-                                               row_of + (size_t)(ti + 1) * n, p, P, active);
+        const real dt = (real)(times[ti + 1] - times[ti]);
+        const int32_t* row_next = row_of + (size_t)(ti + 1) * n;
+        if constexpr (GENERAL) {
+            if (terminal) general_step<NCAP, FCAP, real, true, PC>(L, MU, n, ti, first_alive[ti], dt, dW, rec, var,
+                                                                   paths, row_next, p, P, active);
+            else general_step<NCAP, FCAP, real, false, PC>(L, MU, n, ti, first_alive[ti], dt, dW, rec, var, paths,
+                                                           row_next, p, P, active);
+        } else {
+            euler_step<NCAP, FCAP, real, LOGSTATE>(L, n, ti, first_alive[ti], dt, dW, rec, lognormal, capped, cap,
+                                                   logcap, paths, row_next, p, P, active);
+        }
     }
 }
 

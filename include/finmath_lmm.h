@@ -63,12 +63,19 @@ extern "C" {
 /* FP64 arithmetic that carries the log-forwards between steps (one exp per alive
  * forward-step instead of the reference's log(exp(.)) round trip) and lets the
  * compiler fuse multiply-adds: the same values as FP64 up to a few units in the last
- * place per step. No adjoint. */
+ * place per step. Applies to the spot measure, log-normal state space and the Euler
+ * schemes (EULER only with an infinite cap: the log-forward kernel caps log-states);
+ * every other configuration runs as FM_PRECISION_FP64. No adjoint. */
 #define FM_PRECISION_FP64_LOGSTATE 2
 
 typedef struct fm_lmm fm_lmm;   /* one simulation: model + Brownian driver + Euler process */
 
-/* Shape and options of a simulation.
+/* Shape and options of a simulation. Every measure, state space and scheme of the
+ * reference is supported: SPOT and TERMINAL (LIBORMarketModelFromCovarianceModel.java:
+ * 1132-1181, numeraire 1020-1066), NORMAL and LOGNORMAL, EULER / PREDICTOR_CORRECTOR
+ * (state carried between steps) and their _FUNCTIONAL forms (state re-derived from the
+ * value), EulerSchemeFromProcessModel.java:232-314. Under TERMINAL, payoffs of forwards
+ * fixed before their payment date are evaluated at the payment date's time index.
  * Replaces the constructor arguments of
  *   TimeDiscretizationFromArray (time/TimeDiscretizationFromArray.java:210-217),
  *   BrownianMotionFromMersenneRandomNumbers(td, factors, paths, seed)
@@ -222,8 +229,10 @@ int fm_lmm_set_volatility_factorization(fm_lmm* ctx, const double* volatility, c
                                         const double* variance_factor);
 
 /* Gradient of V = sum_q weights[q] * value_q (weights NULL -> all 1) by a
- * hand-written reverse sweep of the Euler scheme with forward-state checkpoints
- * every checkpoint_stride steps (the sweep recomputes the states in between):
+ * hand-written reverse sweep of the Euler scheme with forward-state checkpoints.
+ * Spot measure, EULER_FUNCTIONAL (or EULER where it coincides: normal state space
+ * without cap) only; other measures and schemes give FM_ERR_UNSUPPORTED. Checkpoints
+ * are taken every checkpoint_stride steps (the sweep recomputes the states in between):
  *   value                      = V
  *   d_initial_forwards[j]      = dV/dL_j(0)              (n entries, may be NULL)
  *   d_volatility[i*n + j]      = dV/dsigma_j(t_i)        (nT*n entries, may be NULL)

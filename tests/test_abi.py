@@ -49,8 +49,8 @@ def _ctx(**kw):
     (dict(num_paths=0), _abi.IllegalArgumentException),
     (dict(times=np.array([0.0, 0.5, 0.5])), _abi.IllegalArgumentException),
     (dict(tenor=np.array([0.0, 1.0, 0.5])), _abi.IllegalArgumentException),
-    (dict(measure="TERMINAL"), _abi.UnsupportedOperationException),
-    (dict(scheme="PREDICTOR_CORRECTOR"), _abi.UnsupportedOperationException),
+    (dict(num_factors=11), _abi.UnsupportedOperationException),
+    (dict(tenor=np.arange(82) * 0.5), _abi.UnsupportedOperationException),
     (dict(libor_cap=math.nan), _abi.IllegalArgumentException),
     (dict(times=np.array([0.25, 0.5, 1.0])), _abi.UnsupportedOperationException),
 ])

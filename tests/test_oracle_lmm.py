@@ -21,41 +21,121 @@ def tiny_workload(n=4, F=2, P=6):
     return w
 
 
-def brute_force_paths(tables, P):
-    """log-Euler under the spot measure, written from the SDE:
-    dlog L_j = (sum_{l=m(t)+1..j} d L_l/(1+d L_l) lambda_j.lambda_l - var_j/2) dt + lambda_j . dW."""
+def brute_force_paths(tables, P, measure=0, scheme=2, state_space=1, cap=1e5):
+    """Euler scheme per path, written from the SDE of the forward rates:
+      log-normal: d log L_j = (mu_j - var_j / 2) dt + lambda_j . dW
+      normal:     d L_j     =  mu_j dt             + lambda_j . dW
+    with w_l = d L_l / (1 + d L_l) (log-normal) or d / (1 + d L_l) (normal) and
+      spot measure:     mu_j =  sum_{l = m(t)+1 .. j}  w_l lambda_j . lambda_l
+      terminal measure: mu_j = -sum_{l = j+1 .. n-1}   w_l lambda_j . lambda_l.
+    The predictor-corrector (scheme 1, 3) steps with the average of the drifts at the
+    start values and at the predicted values; schemes 0, 1 carry the state, 2, 3
+    re-derive it from the value (they differ only when the cap binds)."""
     times, tenor = tables.times, tables.tenor
     nT, n = times.size - 1, tenor.size - 1
     F = tables.factors.shape[1]
     dW = O.brownian(times, F, P, tables.wl.seed)
+    lognormal = state_space == 1
+    to_value = (lambda y: min(math.exp(y), cap)) if lognormal else (lambda y: min(y, cap))
     L = np.empty((nT + 1, n, P))
-    L[0] = np.minimum(np.exp(np.log(tables.L0)), 1e5)[:, None]
+    Y0 = np.log(np.maximum(tables.L0, 0.0)) if lognormal else tables.L0.copy()
+    L[0] = np.array([to_value(y) for y in Y0])[:, None]
+    state = np.repeat(Y0[:, None], P, axis=1)
+
+    def drift(i, Lv, first):
+        lam = tables.lam[i]                                         # [n][F]
+        mu = np.zeros(n)
+        for j in range(first, n):
+            ls = range(first, j + 1) if measure == 0 else range(j + 1, n)
+            sgn = 1.0 if measure == 0 else -1.0
+            for l in ls:
+                d = tenor[l + 1] - tenor[l]
+                w = d * Lv[l] / (1 + d * Lv[l]) if lognormal else d / (1 + d * Lv[l])
+                mu[j] += sgn * w * float(lam[j] @ lam[l])
+            if lognormal:
+                mu[j] -= 0.5 * tables.var[i, j]
+        return mu
+
     for i in range(nT):
         dt = times[i + 1] - times[i]
         first = int(np.searchsorted(tenor, times[i], side="right"))
         L[i + 1] = L[i]
         for p in range(P):
-            lam = tables.lam[i]                                     # [n][F]
+            y0 = state[:, p] if scheme in (0, 1) else (np.log(L[i, :, p]) if lognormal else L[i, :, p].copy())
+            mu = drift(i, L[i, :, p], first)
+            y = y0 + mu * dt + tables.lam[i] @ dW[i, :, p]
+            if scheme in (1, 3):
+                mup = drift(i, np.array([to_value(v) for v in y]), first)
+                y = y0 + 0.5 * (mu + mup) * dt + tables.lam[i] @ dW[i, :, p]
             for j in range(first, n):
-                mu = 0.0
-                for l in range(first, j + 1):
-                    d = tenor[l + 1] - tenor[l]
-                    mu += d * L[i, l, p] / (1 + d * L[i, l, p]) * float(lam[j] @ lam[l])
-                mu -= 0.5 * tables.var[i, j]
-                y = math.log(L[i, j, p]) + mu * dt + float(lam[j] @ dW[i, :, p])
-                L[i + 1, j, p] = min(math.exp(y), 1e5)
+                state[j, p] = y[j]
+                L[i + 1, j, p] = to_value(y[j])
     return L
 
 
-def test_process_against_brute_force():
+@pytest.mark.parametrize("measure", [0, 1])
+@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
+@pytest.mark.parametrize("state_space", [1, 0])
+def test_process_against_brute_force(measure, scheme, state_space):
+    wl = tiny_workload()
+    # the normal state space takes absolute volatilities
+    t = O.ModelTables(wl, vol_params=(0.2, 0.0, 0.25, 0.1) if state_space == 1 else (0.006, 0.0, 0.25, 0.002))
+    P = 6
+    om = O.OracleModel(t, P, measure=measure, scheme=scheme, state_space=state_space)
+    ref = brute_force_paths(t, P, measure, scheme, state_space)
+    for i in range(wl.num_time_steps + 1):
+        for j in range(wl.num_libors):
+            assert np.allclose(om.process_value(i, j), ref[i, j], rtol=1e-12, atol=1e-16), (i, j)
+
+
+@pytest.mark.parametrize("measure", [0, 1])
+@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
+def test_binding_cap_against_brute_force(measure, scheme):
+    # a cap that binds on many path-steps: the carried schemes keep the uncapped state,
+    # the functional ones restart from log(cap)
+    wl = tiny_workload(P=8)
+    t = O.ModelTables(wl, vol_params=(0.6, 0.0, 0.25, 0.3))
+    P, cap = 8, 0.04
+    om = O.OracleModel(t, P, measure=measure, scheme=scheme, cap=cap)
+    ref = brute_force_paths(t, P, measure, scheme, 1, cap=cap)
+    assert np.mean(ref[1:] == cap) > 0.05
+    for i in range(wl.num_time_steps + 1):
+        for j in range(wl.num_libors):
+            assert np.allclose(om.process_value(i, j), ref[i, j], rtol=1e-12, atol=0), (i, j)
+
+
+def test_terminal_numeraire_against_brute_force():
+    # N(T_k) = P(T_k; T_n) = prod_{l >= k} 1 / (1 + d L_l(T_k))
     wl = tiny_workload()
     t = O.ModelTables(wl)
     P = 6
-    om = O.OracleModel(t, P)
-    ref = brute_force_paths(t, P)
-    for i in range(wl.num_time_steps + 1):
-        for j in range(wl.num_libors):
-            assert np.allclose(om.process_value(i, j), ref[i, j], rtol=1e-13, atol=0)
+    om = O.OracleModel(t, P, measure=1, with_discount=False)
+    ref = brute_force_paths(t, P, measure=1)
+    n = wl.num_libors
+    for k in range(n + 1):
+        ti = k               # simulation grid = tenor grid
+        expect = np.ones(P)
+        for l in range(k, n):
+            expect = expect / (1 + 0.5 * ref[ti, l])
+        assert np.allclose(om.numeraire(t.tenor[k]), expect, rtol=1e-13, atol=0), k
+
+
+@pytest.mark.parametrize("measure,scheme", [(0, 2), (0, 1), (1, 2), (1, 1), (1, 0)])
+def test_bonds_are_martingale_prices_without_discount_curve(measure, scheme):
+    # N(0) E[1 / N(T_k)] = P(0; T_k) = prod_{l < k} 1 / (1 + d L_l(0)) up to the Monte Carlo
+    # error (the Euler bias of the drift is far below it at these sizes); a wrong drift
+    # sign or summation range in either measure moves the bonds by several errors
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    P = 20000
+    om = O.OracleModel(t, P, measure=measure, scheme=scheme, with_discount=False)
+    N0 = om.numeraire(0.0)
+    assert np.ptp(N0) == 0.0
+    for k in range(wl.num_libors + 1):
+        x = N0[0] / om.numeraire(t.tenor[k])
+        m, se = float(np.mean(x)), float(np.std(x) / math.sqrt(P))
+        exact = float(np.prod(1.0 / (1.0 + 0.5 * t.L0[:k])))
+        assert abs(m - exact) < 4.0 * se + 1e-12, (k, m, exact, se)
 
 
 def test_brownian_increments_are_the_generator_stream():
@@ -90,6 +170,30 @@ def test_zero_volatility_closed_form():
     v, err = om.swaption(s)
     assert abs(v - max(annuity_swap, 0.0) / N_e) < 1e-14
     assert err == 0.0
+
+
+@pytest.mark.parametrize("scheme", [1, 2])
+def test_terminal_zero_volatility_closed_form(scheme):
+    # zero volatility: deterministic forwards, the same prices in both measures
+    # (N(0) E[payoff / N(T_e)] with N(T) = P(T; T_n) under the terminal measure)
+    wl = W.c0()
+    t = O.ModelTables(wl, vol_params=(0.0, 0.0, 0.0, 0.0))
+    om = O.OracleModel(t, 64, measure=1, scheme=scheme, with_discount=False)
+    L = t.L0
+    n = wl.num_libors
+    for k in range(n + 1):
+        assert np.allclose(om.numeraire(t.tenor[k]), np.prod(1.0 / (1.0 + 0.5 * L[k:])), rtol=1e-14, atol=0)
+    s = wl.products[0]
+    e = int(round(s["exercise"] / 0.5))
+    K = s["strike"][0]
+    annuity_swap, df = 0.0, 1.0
+    for j in range(e, e + len(s["fixing"])):
+        df /= 1 + L[j] * 0.5
+        annuity_swap += (L[j] - K) * 0.5 * df
+    P_e = np.prod(1.0 / (1.0 + L[:e] * 0.5))
+    v, err = om.swaption(s)
+    assert abs(v - max(annuity_swap, 0.0) * P_e) < 1e-14
+    assert err < 1e-15
 
 
 def test_adjusted_numeraire_reprices_discount_curve():
