@@ -14,6 +14,9 @@
 // Products are evaluated on the fly at their exercise/fixing time index and reduced
 // per wavefront (sum and centred second moment); the numeraire is accumulated per
 // path in double precision (SPOT measure, LMM:1050-1066).
+// The general variant (kVarGeneral*) adds the terminal measure (components in
+// descending order, mu_j from the later components, N(T_k) = prod_{l>=k} 1/(1+L_l d_l)),
+// the carried state of the EULER / PREDICTOR_CORRECTOR schemes and the corrector step.
 //
 // Model data layout: one record of rec_size_of(FCAP) reals per (time step,
 // component): the factor loadings padded with zeros to FCAP, then variance*(-0.5),
