@@ -36,6 +36,7 @@ sys.path.insert(0, os.path.join(REPO, "finmath-lib_amd"))
 import numpy as np  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X published FP64 peak (vector and matrix)
+HBM_PEAK_GBPS = 8000.0       # MI355X published HBM3E bandwidth
 # HBM bytes per launch of lmm_fused_kernel<40,5,double> on the 40 x 5 x 1,000,000-path strip:
 # FETCH_SIZE and WRITE_SIZE counter passes scaled by a plain 1 GiB copy
 # (profiles/lmm_fused_fp64_c1_hbm_traffic.md)
@@ -154,6 +155,45 @@ def cpu_baseline(wl_gpu, paths):
             "seconds": round(dt, 3)}
 
 
+def run_rv_algebra(steps):
+    """Device random-variable programs (include/finmath_rv.h) on 2^26 realisations: one
+    step of the swaption recursion, (x * y + z) / (1.0 + y * 0.5) capped at 5, applied once
+    and four times, materialised (3 inputs read, 1 written) and averaged without
+    materialising (3 inputs read). Algorithmic bytes over the kernel time on the library's
+    stream, timed around `steps` calls with one synchronisation at the end."""
+    from finmath_hip import randomvariable as R
+    n = 1 << 26
+    rng = np.random.default_rng(1)
+    X = R.RandomVariable.from_array(rng.standard_normal(n))
+    Y = R.RandomVariable.from_array(rng.uniform(0.5, 2.0, n))
+    Z = R.RandomVariable.from_array(rng.uniform(-1.0, 1.0, n))
+
+    def prog(k):
+        e = X
+        for _ in range(k):
+            e = e.mult(Y).add(Z).discount(Y, 0.5).cap(5.0)
+        return e
+
+    out = {"workload": "2^26 double realisations, program (x*y + z)/(1.0 + y*0.5) capped, 3 inputs",
+           "peak_GBps": HBM_PEAK_GBPS}
+    for k in (1, 4):
+        prog(k).cache().getAverage()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = prog(k).cache()
+        r.getAverage()                       # synchronises the stream
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            prog(k).getAverage()
+        t2 = time.perf_counter()
+        ms_store = (t1 - t0) * 1e3 / steps
+        ms_avg = (t2 - t1) * 1e3 / steps
+        out[f"ops{4 * k}"] = {
+            "materialise_ms": ms_store, "materialise_GBps": 32 * n / (ms_store * 1e-3) / 1e9,
+            "average_ms": ms_avg, "average_GBps": 24 * n / (ms_avg * 1e-3) / 1e9}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,12 +205,13 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-logstate", action="store_true")
     ap.add_argument("--no-bermudan", action="store_true")
+    ap.add_argument("--no-rv", action="store_true", help="skip the device random-variable algebra leg")
     ap.add_argument("--only-fp64", action="store_true", help="the headline leg only (for counter passes)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.only_fp64:
-        args.no_fp32 = args.no_logstate = args.no_bermudan = args.no_greeks = args.no_cpu = True
+        args.no_fp32 = args.no_logstate = args.no_bermudan = args.no_greeks = args.no_cpu = args.no_rv = True
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path axis would shard across GPUs "
               "as described in DESIGN.md, but this build measures one MI355X.", file=sys.stderr)
@@ -254,6 +295,8 @@ def main():
             "adjoint_stage_ms_stride4": {"generator": gst[0], "forward_path_kernel": gst[1], "reverse_sweep": gst[3],
                                          "call": gst[4]},
             "sum_delta": float(np.sum(g["delta"])), "parameter_vega": [float(x) for x in g["parameterVega"]]}
+    if not args.no_rv:
+        line["rv_algebra"] = run_rv_algebra(args.steps)
     if not args.no_cpu:
         cb = cpu_baseline(wl, args.cpu_paths)
         line["cpu_baseline"] = cb

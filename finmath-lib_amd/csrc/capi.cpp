@@ -945,6 +945,64 @@ int fm_lmm_get_numeraire(fm_lmm* c, double time, double* out) {
     FM_CATCH
 }
 
+int fm_lmm_process_value_rv(fm_lmm* c, int32_t t, int32_t j, fm_rv** out) {
+    FM_TRY
+    require(c && out, FM_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(c->mu);
+    require(t >= 0 && t <= c->nT() && j >= 0 && j < c->n(), FM_ERR_INVALID_ARGUMENT, "process index out of range");
+    require(c->simulated, FM_ERR_STATE, "process not simulated (call fm_lmm_simulate)");
+    DeviceGuard g(c->device);
+    const int r = c->row_of[(size_t)t * c->n() + j];
+    if (r < 0) { *out = rv_constant(c->times[(size_t)t], c->init_value[(size_t)j]); return FM_OK; }
+    const char* src = (const char*)c->d_paths.p + (size_t)r * c->P() * c->real_size();
+    *out = rv_from_device(c->device, src, c->fp32(), c->P(), c->times[(size_t)t]);
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_numeraire_rv(fm_lmm* c, double time, fm_rv** out) {
+    FM_TRY
+    require(c && out, FM_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(c->mu);
+    require(c->simulated, FM_ERR_STATE, "process not simulated (call fm_lmm_simulate)");
+    DeviceGuard g(c->device);
+    const int k = c->libor_index(time);
+    require(k >= 0 && k <= c->n(), FM_ERR_UNSUPPORTED, "numeraire requested off the tenor grid");
+    const int known = c->terminal() ? k : std::max(k - 1, 0);
+    require(c->tenor_ti[(size_t)known] <= c->nT(), FM_ERR_INVALID_ARGUMENT, "numeraire date beyond the simulation");
+    const bool scalar_num = c->terminal() ? (k == 0 || k == c->n()) : k == 0;
+    const std::vector<double> adj = c->has_pd ? adjustment_chain(c) : std::vector<double>();
+    if (scalar_num) {
+        double v = c->terminal() && k == 0 ? c->n0_terminal : 1.0;
+        if (c->has_pd) v = (1.0 / adj[(size_t)k]) * (v * c->zb[(size_t)k]);
+        *out = rv_constant(time, v);
+        return FM_OK;
+    }
+    fm_rv* raw = rv_from_device(c->device, c->d_num.as<double>() + (size_t)k * c->P(), false, c->P(), time);
+    if (!c->has_pd) { *out = raw; return FM_OK; }
+    // N(T) * avg(N(0)/N(T)) / P^d(T) (LMM:869-874); spot N(T_1) is a Scalar: (1/adj) (N zb)
+    const bool scalar_adj = !c->terminal() && k <= 1;
+    try {
+        *out = rv_scale_numeraire(raw, c->zb[(size_t)k], adj[(size_t)k], scalar_adj);
+    } catch (...) {
+        fm_rv_release(raw);
+        throw;
+    }
+    fm_rv_release(raw);
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_weights_rv(fm_lmm* c, fm_rv** out) {
+    FM_TRY
+    require(c && out, FM_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = rv_constant(0.0, 1.0 / (double)c->P());
+    return FM_OK;
+    FM_CATCH
+}
+
 int fm_lmm_get_brownian_increment(fm_lmm* c, int32_t t, int32_t k, double* out) {
     FM_TRY
     require(c && out, FM_ERR_INVALID_ARGUMENT, "NULL argument");

@@ -8,6 +8,7 @@
 #include <stdexcept>
 
 #include "finmath_lmm.h"
+#include "finmath_rv.h"
 
 namespace fm {
 
@@ -204,5 +205,12 @@ struct BermStep {
 int berm_num_blocks(int P);
 void launch_berm_step(const BermStep& s, int kind, void* stream);
 void launch_berm_solve(const double* partials, int num_rows, int K, int P, double* coef, void* stream);
+
+// ---- device random variables (rv.hip) ---------------------------------------------------
+// Copy of P device realisations (double, or float widened to double) as a random variable.
+fm_rv* rv_from_device(int device, const void* src, bool is_float, int64_t size, double time);
+fm_rv* rv_constant(double time, double value);
+// Numeraire adjustment (x zb) / adj, or (1/adj) (x zb) in the Scalar form (new random variable).
+fm_rv* rv_scale_numeraire(fm_rv* x, double zb, double adj, bool scalar_form);
 
 } // namespace fm

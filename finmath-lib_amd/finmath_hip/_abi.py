@@ -1,4 +1,4 @@
-"""ctypes binding of the C ABI declared in include/finmath_lmm.h.
+"""ctypes binding of the C ABI declared in include/finmath_lmm.h and include/finmath_rv.h.
 
 The binding is the Python face of the drop-in boundary: plain pointers and sizes,
 status codes mapped onto the exception types the reference raises. It loads the
@@ -41,6 +41,10 @@ EXPORTED_SYMBOLS = (
     "fm_lmm_bermudan_exercise_times",
     "fm_mt19937_words", "fm_mt19937_doubles",
     "fm_last_error", "fm_lmm_last_timings", "fm_device_info",
+    # include/finmath_rv.h
+    "fm_rv_from_host", "fm_rv_constant", "fm_rv_release", "fm_rv_info", "fm_rv_to_host", "fm_rv_device_ptr",
+    "fm_rv_evaluate", "fm_rv_evaluate_average", "fm_rv_average", "fm_rv_variance",
+    "fm_lmm_process_value_rv", "fm_lmm_numeraire_rv", "fm_lmm_weights_rv",
 )
 
 
@@ -80,6 +84,11 @@ class FmLmmConfig(ctypes.Structure):
         ("libor_cap", c_double), ("store_paths", c_int32), ("checkpoint_stride", c_int32),
         ("regenerate_increments", c_int32),
     ]
+
+
+class FmRvInstr(ctypes.Structure):
+    _fields_ = [("op", c_int32), ("dst", ctypes.c_int16), ("a", ctypes.c_int16), ("b", ctypes.c_int16),
+                ("c", ctypes.c_int16), ("imm", c_double)]
 
 
 class FmProduct(ctypes.Structure):
@@ -129,6 +138,21 @@ def load() -> ctypes.CDLL:
         "fm_last_error": ([], c_char_p),
         "fm_lmm_last_timings": ([c_void_p, dp, c_int32], c_int32),
         "fm_device_info": ([ctypes.c_char_p, c_int32], c_int32),
+        "fm_rv_from_host": ([c_double, c_int64, dp, POINTER(c_void_p)], c_int32),
+        "fm_rv_constant": ([c_double, c_double, POINTER(c_void_p)], c_int32),
+        "fm_rv_release": ([c_void_p], None),
+        "fm_rv_info": ([c_void_p, dp, POINTER(c_int64), POINTER(c_int32), dp], c_int32),
+        "fm_rv_to_host": ([c_void_p, c_int64, dp], c_int32),
+        "fm_rv_device_ptr": ([c_void_p, POINTER(c_void_p)], c_int32),
+        "fm_rv_evaluate": ([c_int32, POINTER(c_void_p), c_int32, POINTER(FmRvInstr), c_int32, c_double,
+                            POINTER(c_void_p)], c_int32),
+        "fm_rv_evaluate_average": ([c_int32, POINTER(c_void_p), c_int32, POINTER(FmRvInstr), c_int32, c_void_p,
+                                    dp, dp], c_int32),
+        "fm_rv_average": ([c_void_p, dp], c_int32),
+        "fm_rv_variance": ([c_void_p, dp], c_int32),
+        "fm_lmm_process_value_rv": ([c_void_p, c_int32, c_int32, POINTER(c_void_p)], c_int32),
+        "fm_lmm_numeraire_rv": ([c_void_p, c_double, POINTER(c_void_p)], c_int32),
+        "fm_lmm_weights_rv": ([c_void_p, POINTER(c_void_p)], c_int32),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

@@ -1,0 +1,367 @@
+"""Device random variables with the reference's RandomVariable interface.
+
+`RandomVariable` follows net.finmath.stochastic.RandomVariable as implemented by
+RandomVariableFromDoubleArray (montecarlo/RandomVariableFromDoubleArray.java): immutable,
+element-wise operations, compensated averages. Operations are recorded lazily, as
+RandomVariableLazyEvaluation (montecarlo/RandomVariableLazyEvaluation.java) does, and
+compiled into one register program of the C ABI (include/finmath_rv.h) when a value is
+needed: materialising an expression, or averaging it without materialising, is one pass
+over device memory. Constants behave like RandomVariableFromDoubleArray's deterministic
+values (not like Scalar, whose discount formula differs).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from ctypes import POINTER, c_double, c_int32, c_int64, c_void_p
+
+import numpy as np
+
+from . import _abi
+
+OP = dict(LOAD=0, CONST=1, ADD=2, SUB=3, MULT=4, DIV=5, CAP=6, FLOOR=7, POW=8, EXP=9, LOG=10, SQRT=11, ABS=12,
+          INVERT=13, SQUARED=14, ADD_PRODUCT=15, ADD_RATIO=16, SUB_RATIO=17, ACCRUE=18, DISCOUNT=19, CHOOSE=20,
+          ISNAN=21, EXPM1=22, SIN=23, COS=24)
+MAX_INSTRUCTIONS, MAX_INPUTS, REGISTERS = 64, 16, 16
+
+
+class _Handle:
+    """Owns one fm_rv* of the library."""
+
+    def __init__(self, ptr: int):
+        self.ptr = ptr
+
+    def __del__(self):
+        if self.ptr:
+            try:
+                _abi.load().fm_rv_release(c_void_p(self.ptr))
+            except Exception:
+                pass
+            self.ptr = None
+
+
+class _Leaf:
+    __slots__ = ("handle", "time", "det", "value", "size")
+
+    def __init__(self, handle, time, det, value, size):
+        self.handle, self.time, self.det, self.value, self.size = handle, time, det, value, size
+
+
+class _Op:
+    __slots__ = ("op", "args", "imm", "time", "size", "det")
+
+    def __init__(self, op, args, imm=0.0):
+        self.op, self.args, self.imm = op, tuple(args), float(imm)
+        self.time = max(a.time for a in self.args)
+        sizes = {a.size for a in self.args if not a.det}
+        if len(sizes) > 1:
+            raise _abi.IllegalArgumentException(f"random variables of different sizes {sorted(sizes)}")
+        self.size = sizes.pop() if sizes else 1
+        self.det = all(a.det for a in self.args)
+
+
+def _leaf_from_ptr(ptr: int) -> _Leaf:
+    lib = _abi.load()
+    t, v = c_double(), c_double()
+    n, d = c_int64(), c_int32()
+    _abi.check(lib.fm_rv_info(c_void_p(ptr), ctypes.byref(t), ctypes.byref(n), ctypes.byref(d), ctypes.byref(v)))
+    return _Leaf(_Handle(ptr), t.value, bool(d.value), v.value, n.value)
+
+
+class _Overflow(Exception):
+    pass
+
+
+def _compile(root):
+    """Register program for the DAG under `root`: (inputs, instructions, result register).
+    Shared sub-expressions are computed once; a register is freed after its last use."""
+    order, seen = [], set()
+    stack = [(root, False)]                 # iterative post-order (expressions can be deep)
+    while stack:
+        node, expanded = stack.pop()
+        if expanded:
+            order.append(node)
+            continue
+        if id(node) in seen:
+            continue
+        seen.add(id(node))
+        stack.append((node, True))
+        if isinstance(node, _Op):
+            for a in reversed(node.args):
+                if id(a) not in seen:
+                    stack.append((a, False))
+    uses = {}
+    for node in order:
+        if isinstance(node, _Op):
+            for a in node.args:
+                uses[id(a)] = uses.get(id(a), 0) + 1
+    inputs, input_of, reg_of, free = [], {}, {}, list(range(REGISTERS - 1, -1, -1))
+    prog = []
+    for node in order:
+        if isinstance(node, _Leaf):
+            if node.det:
+                ins = (OP["CONST"], 0, 0, 0, node.value)
+            else:
+                key = id(node.handle)
+                if key not in input_of:
+                    if len(inputs) == MAX_INPUTS:
+                        raise _Overflow()
+                    input_of[key] = len(inputs)
+                    inputs.append(node.handle)
+                ins = (OP["LOAD"], input_of[key], 0, 0, 0.0)
+        else:
+            regs = [reg_of[id(a)] for a in node.args] + [0] * (3 - len(node.args))
+            ins = (node.op, regs[0], regs[1], regs[2], node.imm)
+            for a in node.args:                       # free the registers whose last use this is
+                uses[id(a)] -= 1
+                if uses[id(a)] == 0 and a is not root:
+                    free.append(reg_of[id(a)])
+        if not free:
+            raise _Overflow()
+        dst = free.pop()
+        reg_of[id(node)] = dst
+        prog.append((ins[0], dst, ins[1], ins[2], ins[3], ins[4]))
+        if len(prog) > MAX_INSTRUCTIONS:
+            raise _Overflow()
+    return inputs, prog, reg_of[id(root)]
+
+
+def _program_args(inputs, prog):
+    arr = (c_void_p * max(len(inputs), 1))(*[h.ptr for h in inputs])
+    ins = (_abi.FmRvInstr * len(prog))()
+    for k, (op, dst, a, b, c, imm) in enumerate(prog):
+        ins[k].op, ins[k].dst, ins[k].a, ins[k].b, ins[k].c, ins[k].imm = op, dst, a, b, c, imm
+    return arr, ins
+
+
+def _subtree_size(node):
+    count, stack, seen = 0, [node], set()
+    while stack:
+        n = stack.pop()
+        if id(n) in seen:
+            continue
+        seen.add(id(n))
+        count += 1
+        if isinstance(n, _Op):
+            stack.extend(n.args)
+    return count
+
+
+def _materialize(node) -> _Leaf:
+    if isinstance(node, _Leaf):
+        return node
+    while True:
+        try:
+            inputs, prog, res = _compile(node)
+            break
+        except _Overflow:
+            # evaluate the largest operand on its own first, then retry
+            big = max((a for a in node.args if isinstance(a, _Op)), key=_subtree_size)
+            node = _Op(node.op, [(_materialize(a) if a is big else a) for a in node.args], node.imm)
+    arr, ins = _program_args(inputs, prog)
+    out = c_void_p()
+    _abi.check(_abi.load().fm_rv_evaluate(len(inputs), arr, len(prog), ins, res, node.time, ctypes.byref(out)))
+    return _leaf_from_ptr(out.value)
+
+
+def _as_node(x):
+    if isinstance(x, RandomVariable):
+        return x._node
+    return _Leaf(None, 0.0, True, float(x), 1)
+
+
+class RandomVariable:
+    """A random variable on the device (RandomVariableFromDoubleArray semantics)."""
+
+    __slots__ = ("_node",)
+
+    def __init__(self, node):
+        self._node = node
+
+    # ---- construction (RandomVariableFactory.createRandomVariable, RandomVariableFactory.java:22-95)
+    @staticmethod
+    def from_array(values, time: float = 0.0) -> "RandomVariable":
+        v = np.ascontiguousarray(np.asarray(values, dtype=np.float64))
+        out = c_void_p()
+        _abi.check(_abi.load().fm_rv_from_host(time, v.size, v.ctypes.data_as(POINTER(c_double)), ctypes.byref(out)))
+        return RandomVariable(_leaf_from_ptr(out.value))
+
+    @staticmethod
+    def constant(value: float, time: float = 0.0) -> "RandomVariable":
+        return RandomVariable(_Leaf(None, time, True, float(value), 1))
+
+    @staticmethod
+    def _from_ptr(ptr: int) -> "RandomVariable":
+        return RandomVariable(_leaf_from_ptr(ptr))
+
+    # ---- properties
+    def getFiltrationTime(self) -> float:
+        return self._node.time
+
+    def isDeterministic(self) -> bool:
+        return self._node.det
+
+    def size(self) -> int:
+        return self._node.size
+
+    def _leaf(self) -> _Leaf:
+        if not isinstance(self._node, _Leaf):
+            self._node = _materialize(self._node)
+        return self._node
+
+    def cache(self) -> "RandomVariable":
+        self._leaf()
+        return self
+
+    def doubleValue(self) -> float:
+        leaf = self._leaf()
+        if leaf.det:
+            return leaf.value
+        raise _abi.UnsupportedOperationException("doubleValue of a stochastic random variable")
+
+    def getRealizations(self, size: int | None = None) -> np.ndarray:
+        leaf = self._leaf()
+        n = leaf.size if size is None else int(size)
+        out = np.empty(n)
+        if leaf.det:
+            out[:] = leaf.value
+            return out
+        _abi.check(_abi.load().fm_rv_to_host(c_void_p(leaf.handle.ptr), n, out.ctypes.data_as(POINTER(c_double))))
+        return out
+
+    def get(self, i: int) -> float:
+        leaf = self._leaf()
+        return leaf.value if leaf.det else float(self.getRealizations()[i])
+
+    def devicePointer(self) -> int:
+        leaf = self._leaf()
+        return 0 if leaf.det else leaf.handle.ptr
+
+    # ---- reductions (RandomVariableFromDoubleArray.java:287-428)
+    def _reduce(self, probabilities, want_variance):
+        node = self._node
+        if isinstance(node, _Op) and node.det:
+            node = self._leaf()
+        if isinstance(node, _Leaf) and node.det and probabilities is None:
+            return node.value, 0.0
+        while True:
+            try:
+                inputs, prog, res = _compile(node)
+                break
+            except _Overflow:
+                node = self._leaf()
+        prob = None
+        if probabilities is not None:
+            prob = probabilities if isinstance(probabilities, RandomVariable) else RandomVariable.constant(probabilities)
+            pl = prob._leaf()
+            if pl.det:
+                cp = c_void_p()
+                _abi.check(_abi.load().fm_rv_constant(pl.time, pl.value, ctypes.byref(cp)))
+                prob_handle = _Handle(cp.value)
+            else:
+                prob_handle = pl.handle
+        arr, ins = _program_args(inputs, prog)
+        avg, var = c_double(), c_double()
+        _abi.check(_abi.load().fm_rv_evaluate_average(
+            len(inputs), arr, len(prog), ins, res, c_void_p(prob_handle.ptr) if prob is not None else None,
+            ctypes.byref(avg), ctypes.byref(var) if want_variance else None))
+        return avg.value, var.value
+
+    def getAverage(self, probabilities=None) -> float:
+        return self._reduce(probabilities, False)[0]
+
+    def getVariance(self, probabilities=None) -> float:
+        if self.isDeterministic():
+            return 0.0
+        return self._reduce(probabilities, True)[1]
+
+    def getStandardDeviation(self, probabilities=None) -> float:
+        return 0.0 if self.isDeterministic() else math.sqrt(self.getVariance(probabilities))
+
+    def getStandardError(self) -> float:
+        if self.isDeterministic():
+            return 0.0
+        return self.getStandardDeviation() / math.sqrt(self.size())
+
+    def average(self) -> "RandomVariable":
+        return RandomVariable.constant(self.getAverage())
+
+    # ---- element operations
+    def _op(self, name, *args, imm=0.0):
+        return RandomVariable(_Op(OP[name], [self._node] + [_as_node(a) for a in args], imm))
+
+    def add(self, x): return self._op("ADD", x)
+    def sub(self, x): return self._op("SUB", x)
+    def bus(self, x): return RandomVariable(_Op(OP["SUB"], [_as_node(x), self._node]))
+    def mult(self, x): return self._op("MULT", x)
+    def div(self, x): return self._op("DIV", x)
+    def vid(self, x): return RandomVariable(_Op(OP["DIV"], [_as_node(x), self._node]))
+    def cap(self, x): return self._op("CAP", x)
+    def floor(self, x): return self._op("FLOOR", x)
+    def pow(self, exponent): return self._op("POW", exponent)
+    def exp(self): return self._op("EXP")
+    def expm1(self): return self._op("EXPM1")
+    def log(self): return self._op("LOG")
+    def sqrt(self): return self._op("SQRT")
+    def abs(self): return self._op("ABS")
+    def invert(self): return self._op("INVERT")
+    def squared(self): return self._op("SQUARED")
+    def sin(self): return self._op("SIN")
+    def cos(self): return self._op("COS")
+    def isNaN(self): return self._op("ISNAN")
+
+    def accrue(self, rate, periodLength: float):
+        """this * (1 + rate * periodLength) (RandomVariableFromDoubleArray.java:1279-1305)."""
+        return self._op("ACCRUE", rate, imm=periodLength)
+
+    def discount(self, rate, periodLength: float):
+        """this / (1.0 + rate * periodLength) (RandomVariableFromDoubleArray.java:1308-1333)."""
+        return self._op("DISCOUNT", rate, imm=periodLength)
+
+    def choose(self, valueIfTriggerNonNegative, valueIfTriggerNegative):
+        """this >= 0 ? a : b (RandomVariableFromDoubleArray.java:1342-1363)."""
+        return self._op("CHOOSE", valueIfTriggerNonNegative, valueIfTriggerNegative)
+
+    def addProduct(self, factor1, factor2):
+        return self._op("ADD_PRODUCT", factor1, factor2)
+
+    def addSumProduct(self, factor1, factor2):
+        """Sequential addProduct over the lists (RandomVariableFromDoubleArray.java:1430-1437)."""
+        r = self
+        for f1, f2 in zip(factor1, factor2):
+            r = r.addProduct(f1, f2)
+        return r
+
+    def addRatio(self, numerator, denominator): return self._op("ADD_RATIO", numerator, denominator)
+    def subRatio(self, numerator, denominator): return self._op("SUB_RATIO", numerator, denominator)
+
+    __add__ = add
+    __sub__ = sub
+    __mul__ = mult
+    __truediv__ = div
+
+    def __radd__(self, x): return RandomVariable(_Op(OP["ADD"], [_as_node(x), self._node]))
+    def __rsub__(self, x): return self.bus(x)
+    def __rmul__(self, x): return RandomVariable(_Op(OP["MULT"], [_as_node(x), self._node]))
+    def __rtruediv__(self, x): return self.vid(x)
+    def __neg__(self): return self.mult(-1.0)
+
+
+def process_value(ctx, time_index: int, component: int) -> RandomVariable:
+    """getProcessValue(timeIndex, component) of a simulated context (fm_lmm_process_value_rv)."""
+    out = c_void_p()
+    _abi.check(_abi.load().fm_lmm_process_value_rv(ctx.h, int(time_index), int(component), ctypes.byref(out)))
+    return RandomVariable._from_ptr(out.value)
+
+
+def numeraire(ctx, time: float) -> RandomVariable:
+    """getNumeraire(time) of a simulated context (fm_lmm_numeraire_rv)."""
+    out = c_void_p()
+    _abi.check(_abi.load().fm_lmm_numeraire_rv(ctx.h, float(time), ctypes.byref(out)))
+    return RandomVariable._from_ptr(out.value)
+
+
+def monte_carlo_weights(ctx) -> RandomVariable:
+    out = c_void_p()
+    _abi.check(_abi.load().fm_lmm_weights_rv(ctx.h, ctypes.byref(out)))
+    return RandomVariable._from_ptr(out.value)
