@@ -168,6 +168,10 @@ void validate_config(const fm_lmm_config* c) {
     require(c->precision == FM_PRECISION_FP64 || c->precision == FM_PRECISION_FP32_MIXED ||
             c->precision == FM_PRECISION_FP64_LOGSTATE, FM_ERR_INVALID_ARGUMENT, "unknown precision");
     require(!std::isnan(c->libor_cap), FM_ERR_INVALID_ARGUMENT, "liborCap must not be NaN");
+    require(c->path_offset >= 0, FM_ERR_INVALID_ARGUMENT, "path offset must not be negative");
+    // the generator jumps at most 2^41 words (the cached jump polynomials x^(2^e), e <= 40)
+    const double words = 2.0 * ((double)c->path_offset + (double)c->num_paths) * c->num_time_steps * c->num_factors;
+    require(words < 2199023255552.0, FM_ERR_UNSUPPORTED, "path range beyond 2^41 generator words");
     require(c->num_libors <= 80 && c->num_factors <= 10, FM_ERR_UNSUPPORTED,
             "the fused kernel holds at most 80 forward rates and 10 factors");
 }
@@ -273,6 +277,9 @@ void ensure_states(fm_lmm* c) {
     c->lay = choose_layout(total);
     uint32_t seedw[624];
     mt_seed_window((int64_t)c->cfg.seed, seedw);
+    // path p of the seed's stream starts at word 2 p nT F: jump to the first path of this context
+    if (c->cfg.path_offset > 0)
+        mt_jump_window_by(seedw, 2ull * (uint64_t)c->cfg.path_offset * (uint64_t)c->nT() * (uint64_t)c->F());
     uint32_t* st = (uint32_t*)c->d_states.get((size_t)c->lay.num_sub * 624 * sizeof(uint32_t));
     uint64_t* poly = (uint64_t*)c->d_poly.get((size_t)64 * 312 * sizeof(uint64_t));
     launch_jump_tree(c->lay, seedw, st, poly, c->stream);
@@ -1203,6 +1210,122 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     if (d_volatility) for (size_t i = 0; i < (size_t)nT * n; i++) d_volatility[i] = grad[i];
     if (d_initial_forwards) for (int j = 0; j < n; j++) d_initial_forwards[j] = grad[(size_t)nT * n + j];
     for (double x : grad) require(std::isfinite(x), FM_ERR_NUMERICAL, "adjoint is not finite");
+    return FM_OK;
+    FM_CATCH
+}
+
+// ---- serialisation: "FMLMMSER", version, configuration, grids, tables (native little-endian)
+extern "C++" {
+namespace {
+constexpr char kSerMagic[8] = {'F', 'M', 'L', 'M', 'M', 'S', 'E', 'R'};
+constexpr int32_t kSerVersion = 1;
+
+struct Writer {
+    std::vector<unsigned char> b;
+    template <class T> void put(const T& v) {
+        const unsigned char* p = (const unsigned char*)&v;
+        b.insert(b.end(), p, p + sizeof(T));
+    }
+    void put_doubles(const std::vector<double>& v) {
+        put<int64_t>((int64_t)v.size());
+        const unsigned char* p = (const unsigned char*)v.data();
+        b.insert(b.end(), p, p + v.size() * sizeof(double));
+    }
+};
+
+struct Reader {
+    const unsigned char* p;
+    int64_t left;
+    template <class T> T get() {
+        require(left >= (int64_t)sizeof(T), FM_ERR_INVALID_ARGUMENT, "serialised context is truncated");
+        T v;
+        std::memcpy(&v, p, sizeof(T));
+        p += sizeof(T);
+        left -= (int64_t)sizeof(T);
+        return v;
+    }
+    std::vector<double> get_doubles(int64_t expect) {
+        const int64_t n = get<int64_t>();
+        require(n == expect, FM_ERR_INVALID_ARGUMENT, "serialised table has an unexpected length");
+        require(left >= n * (int64_t)sizeof(double), FM_ERR_INVALID_ARGUMENT, "serialised context is truncated");
+        std::vector<double> v((size_t)n);
+        std::memcpy(v.data(), p, (size_t)n * sizeof(double));
+        p += n * (int64_t)sizeof(double);
+        left -= n * (int64_t)sizeof(double);
+        return v;
+    }
+};
+} // namespace
+} // extern "C++"
+
+int fm_lmm_serialize(fm_lmm* c, void* buffer, int64_t* size) {
+    FM_TRY
+    require(c && size, FM_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    Writer w;
+    w.b.insert(w.b.end(), kSerMagic, kSerMagic + 8);
+    w.put<int32_t>(kSerVersion);
+    const fm_lmm_config& g = c->cfg;
+    for (int32_t v : {g.num_time_steps, g.num_libors, g.num_factors, g.num_paths, g.seed, g.measure, g.state_space,
+                      g.scheme, g.precision, g.store_paths, g.checkpoint_stride, g.regenerate_increments})
+        w.put<int32_t>(v);
+    w.put<int64_t>(g.path_offset);
+    w.put<double>(g.libor_cap);
+    w.put_doubles(c->times);
+    w.put_doubles(c->tenor);
+    w.put<int32_t>(c->has_L0); w.put<int32_t>(c->has_cov); w.put<int32_t>(c->has_pd); w.put<int32_t>(c->has_vf);
+    if (c->has_L0) w.put_doubles(c->L0);
+    if (c->has_cov) { w.put_doubles(c->lam); w.put_doubles(c->var); }
+    if (c->has_pd) w.put_doubles(c->pd);
+    if (c->has_vf) { w.put_doubles(c->vf_sig); w.put_doubles(c->vf_fac); w.put_doubles(c->vf_c); }
+    const int64_t need = (int64_t)w.b.size();
+    if (buffer) {
+        require(*size >= need, FM_ERR_INVALID_ARGUMENT, "buffer too small for the serialised context");
+        std::memcpy(buffer, w.b.data(), (size_t)need);
+    }
+    *size = need;
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_deserialize(const void* buffer, int64_t size, fm_lmm** out) {
+    FM_TRY
+    require(buffer && out && size > 0, FM_ERR_INVALID_ARGUMENT, "invalid arguments");
+    *out = nullptr;
+    Reader r{(const unsigned char*)buffer, size};
+    char magic[8];
+    for (char& m : magic) m = r.get<char>();
+    require(std::memcmp(magic, kSerMagic, 8) == 0, FM_ERR_INVALID_ARGUMENT, "not a serialised context");
+    require(r.get<int32_t>() == kSerVersion, FM_ERR_INVALID_ARGUMENT, "unsupported serialisation version");
+    fm_lmm_config g{};
+    g.num_time_steps = r.get<int32_t>(); g.num_libors = r.get<int32_t>(); g.num_factors = r.get<int32_t>();
+    g.num_paths = r.get<int32_t>(); g.seed = r.get<int32_t>(); g.measure = r.get<int32_t>();
+    g.state_space = r.get<int32_t>(); g.scheme = r.get<int32_t>(); g.precision = r.get<int32_t>();
+    g.store_paths = r.get<int32_t>(); g.checkpoint_stride = r.get<int32_t>(); g.regenerate_increments = r.get<int32_t>();
+    g.path_offset = r.get<int64_t>();
+    g.libor_cap = r.get<double>();
+    require(g.num_time_steps >= 1 && g.num_libors >= 1 && g.num_factors >= 1 && g.num_time_steps < (1 << 24) &&
+            g.num_libors <= 80 && g.num_factors <= 10, FM_ERR_INVALID_ARGUMENT, "serialised shape out of range");
+    const int nT = g.num_time_steps, n = g.num_libors, F = g.num_factors;
+    const std::vector<double> times = r.get_doubles((int64_t)nT + 1), tenor = r.get_doubles((int64_t)n + 1);
+    g.times = times.data();
+    g.tenor = tenor.data();
+    const int32_t hL0 = r.get<int32_t>(), hcov = r.get<int32_t>(), hpd = r.get<int32_t>(), hvf = r.get<int32_t>();
+    std::vector<double> L0, lam, var, pd, sig, fac, cv;
+    if (hL0) L0 = r.get_doubles(n);
+    if (hcov) { lam = r.get_doubles((int64_t)nT * n * F); var = r.get_doubles((int64_t)nT * n); }
+    if (hpd) pd = r.get_doubles((int64_t)n + 1);
+    if (hvf) { sig = r.get_doubles((int64_t)nT * n); fac = r.get_doubles((int64_t)n * F); cv = r.get_doubles(n); }
+    require(r.left == 0, FM_ERR_INVALID_ARGUMENT, "trailing bytes after the serialised context");
+    fm_lmm* c = nullptr;
+    int rc = fm_lmm_create(&g, &c);
+    if (rc != FM_OK) return rc;
+    std::unique_ptr<fm_lmm, void (*)(fm_lmm*)> guard(c, fm_lmm_destroy);
+    if (hL0 && (rc = fm_lmm_set_initial_forwards(c, L0.data())) != FM_OK) return rc;
+    if (hcov && (rc = fm_lmm_set_covariance(c, lam.data(), var.data())) != FM_OK) return rc;
+    if (hpd && (rc = fm_lmm_set_discount_curve(c, pd.data())) != FM_OK) return rc;
+    if (hvf && (rc = fm_lmm_set_volatility_factorization(c, sig.data(), fac.data(), cv.data())) != FM_OK) return rc;
+    *out = guard.release();
     return FM_OK;
     FM_CATCH
 }

@@ -26,6 +26,8 @@ int mt_jump_words();
 const std::vector<uint64_t>& mt_jump_poly_pow2(int e);     // x^(2^e) mod phi
 const std::vector<uint64_t>& mt_char_poly();
 void mt_jump_window_host(const uint32_t* src624, const std::vector<uint64_t>& c, uint32_t* dst624);
+// Advance a 624-word window by `words` outputs (words < 2^41).
+void mt_jump_window_by(uint32_t* window624, uint64_t words);
 
 // commons-math3 MersenneTwister(long) seeding (init_by_array with key {hi, lo}).
 void mt_seed_window(int64_t seed, uint32_t* window624);

@@ -192,4 +192,15 @@ void mt_jump_window_host(const uint32_t* src624, const std::vector<uint64_t>& c,
     }
 }
 
+// Advance a window by `words` outputs: the product of the jumps x^(2^e) of the set bits
+// (jumps commute), each applied on the host.
+void mt_jump_window_by(uint32_t* window624, uint64_t words) {
+    uint32_t tmp[624];
+    for (int e = 0; words != 0; e++, words >>= 1) {
+        if (!(words & 1u)) continue;
+        mt_jump_window_host(window624, mt_jump_poly_pow2(e), tmp);
+        std::memcpy(window624, tmp, sizeof(tmp));
+    }
+}
+
 } // namespace fm

@@ -9,6 +9,7 @@ from __future__ import annotations
 import ctypes
 import math
 import os
+import struct
 from ctypes import POINTER, c_char_p, c_double, c_int32, c_int64, c_uint32, c_uint64, c_void_p
 
 import numpy as np
@@ -38,7 +39,7 @@ EXPORTED_SYMBOLS = (
     "fm_lmm_set_discount_curve", "fm_lmm_simulate", "fm_lmm_get_process_value",
     "fm_lmm_process_value_device_ptr", "fm_lmm_get_numeraire", "fm_lmm_get_brownian_increment",
     "fm_lmm_value_products", "fm_lmm_set_volatility_factorization", "fm_lmm_adjoint",
-    "fm_lmm_bermudan_exercise_times",
+    "fm_lmm_bermudan_exercise_times", "fm_lmm_serialize", "fm_lmm_deserialize",
     "fm_mt19937_words", "fm_mt19937_doubles",
     "fm_last_error", "fm_lmm_last_timings", "fm_device_info",
     # include/finmath_rv.h
@@ -82,7 +83,7 @@ class FmLmmConfig(ctypes.Structure):
         ("num_factors", c_int32), ("num_paths", c_int32), ("seed", c_int32),
         ("measure", c_int32), ("state_space", c_int32), ("scheme", c_int32), ("precision", c_int32),
         ("libor_cap", c_double), ("store_paths", c_int32), ("checkpoint_stride", c_int32),
-        ("regenerate_increments", c_int32),
+        ("regenerate_increments", c_int32), ("path_offset", c_int64),
     ]
 
 
@@ -133,6 +134,8 @@ def load() -> ctypes.CDLL:
         "fm_lmm_set_volatility_factorization": ([c_void_p, dp, dp, dp], c_int32),
         "fm_lmm_adjoint": ([c_void_p, POINTER(FmProduct), c_int32, dp, dp, dp, dp], c_int32),
         "fm_lmm_bermudan_exercise_times": ([c_void_p, c_int32, dp], c_int32),
+        "fm_lmm_serialize": ([c_void_p, c_void_p, POINTER(c_int64)], c_int32),
+        "fm_lmm_deserialize": ([c_void_p, c_int64, POINTER(c_void_p)], c_int32),
         "fm_mt19937_words": ([c_int64, c_uint64, c_uint64, POINTER(c_uint32)], c_int32),
         "fm_mt19937_doubles": ([c_int64, c_uint64, c_uint64, dp], c_int32),
         "fm_last_error": ([], c_char_p),
@@ -245,7 +248,7 @@ class LmmContext:
 
     def __init__(self, times, tenor, num_factors: int, num_paths: int, seed: int, *, measure="SPOT",
                  state_space="LOGNORMAL", scheme="EULER_FUNCTIONAL", precision="FP64", libor_cap=1e5,
-                 store_paths=False, checkpoint_stride=4, regenerate_increments=False):
+                 store_paths=False, checkpoint_stride=4, regenerate_increments=False, path_offset=0):
         self.lib = load()
         self.times = as_f64(times)
         self.tenor = as_f64(tenor)
@@ -265,6 +268,7 @@ class LmmContext:
         cfg.store_paths = 1 if store_paths else 0
         cfg.checkpoint_stride = int(checkpoint_stride)
         cfg.regenerate_increments = 1 if regenerate_increments else 0
+        cfg.path_offset = int(path_offset)
         self.cfg = cfg
         h = c_void_p()
         check(self.lib.fm_lmm_create(ctypes.byref(cfg), ctypes.byref(h)))
@@ -340,6 +344,31 @@ class LmmContext:
         c = None if variance_factor is None else as_f64(variance_factor)
         check(self.lib.fm_lmm_set_volatility_factorization(self.h, _dptr(v), _dptr(f),
                                                            POINTER(c_double)() if c is None else _dptr(c)))
+
+    def serialize(self) -> bytes:
+        """Configuration and tables as bytes (fm_lmm_serialize); the paths are regenerated."""
+        n = c_int64(0)
+        check(self.lib.fm_lmm_serialize(self.h, None, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        check(self.lib.fm_lmm_serialize(self.h, buf, ctypes.byref(n)))
+        return buf.raw[:n.value]
+
+    @classmethod
+    def deserialize(cls, blob: bytes) -> "LmmContext":
+        """A context that regenerates the serialised one's increments and paths (fm_lmm_deserialize)."""
+        lib = load()
+        h = c_void_p()
+        buf = ctypes.create_string_buffer(bytes(blob), len(blob))
+        check(lib.fm_lmm_deserialize(buf, len(blob), ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self.lib = lib
+        self.h = h
+        # shape from the blob header (version 1: magic, version, then nT, n, F, P as int32)
+        nT, n, F, P = struct.unpack_from("<4i", bytes(blob), 12)
+        self.nT, self.n, self.F, self.P = nT, n, F, P
+        self.times = self.tenor = None
+        self.cfg = None
+        return self
 
     def bermudan_exercise_times(self, bermudan: int = 0) -> np.ndarray:
         """exerciseTime of BermudanSwaption.getValues per path (+inf: never exercised)."""

@@ -428,6 +428,7 @@ void EulerSchemeFromProcessModel::ensureContext(bool storePaths) {
     cfg.store_paths = storePaths ? 1 : 0;
     cfg.checkpoint_stride = ckptStride_;
     cfg.regenerate_increments = regenerate_ ? 1 : 0;
+    cfg.path_offset = 0;
     if (driver_->getNumberOfFactors() != model_->getCovarianceModel()->getNumberOfFactors())
         throw IllegalArgumentException("number of factors of the driver and of the covariance model differ");
     fm_lmm* c = nullptr;

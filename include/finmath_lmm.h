@@ -103,6 +103,11 @@ typedef struct fm_lmm_config {
                                             kept in device memory, as the reference caches them
                                             (BrownianMotionFromMersenneRandomNumbers.java:123-136);
                                             1: regenerated by every pricing/simulation call */
+    int64_t       path_offset;      /* first path of this context in the seed's stream: the context
+                                       simulates paths [path_offset, path_offset + num_paths) of a
+                                       run with this seed (path p uses the words from 2 p nT F on,
+                                       BrownianMotionFromMersenneRandomNumbers.java:167-178), so
+                                       path ranges can be resumed or split across devices */
 } fm_lmm_config;
 
 /* Create a context (allocates device memory, seeds the generator).
@@ -255,6 +260,18 @@ int fm_lmm_adjoint(fm_lmm* ctx, const fm_product* products, int32_t count, const
  * [offset, offset+count) and nextDouble() values [offset, offset+count). */
 int fm_mt19937_words(int64_t seed, uint64_t offset, uint64_t count, uint32_t* out);
 int fm_mt19937_doubles(int64_t seed, uint64_t offset, uint64_t count, double* out);
+
+/* ---- serialisation --------------------------------------------------------------
+ * The reference serialises a model and its Brownian motion by their parameters and
+ * seed and regenerates the transient state (increments, process) on demand
+ * (models/LIBORMarketModelFromCovarianceModel.java:1734-1745,
+ *  montecarlo/BrownianMotionFromMersenneRandomNumbers.java:253-257). Likewise:
+ * fm_lmm_serialize writes the configuration (seed, path range, grids, options) and
+ * every table set on the context into a versioned little-endian byte string; NULL
+ * buffer -> *size receives the needed size. fm_lmm_deserialize creates a context that
+ * regenerates the same increments and paths bit for bit. */
+int fm_lmm_serialize(fm_lmm* ctx, void* buffer, int64_t* size);
+int fm_lmm_deserialize(const void* buffer, int64_t size, fm_lmm** out);
 
 /* ---- diagnostics --------------------------------------------------------------- */
 const char* fm_last_error(void);

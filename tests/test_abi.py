@@ -66,3 +66,14 @@ def test_reference_validation_message():
     # BrownianMotionFromMersenneRandomNumbers.java:77-78 wording
     with pytest.raises(_abi.IllegalArgumentException, match="Number of factors must be greater or equal 1"):
         _ctx(num_factors=0)
+
+
+@pytest.mark.parametrize("blob", [b"", b"garbage-bytes-not-a-context", b"FMLMMSER\x02\x00\x00\x00" + bytes(64),
+                                  b"FMLMMSER\x01\x00\x00\x00" + bytes(8)])
+def test_malformed_serialised_context_is_rejected_before_device_use(blob):
+    if not blob:
+        with pytest.raises(_abi.IllegalArgumentException):
+            _abi.check(_abi.load().fm_lmm_deserialize(None, 0, None))
+        return
+    with pytest.raises(_abi.IllegalArgumentException):
+        _abi.LmmContext.deserialize(blob)
