@@ -194,6 +194,39 @@ def run_rv_algebra(steps):
     return out
 
 
+def run_calibration(paths):
+    """getCloneCalibrated of the four-parameter volatility to the c1 strip (30 swaptions),
+    targets priced under (0.2, 0.05, 0.3, 0.1) on the calibration's Brownian motion, start
+    (0.15, 0, 0.25, 0.15); every Levenberg-Marquardt evaluation is one fused pricing pass."""
+    from finmath_hip import _finmath_host as H
+    from finmath_hip import workloads as W
+    wl = W.c1()
+    td = H.TimeDiscretizationFromArray(0.0, wl.num_time_steps, wl.period)
+    tenor = H.TimeDiscretizationFromArray(0.0, wl.num_libors, wl.period)
+    fc = H.ForwardCurveInterpolation.createForwardCurveFromForwards(
+        "forwardCurve", list(tenor.getAsDoubleArray()[:-1]), list(wl.forwards), wl.period)
+
+    def model(params):
+        vol = H.LIBORVolatilityModelFourParameterExponentialForm(td, tenor, *params, True)
+        corr = H.LIBORCorrelationModelExponentialDecay(td, tenor, wl.num_factors, wl.corr_decay)
+        cov = H.LIBORCovarianceModelFromVolatilityAndCorrelation(td, tenor, vol, corr)
+        return cov, H.LIBORMarketModelFromCovarianceModel.of(tenor, None, fc, H.DiscountCurveFromForwardCurve(fc), None,
+                                                             cov, [], {"measure": "SPOT", "stateSpace": "LOGNORMAL"})
+    products = [H.Swaption(s["exercise"], list(s["fixing"]), list(s["payment"]), list(s["strike"])) for s in wl.products]
+    _, mtrue = model([0.2, 0.05, 0.3, 0.1])
+    bm = H.BrownianMotionFromMersenneRandomNumbers(td, wl.num_factors, paths, 31415)
+    sim = H.LIBORMonteCarloSimulationFromLIBORModel(H.EulerSchemeFromProcessModel(mtrue, bm))
+    targets = [r["value"] for r in sim.getValues(products)]
+    cov0, m0 = model([0.15, 0.0, 0.25, 0.15])
+    calib = [H.CalibrationProduct(p, v, 1.0) for p, v in zip(products, targets)]
+    t0 = time.perf_counter()
+    _, info = cov0.getCloneCalibrated(m0, calib, {"numberOfPaths": paths, "accuracy": 1e-12, "maxIterations": 200})
+    dt = time.perf_counter() - t0
+    return {"paths": paths, "seconds": dt, "iterations": info["iterations"], "valuations": info["valuations"],
+            "ms_per_valuation": 1e3 * dt / max(1, info["valuations"]), "rms_error": info["rootMeanSquaredError"],
+            "parameters": info["parameters"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,12 +239,14 @@ def main():
     ap.add_argument("--no-logstate", action="store_true")
     ap.add_argument("--no-bermudan", action="store_true")
     ap.add_argument("--no-rv", action="store_true", help="skip the device random-variable algebra leg")
+    ap.add_argument("--no-calibration", action="store_true", help="skip the covariance calibration leg")
     ap.add_argument("--only-fp64", action="store_true", help="the headline leg only (for counter passes)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.only_fp64:
         args.no_fp32 = args.no_logstate = args.no_bermudan = args.no_greeks = args.no_cpu = args.no_rv = True
+        args.no_calibration = True
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path axis would shard across GPUs "
               "as described in DESIGN.md, but this build measures one MI355X.", file=sys.stderr)
@@ -297,6 +332,11 @@ def main():
             "sum_delta": float(np.sum(g["delta"])), "parameter_vega": [float(x) for x in g["parameterVega"]]}
     if not args.no_rv:
         line["rv_algebra"] = run_rv_algebra(args.steps)
+    if not args.no_calibration:
+        line["calibration"] = {
+            "workload": "c1 model, 4 volatility parameters to the 30-swaption strip, Levenberg-Marquardt with "
+                        "forward-difference Jacobian, one fused pricing pass per evaluation, cached increments",
+            "runs": [run_calibration(2000), run_calibration(100_000)]}
     if not args.no_cpu:
         cb = cpu_baseline(wl, args.cpu_paths)
         line["cpu_baseline"] = cb

@@ -120,6 +120,27 @@ PYBIND11_MODULE(_finmath_host, m) {
         })
         .def("getCovariance", &LIBORCovarianceModelFromVolatilityAndCorrelation::getCovariance)
         .def("getNumberOfFactors", &LIBORCovarianceModelFromVolatilityAndCorrelation::getNumberOfFactors)
+        .def("getParameter", [](const LIBORCovarianceModelFromVolatilityAndCorrelation& c) { return getParameter(c); })
+        .def("getCloneWithModifiedParameters", [](const LIBORCovarianceModelFromVolatilityAndCorrelation& c,
+                                                  std::vector<double> p) {
+            return std::const_pointer_cast<LIBORCovarianceModelFromVolatilityAndCorrelation>(getCloneWithModifiedParameters(c, p));
+        })
+        .def("getCloneCalibrated", [](const LIBORCovarianceModelFromVolatilityAndCorrelation& c,
+                                      std::shared_ptr<LIBORMarketModelFromCovarianceModel> model,
+                                      const std::vector<CalibrationProduct>& products, std::map<std::string, double> params) {
+            CalibrationResult r;
+            {
+                py::gil_scoped_release rel;
+                r = getCloneCalibrated(c, model, products, params);
+            }
+            py::dict info;
+            info["parameters"] = r.parameters;
+            info["iterations"] = r.iterations;
+            info["rootMeanSquaredError"] = r.rootMeanSquaredError;
+            info["valuations"] = r.valuations;
+            return py::make_tuple(std::const_pointer_cast<LIBORCovarianceModelFromVolatilityAndCorrelation>(r.covarianceModel), info);
+        }, py::arg("calibrationModel"), py::arg("calibrationProducts"),
+           py::arg("calibrationParameters") = std::map<std::string, double>{})
         .def("tables", [](const LIBORCovarianceModelFromVolatilityAndCorrelation& c, const TimeDiscretizationFromArray& sim) {
             std::vector<double> lam, var;
             c.tables(sim, lam, var);
@@ -130,6 +151,30 @@ PYBIND11_MODULE(_finmath_host, m) {
             std::copy(var.begin(), var.end(), V.mutable_data());
             return py::make_tuple(L, V);
         });
+
+    py::class_<LevenbergMarquardt>(m, "LevenbergMarquardt")
+        .def(py::init<std::vector<double>, std::vector<double>, int>(), py::arg("initialParameters"),
+             py::arg("targetValues"), py::arg("maxIteration"))
+        .def("setWeights", &LevenbergMarquardt::setWeights, py::return_value_policy::reference_internal)
+        .def("setParameterSteps", &LevenbergMarquardt::setParameterSteps, py::return_value_policy::reference_internal)
+        .def("setErrorTolerance", &LevenbergMarquardt::setErrorTolerance, py::return_value_policy::reference_internal)
+        .def("run", [](LevenbergMarquardt& lm, py::function f) {
+            lm.run([&](const std::vector<double>& p, std::vector<double>& v) {
+                py::list out = f(p).cast<py::list>();
+                v.assign(out.size(), 0.0);
+                for (size_t i = 0; i < out.size(); i++) v[i] = out[i].cast<double>();
+            });
+        }, py::arg("setValues"))
+        .def("getBestFitParameters", &LevenbergMarquardt::getBestFitParameters)
+        .def("getIterations", &LevenbergMarquardt::getIterations)
+        .def("getRootMeanSquaredError", &LevenbergMarquardt::getRootMeanSquaredError);
+
+    py::class_<CalibrationProduct>(m, "CalibrationProduct")
+        .def(py::init([](std::shared_ptr<Product> p, double target, double weight) {
+            return CalibrationProduct{p, target, weight};
+        }), py::arg("product"), py::arg("targetValue"), py::arg("weight") = 1.0)
+        .def_readonly("targetValue", &CalibrationProduct::targetValue)
+        .def_readonly("weight", &CalibrationProduct::weight);
 
     py::class_<LIBORMarketModelFromCovarianceModel, std::shared_ptr<LIBORMarketModelFromCovarianceModel>>(
         m, "LIBORMarketModelFromCovarianceModel")

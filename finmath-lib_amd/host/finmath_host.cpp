@@ -283,8 +283,8 @@ std::vector<double> factorReduction(const std::vector<double>& corr, int n, int 
 }
 
 LIBORCorrelationModelExponentialDecay::LIBORCorrelationModelExponentialDecay(TimeDiscretizationPtr, TimeDiscretizationPtr tenor,
-                                                                             int numberOfFactors, double a, bool)
-    : n_(tenor->getNumberOfTimeSteps()), m_(numberOfFactors), a_(a) {
+                                                                             int numberOfFactors, double a, bool calibrateable)
+    : n_(tenor->getNumberOfTimeSteps()), m_(numberOfFactors), a_(a), calibrateable_(calibrateable) {
     a = std::max(a, 0.0);
     std::vector<double> rho((size_t)n_ * n_);
     for (int r = 0; r < n_; r++)
@@ -671,6 +671,234 @@ void BermudanSwaption::lower(std::deque<std::vector<double>>& arena, fm_product&
     o.is_cancelable = isCallable ? 0 : 1;
     o.basis = provider ? provider->basisId() : FM_BASIS_REFERENCE;
     o.notional = 1.0;
+}
+
+// ---- optimizer/LevenbergMarquardt.java -------------------------------------------------------
+LevenbergMarquardt::LevenbergMarquardt(std::vector<double> initialParameters, std::vector<double> targetValues,
+                                       int maxIteration)
+    : initial_(std::move(initialParameters)), targets_(std::move(targetValues)), maxIteration_(maxIteration) {
+    weights_.assign(targets_.size(), 1.0);
+}
+
+LevenbergMarquardt& LevenbergMarquardt::setWeights(std::vector<double> weights) {
+    if (weights.size() != targets_.size()) throw IllegalArgumentException("one weight per target value");
+    weights_ = std::move(weights);
+    return *this;
+}
+
+LevenbergMarquardt& LevenbergMarquardt::setParameterSteps(std::vector<double> parameterSteps) {
+    if (parameterSteps.size() != initial_.size()) throw IllegalArgumentException("one step per parameter");
+    steps_ = std::move(parameterSteps);
+    return *this;
+}
+
+LevenbergMarquardt& LevenbergMarquardt::setErrorTolerance(double errorTolerance) {
+    tolerance_ = errorTolerance;
+    return *this;
+}
+
+double LevenbergMarquardt::meanSquaredError(const std::vector<double>& v) const {       // :733-742
+    double e = 0.0;
+    for (size_t k = 0; k < v.size(); k++) {
+        const double dev = v[k] - targets_[k];
+        e += weights_[k] * dev * dev;
+    }
+    return e / (double)v.size();
+}
+
+void LevenbergMarquardt::run(const ObjectiveFunction& setValues) {                          // :640-731
+    const int np = (int)initial_.size(), nv = (int)targets_.size();
+    std::vector<double> parameterTest = initial_, valueTest((size_t)nv), valueCurrent((size_t)nv);
+    std::vector<std::vector<double>> derivative((size_t)np, std::vector<double>((size_t)nv));
+    parameterCurrent_ = initial_;
+    errorMeanSquaredCurrent_ = INFINITY;
+    double rmsChange = INFINITY;
+    bool derivativeValid = false;
+    iteration_ = 0;
+    while (true) {
+        iteration_++;
+        setValues(parameterTest, valueTest);
+        const double errorTest = meanSquaredError(valueTest);
+        if (errorTest < errorMeanSquaredCurrent_) {                 // false for NaN: the point is rejected
+            rmsChange = std::sqrt(errorMeanSquaredCurrent_) - std::sqrt(errorTest);
+            parameterCurrent_ = parameterTest;
+            valueCurrent = valueTest;
+            errorMeanSquaredCurrent_ = errorTest;
+            derivativeValid = false;
+            lambda_ /= lambdaDivisor_;
+        } else {
+            rmsChange = std::sqrt(errorTest) - std::sqrt(errorMeanSquaredCurrent_);
+            lambda_ *= lambdaMultiplicator_;
+        }
+        // done(): :623-637
+        if (iteration_ > maxIteration_ || rmsChange <= tolerance_ || std::isinf(lambda_)) break;
+        // updateParameterTest(): :749-815
+        if (!derivativeValid) {                                     // forward differences (:550-620)
+            for (int i = 0; i < np; i++) {
+                std::vector<double> p = parameterCurrent_;
+                const double h = steps_.empty() ? (std::fabs(p[(size_t)i]) + 1) * 1e-8 : steps_[(size_t)i];
+                p[(size_t)i] += h;
+                std::vector<double>& d = derivative[(size_t)i];
+                setValues(p, d);
+                for (int k = 0; k < nv; k++) {
+                    d[(size_t)k] = (d[(size_t)k] - valueCurrent[(size_t)k]) / h;
+                    if (std::isnan(d[(size_t)k])) d[(size_t)k] = 0.0;
+                }
+            }
+            derivativeValid = true;
+        }
+        std::vector<double> increment((size_t)np, 0.0);
+        bool invalid = true;
+        while (invalid && std::isfinite(lambda_)) {
+            invalid = false;
+            std::vector<double> H((size_t)np * np), beta((size_t)np);
+            for (int i = 0; i < np; i++) {
+                for (int j = i; j < np; j++) {
+                    double h = 0.0;
+                    for (int k = 0; k < nv; k++) h += weights_[(size_t)k] * derivative[(size_t)i][(size_t)k] * derivative[(size_t)j][(size_t)k];
+                    if (i == j) h = (h == 0.0) ? lambda_ : h * (1 + lambda_);   // LEVENBERG_MARQUARDT regularisation
+                    H[(size_t)i * np + j] = H[(size_t)j * np + i] = h;
+                }
+                double b = 0.0;
+                for (int k = 0; k < nv; k++)
+                    b += weights_[(size_t)k] * (targets_[(size_t)k] - valueCurrent[(size_t)k]) * derivative[(size_t)i][(size_t)k];
+                beta[(size_t)i] = b;
+            }
+            // pseudo-inverse solve (LinearAlgebra.solveLinearEquationSVD): eigen-decomposition of the
+            // symmetric H, singular values below max(n sigma_max 2^-52, sqrt(2^-1022)) dropped
+            std::vector<double> V = H, ev;
+            symmetric_eigen(V, np, ev);
+            double smax = 0.0;
+            for (double e : ev) smax = std::max(smax, std::fabs(e));
+            const double tol = std::max(np * smax * 0x1.0p-52, std::sqrt(0x1.0p-1022));
+            for (int j = 0; j < np; j++) {
+                if (!(std::fabs(ev[(size_t)j]) > tol)) continue;
+                double vb = 0.0;
+                for (int i = 0; i < np; i++) vb += V[(size_t)i * np + j] * beta[(size_t)i];
+                for (int i = 0; i < np; i++) increment[(size_t)i] += V[(size_t)i * np + j] * (vb / ev[(size_t)j]);
+            }
+            for (double x : increment)
+                if (!std::isfinite(x)) invalid = true;
+            if (invalid) {
+                lambda_ *= 16;
+                std::fill(increment.begin(), increment.end(), 0.0);
+            }
+        }
+        for (int i = 0; i < np; i++) parameterTest[(size_t)i] = parameterCurrent_[(size_t)i] + increment[(size_t)i];
+    }
+}
+
+// ---- calibration of the covariance model -----------------------------------------------------
+std::vector<double> getParameter(const LIBORCovarianceModelFromVolatilityAndCorrelation& cov) {
+    std::vector<double> p = cov.getVolatilityModel()->getParameter();
+    if (cov.getCorrelationModel()->isCalibrateable()) p.push_back(cov.getCorrelationModel()->getParameter());
+    return p;
+}
+
+CovarianceModelPtr getCloneWithModifiedParameters(const LIBORCovarianceModelFromVolatilityAndCorrelation& cov,
+                                                  const std::vector<double>& parameters) {
+    size_t used = 0;
+    VolatilityModelPtr vol = cov.getVolatilityModel();
+    auto four = std::dynamic_pointer_cast<const LIBORVolatilityModelFourParameterExponentialForm>(vol);
+    if (four && four->isCalibrateable) {
+        if (parameters.size() < 4) throw IllegalArgumentException("four volatility parameters expected");
+        vol = std::make_shared<LIBORVolatilityModelFourParameterExponentialForm>(
+            cov.getTimeDiscretization(), cov.getLiborPeriodDiscretization(), parameters[0], parameters[1], parameters[2],
+            parameters[3], true);
+        used = 4;
+    } else if (!vol->getParameter().empty()) {
+        throw UnsupportedOperationException("only the four-parameter volatility form is calibrated");
+    }
+    CorrelationModelPtr corr = cov.getCorrelationModel();
+    if (corr->isCalibrateable()) {
+        if (parameters.size() < used + 1) throw IllegalArgumentException("correlation parameter expected");
+        corr = std::make_shared<LIBORCorrelationModelExponentialDecay>(cov.getTimeDiscretization(),
+                                                                      cov.getLiborPeriodDiscretization(),
+                                                                      corr->getNumberOfFactors(), parameters[used], true);
+        used++;
+    }
+    if (used != parameters.size()) throw IllegalArgumentException("number of parameters does not match the model");
+    return std::make_shared<LIBORCovarianceModelFromVolatilityAndCorrelation>(cov.getTimeDiscretization(),
+                                                                             cov.getLiborPeriodDiscretization(), vol, corr);
+}
+
+CalibrationResult getCloneCalibrated(const LIBORCovarianceModelFromVolatilityAndCorrelation& cov,
+                                     LIBORMarketModelPtr model, const std::vector<CalibrationProduct>& products,
+                                     const std::map<std::string, double>& params) {
+    auto opt = [&](const char* k, double def) { auto it = params.find(k); return it == params.end() ? def : it->second; };
+    const int numberOfPaths = (int)opt("numberOfPaths", 2000);
+    const int seed = (int)opt("seed", 31415);
+    const int maxIterations = (int)opt("maxIterations", 400);
+    const double accuracy = opt("accuracy", 1e-7);
+    const double parameterStep = opt("parameterStep", 1e-4);
+    const int precision = (int)opt("precision", FM_PRECISION_FP64);
+    if (!model) throw IllegalArgumentException("calibration model must be given");
+    if (products.empty()) throw IllegalArgumentException("no calibration products");
+    const std::vector<double> initial = getParameter(cov);
+    if (initial.empty()) throw IllegalArgumentException("the covariance model has no calibrateable parameters");
+
+    // one context, one Brownian motion on the covariance model's time discretization
+    const auto td = cov.getTimeDiscretization();
+    const auto& times = td->getAsDoubleArray();
+    const auto& tenor = model->getLiborPeriodDiscretization()->getAsDoubleArray();
+    fm_lmm_config cfg{};
+    cfg.num_time_steps = (int32_t)times.size() - 1;
+    cfg.times = times.data();
+    cfg.num_libors = (int32_t)tenor.size() - 1;
+    cfg.tenor = tenor.data();
+    cfg.num_factors = cov.getNumberOfFactors();
+    cfg.num_paths = numberOfPaths;
+    cfg.seed = seed;
+    cfg.measure = model->measure;
+    cfg.state_space = model->stateSpace;
+    cfg.scheme = FM_SCHEME_EULER_FUNCTIONAL;
+    cfg.precision = precision;
+    cfg.libor_cap = model->liborCap;
+    cfg.regenerate_increments = 0;
+    fm_lmm* ctx = nullptr;
+    check(fm_lmm_create(&cfg, &ctx));
+    std::unique_ptr<fm_lmm, void (*)(fm_lmm*)> guard(ctx, fm_lmm_destroy);
+    const std::vector<double> L0 = model->getInitialForwards();
+    check(fm_lmm_set_initial_forwards(ctx, L0.data()));
+    if (model->getDiscountCurve()) {
+        std::vector<double> pd(tenor.size());
+        for (size_t k = 0; k < tenor.size(); k++) pd[k] = model->getDiscountCurve()->getDiscountFactor(tenor[k]);
+        check(fm_lmm_set_discount_curve(ctx, pd.data()));
+    }
+    std::deque<std::vector<double>> arena;
+    std::vector<fm_product> lowered(products.size());
+    for (size_t q = 0; q < products.size(); q++) {
+        if (!products[q].product) throw IllegalArgumentException("calibration product is null");
+        products[q].product->lower(arena, lowered[q]);
+    }
+    int valuations = 0;
+    std::vector<double> lam, var, vals(products.size());
+    auto setValues = [&](const std::vector<double>& p, std::vector<double>& out) {
+        out.assign(products.size(), NAN);
+        try {
+            const CovarianceModelPtr c = getCloneWithModifiedParameters(cov, p);
+            c->tables(*td, lam, var);
+            check(fm_lmm_set_covariance(ctx, lam.data(), var.data()));
+            check(fm_lmm_value_products(ctx, lowered.data(), (int32_t)lowered.size(), vals.data(), nullptr));
+            valuations++;
+            for (size_t q = 0; q < products.size(); q++)
+                out[q] = (vals[q] - products[q].targetValue) * products[q].weight;
+        } catch (const CalculationException&) {
+            // a failed valuation (e.g. a non-finite price) is a rejected point
+        } catch (const IllegalArgumentException&) {
+            // parameters whose tables are not finite: rejected as well
+        }
+    };
+    LevenbergMarquardt lm(initial, std::vector<double>(products.size(), 0.0), maxIterations);
+    lm.setErrorTolerance(accuracy).setParameterSteps(std::vector<double>(initial.size(), parameterStep));
+    lm.run(setValues);
+    CalibrationResult r;
+    r.parameters = lm.getBestFitParameters();
+    r.covarianceModel = getCloneWithModifiedParameters(cov, r.parameters);
+    r.iterations = lm.getIterations();
+    r.rootMeanSquaredError = lm.getRootMeanSquaredError();
+    r.valuations = valuations;
+    return r;
 }
 
 } // namespace finmath

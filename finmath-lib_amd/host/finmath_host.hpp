@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -128,10 +129,12 @@ public:
     double getCorrelation(int, int c1, int c2) const { return corr_[(size_t)c1 * n_ + c2]; }
     int getNumberOfFactors() const { return m_; }
     double getParameter() const { return a_; }
+    bool isCalibrateable() const { return calibrateable_; }
     const std::vector<double>& factorMatrix() const { return factor_; }
 private:
     int n_, m_;
     double a_;
+    bool calibrateable_ = false;
     std::vector<double> factor_, corr_;
 };
 using CorrelationModelPtr = std::shared_ptr<const LIBORCorrelationModelExponentialDecay>;
@@ -390,5 +393,68 @@ public:
 private:
     std::vector<int32_t> exerciseFlags_;
 };
+
+// ---- optimizer/LevenbergMarquardt.java -----------------------------------------------------
+// Minimises sum_k w_k (f_k(x) - y_k)^2. Each step solves (J^T W J with its diagonal scaled by
+// 1 + lambda) dx = J^T W (y - f) by the pseudo-inverse; lambda is divided by 3 after an
+// improvement, multiplied by 2 after a rejected point and by 16 when the step cannot be
+// solved; the Jacobian is a forward difference with the given parameter steps; the solver
+// stops when the RMS error improves by no more than the tolerance, after maxIteration
+// iterations, or when lambda becomes infinite (LevenbergMarquardt.java:623-815). A point
+// with a NaN value is rejected.
+class LevenbergMarquardt {
+public:
+    using ObjectiveFunction = std::function<void(const std::vector<double>& parameters, std::vector<double>& values)>;
+    LevenbergMarquardt(std::vector<double> initialParameters, std::vector<double> targetValues, int maxIteration);
+    LevenbergMarquardt& setWeights(std::vector<double> weights);
+    LevenbergMarquardt& setParameterSteps(std::vector<double> parameterSteps);
+    LevenbergMarquardt& setErrorTolerance(double errorTolerance);
+    void run(const ObjectiveFunction& setValues);
+    std::vector<double> getBestFitParameters() const { return parameterCurrent_; }
+    int getIterations() const { return iteration_; }
+    double getRootMeanSquaredError() const { return std::sqrt(errorMeanSquaredCurrent_); }
+private:
+    double meanSquaredError(const std::vector<double>& values) const;
+    std::vector<double> initial_, targets_, weights_, steps_;
+    int maxIteration_;
+    double tolerance_ = 0.0;
+    double lambda_ = 0.001, lambdaDivisor_ = 3.0, lambdaMultiplicator_ = 2.0;
+    std::vector<double> parameterCurrent_;
+    double errorMeanSquaredCurrent_ = INFINITY;
+    int iteration_ = 0;
+};
+
+// montecarlo/interestrate/CalibrationProduct.java: a product, its target value and weight.
+struct CalibrationProduct {
+    ProductPtr product;
+    double targetValue = 0.0;
+    double weight = 1.0;
+};
+
+struct CalibrationResult {
+    CovarianceModelPtr covarianceModel;   // the calibrated clone
+    std::vector<double> parameters;       // its calibrated parameters
+    int iterations = 0;
+    double rootMeanSquaredError = 0.0;
+    int valuations = 0;                   // pricing passes on the device
+};
+
+// The parameters of the covariance model, in the reference's order: the four-parameter
+// volatility (a, b, c, d) when it is calibrateable, then the correlation decay when that is
+// calibrateable (LIBORCovarianceModelFromVolatilityAndCorrelation.getParameter).
+std::vector<double> getParameter(const LIBORCovarianceModelFromVolatilityAndCorrelation& covarianceModel);
+CovarianceModelPtr getCloneWithModifiedParameters(const LIBORCovarianceModelFromVolatilityAndCorrelation& covarianceModel,
+                                                  const std::vector<double>& parameters);
+
+// AbstractLIBORCovarianceModelParametric.getCloneCalibrated (covariance/
+// AbstractLIBORCovarianceModelParametric.java:156-276): Levenberg-Marquardt on the calibration
+// errors (value - target) * weight, every evaluation one fused pricing pass of all products on
+// the device, with one Brownian motion (its increments kept in device memory) for all
+// evaluations. calibrationParameters: numberOfPaths (2000), seed (31415), maxIterations (400),
+// accuracy (1e-7), parameterStep (1e-4), precision (FM_PRECISION_FP64).
+CalibrationResult getCloneCalibrated(const LIBORCovarianceModelFromVolatilityAndCorrelation& covarianceModel,
+                                     LIBORMarketModelPtr calibrationModel,
+                                     const std::vector<CalibrationProduct>& calibrationProducts,
+                                     const std::map<std::string, double>& calibrationParameters);
 
 } // namespace finmath

@@ -1,0 +1,93 @@
+"""Calibration of the covariance model on the device (AbstractLIBORCovarianceModelParametric.
+getCloneCalibrated, covariance/AbstractLIBORCovarianceModelParametric.java:156-276), through
+the host mirror.
+
+Targets are the prices of a swaption strip under known volatility parameters, simulated
+on the calibration's own Brownian motion (the reference's defaults: 2,000 paths, seed
+31415), so the calibration error has an exact zero at those parameters: Levenberg-
+Marquardt started away from them must find them, and the calibrated clone must reprice
+the targets.
+"""
+import numpy as np
+import pytest
+
+from finmath_hip import _finmath_host as H
+from finmath_hip import workloads as W
+from test_gpu_parity import say
+
+pytestmark = pytest.mark.gpu
+
+TRUE = [0.2, 0.05, 0.3, 0.1]
+
+
+def build(vol_params, n=20, factors=3, calibrate_corr=False, corr=0.1):
+    td = H.TimeDiscretizationFromArray(0.0, n, 0.5)
+    tenor = H.TimeDiscretizationFromArray(0.0, n, 0.5)
+    fc = H.ForwardCurveInterpolation.createForwardCurveFromForwards(
+        "forwardCurve", list(tenor.getAsDoubleArray()[:-1]), list(0.02 + 0.001 * np.arange(n)), 0.5)
+    vol = H.LIBORVolatilityModelFourParameterExponentialForm(td, tenor, *vol_params, True)
+    cm = H.LIBORCorrelationModelExponentialDecay(td, tenor, factors, corr, calibrate_corr)
+    cov = H.LIBORCovarianceModelFromVolatilityAndCorrelation(td, tenor, vol, cm)
+    model = H.LIBORMarketModelFromCovarianceModel.of(tenor, None, fc, H.DiscountCurveFromForwardCurve(fc), None, cov,
+                                                     [], {"measure": "SPOT", "stateSpace": "LOGNORMAL"})
+    return td, tenor, fc, cov, model
+
+
+def strip(fc):
+    products = []
+    for e in (1.0, 2.0, 3.0, 4.0, 5.0):
+        fix = [e + 0.5 * i for i in range(10)]
+        pay = [f + 0.5 for f in fix]
+        t = H.TimeDiscretizationFromArray(fix + [pay[-1]])
+        atm = H.getForwardSwapRate(t, t, fc)
+        for k in (-0.01, 0.0, 0.01):
+            products.append(H.Swaption(e, fix, pay, [atm + k] * 10))
+    return products
+
+
+def prices(model, td, factors, products, paths=2000, seed=31415):
+    bm = H.BrownianMotionFromMersenneRandomNumbers(td, factors, paths, seed)
+    sim = H.LIBORMonteCarloSimulationFromLIBORModel(H.EulerSchemeFromProcessModel(model, bm))
+    return [r["value"] for r in sim.getValues(products)]
+
+
+def test_calibration_recovers_the_parameters_that_generated_the_targets():
+    say("case calibration four-parameter volatility, 15 swaptions, 2000 paths")
+    td, tenor, fc, cov_true, model_true = build(TRUE)
+    products = strip(fc)
+    targets = prices(model_true, td, 3, products)
+    td, tenor, fc, cov0, model0 = build([0.15, 0.0, 0.25, 0.15])
+    calib = [H.CalibrationProduct(p, v, 1.0) for p, v in zip(products, targets)]
+    cov_cal, info = cov0.getCloneCalibrated(model0, calib, {"accuracy": 1e-12, "maxIterations": 200})
+    say(f"  parameters {info['parameters']} iterations {info['iterations']} valuations {info['valuations']} "
+        f"rms {info['rootMeanSquaredError']:.3e}")
+    assert np.allclose(info["parameters"], TRUE, rtol=0, atol=2e-4)
+    assert info["rootMeanSquaredError"] < 1e-7
+    # the calibrated clone reprices the targets on the same Brownian motion
+    model_cal = H.LIBORMarketModelFromCovarianceModel.of(tenor, None, fc, H.DiscountCurveFromForwardCurve(fc), None,
+                                                         cov_cal, [], {"measure": "SPOT", "stateSpace": "LOGNORMAL"})
+    rep = prices(model_cal, td, 3, products)
+    assert np.max(np.abs(np.array(rep) - np.array(targets))) < 1e-6
+
+
+def test_calibration_with_the_correlation_parameter_reduces_the_error():
+    say("case calibration volatility + correlation decay")
+    td, tenor, fc, _, model_true = build(TRUE, corr=0.2)
+    products = strip(fc)
+    targets = prices(model_true, td, 3, products)
+    td, tenor, fc, cov0, model0 = build([0.15, 0.0, 0.25, 0.15], calibrate_corr=True, corr=0.05)
+    calib = [H.CalibrationProduct(p, v, 1.0) for p, v in zip(products, targets)]
+    start_rms = float(np.sqrt(np.mean((np.array(prices(model0, td, 3, products)) - np.array(targets)) ** 2)))
+    cov_cal, info = cov0.getCloneCalibrated(model0, calib, {"maxIterations": 60})
+    say(f"  parameters {info['parameters']} rms {start_rms:.3e} -> {info['rootMeanSquaredError']:.3e}")
+    assert len(info["parameters"]) == 5
+    assert info["rootMeanSquaredError"] < 0.05 * start_rms
+
+
+def test_calibration_without_calibrateable_parameters_is_refused():
+    td, tenor, fc, cov, model = build(TRUE)
+    fixed = H.LIBORCovarianceModelFromVolatilityAndCorrelation(
+        td, tenor, H.LIBORVolatilityModelFourParameterExponentialForm(td, tenor, *TRUE, False),
+        H.LIBORCorrelationModelExponentialDecay(td, tenor, 3, 0.1, False))
+    with pytest.raises(ValueError):
+        fixed.getCloneCalibrated(model, [H.CalibrationProduct(strip(fc)[0], 0.01, 1.0)], {})
