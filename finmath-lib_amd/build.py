@@ -42,6 +42,8 @@ SOURCES = {
     "lmm_step_pc_fp64.hip": NO_CONTRACT,
     "lmm_step_general_fp32.hip": [],
     "lmm_step_pc_fp32.hip": [],
+    "lmm_step_lv_fp64.hip": NO_CONTRACT,
+    "lmm_step_lv_fp32.hip": [],
     "lmm_reduce.hip": NO_CONTRACT,
     "lmm_bermudan.hip": NO_CONTRACT,
     "rv.hip": NO_CONTRACT,

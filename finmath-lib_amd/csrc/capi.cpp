@@ -81,6 +81,9 @@ struct fm_lmm {
     bool has_L0 = false, has_cov = false, has_pd = false;
     std::vector<double> vf_sig, vf_fac, vf_c;       // volatility factorization for the adjoint
     bool has_vf = false;
+    std::vector<double> lvb;                        // local-volatility offsets b_ij [nT][n]
+    double lv_alpha = 0.0;
+    bool has_lv = false;
     bool tables_dirty = true;
 
     // derived host tables
@@ -98,7 +101,7 @@ struct fm_lmm {
     bool uniform_dt = false;            // all simulation steps have the same length
     std::vector<std::vector<double>> berm_fixing;   // fixing dates of the Bermudans of the last pass
 
-    DevBuf d_times, d_first, d_tenor_ti, d_rec, d_delta, d_L0, d_Y0, d_Y0cap, d_sqrt_dt;
+    DevBuf d_times, d_first, d_tenor_ti, d_rec, d_delta, d_L0, d_Y0, d_Y0cap, d_sqrt_dt, d_lvb;
     DevBuf d_states, d_poly, d_dw, d_row_of, d_paths, d_num;
     DevBuf d_products, d_periods, d_wave_sum, d_wave_m2, d_contrib, d_out, d_swapmeta, d_scale;
     DevBuf d_ckpt, d_adj_num, d_kcur, d_fsig, d_sig, d_cdiag, d_wave_grad, d_grad_out;
@@ -119,8 +122,9 @@ struct fm_lmm {
     bool carry_matters() const {
         return carry() && (cfg.state_space == FM_STATE_SPACE_LOGNORMAL || !std::isinf(cfg.libor_cap));
     }
-    // spot measure, functional (or equivalent) Euler: the kernel the adjoint differentiates
-    bool plain_euler() const { return !terminal() && !pc() && !carry_matters(); }
+    // spot measure, functional (or equivalent) Euler, loadings without local volatility:
+    // the kernel the adjoint differentiates
+    bool plain_euler() const { return !terminal() && !pc() && !carry_matters() && !has_lv; }
     size_t real_size() const { return fp32() ? sizeof(float) : sizeof(double); }
     int process_time_index(double t) const { return jsearch(times, round_tick(t)); }
     int libor_index(double t) const { return jsearch(tenor, round_tick(t)); }
@@ -240,10 +244,17 @@ void prepare_model(fm_lmm* c) {
     upload(c->d_delta, delta, s);
     upload(c->d_sqrt_dt, sq, s);
     upload(c->d_row_of, c->row_of, s);
+    // host staging buffers stay alive until the stream synchronisation below
+    std::vector<float> rf, lvf;
+    std::vector<double> l0d;
+    if (c->has_lv) {
+        if (c->fp32()) { lvf.assign(c->lvb.begin(), c->lvb.end()); upload(c->d_lvb, lvf, s); }
+        else upload(c->d_lvb, c->lvb, s);
+    }
     if (c->fp32()) {
-        std::vector<float> rf(rec.begin(), rec.end());
+        rf.assign(rec.begin(), rec.end());
         upload(c->d_rec, rf, s);
-        std::vector<double> l0d((size_t)n);                 // float-rounded initial state, passed as double
+        l0d.resize((size_t)n);                              // float-rounded initial state, passed as double
         for (int j = 0; j < n; j++) l0d[(size_t)j] = (double)(float)c->init_value[(size_t)j];
         upload(c->d_L0, l0d, s);
     } else {
@@ -723,12 +734,16 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     // EULER with a finite cap caps only the value and runs on the general kernel
     const bool logstate_kernel = c->cfg.precision == FM_PRECISION_FP64_LOGSTATE &&
         c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL && !c->terminal() && !c->pc() &&
-        (!c->carry() || std::isinf(c->cfg.libor_cap));
+        (!c->carry() || std::isinf(c->cfg.libor_cap)) && !c->has_lv;
     if (logstate_kernel) {
         a.Y0 = c->d_Y0cap.as<double>();
         launch_lmm_step_fp64_logstate(a, s);
-    }
-    else if (c->plain_euler()) {
+    } else if (c->has_lv) {
+        a.lv_alpha = c->lv_alpha;
+        a.lvb = c->d_lvb.p;
+        if (c->fp32()) launch_lmm_step_lv_fp32(a, s);
+        else launch_lmm_step_lv_fp64(a, s);
+    } else if (c->plain_euler()) {
         if (c->fp32()) launch_lmm_step_fp32(a, s);
         else launch_lmm_step_fp64(a, s);
     } else if (c->pc()) {
@@ -858,6 +873,27 @@ int fm_lmm_set_covariance(fm_lmm* c, const double* factor_loading, const double*
     for (double v : c->lam) require(std::isfinite(v), FM_ERR_INVALID_ARGUMENT, "factor loadings must be finite");
     for (double v : c->var) require(std::isfinite(v), FM_ERR_INVALID_ARGUMENT, "variances must be finite");
     c->has_cov = true; c->tables_dirty = true; c->simulated = false;
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_set_local_volatility(fm_lmm* c, double alpha, const double* offsets) {
+    FM_TRY
+    require(c != nullptr, FM_ERR_INVALID_ARGUMENT, "NULL context");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (offsets) {
+        require(std::isfinite(alpha), FM_ERR_INVALID_ARGUMENT, "the local-volatility blend must be finite");
+        std::vector<double> b(offsets, offsets + (size_t)c->nT() * c->n());
+        for (double v : b) require(std::isfinite(v), FM_ERR_INVALID_ARGUMENT, "local-volatility offsets must be finite");
+        c->lvb = std::move(b);
+        c->lv_alpha = alpha;
+        c->has_lv = true;
+    } else {
+        c->lvb.clear();
+        c->lv_alpha = 0.0;
+        c->has_lv = false;
+    }
+    c->tables_dirty = true; c->simulated = false;
     return FM_OK;
     FM_CATCH
 }
@@ -1274,10 +1310,12 @@ int fm_lmm_serialize(fm_lmm* c, void* buffer, int64_t* size) {
     w.put_doubles(c->times);
     w.put_doubles(c->tenor);
     w.put<int32_t>(c->has_L0); w.put<int32_t>(c->has_cov); w.put<int32_t>(c->has_pd); w.put<int32_t>(c->has_vf);
+    w.put<int32_t>(c->has_lv);
     if (c->has_L0) w.put_doubles(c->L0);
     if (c->has_cov) { w.put_doubles(c->lam); w.put_doubles(c->var); }
     if (c->has_pd) w.put_doubles(c->pd);
     if (c->has_vf) { w.put_doubles(c->vf_sig); w.put_doubles(c->vf_fac); w.put_doubles(c->vf_c); }
+    if (c->has_lv) { w.put<double>(c->lv_alpha); w.put_doubles(c->lvb); }
     const int64_t need = (int64_t)w.b.size();
     if (buffer) {
         require(*size >= need, FM_ERR_INVALID_ARGUMENT, "buffer too small for the serialised context");
@@ -1311,11 +1349,14 @@ int fm_lmm_deserialize(const void* buffer, int64_t size, fm_lmm** out) {
     g.times = times.data();
     g.tenor = tenor.data();
     const int32_t hL0 = r.get<int32_t>(), hcov = r.get<int32_t>(), hpd = r.get<int32_t>(), hvf = r.get<int32_t>();
-    std::vector<double> L0, lam, var, pd, sig, fac, cv;
+    const int32_t hlv = r.get<int32_t>();
+    std::vector<double> L0, lam, var, pd, sig, fac, cv, lvb;
+    double lv_alpha = 0.0;
     if (hL0) L0 = r.get_doubles(n);
     if (hcov) { lam = r.get_doubles((int64_t)nT * n * F); var = r.get_doubles((int64_t)nT * n); }
     if (hpd) pd = r.get_doubles((int64_t)n + 1);
     if (hvf) { sig = r.get_doubles((int64_t)nT * n); fac = r.get_doubles((int64_t)n * F); cv = r.get_doubles(n); }
+    if (hlv) { lv_alpha = r.get<double>(); lvb = r.get_doubles((int64_t)nT * n); }
     require(r.left == 0, FM_ERR_INVALID_ARGUMENT, "trailing bytes after the serialised context");
     fm_lmm* c = nullptr;
     int rc = fm_lmm_create(&g, &c);
@@ -1325,6 +1366,7 @@ int fm_lmm_deserialize(const void* buffer, int64_t size, fm_lmm** out) {
     if (hcov && (rc = fm_lmm_set_covariance(c, lam.data(), var.data())) != FM_OK) return rc;
     if (hpd && (rc = fm_lmm_set_discount_curve(c, pd.data())) != FM_OK) return rc;
     if (hvf && (rc = fm_lmm_set_volatility_factorization(c, sig.data(), fac.data(), cv.data())) != FM_OK) return rc;
+    if (hlv && (rc = fm_lmm_set_local_volatility(c, lv_alpha, lvb.data())) != FM_OK) return rc;
     *out = guard.release();
     return FM_OK;
     FM_CATCH

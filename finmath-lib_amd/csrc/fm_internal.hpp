@@ -102,6 +102,8 @@ struct StepArgs {
     int terminal;                // TERMINAL measure (general kernel)
     int carry;                   // EULER / PREDICTOR_CORRECTOR: the state is carried (general kernel)
     int pc;                      // predictor-corrector (general kernel)
+    double lv_alpha;             // local-volatility factor (L - L alpha) + b_ij, or L + b_ij for alpha = 0
+    const void* lvb;             // device [nT][n] b_ij (double or float), local-volatility kernels only
     const void* dw;              // device dW [P][nT][F]
     double cap;
     int32_t scheme;
@@ -129,6 +131,9 @@ void launch_lmm_step_general_fp64(const StepArgs& a, void* stream);
 void launch_lmm_step_general_fp32(const StepArgs& a, void* stream);
 void launch_lmm_step_pc_fp64(const StepArgs& a, void* stream);
 void launch_lmm_step_pc_fp32(const StepArgs& a, void* stream);
+// General variant with a local-volatility covariance wrapper (a.pc selects the corrector).
+void launch_lmm_step_lv_fp64(const StepArgs& a, void* stream);
+void launch_lmm_step_lv_fp32(const StepArgs& a, void* stream);
 
 // Reverse sweep (lmm_adjoint.hpp): adjoint arguments and launchers.
 struct AdjArgs {

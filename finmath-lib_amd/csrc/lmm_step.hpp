@@ -16,7 +16,8 @@
 // path in double precision (SPOT measure, LMM:1050-1066).
 // The general variant (kVarGeneral*) adds the terminal measure (components in
 // descending order, mu_j from the later components, N(T_k) = prod_{l>=k} 1/(1+L_l d_l)),
-// the carried state of the EULER / PREDICTOR_CORRECTOR schemes and the corrector step.
+// the carried state of the EULER / PREDICTOR_CORRECTOR schemes, the corrector step and
+// local-volatility factors on the loadings.
 //
 // Model data layout: one record of rec_size_of(FCAP) reals per (time step,
 // component): the factor loadings padded with zeros to FCAP, then variance*(-0.5),
@@ -185,30 +186,54 @@ struct Variant {
     bool carry;        // EULER / PREDICTOR_CORRECTOR: the state is carried between steps
                        // (else the *_FUNCTIONAL schemes re-derive it from the value, Euler:234-237)
     real cap;
+    real lv_alpha;     // local-volatility factor f = (L - L alpha) + b_ij, or L + b_ij for alpha = 0
+    const real* lvb;   // b_ij, [nT][n] (local-volatility kernels only)
 };
+
+// Local-volatility factor of component j at time index i from its value L at that time
+// index: DisplacedLocalVolatilityModel L + d (covariance/DisplacedLocalVolatilityModel.java:175-186),
+// BlendedLocalVolatilityModel (L - L a) + a f0 (covariance/BlendedLocalVolatilityModel.java:265-283);
+// every factor loading of the component is multiplied by it.
+template <typename real>
+__device__ __forceinline__ real local_vol_factor(real L, const Variant<real>& v, int i, int j, int n) {
+    real x = L;
+    if (v.lv_alpha != (real)0) x = x - x * v.lv_alpha;
+    return x + v.lvb[(size_t)i * n + j];
+}
 
 // Drift of one component and the update of the factor sums S, in the measure's order
 // (LMM:1132-1181): SPOT adds w lambda_j to S and then takes mu_j = S . lambda_j;
 // TERMINAL takes mu_j = S . lambda_j of the later components first and then adds
 // w lambda_j. The record's d/d and 1/d hold -1 and -1/d under TERMINAL
 // (Scalar.of(-d).discount(L, d), stochastic/Scalar.java:320-328), so w carries the sign.
-template <int FCAP, typename real, bool TERMINAL>
-__device__ __forceinline__ real drift_component(real (&S)[FCAP], const Rec<FCAP, real>& R, real Lj, bool lognormal) {
+// LV: the loadings are lambda_jk * f; the log-normal variance is then the sum of their
+// squares (AbstractLIBORCovarianceModel.getCovariance, covariance/AbstractLIBORCovarianceModel.java:79-92)
+// instead of the record's sigma_j sigma_j rho_jj.
+template <int FCAP, typename real, bool TERMINAL, bool LV = false>
+__device__ __forceinline__ real drift_component(real (&S)[FCAP], const Rec<FCAP, real>& R, real Lj, bool lognormal,
+                                                real f = (real)1) {
+    auto lam = [&](int k) -> real { return LV ? R.lam(k) * f : R.lam(k); };
     real w = dev_div<real>((real)1.0, Lj * R.c1() + R.c2());
     if (lognormal) w = w * Lj;
     real mu;
     if (TERMINAL) {
-        mu = S[0] * R.lam(0);
+        mu = S[0] * lam(0);
 #pragma unroll
-        for (int k = 1; k < FCAP; k++) mu = mu + S[k] * R.lam(k);
+        for (int k = 1; k < FCAP; k++) mu = mu + S[k] * lam(k);
 #pragma unroll
-        for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * R.lam(k);
+        for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * lam(k);
     } else {
 #pragma unroll
-        for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * R.lam(k);
-        mu = S[0] * R.lam(0);
+        for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * lam(k);
+        mu = S[0] * lam(0);
 #pragma unroll
-        for (int k = 1; k < FCAP; k++) mu = mu + S[k] * R.lam(k);
+        for (int k = 1; k < FCAP; k++) mu = mu + S[k] * lam(k);
+    }
+    if (LV && lognormal) {
+        real c = lam(0) * lam(0);
+#pragma unroll
+        for (int k = 1; k < FCAP; k++) c = c + lam(k) * lam(k);
+        return mu + c * (real)(-0.5);
     }
     return mu + R.hv();
 }
@@ -217,7 +242,9 @@ __device__ __forceinline__ real drift_component(real (&S)[FCAP], const Rec<FCAP,
 // (carried schemes) of each component. The predictor pass follows Euler:232-276; with PC
 // the corrector re-evaluates the drift on the predicted values and adds
 // ((mu_pred - mu) / 2) dt to the state (Euler:292-314). MU keeps the predictor's drift.
-template <int NCAP, int FCAP, typename real, bool TERMINAL, bool PC>
+// LV: the loadings of the drift and the diffusion carry the local-volatility factor of the
+// value the drift is evaluated on (the start value; the predicted value in the corrector).
+template <int NCAP, int FCAP, typename real, bool TERMINAL, bool PC, bool LV>
 __device__ __forceinline__ void general_step(real (&X)[NCAP], real (&MU)[NCAP], int n, int i, int first, real dt,
                                              const real (&dW)[FCAP], const real* __restrict__ rec,
                                              const Variant<real>& v, real* __restrict__ paths,
@@ -231,11 +258,12 @@ __device__ __forceinline__ void general_step(real (&X)[NCAP], real (&MU)[NCAP], 
         if (j < first || j >= n) continue;                  // uniform
         const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, i, j);
         const real Lj = v.carry ? to_value<real>(X[j], v.lognormal, v.capped, v.cap) : X[j];
-        const real mu = drift_component<FCAP, real, TERMINAL>(S, R, Lj, v.lognormal);
+        const real f = LV ? local_vol_factor<real>(Lj, v, i, j, n) : (real)1;
+        const real mu = drift_component<FCAP, real, TERMINAL, LV>(S, R, Lj, v.lognormal, f);
         real Y = v.carry ? X[j] : (v.lognormal ? dev_log<real>(Lj) : Lj);
         Y = Y + mu * dt;
 #pragma unroll
-        for (int k = 0; k < FCAP; k++) Y = Y + dW[k] * R.lam(k);
+        for (int k = 0; k < FCAP; k++) Y = Y + dW[k] * (LV ? R.lam(k) * f : R.lam(k));
         if (PC) {
             MU[j] = mu;
             X[j] = Y;
@@ -254,7 +282,8 @@ __device__ __forceinline__ void general_step(real (&X)[NCAP], real (&MU)[NCAP], 
             if (j < first || j >= n) continue;
             const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, i, j);
             const real Lp = to_value<real>(X[j], v.lognormal, v.capped, v.cap);    // predicted value
-            const real mup = drift_component<FCAP, real, TERMINAL>(S, R, Lp, v.lognormal);
+            const real f = LV ? local_vol_factor<real>(Lp, v, i, j, n) : (real)1;
+            const real mup = drift_component<FCAP, real, TERMINAL, LV>(S, R, Lp, v.lognormal, f);
             const real Y = X[j] + ((mup - MU[j]) * (real)0.5) * dt;
             const real Ln = to_value<real>(Y, v.lognormal, v.capped, v.cap);
             X[j] = v.carry ? Y : Ln;
@@ -276,7 +305,9 @@ __device__ __forceinline__ void load_increments(real (&dW)[FCAP], const real* __
 //                   capped as log-states, contraction allowed
 //   kVarGeneral     spot or terminal measure, any state space, functional or carried state
 //   kVarGeneralPC   kVarGeneral with the predictor-corrector
-constexpr int kVarFunctional = 0, kVarLogState = 1, kVarGeneral = 2, kVarGeneralPC = 3;
+//   kVarGeneralLV, kVarGeneralPCLV  the same with a local-volatility covariance wrapper
+constexpr int kVarFunctional = 0, kVarLogState = 1, kVarGeneral = 2, kVarGeneralPC = 3, kVarGeneralLV = 4,
+              kVarGeneralPCLV = 5;
 
 template <int NCAP, int FCAP, typename real, int VAR = kVarFunctional>
 __global__ void __launch_bounds__(kStepThreads)
@@ -292,7 +323,8 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
 {
     constexpr bool LOGSTATE = VAR == kVarLogState;
     constexpr bool GENERAL = VAR >= kVarGeneral;
-    constexpr bool PC = VAR == kVarGeneralPC;
+    constexpr bool PC = VAR == kVarGeneralPC || VAR == kVarGeneralPCLV;
+    constexpr bool LV = VAR >= kVarGeneralLV;
     const int64_t p = (int64_t)blockIdx.x * kStepThreads + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int64_t wave = p >> 6;
@@ -304,7 +336,7 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
     const bool lognormal = a.lognormal != 0;
     const bool capped = a.capped != 0;
     const real cap = (real)a.cap;
-    const Variant<real> var{lognormal, capped, GENERAL && a.carry != 0, cap};
+    const Variant<real> var{lognormal, capped, GENERAL && a.carry != 0, cap, (real)a.lv_alpha, (const real*)a.lvb};
     const bool terminal = GENERAL && a.terminal != 0;
     // the array holds states (carried schemes) or values; val gives the value
     const real logcap = (real)a.logcap;
@@ -456,10 +488,10 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
         const real dt = (real)(times[ti + 1] - times[ti]);
         const int32_t* row_next = row_of + (size_t)(ti + 1) * n;
         if constexpr (GENERAL) {
-            if (terminal) general_step<NCAP, FCAP, real, true, PC>(L, MU, n, ti, first_alive[ti], dt, dW, rec, var,
-                                                                   paths, row_next, p, P, active);
-            else general_step<NCAP, FCAP, real, false, PC>(L, MU, n, ti, first_alive[ti], dt, dW, rec, var, paths,
-                                                           row_next, p, P, active);
+            if (terminal) general_step<NCAP, FCAP, real, true, PC, LV>(L, MU, n, ti, first_alive[ti], dt, dW, rec,
+                                                                       var, paths, row_next, p, P, active);
+            else general_step<NCAP, FCAP, real, false, PC, LV>(L, MU, n, ti, first_alive[ti], dt, dW, rec, var,
+                                                               paths, row_next, p, P, active);
         } else {
             euler_step<NCAP, FCAP, real, LOGSTATE>(L, n, ti, first_alive[ti], dt, dW, rec, lognormal, capped, cap,
                                                    logcap, paths, row_next, p, P, active);

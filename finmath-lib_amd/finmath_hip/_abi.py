@@ -36,7 +36,7 @@ BASIS_REFERENCE, BASIS_SWAP_RATE = 0, 1
 # Every symbol include/finmath_lmm.h declares.
 EXPORTED_SYMBOLS = (
     "fm_lmm_create", "fm_lmm_destroy", "fm_lmm_set_initial_forwards", "fm_lmm_set_covariance",
-    "fm_lmm_set_discount_curve", "fm_lmm_simulate", "fm_lmm_get_process_value",
+    "fm_lmm_set_discount_curve", "fm_lmm_set_local_volatility", "fm_lmm_simulate", "fm_lmm_get_process_value",
     "fm_lmm_process_value_device_ptr", "fm_lmm_get_numeraire", "fm_lmm_get_brownian_increment",
     "fm_lmm_value_products", "fm_lmm_set_volatility_factorization", "fm_lmm_adjoint",
     "fm_lmm_bermudan_exercise_times", "fm_lmm_serialize", "fm_lmm_deserialize",
@@ -125,6 +125,7 @@ def load() -> ctypes.CDLL:
         "fm_lmm_set_initial_forwards": ([c_void_p, dp], c_int32),
         "fm_lmm_set_covariance": ([c_void_p, dp, dp], c_int32),
         "fm_lmm_set_discount_curve": ([c_void_p, dp], c_int32),
+        "fm_lmm_set_local_volatility": ([c_void_p, c_double, dp], c_int32),
         "fm_lmm_simulate": ([c_void_p], c_int32),
         "fm_lmm_get_process_value": ([c_void_p, c_int32, c_int32, dp], c_int32),
         "fm_lmm_process_value_device_ptr": ([c_void_p, c_int32, c_int32, POINTER(c_void_p)], c_int32),
@@ -305,6 +306,16 @@ class LmmContext:
         if lam.size != self.nT * self.n * self.F or var.size != self.nT * self.n:
             raise IllegalArgumentException("covariance table shapes do not match the configuration")
         check(self.lib.fm_lmm_set_covariance(self.h, _dptr(lam), _dptr(var)))
+
+    def set_local_volatility(self, alpha, offsets):
+        """Local-volatility factor f = (L - L alpha) + b_ij (L + b_ij for alpha = 0) on every
+        loading of component j at time index i; offsets is b as [nT][n] (a scalar is
+        broadcast), None removes the wrapper."""
+        if offsets is None:
+            check(self.lib.fm_lmm_set_local_volatility(self.h, 0.0, POINTER(c_double)()))
+            return
+        b = np.broadcast_to(as_f64(offsets), (self.nT, self.n)).copy()
+        check(self.lib.fm_lmm_set_local_volatility(self.h, float(alpha), _dptr(b)))
 
     def set_discount_curve(self, df):
         if df is None:

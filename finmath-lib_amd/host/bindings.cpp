@@ -150,7 +150,43 @@ PYBIND11_MODULE(_finmath_host, m) {
             std::copy(lam.begin(), lam.end(), L.mutable_data());
             std::copy(var.begin(), var.end(), V.mutable_data());
             return py::make_tuple(L, V);
+        })
+        .def("localVolatility", [](const LIBORCovarianceModelFromVolatilityAndCorrelation& c,
+                                   const TimeDiscretizationFromArray& sim) -> py::object {
+            double alpha = 0.0;
+            std::vector<double> offsets;
+            if (!c.localVolatility(sim, alpha, offsets)) return py::none();
+            const int nT = sim.getNumberOfTimeSteps(), n = c.getLiborPeriodDiscretization()->getNumberOfTimeSteps();
+            py::array_t<double> B({nT, n});
+            std::copy(offsets.begin(), offsets.end(), B.mutable_data());
+            return py::make_tuple(alpha, B);
         });
+
+    py::class_<DisplacedLocalVolatilityModel, LIBORCovarianceModelFromVolatilityAndCorrelation,
+               std::shared_ptr<DisplacedLocalVolatilityModel>>(m, "DisplacedLocalVolatilityModel")
+        .def(py::init([](std::shared_ptr<LIBORCovarianceModelFromVolatilityAndCorrelation> cov, double displacement,
+                         bool isCalibrateable) {
+            return std::make_shared<DisplacedLocalVolatilityModel>(cov, displacement, isCalibrateable);
+        }), py::arg("covarianceModel"), py::arg("displacement"), py::arg("isCalibrateable"))
+        .def("getBaseCovarianceModel", [](const DisplacedLocalVolatilityModel& c) {
+            return std::const_pointer_cast<LIBORCovarianceModelFromVolatilityAndCorrelation>(c.getBaseCovarianceModel());
+        })
+        .def("getDisplacement", &DisplacedLocalVolatilityModel::getDisplacement);
+
+    py::class_<BlendedLocalVolatilityModel, LIBORCovarianceModelFromVolatilityAndCorrelation,
+               std::shared_ptr<BlendedLocalVolatilityModel>>(m, "BlendedLocalVolatilityModel")
+        .def(py::init([](std::shared_ptr<LIBORCovarianceModelFromVolatilityAndCorrelation> cov,
+                         std::shared_ptr<ForwardCurveInterpolation> forwardCurve, double displacement, bool isCalibrateable) {
+            return std::make_shared<BlendedLocalVolatilityModel>(cov, forwardCurve, displacement, isCalibrateable);
+        }), py::arg("covarianceModel"), py::arg("forwardCurve"), py::arg("displacement"), py::arg("isCalibrateable"))
+        .def(py::init([](std::shared_ptr<LIBORCovarianceModelFromVolatilityAndCorrelation> cov, double displacement,
+                         bool isCalibrateable) {
+            return std::make_shared<BlendedLocalVolatilityModel>(cov, nullptr, displacement, isCalibrateable);
+        }), py::arg("covarianceModel"), py::arg("displacement"), py::arg("isCalibrateable"))
+        .def("getBaseCovarianceModel", [](const BlendedLocalVolatilityModel& c) {
+            return std::const_pointer_cast<LIBORCovarianceModelFromVolatilityAndCorrelation>(c.getBaseCovarianceModel());
+        })
+        .def("getDisplacement", &BlendedLocalVolatilityModel::getDisplacement);
 
     py::class_<LevenbergMarquardt>(m, "LevenbergMarquardt")
         .def(py::init<std::vector<double>, std::vector<double>, int>(), py::arg("initialParameters"),

@@ -313,6 +313,18 @@ double LIBORCovarianceModelFromVolatilityAndCorrelation::getCovariance(int timeI
     return (vol_->getVolatility(timeIndex, c1) * vol_->getVolatility(timeIndex, c2)) * corr_->getCorrelation(timeIndex, c1, c2);
 }
 
+int LIBORCovarianceModelFromVolatilityAndCorrelation::covarianceTimeIndex(double time) const {
+    int ci = td_->getTimeIndex(time);
+    return ci < 0 ? -ci - 2 : ci;
+}
+
+bool LIBORCovarianceModelFromVolatilityAndCorrelation::localVolatility(const TimeDiscretizationFromArray&, double& alpha,
+                                                                       std::vector<double>& offsets) const {
+    alpha = 0.0;
+    offsets.clear();
+    return false;
+}
+
 void LIBORCovarianceModelFromVolatilityAndCorrelation::tables(const TimeDiscretizationFromArray& sim,
                                                               std::vector<double>& lam, std::vector<double>& var) const {
     const int nT = sim.getNumberOfTimeSteps(), n = tenor_->getNumberOfTimeSteps(), m = getNumberOfFactors();
@@ -320,14 +332,65 @@ void LIBORCovarianceModelFromVolatilityAndCorrelation::tables(const TimeDiscreti
     var.assign((size_t)nT * n, 0.0);
     for (int i = 0; i < nT; i++) {
         // AbstractLIBORCovarianceModel.getFactorLoading(double time, ...): round-down time index (:54-60,69-76)
-        int ci = td_->getTimeIndex(sim.getTime(i));
-        if (ci < 0) ci = -ci - 2;
+        const int ci = covarianceTimeIndex(sim.getTime(i));
         for (int j = 0; j < n; j++) {
             const std::vector<double> f = getFactorLoading(ci, j);
             for (int k = 0; k < m; k++) lam[((size_t)i * n + j) * m + k] = f[(size_t)k];
             var[(size_t)i * n + j] = getCovariance(ci, j, j);
         }
     }
+}
+
+// ---- local-volatility wrappers -------------------------------------------------------------
+static const LIBORCovarianceModelFromVolatilityAndCorrelation& unwrapped(const CovarianceModelPtr& base) {
+    if (!base) throw IllegalArgumentException("covariance model must be given");
+    double a;
+    std::vector<double> o;
+    if (base->localVolatility(*base->getTimeDiscretization(), a, o))
+        throw UnsupportedOperationException("local-volatility wrappers do not nest");
+    return *base;
+}
+
+DisplacedLocalVolatilityModel::DisplacedLocalVolatilityModel(CovarianceModelPtr covarianceModel, double displacement,
+                                                             bool isCalibrateable)
+    : LIBORCovarianceModelFromVolatilityAndCorrelation(unwrapped(covarianceModel)),
+      base_(std::move(covarianceModel)), displacement_(displacement), calibrateable_(isCalibrateable) {
+    if (!std::isfinite(displacement)) throw IllegalArgumentException("displacement must be finite");
+}
+
+bool DisplacedLocalVolatilityModel::localVolatility(const TimeDiscretizationFromArray& sim, double& alpha,
+                                                    std::vector<double>& offsets) const {
+    // L.add(d) (DisplacedLocalVolatilityModel.java:179)
+    alpha = 0.0;
+    offsets.assign((size_t)sim.getNumberOfTimeSteps() * getLiborPeriodDiscretization()->getNumberOfTimeSteps(),
+                   displacement_);
+    return true;
+}
+
+BlendedLocalVolatilityModel::BlendedLocalVolatilityModel(CovarianceModelPtr covarianceModel, ForwardCurvePtr forwardCurve,
+                                                         double displacement, bool isCalibrateable)
+    : LIBORCovarianceModelFromVolatilityAndCorrelation(unwrapped(covarianceModel)),
+      base_(std::move(covarianceModel)), fc_(std::move(forwardCurve)), displacement_(displacement),
+      calibrateable_(isCalibrateable) {
+    if (!std::isfinite(displacement)) throw IllegalArgumentException("displacement must be finite");
+}
+
+bool BlendedLocalVolatilityModel::localVolatility(const TimeDiscretizationFromArray& sim, double& alpha,
+                                                  std::vector<double>& offsets) const {
+    // L.sub(L.mult(a)).add(a.mult(f0(max(T_j - t, 0)))) at the covariance model's round-down
+    // time index t (BlendedLocalVolatilityModel.java:268-276)
+    const int nT = sim.getNumberOfTimeSteps(), n = getLiborPeriodDiscretization()->getNumberOfTimeSteps();
+    alpha = displacement_;
+    offsets.assign((size_t)nT * n, 0.0);
+    for (int i = 0; i < nT; i++) {
+        const double t = getTimeDiscretization()->getTime(covarianceTimeIndex(sim.getTime(i)));
+        for (int j = 0; j < n; j++) {
+            double forward = 1.0;
+            if (fc_) forward = fc_->getForward(std::max(getLiborPeriodDiscretization()->getTime(j) - t, 0.0));
+            offsets[(size_t)i * n + j] = displacement_ * forward;
+        }
+    }
+    return true;
 }
 
 // ---- LIBOR market model ------------------------------------------------------------------
@@ -439,6 +502,10 @@ void EulerSchemeFromProcessModel::ensureContext(bool storePaths) {
         std::vector<double> lam, var;
         model_->getCovarianceModel()->tables(*driver_->getTimeDiscretization(), lam, var);
         check(fm_lmm_set_covariance(c, lam.data(), var.data()));
+        double lvAlpha = 0.0;
+        std::vector<double> lvOffsets;
+        if (model_->getCovarianceModel()->localVolatility(*driver_->getTimeDiscretization(), lvAlpha, lvOffsets))
+            check(fm_lmm_set_local_volatility(c, lvAlpha, lvOffsets.data()));
         if (model_->getDiscountCurve()) {
             std::vector<double> pd(tenor.size());
             for (size_t k = 0; k < tenor.size(); k++) pd[k] = model_->getDiscountCurve()->getDiscountFactor(tenor[k]);
@@ -790,6 +857,16 @@ void LevenbergMarquardt::run(const ObjectiveFunction& setValues) {              
 
 // ---- calibration of the covariance model -----------------------------------------------------
 std::vector<double> getParameter(const LIBORCovarianceModelFromVolatilityAndCorrelation& cov) {
+    if (auto* w = dynamic_cast<const DisplacedLocalVolatilityModel*>(&cov)) {   // DisplacedLocalVolatilityModel.java:121-137
+        std::vector<double> p = getParameter(*w->getBaseCovarianceModel());
+        if (w->isCalibrateable()) p.push_back(w->getDisplacement());
+        return p;
+    }
+    if (auto* w = dynamic_cast<const BlendedLocalVolatilityModel*>(&cov)) {     // BlendedLocalVolatilityModel.java:211-228
+        std::vector<double> p = getParameter(*w->getBaseCovarianceModel());
+        if (w->isCalibrateable()) p.push_back(w->getDisplacement());
+        return p;
+    }
     std::vector<double> p = cov.getVolatilityModel()->getParameter();
     if (cov.getCorrelationModel()->isCalibrateable()) p.push_back(cov.getCorrelationModel()->getParameter());
     return p;
@@ -797,6 +874,26 @@ std::vector<double> getParameter(const LIBORCovarianceModelFromVolatilityAndCorr
 
 CovarianceModelPtr getCloneWithModifiedParameters(const LIBORCovarianceModelFromVolatilityAndCorrelation& cov,
                                                   const std::vector<double>& parameters) {
+    // wrappers: the base model takes the leading parameters, a calibrateable displacement or
+    // blend the last one (DisplacedLocalVolatilityModel.java:140-166, BlendedLocalVolatilityModel.java:240-257)
+    auto split = [&](bool calibrateable, double current, std::vector<double>& base) {
+        if (!calibrateable) { base = parameters; return current; }
+        if (parameters.empty()) throw IllegalArgumentException("the displacement parameter is missing");
+        base.assign(parameters.begin(), parameters.end() - 1);
+        return parameters.back();
+    };
+    if (auto* w = dynamic_cast<const DisplacedLocalVolatilityModel*>(&cov)) {
+        std::vector<double> base;
+        const double d = split(w->isCalibrateable(), w->getDisplacement(), base);
+        return std::make_shared<DisplacedLocalVolatilityModel>(getCloneWithModifiedParameters(*w->getBaseCovarianceModel(), base),
+                                                               d, w->isCalibrateable());
+    }
+    if (auto* w = dynamic_cast<const BlendedLocalVolatilityModel*>(&cov)) {
+        std::vector<double> base;
+        const double a = split(w->isCalibrateable(), w->getDisplacement(), base);
+        return std::make_shared<BlendedLocalVolatilityModel>(getCloneWithModifiedParameters(*w->getBaseCovarianceModel(), base),
+                                                             w->getForwardCurve(), a, w->isCalibrateable());
+    }
     size_t used = 0;
     VolatilityModelPtr vol = cov.getVolatilityModel();
     auto four = std::dynamic_pointer_cast<const LIBORVolatilityModelFourParameterExponentialForm>(vol);
@@ -872,13 +969,16 @@ CalibrationResult getCloneCalibrated(const LIBORCovarianceModelFromVolatilityAnd
         products[q].product->lower(arena, lowered[q]);
     }
     int valuations = 0;
-    std::vector<double> lam, var, vals(products.size());
+    std::vector<double> lam, var, lvOffsets, vals(products.size());
     auto setValues = [&](const std::vector<double>& p, std::vector<double>& out) {
         out.assign(products.size(), NAN);
         try {
             const CovarianceModelPtr c = getCloneWithModifiedParameters(cov, p);
             c->tables(*td, lam, var);
             check(fm_lmm_set_covariance(ctx, lam.data(), var.data()));
+            double lvAlpha = 0.0;
+            const bool lv = c->localVolatility(*td, lvAlpha, lvOffsets);
+            check(fm_lmm_set_local_volatility(ctx, lvAlpha, lv ? lvOffsets.data() : nullptr));
             check(fm_lmm_value_products(ctx, lowered.data(), (int32_t)lowered.size(), vals.data(), nullptr));
             valuations++;
             for (size_t q = 0; q < products.size(); q++)

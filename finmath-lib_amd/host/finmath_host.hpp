@@ -150,6 +150,7 @@ public:
     LIBORCovarianceModelFromVolatilityAndCorrelation(TimeDiscretizationPtr td, TimeDiscretizationPtr tenor,
                                                      VolatilityModelPtr vol, CorrelationModelPtr corr)
         : td_(std::move(td)), tenor_(std::move(tenor)), vol_(std::move(vol)), corr_(std::move(corr)) {}
+    virtual ~LIBORCovarianceModelFromVolatilityAndCorrelation() = default;
     std::vector<double> getFactorLoading(int timeIndex, int component) const;
     double getCovariance(int timeIndex, int component1, int component2) const;
     int getNumberOfFactors() const { return corr_->getNumberOfFactors(); }
@@ -160,12 +161,60 @@ public:
     // tables for the engine: factor loading [nT][n][F], variance [nT][n] on the simulation grid
     void tables(const TimeDiscretizationFromArray& simulationTimes, std::vector<double>& factorLoading,
                 std::vector<double>& variance) const;
+    // Local-volatility factor of a wrapper for the engine, on the simulation grid: the
+    // loadings of component j at simulation time index i are multiplied by
+    // (L_j - L_j alpha) + offsets[i n + j] (L_j + offsets[i n + j] for alpha = 0). False
+    // for a model without wrapper.
+    virtual bool localVolatility(const TimeDiscretizationFromArray& simulationTimes, double& alpha,
+                                 std::vector<double>& offsets) const;
+protected:
+    // round-down index of a simulation time on the covariance model's grid
+    // (AbstractLIBORCovarianceModel.getFactorLoading(double time, ...), :54-60)
+    int covarianceTimeIndex(double time) const;
 private:
     TimeDiscretizationPtr td_, tenor_;
     VolatilityModelPtr vol_;
     CorrelationModelPtr corr_;
 };
 using CovarianceModelPtr = std::shared_ptr<const LIBORCovarianceModelFromVolatilityAndCorrelation>;
+
+// Factor loadings of the base model times L_j(t) + d
+// (covariance/DisplacedLocalVolatilityModel.java:67-71,175-186); the parameters are the base
+// model's, followed by the displacement when it is calibrateable (:120-166).
+class DisplacedLocalVolatilityModel : public LIBORCovarianceModelFromVolatilityAndCorrelation {
+public:
+    DisplacedLocalVolatilityModel(CovarianceModelPtr covarianceModel, double displacement, bool isCalibrateable);
+    CovarianceModelPtr getBaseCovarianceModel() const { return base_; }
+    double getDisplacement() const { return displacement_; }
+    bool isCalibrateable() const { return calibrateable_; }
+    bool localVolatility(const TimeDiscretizationFromArray& simulationTimes, double& alpha,
+                         std::vector<double>& offsets) const override;
+private:
+    CovarianceModelPtr base_;
+    double displacement_;
+    bool calibrateable_;
+};
+
+// Factor loadings of the base model times (L_j(t) - L_j(t) a) + a f0(max(T_j - t, 0)), with
+// f0 the given forward curve (1 without curve) (covariance/BlendedLocalVolatilityModel.java:
+// 79-86,265-283); the parameters are the base model's, followed by the blend a when it is
+// calibrateable (:211-262; the clone leaves this model unchanged).
+class BlendedLocalVolatilityModel : public LIBORCovarianceModelFromVolatilityAndCorrelation {
+public:
+    BlendedLocalVolatilityModel(CovarianceModelPtr covarianceModel, ForwardCurvePtr forwardCurve, double displacement,
+                                bool isCalibrateable);
+    CovarianceModelPtr getBaseCovarianceModel() const { return base_; }
+    ForwardCurvePtr getForwardCurve() const { return fc_; }
+    double getDisplacement() const { return displacement_; }
+    bool isCalibrateable() const { return calibrateable_; }
+    bool localVolatility(const TimeDiscretizationFromArray& simulationTimes, double& alpha,
+                         std::vector<double>& offsets) const override;
+private:
+    CovarianceModelPtr base_;
+    ForwardCurvePtr fc_;
+    double displacement_;
+    bool calibrateable_;
+};
 
 // ---- montecarlo/interestrate/models/LIBORMarketModelFromCovarianceModel.java -----
 class LIBORMarketModelFromCovarianceModel {
@@ -441,7 +490,8 @@ struct CalibrationResult {
 
 // The parameters of the covariance model, in the reference's order: the four-parameter
 // volatility (a, b, c, d) when it is calibrateable, then the correlation decay when that is
-// calibrateable (LIBORCovarianceModelFromVolatilityAndCorrelation.getParameter).
+// calibrateable (LIBORCovarianceModelFromVolatilityAndCorrelation.getParameter), then a
+// local-volatility wrapper's displacement or blend when that is calibrateable.
 std::vector<double> getParameter(const LIBORCovarianceModelFromVolatilityAndCorrelation& covarianceModel);
 CovarianceModelPtr getCloneWithModifiedParameters(const LIBORCovarianceModelFromVolatilityAndCorrelation& covarianceModel,
                                                   const std::vector<double>& parameters);

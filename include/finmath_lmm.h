@@ -131,6 +131,22 @@ int fm_lmm_set_initial_forwards(fm_lmm* ctx, const double* forwards);
  *  models/covariance/LIBORCovarianceModelFromVolatilityAndCorrelation.java:48-93). */
 int fm_lmm_set_covariance(fm_lmm* ctx, const double* factor_loading, const double* variance);
 
+/* Local-volatility wrapper of the covariance model: every factor loading of component j
+ * at simulation time index i is multiplied by a per-path factor of its value L_j(t_i),
+ *   f = (L_j - L_j * alpha) + offsets[i*n + j]   (alpha != 0)
+ *   f = L_j + offsets[i*n + j]                   (alpha == 0),
+ * in the drift (with the predicted value in the corrector step) and in the diffusion;
+ * in the log-normal state space the drift adjustment then uses the variance
+ * sum_k (lambda_jk f)^2 instead of the covariance table. offsets == NULL removes the
+ * wrapper. The adjoint refuses a context with the wrapper.
+ * Replaces DisplacedLocalVolatilityModel.getFactorLoading (displacement d: alpha = 0,
+ * offsets = d; models/covariance/DisplacedLocalVolatilityModel.java:175-186) and
+ * BlendedLocalVolatilityModel.getFactorLoading (blend a, forward curve f0: alpha = a,
+ * offsets = a * f0(max(T_j - t_i, 0)), or a without curve;
+ * models/covariance/BlendedLocalVolatilityModel.java:265-283), with the variance of
+ * AbstractLIBORCovarianceModel.getCovariance (models/covariance/AbstractLIBORCovarianceModel.java:79-92). */
+int fm_lmm_set_local_volatility(fm_lmm* ctx, double alpha, const double* offsets);
+
 /* Discount factors P^d(T_k), k <= n, of the model's discount curve, or NULL for a
  * model without discount curve (no numeraire adjustment).
  * Replaces the deterministic numeraire adjustment of

@@ -555,6 +555,9 @@ struct Model {
     double capv;
     int scheme, measure, ss;
     bool has_disc;
+    bool has_lv = false;                          // local-volatility wrapper of the covariance model
+    double lv_alpha = 0.0;
+    std::vector<double> lvb;                      // [nT][n] offsets
     std::vector<std::vector<RV>> proc;            // [nT+1][n]
     std::vector<std::vector<RV>> dW;              // [nT][F]
     std::map<int, RV> numeraires;                 // unadjusted at tenor index
@@ -566,6 +569,31 @@ struct Model {
 
     RV lambda(int ti, int j, int k) const { return mult(scalar(vol[ti * n + j]), fac[j * F + k]); }   // FromVolAndCorr:48-57
     RV variance_(int ti, int j) const { return mult(mult(scalar(vol[ti * n + j]), scalar(vol[ti * n + j])), 1.0); } // :83-93
+
+    // Factor loadings of component j at time index ti on the realisation `real`: the
+    // covariance model's, times the local-volatility factor of L_j when a wrapper is set:
+    // DisplacedLocalVolatilityModel L + d (DisplacedLocalVolatilityModel.java:175-186),
+    // BlendedLocalVolatilityModel L - L a + a f0 (BlendedLocalVolatilityModel.java:265-283).
+    std::vector<RV> loadings(int ti, int j, const std::vector<RV>& real) const {
+        std::vector<RV> l(F);
+        for (int k = 0; k < F; k++) l[k] = lambda(ti, j, k);
+        if (has_lv) {
+            RV f = real[j];
+            if (lv_alpha != 0.0) f = sub(f, mult(f, lv_alpha));
+            f = add(f, lvb[(size_t)ti * n + j]);
+            for (int k = 0; k < F; k++) l[k] = mult(l[k], f);
+        }
+        return l;
+    }
+    // Variance of the log-coordinate drift adjustment: sigma_j sigma_j rho_jj of the
+    // covariance model, or, under a local-volatility wrapper, the sum of the squared
+    // loadings (AbstractLIBORCovarianceModel.getCovariance, AbstractLIBORCovarianceModel.java:79-92).
+    RV variance_of(int ti, int j, const std::vector<RV>& l) const {
+        if (!has_lv) return variance_(ti, j);
+        RV c = mult(l[0], l[0]);
+        for (int k = 1; k < F; k++) c = addProduct(c, l[k], l[k]);
+        return c;
+    }
 
     RV transform(const RV& y) const {                                            // LMM:1200-1212
         RV v = y;
@@ -588,14 +616,16 @@ struct Model {
         RV zero = scalar(0.0);
         for (int j = first; j < n; j++) { d[j] = zero; has[j] = true; }
         std::vector<RV> S(F, zero);
+        std::vector<std::vector<RV>> lam(n);
+        for (int j = first; j < n; j++) lam[j] = loadings(ti, j, real);
         if (measure == MEASURE_SPOT) {
             for (int j = first; j < n; j++) {
                 double dl = tenor[j + 1] - tenor[j];
                 RV w = discount(scalar(dl), real[j], dl);
                 if (ss == SS_LOGNORMAL) w = mult(w, real[j]);
-                for (int k = 0; k < F; k++) S[k] = addProduct(S[k], w, lambda(ti, j, k));
+                for (int k = 0; k < F; k++) S[k] = addProduct(S[k], w, lam[j][k]);
                 RV r = d[j];
-                for (int k = 0; k < F; k++) r = addProduct(r, S[k], lambda(ti, j, k));   // RandomVariable.java:659-666
+                for (int k = 0; k < F; k++) r = addProduct(r, S[k], lam[j][k]);   // RandomVariable.java:659-666
                 d[j] = r;
             }
         } else {
@@ -604,13 +634,13 @@ struct Model {
                 RV w = discount(scalar(-dl), real[j], dl);
                 if (ss == SS_LOGNORMAL) w = mult(w, real[j]);
                 RV r = d[j];
-                for (int k = 0; k < F; k++) r = addProduct(r, S[k], lambda(ti, j, k));
+                for (int k = 0; k < F; k++) r = addProduct(r, S[k], lam[j][k]);
                 d[j] = r;
-                for (int k = 0; k < F; k++) S[k] = addProduct(S[k], w, lambda(ti, j, k));
+                for (int k = 0; k < F; k++) S[k] = addProduct(S[k], w, lam[j][k]);
             }
         }
         if (ss == SS_LOGNORMAL)
-            for (int j = first; j < n; j++) d[j] = addProduct(d[j], variance_(ti, j), -0.5);
+            for (int j = first; j < n; j++) d[j] = addProduct(d[j], variance_of(ti, j, lam[j]), -0.5);
         std::vector<RV> out(n);
         for (int j = 0; j < n; j++) if (has[j]) out[j] = d[j];
         (void)has;
@@ -646,7 +676,8 @@ struct Model {
                 if (j < first) continue;
                 RV cur = (scheme == SCHEME_EULER_FUNCTIONAL || scheme == SCHEME_PC_FUNCTIONAL) ? transform_inv(proc[ti - 1][j]) : state[j];
                 cur = addProduct(cur, d[j], dt);                                 // Euler:248
-                for (int k = 0; k < F; k++) cur = addProduct(cur, lambda(ti - 1, j, k), dW[ti - 1][k]);   // Euler:252
+                const std::vector<RV> lam = loadings(ti - 1, j, proc[ti - 1]);       // Euler:240
+                for (int k = 0; k < F; k++) cur = addProduct(cur, lam[k], dW[ti - 1][k]);   // Euler:252
                 state[j] = cur;
                 next[j] = transform(cur);                                        // Euler:255
             }
@@ -751,6 +782,8 @@ typedef struct orc_lmm_spec {
     int scheme, measure, state_space;
     const double* pd;                  // n+1 discount-curve factors at tenor dates, or NULL
     int threads;
+    double lv_alpha;                   // local-volatility factor (L - L alpha) + b, or L + b for alpha = 0
+    const double* lv_offset;           // [nT][n] b, or NULL (no local-volatility wrapper)
 } orc_lmm_spec;
 
 static std::string g_err;
@@ -768,6 +801,11 @@ void* orc_lmm_create(const orc_lmm_spec* s) {
         m->capv = s->cap; m->scheme = s->scheme; m->measure = s->measure; m->ss = s->state_space;
         m->has_disc = s->pd != nullptr;
         if (m->has_disc) m->pd.assign(s->pd, s->pd + s->n + 1);
+        m->has_lv = s->lv_offset != nullptr;
+        if (m->has_lv) {
+            m->lv_alpha = s->lv_alpha;
+            m->lvb.assign(s->lv_offset, s->lv_offset + (size_t)s->nT * s->n);
+        }
         m->simulate(s->seed, s->threads);
         return m;
     } catch (const std::exception& e) { g_err = e.what(); return nullptr; }

@@ -152,6 +152,7 @@ class Spec(ctypes.Structure):
         ("F", c_int), ("P", c_int), ("seed", c_int32), ("L0", POINTER(c_double)), ("vol", POINTER(c_double)),
         ("factors", POINTER(c_double)), ("cap", c_double), ("scheme", c_int), ("measure", c_int),
         ("state_space", c_int), ("pd", POINTER(c_double)), ("threads", c_int),
+        ("lv_alpha", c_double), ("lv_offset", POINTER(c_double)),
     ]
 
 
@@ -180,7 +181,8 @@ class ModelTables:
 
 class OracleModel:
     def __init__(self, tables: ModelTables, num_paths=None, *, cap=1e5, scheme=2, measure=0, state_space=1,
-                 with_discount=True, threads=8, seed=None):
+                 with_discount=True, threads=8, seed=None, local_vol=None):
+        """local_vol: (alpha, offsets) of a local-volatility wrapper, offsets [nT][n] or a scalar."""
         wl = tables.wl
         self.t = tables
         self.P = int(num_paths or wl.num_paths)
@@ -201,6 +203,14 @@ class OracleModel:
         s.scheme, s.measure, s.state_space = scheme, measure, state_space
         s.pd = _dp(self._keep[5]) if with_discount else POINTER(c_double)()
         s.threads = threads
+        if local_vol is not None:
+            nT = tables.times.size - 1
+            self._keep.append(np.ascontiguousarray(np.broadcast_to(f64(local_vol[1]), (nT, self.n))))
+            s.lv_alpha = float(local_vol[0])
+            s.lv_offset = _dp(self._keep[-1])
+        else:
+            s.lv_alpha = 0.0
+            s.lv_offset = POINTER(c_double)()
         self.spec = s
         self.h = lib().orc_lmm_create(ctypes.byref(s))
         if not self.h:

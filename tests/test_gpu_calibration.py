@@ -91,3 +91,29 @@ def test_calibration_without_calibrateable_parameters_is_refused():
         H.LIBORCorrelationModelExponentialDecay(td, tenor, 3, 0.1, False))
     with pytest.raises(ValueError):
         fixed.getCloneCalibrated(model, [H.CalibrationProduct(strip(fc)[0], 0.01, 1.0)], {})
+
+
+def test_calibration_under_a_displaced_wrapper_in_the_normal_state_space():
+    # the reference's LIBORMarketModelCalibrationTest DISPLACED set-up: normal state space,
+    # DisplacedLocalVolatilityModel(base, 1/0.25, not calibrateable) over a calibrateable base
+    say("case calibration displaced wrapper d=4, normal state space, 15 swaptions, 2000 paths")
+    true_vol = [0.004, 0.001, 0.3, 0.002]
+
+    def displaced(vol_params):
+        td, tenor, fc, cov, _ = build(vol_params)
+        wrapped = H.DisplacedLocalVolatilityModel(cov, 1.0 / 0.25, False)
+        model = H.LIBORMarketModelFromCovarianceModel.of(tenor, None, fc, H.DiscountCurveFromForwardCurve(fc), None,
+                                                         wrapped, [], {"measure": "SPOT", "stateSpace": "NORMAL"})
+        return td, tenor, fc, wrapped, model
+
+    td, tenor, fc, _, model_true = displaced(true_vol)
+    products = strip(fc)
+    targets = prices(model_true, td, 3, products)
+    td, tenor, fc, cov0, model0 = displaced([0.003, 0.0, 0.25, 0.003])
+    assert list(cov0.getParameter()) == [0.003, 0.0, 0.25, 0.003]
+    calib = [H.CalibrationProduct(p, v, 1.0) for p, v in zip(products, targets)]
+    cov_cal, info = cov0.getCloneCalibrated(model0, calib, {"accuracy": 1e-12, "maxIterations": 200})
+    say(f"  parameters {info['parameters']} iterations {info['iterations']} rms {info['rootMeanSquaredError']:.3e}")
+    assert isinstance(cov_cal, H.DisplacedLocalVolatilityModel) and cov_cal.getDisplacement() == 4.0
+    assert np.allclose(info["parameters"], true_vol, rtol=0, atol=2e-5)
+    assert info["rootMeanSquaredError"] < 1e-8

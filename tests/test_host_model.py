@@ -96,3 +96,52 @@ def test_model_rejects_calibration_and_unknown_properties():
         H.LIBORMarketModelFromCovarianceModel.of(tenor, None, fc, None, None, cov, [object()], {})
     with pytest.raises(ValueError):
         H.LIBORMarketModelFromCovarianceModel.of(tenor, None, fc, None, None, cov, [], {"measure": "FOO"})
+
+
+def test_local_volatility_wrappers_describe_their_factor():
+    # DisplacedLocalVolatilityModel: L + d everywhere; BlendedLocalVolatilityModel:
+    # (L - L a) + a f0(max(T_j - t_i, 0)), 1 in place of f0 without a curve
+    wl = W.c0()
+    td, tenor, fc, vol, corr, cov, model = build_model(wl)
+    assert cov.localVolatility(td) is None
+    alpha, b = H.DisplacedLocalVolatilityModel(cov, 4.0, False).localVolatility(td)
+    assert alpha == 0.0 and b.shape == (wl.num_time_steps, wl.num_libors) and np.all(b == 4.0)
+    t = O.ModelTables(wl)
+    alpha, b = H.BlendedLocalVolatilityModel(cov, fc, 0.3, False).localVolatility(td)
+    times, T = td.getAsDoubleArray(), tenor.getAsDoubleArray()
+    expect = np.array([[0.3 * O.forward_curve(t.node_t, t.node_f, max(T[j] - times[i], 0.0))
+                        for j in range(wl.num_libors)] for i in range(wl.num_time_steps)])
+    assert alpha == 0.3 and np.array_equal(b, expect)
+    # components past their fixing take f0(0)
+    assert b[-1, 0] == 0.3 * O.forward_curve(t.node_t, t.node_f, 0.0)
+    alpha, b = H.BlendedLocalVolatilityModel(cov, 0.3, False).localVolatility(td)
+    assert np.all(b == 0.3)
+    # the wrapper keeps the base model's tables
+    lam0, var0 = cov.tables(td)
+    lam1, var1 = H.DisplacedLocalVolatilityModel(cov, 4.0, False).tables(td)
+    assert np.array_equal(lam0, lam1) and np.array_equal(var0, var1)
+
+
+def test_local_volatility_wrapper_parameters_and_clones():
+    wl = W.c0()
+    td, tenor, fc, _, corr, _, model = build_model(wl)
+    vol = H.LIBORVolatilityModelFourParameterExponentialForm(td, tenor, 0.2, 0.0, 0.25, 0.1, True)
+    cov = H.LIBORCovarianceModelFromVolatilityAndCorrelation(td, tenor, vol, corr)
+    fixed = H.DisplacedLocalVolatilityModel(cov, 4.0, False)
+    assert list(fixed.getParameter()) == [0.2, 0.0, 0.25, 0.1]
+    free = H.DisplacedLocalVolatilityModel(cov, 4.0, True)
+    assert list(free.getParameter()) == [0.2, 0.0, 0.25, 0.1, 4.0]
+    c = free.getCloneWithModifiedParameters([0.3, 0.01, 0.2, 0.05, 2.5])
+    assert isinstance(c, H.DisplacedLocalVolatilityModel)
+    assert c.getDisplacement() == 2.5 and list(c.getBaseCovarianceModel().getParameter()) == [0.3, 0.01, 0.2, 0.05]
+    assert list(free.getParameter()) == [0.2, 0.0, 0.25, 0.1, 4.0]          # the original is unchanged
+    c = fixed.getCloneWithModifiedParameters([0.3, 0.01, 0.2, 0.05])
+    assert isinstance(c, H.DisplacedLocalVolatilityModel) and c.getDisplacement() == 4.0
+    blended = H.BlendedLocalVolatilityModel(cov, fc, 0.5, True)
+    c = blended.getCloneWithModifiedParameters([0.3, 0.01, 0.2, 0.05, 0.7])
+    assert isinstance(c, H.BlendedLocalVolatilityModel) and c.getDisplacement() == 0.7
+    assert blended.getDisplacement() == 0.5
+    with pytest.raises(ValueError):
+        free.getCloneWithModifiedParameters([0.3, 0.01, 0.2, 0.05])         # base model takes four
+    with pytest.raises(Exception):
+        H.DisplacedLocalVolatilityModel(free, 1.0, False)                  # wrappers do not nest

@@ -21,7 +21,7 @@ def tiny_workload(n=4, F=2, P=6):
     return w
 
 
-def brute_force_paths(tables, P, measure=0, scheme=2, state_space=1, cap=1e5):
+def brute_force_paths(tables, P, measure=0, scheme=2, state_space=1, cap=1e5, local_vol=None):
     """Euler scheme per path, written from the SDE of the forward rates:
       log-normal: d log L_j = (mu_j - var_j / 2) dt + lambda_j . dW
       normal:     d L_j     =  mu_j dt             + lambda_j . dW
@@ -30,7 +30,10 @@ def brute_force_paths(tables, P, measure=0, scheme=2, state_space=1, cap=1e5):
       terminal measure: mu_j = -sum_{l = j+1 .. n-1}   w_l lambda_j . lambda_l.
     The predictor-corrector (scheme 1, 3) steps with the average of the drifts at the
     start values and at the predicted values; schemes 0, 1 carry the state, 2, 3
-    re-derive it from the value (they differ only when the cap binds)."""
+    re-derive it from the value (they differ only when the cap binds).
+    local_vol = (alpha, b[nT][n]): the loadings of L_j are scaled by ((1 - alpha) L_j + b_ij),
+    with L_j the value the drift or the diffusion is evaluated on, and the log-normal
+    variance is the squared norm of the scaled loading."""
     times, tenor = tables.times, tables.tenor
     nT, n = times.size - 1, tenor.size - 1
     F = tables.factors.shape[1]
@@ -42,8 +45,15 @@ def brute_force_paths(tables, P, measure=0, scheme=2, state_space=1, cap=1e5):
     L[0] = np.array([to_value(y) for y in Y0])[:, None]
     state = np.repeat(Y0[:, None], P, axis=1)
 
-    def drift(i, Lv, first):
+    def loading(i, Lv):
         lam = tables.lam[i]                                         # [n][F]
+        if local_vol is None:
+            return lam
+        alpha, b = local_vol[0], np.broadcast_to(local_vol[1], (nT, n))
+        return lam * ((1.0 - alpha) * Lv + b[i])[:, None]
+
+    def drift(i, Lv, first):
+        lam = loading(i, Lv)
         mu = np.zeros(n)
         for j in range(first, n):
             ls = range(first, j + 1) if measure == 0 else range(j + 1, n)
@@ -53,7 +63,7 @@ def brute_force_paths(tables, P, measure=0, scheme=2, state_space=1, cap=1e5):
                 w = d * Lv[l] / (1 + d * Lv[l]) if lognormal else d / (1 + d * Lv[l])
                 mu[j] += sgn * w * float(lam[j] @ lam[l])
             if lognormal:
-                mu[j] -= 0.5 * tables.var[i, j]
+                mu[j] -= 0.5 * (tables.var[i, j] if local_vol is None else float(lam[j] @ lam[j]))
         return mu
 
     for i in range(nT):
@@ -63,10 +73,11 @@ def brute_force_paths(tables, P, measure=0, scheme=2, state_space=1, cap=1e5):
         for p in range(P):
             y0 = state[:, p] if scheme in (0, 1) else (np.log(L[i, :, p]) if lognormal else L[i, :, p].copy())
             mu = drift(i, L[i, :, p], first)
-            y = y0 + mu * dt + tables.lam[i] @ dW[i, :, p]
+            diffusion = loading(i, L[i, :, p]) @ dW[i, :, p]
+            y = y0 + mu * dt + diffusion
             if scheme in (1, 3):
                 mup = drift(i, np.array([to_value(v) for v in y]), first)
-                y = y0 + 0.5 * (mu + mup) * dt + tables.lam[i] @ dW[i, :, p]
+                y = y0 + 0.5 * (mu + mup) * dt + diffusion
             for j in range(first, n):
                 state[j, p] = y[j]
                 L[i + 1, j, p] = to_value(y[j])
@@ -102,6 +113,47 @@ def test_binding_cap_against_brute_force(measure, scheme):
     for i in range(wl.num_time_steps + 1):
         for j in range(wl.num_libors):
             assert np.allclose(om.process_value(i, j), ref[i, j], rtol=1e-12, atol=0), (i, j)
+
+
+@pytest.mark.parametrize("measure", [0, 1])
+@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
+@pytest.mark.parametrize("state_space", [0, 1])
+@pytest.mark.parametrize("kind", ["displaced", "blended"])
+def test_local_volatility_against_brute_force(measure, scheme, state_space, kind):
+    # DisplacedLocalVolatilityModel (L + d) and BlendedLocalVolatilityModel ((L - L a) + a f0)
+    # on normal-state-space volatilities, as the reference's calibration tests set them up;
+    # the blend's offset varies by time index and component
+    wl = tiny_workload()
+    t = O.ModelTables(wl, vol_params=(0.06, 0.0, 0.25, 0.02))
+    nT, n = wl.num_time_steps, wl.num_libors
+    if kind == "displaced":
+        lv = (0.0, np.full((nT, n), 0.25))
+    else:
+        a = 0.4
+        lv = (a, a * (0.03 + 0.002 * np.add.outer(np.arange(nT), np.arange(n))))
+    P = 6
+    om = O.OracleModel(t, P, measure=measure, scheme=scheme, state_space=state_space, local_vol=lv)
+    ref = brute_force_paths(t, P, measure, scheme, state_space, local_vol=lv)
+    plain = brute_force_paths(t, P, measure, scheme, state_space)
+    assert np.max(np.abs(ref - plain)) > 1e-4          # the wrapper changes the paths
+    for i in range(wl.num_time_steps + 1):
+        for j in range(wl.num_libors):
+            assert np.allclose(om.process_value(i, j), ref[i, j], rtol=1e-12, atol=1e-16), (i, j)
+
+
+def test_full_blend_with_unit_offset_is_the_plain_model():
+    # blend a = 1 with offset 1: f = (L - L) + 1 = 1 exactly, so in the normal state space
+    # (no variance term, which the wrapper sums from the loadings) the paths are the plain
+    # model's bit for bit
+    wl = tiny_workload()
+    t = O.ModelTables(wl, vol_params=(0.006, 0.0, 0.25, 0.002))
+    P = 6
+    for scheme in (0, 2, 3):
+        om = O.OracleModel(t, P, scheme=scheme, state_space=0, local_vol=(1.0, 1.0))
+        plain = O.OracleModel(t, P, scheme=scheme, state_space=0)
+        for i in range(wl.num_time_steps + 1):
+            for j in range(wl.num_libors):
+                assert np.array_equal(om.process_value(i, j), plain.process_value(i, j))
 
 
 def test_terminal_numeraire_against_brute_force():
