@@ -274,6 +274,7 @@ class LmmContext:
         h = c_void_p()
         check(self.lib.fm_lmm_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
+        self.discount_factors = None
         self.n = cfg.num_libors
         self.nT = cfg.num_time_steps
         self.F = cfg.num_factors
@@ -320,9 +321,11 @@ class LmmContext:
     def set_discount_curve(self, df):
         if df is None:
             check(self.lib.fm_lmm_set_discount_curve(self.h, POINTER(c_double)()))
+            self.discount_factors = None
         else:
             a = as_f64(df)
             check(self.lib.fm_lmm_set_discount_curve(self.h, _dptr(a)))
+            self.discount_factors = a.copy()
 
     def simulate(self):
         check(self.lib.fm_lmm_simulate(self.h))
@@ -379,6 +382,7 @@ class LmmContext:
         self.nT, self.n, self.F, self.P = nT, n, F, P
         self.times = self.tenor = None
         self.cfg = None
+        self.discount_factors = None
         return self
 
     def bermudan_exercise_times(self, bermudan: int = 0) -> np.ndarray:

@@ -365,3 +365,198 @@ def monte_carlo_weights(ctx) -> RandomVariable:
     out = c_void_p()
     _abi.check(_abi.load().fm_lmm_weights_rv(ctx.h, ctypes.byref(out)))
     return RandomVariable._from_ptr(out.value)
+
+
+# ---- the model's getters on device random variables, with the off-grid rules ------------------
+
+_TICK = 1.0 / (365.0 * 24.0)
+
+
+def _round_tick(t: float) -> float:
+    """TimeDiscretizationFromArray tick rounding (time/TimeDiscretizationFromArray.java:387-389)."""
+    return float(np.rint(t / _TICK) * _TICK)
+
+
+def _time_index(grid: np.ndarray, t: float) -> int:
+    """TimeDiscretization.getTimeIndex: the index of t on the grid, else -(insertion point) - 1."""
+    t = _round_tick(t)
+    k = int(np.searchsorted(grid, t))
+    if k < grid.size and grid[k] == t:
+        return k
+    return -k - 1
+
+
+class LIBORMonteCarloSimulation:
+    """getLIBOR / getForwardRate / getNumeraire / getMonteCarloWeights of a simulated context
+    (store_paths = 1) as device random variables, for any time: the rules of
+    LIBORMarketModelFromCovarianceModel for forwards and numeraires off the tenor grid
+    (LIBORMarketModelFromCovarianceModel.java:883-906,968-1015,1236-1381) composed from the
+    context's process values, with the interpolation LOG_LINEAR_UNCORRECTED (the reference's
+    default) or LINEAR and simulation times rounded ROUND_NEAREST.
+
+    forward_curve: the model's forward curve (getForward(fixingTime, paymentOffset), as the host
+    mirror's ForwardCurveInterpolation), needed for interpolated forwards. On-grid numeraires
+    are the engine's own (fm_lmm_numeraire_rv); off the grid the numeraire is composed here, in
+    RandomVariableFromDoubleArray arithmetic, so it can differ from a Scalar chain in the last
+    place.
+    """
+
+    def __init__(self, ctx, forward_curve=None, interpolation: str = "LOG_LINEAR_UNCORRECTED"):
+        if ctx.times is None or ctx.cfg is None:
+            raise _abi.IllegalArgumentException("the context must be created with its time grids")
+        if interpolation not in ("LOG_LINEAR_UNCORRECTED", "LINEAR"):
+            raise _abi.UnsupportedOperationException(f"interpolation {interpolation} is not supported")
+        self.ctx = ctx
+        self.forward_curve = forward_curve
+        self.interpolation = interpolation
+        self.times = np.array([_round_tick(t) for t in ctx.times])
+        self.tenor = np.array([_round_tick(t) for t in ctx.tenor])
+        self.terminal = ctx.cfg.measure == _abi.MEASURE["TERMINAL"]
+        self._numeraires = {}
+        self._adjustments = None
+
+    # -- model primitives --
+    def getLIBOR(self, time_index: int, libor_index: int) -> RandomVariable:
+        return process_value(self.ctx, time_index, libor_index)
+
+    def getMonteCarloWeights(self, time: float = 0.0) -> RandomVariable:
+        return monte_carlo_weights(self.ctx)
+
+    def _process_time_index(self, time: float) -> int:
+        """Nearest simulation time index (LMM:1244-1252, ROUND_NEAREST)."""
+        ti = _time_index(self.times, time)
+        if ti < 0:
+            ti = -ti - 2
+            if time - self.times[ti] > self.times[ti + 1] - time:
+                ti += 1
+        return ti
+
+    def _curve_forward(self, fixing: float, period: float) -> float:
+        if self.forward_curve is None:
+            raise _abi.IllegalArgumentException("interpolated forwards need the model's forward curve")
+        fc = self.forward_curve
+        return float(fc(fixing, period) if callable(fc) else fc.getForward(fixing, period))
+
+    def _one_plus_interpolated(self, ti: int, start: float, j: int) -> RandomVariable:
+        """getOnePlusInterpolatedLIBORDt (LMM:1324-1381)."""
+        ts, te = self.tenor[j], self.tenor[j + 1]
+        tdt = te - ts
+        if ts < self.times[ti]:
+            fi = _time_index(self.times, ts)
+            if fi < 0:
+                raise _abi.IllegalArgumentException("tenor discretization not part of time discretization")
+            ti = min(ti, fi)
+        long_ = self.getLIBOR(ti, j).mult(tdt).add(1.0)
+        small = te - start
+        alpha = small / tdt
+        if self.interpolation == "LINEAR":
+            interp = long_.mult(alpha).add(1 - alpha)
+        else:
+            interp = long_.log().mult(alpha).exp()
+        a_long = 1 + self._curve_forward(ts, tdt) * tdt
+        a_short = 1 + self._curve_forward(start, small) * small
+        if self.interpolation == "LINEAR":
+            a_interp = a_long * alpha + (1 - alpha)
+        else:
+            a_interp = math.exp(math.log(a_long) * alpha)
+        return interp.mult(a_short / a_interp)
+
+    def getForwardRate(self, time: float, start: float, end: float) -> RandomVariable:
+        """getForwardRate (LMM:1236-1305)."""
+        si, ei = _time_index(self.tenor, start), _time_index(self.tenor, end)
+        time = min(time, start)
+        ti = self._process_time_index(time)
+        if ei < 0:
+            pe = (-ei - 1) - 1
+            next_end = float(self.tenor[pe + 1])
+            long_ = self.getForwardRate(time, start, next_end).mult(next_end - start).add(1.0)
+            return long_.div(self._one_plus_interpolated(ti, end, pe)).sub(1.0).div(end - start)
+        if si < 0:
+            ps = (-si - 1) - 1
+            next_start = float(self.tenor[ps + 1])
+            if next_start > end:
+                raise _abi.IllegalArgumentException("interpolation not possible")
+            if next_start == end:
+                return self._one_plus_interpolated(ti, start, ps).sub(1.0).div(end - start)
+            long_ = self.getForwardRate(time, next_start, end).mult(end - next_start).add(1.0)
+            return long_.mult(self._one_plus_interpolated(ti, start, ps)).sub(1.0).div(end - start)
+        if si + 1 == ei:
+            return self.getLIBOR(ti, si)
+        acc = None
+        for j in range(si, ei):
+            d = float(self.tenor[j + 1] - self.tenor[j])
+            L = self.getLIBOR(ti, j)
+            acc = L.mult(d).add(1.0) if acc is None else acc.accrue(L, d)
+        return acc.sub(1.0).div(end - start)
+
+    # -- numeraire --
+    def _unadjusted_at(self, k: int) -> RandomVariable:
+        """getNumerairetUnAdjustedAtLIBORIndex (LMM:1017-1074) from the process values."""
+        if k in self._numeraires:
+            return self._numeraires[k]
+        n = self.tenor.size - 1
+        if self.terminal:
+            ti = _time_index(self.times, self.tenor[k])
+            ti = -ti - 1 if ti < 0 else ti
+            N = RandomVariable.constant(1.0)
+            for l in range(k, n):
+                N = N.discount(self.getLIBOR(ti, l), float(self.tenor[l + 1] - self.tenor[l]))
+        elif k == 0:
+            N = RandomVariable.constant(1.0)
+        else:
+            ti = _time_index(self.times, self.tenor[k - 1])
+            ti = -ti - 1 if ti < 0 else ti
+            N = self._unadjusted_at(k - 1).accrue(self.getLIBOR(ti, k - 1), float(self.tenor[k] - self.tenor[k - 1]))
+        N = N.cache()
+        self._numeraires[k] = N
+        return N
+
+    def _unadjusted(self, time: float) -> RandomVariable:
+        """getNumerairetUnAdjusted (LMM:968-1015)."""
+        k = _time_index(self.tenor, time)
+        if k >= 0:
+            return self._unadjusted_at(k)
+        upper = -k - 1
+        if upper - 1 < 0:
+            raise _abi.IllegalArgumentException(f"numeraire requested for time {time}: unsupported")
+        if self.terminal:
+            N = RandomVariable.constant(1.0)
+            for l in range(upper, self.tenor.size - 1):
+                ti = _time_index(self.times, min(time, self.tenor[l]))
+                if ti < 0:
+                    raise _abi.IllegalArgumentException("terminal numeraire off the simulation grid")
+                N = N.discount(self.getLIBOR(ti, l), float(self.tenor[l + 1] - self.tenor[l]))
+        else:
+            N = self._unadjusted_at(upper)
+        T_up = float(self.tenor[upper])
+        return N.discount(self.getForwardRate(time, time, T_up), T_up - time)
+
+    def _adjustment(self, time: float) -> float:
+        """getNumeraireDefaultableZeroBondAsOfTimeZero (LMM:883-944): the discount curve's chain
+        on the tenor grid (Scalar.discount), log-linear in between."""
+        pd = self.ctx.discount_factors
+        if self._adjustments is None:
+            a = [float(pd[0])]
+            for i in range(self.tenor.size - 1):
+                ts = float(self.tenor[i + 1] - self.tenor[i])
+                fr = (pd[i] / pd[i + 1] - 1.0) / ts
+                a.append(1.0 / (fr * (ts / a[-1]) + 1.0 / a[-1]))
+            self._adjustments = a
+        k = _time_index(self.tenor, time)
+        if k >= 0:
+            return self._adjustments[k]
+        prev = min(-k - 2, self.tenor.size - 2)
+        tp, tn = float(self.tenor[prev]), float(self.tenor[prev + 1])
+        ap, an = self._adjustments[prev], self._adjustments[prev + 1]
+        return ap * math.pow(an / ap, (time - tp) / (tn - tp))
+
+    def getNumeraire(self, time: float) -> RandomVariable:
+        """getNumeraire (LMM:860-876): the engine's numeraire on the tenor grid, the composed
+        one (adjusted by the discount curve, when the context has one) off it."""
+        if _time_index(self.tenor, time) >= 0:
+            return numeraire(self.ctx, time)
+        N = self._unadjusted(time)
+        if self.ctx.discount_factors is not None:
+            zb = N.invert().mult(self._unadjusted(0.0)).getAverage()
+            N = N.mult(zb).div(self._adjustment(time))
+        return N.cache()

@@ -555,6 +555,8 @@ struct Model {
     double capv;
     int scheme, measure, ss;
     bool has_disc;
+    std::vector<double> fc_t, fc_f;               // forward curve nodes (analytic interpolation adjustment)
+    double fc_offset = 0.0;                       // the curve's payment offset
     bool has_lv = false;                          // local-volatility wrapper of the covariance model
     double lv_alpha = 0.0;
     std::vector<double> lvb;                      // [nT][n] offsets
@@ -707,7 +709,36 @@ struct Model {
 
     RV libor_at(int ti, int j) const { return proc[ti][j]; }                   // LMM:1307-1312
 
-    // LMM.getForwardRate (LMM:1237-1305), on-grid and multi-period cases.
+    // ForwardCurveInterpolation.getForward(model, fixingTime, paymentOffset)
+    // (marketdata/model/curves/ForwardCurveInterpolation.java:337-347)
+    double curve_forward(double fixing, double payment_offset) const {
+        if (fc_t.empty()) throw std::runtime_error("interpolated forwards need the model's forward curve");
+        double f = orc_forward_curve(fc_t.data(), fc_f.data(), (int)fc_t.size(), fixing);
+        if (payment_offset != fc_offset)
+            f = (std::exp(std::log(1 + f * fc_offset) * payment_offset / fc_offset) - 1.0) / payment_offset;
+        return f;
+    }
+
+    // getOnePlusInterpolatedLIBORDt (LMM:1324-1381), LOG_LINEAR_UNCORRECTED: the long forward
+    // of tenor period j at time index ti (its fixing time index once fixed), raised to the
+    // power alpha = (T_{j+1} - S) / (T_{j+1} - T_j), times the curve's short/interpolated ratio.
+    RV one_plus_interpolated(int ti, double S, int j) const {
+        const double ts = tenor[j], te = tenor[j + 1], tdt = te - ts;
+        if (ts < times[ti]) {
+            const int fi = process_time_index(ts);
+            if (fi < 0) throw std::runtime_error("tenor discretization not part of time discretization");
+            ti = std::min(ti, fi);
+        }
+        RV long_ = add(mult(libor_at(ti, j), tdt), 1.0);
+        const double small = te - S, alpha = small / tdt;
+        RV r = expv(mult(logv(long_), alpha));
+        const double a_long = 1 + curve_forward(ts, tdt) * tdt;
+        const double a_short = 1 + curve_forward(S, small) * small;
+        const double a_interp = std::exp(std::log(a_long) * alpha);
+        return mult(r, a_short / a_interp);
+    }
+
+    // LMM.getForwardRate (LMM:1237-1305): on the grid, several periods, off-grid ends.
     RV forward_rate(double time, double start, double end) const {
         int si = libor_period_index(start), ei = libor_period_index(end);
         time = std::min(time, start);
@@ -716,7 +747,22 @@ struct Model {
             ti = -ti - 2;
             if (time - times[ti] > times[ti + 1] - time) ti++;                   // ROUND_NEAREST
         }
-        if (si < 0 || ei < 0) throw std::runtime_error("off-tenor forward rates are not restated");
+        if (ei < 0) {                                                            // LMM:1258-1265
+            const int pe = (-ei - 1) - 1;
+            const double next_end = tenor[pe + 1];
+            RV long_ = add(mult(forward_rate(time, start, next_end), next_end - start), 1.0);
+            RV interp = one_plus_interpolated(ti, end, pe);
+            return divd(sub(div(long_, interp), 1.0), end - start);
+        }
+        if (si < 0) {                                                            // LMM:1268-1280
+            const int ps = (-si - 1) - 1;
+            const double next_start = tenor[ps + 1];
+            if (next_start > end) throw std::runtime_error("interpolation not possible");
+            if (next_start == end) return divd(sub(one_plus_interpolated(ti, start, ps), 1.0), end - start);
+            RV long_ = add(mult(forward_rate(time, next_start, end), end - next_start), 1.0);
+            RV interp = one_plus_interpolated(ti, start, ps);
+            return divd(sub(mult(long_, interp), 1.0), end - start);
+        }
         if (si + 1 == ei) return libor_at(ti, si);
         RV acc; bool have = false;
         for (int pi = si; pi < ei; pi++) {
@@ -752,13 +798,43 @@ struct Model {
         return N;
     }
 
-    // getNumeraire (LMM:860-876), tenor-grid times.
+    // getNumerairetUnAdjusted (LMM:968-1015): off the tenor grid, the numeraire of the next
+    // tenor date (spot) or the bond chain from the next tenor date (terminal, which needs the
+    // time on the simulation grid), discounted by the interpolated short forward.
+    RV numeraire_unadjusted(double T) {
+        const int k = libor_period_index(T);
+        if (k >= 0) return numeraire_unadjusted_at(k);
+        const int upper = -k - 1, lower = upper - 1;
+        if (lower < 0) throw std::runtime_error("numeraire requested before the first tenor date");
+        RV N;
+        if (measure == MEASURE_TERMINAL) {
+            N = scalar(1.0);
+            for (int l = upper; l <= n - 1; l++) {
+                const int ti = process_time_index(std::min(T, tenor[l]));
+                if (ti < 0) throw std::runtime_error("time not on the simulation grid");
+                N = discount(N, libor_at(ti, l), tenor[l + 1] - tenor[l]);
+            }
+        } else {
+            N = numeraire_unadjusted_at(upper);
+        }
+        return discount(N, forward_rate(T, T, tenor[upper]), tenor[upper] - T);
+    }
+
+    // getNumeraireDefaultableZeroBondAsOfTimeZero (LMM:883-906): the adjustment chain on the
+    // tenor grid, log-linear in between.
+    double adjustment_at(double T) const {
+        const int k = libor_period_index(T);
+        if (k >= 0) return adjchain[k];
+        const int prev = std::min(-k - 2, n + 1 - 2), next = prev + 1;
+        const double tp = tenor[prev], tn = tenor[next];
+        return adjchain[prev] * std::pow(adjchain[next] / adjchain[prev], (T - tp) / (tn - tp));
+    }
+
+    // getNumeraire (LMM:860-876).
     RV numeraire(double T) {
-        int k = libor_period_index(T);
-        if (k < 0) throw std::runtime_error("numeraire off the tenor grid is not restated");
-        RV N = numeraire_unadjusted_at(k);
+        RV N = numeraire_unadjusted(T);
         if (has_disc) {
-            RV adj = scalar(adjchain[k]);
+            RV adj = scalar(adjustment_at(T));
             double zb = average(mult(invert(N), numeraire_unadjusted_at(0)));
             N = div(mult(N, zb), adj);
         }
@@ -784,6 +860,10 @@ typedef struct orc_lmm_spec {
     int threads;
     double lv_alpha;                   // local-volatility factor (L - L alpha) + b, or L + b for alpha = 0
     const double* lv_offset;           // [nT][n] b, or NULL (no local-volatility wrapper)
+    int fc_n;                          // forward curve nodes (interpolated forwards), 0 = none
+    const double* fc_t;
+    const double* fc_f;
+    double fc_offset;                  // the curve's payment offset
 } orc_lmm_spec;
 
 static std::string g_err;
@@ -801,6 +881,11 @@ void* orc_lmm_create(const orc_lmm_spec* s) {
         m->capv = s->cap; m->scheme = s->scheme; m->measure = s->measure; m->ss = s->state_space;
         m->has_disc = s->pd != nullptr;
         if (m->has_disc) m->pd.assign(s->pd, s->pd + s->n + 1);
+        if (s->fc_n > 0) {
+            m->fc_t.assign(s->fc_t, s->fc_t + s->fc_n);
+            m->fc_f.assign(s->fc_f, s->fc_f + s->fc_n);
+            m->fc_offset = s->fc_offset;
+        }
         m->has_lv = s->lv_offset != nullptr;
         if (m->has_lv) {
             m->lv_alpha = s->lv_alpha;
@@ -825,6 +910,10 @@ int orc_lmm_brownian_increment(void* h, int t, int k, double* out) {
 }
 int orc_lmm_numeraire(void* h, double T, double* out) {
     try { auto* m = (orc::Model*)h; fill(m->numeraire(T), m->P, out); return 0; }
+    catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+int orc_lmm_forward_rate(void* h, double time, double start, double end, double* out) {
+    try { auto* m = (orc::Model*)h; fill(m->forward_rate(time, start, end), m->P, out); return 0; }
     catch (const std::exception& e) { g_err = e.what(); return -1; }
 }
 int orc_lmm_numeraire_unadjusted(void* h, int k, double* out) {

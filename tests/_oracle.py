@@ -54,6 +54,7 @@ def lib():
         L.orc_lmm_brownian_increment.argtypes = [c_void_p, c_int, c_int, dp]
         L.orc_lmm_numeraire.argtypes = [c_void_p, c_double, dp]
         L.orc_lmm_numeraire_unadjusted.argtypes = [c_void_p, c_int, dp]
+        L.orc_lmm_forward_rate.argtypes = [c_void_p, c_double, c_double, c_double, dp]
         L.orc_lmm_numeraire_adjustment.argtypes = [c_void_p, c_int]
         L.orc_lmm_numeraire_adjustment.restype = c_double
         L.orc_lmm_swaption.argtypes = [c_void_p, c_double, c_int, dp, dp, dp, dp, c_double, dp, dp, dp, dp]
@@ -153,6 +154,7 @@ class Spec(ctypes.Structure):
         ("factors", POINTER(c_double)), ("cap", c_double), ("scheme", c_int), ("measure", c_int),
         ("state_space", c_int), ("pd", POINTER(c_double)), ("threads", c_int),
         ("lv_alpha", c_double), ("lv_offset", POINTER(c_double)),
+        ("fc_n", c_int), ("fc_t", POINTER(c_double)), ("fc_f", POINTER(c_double)), ("fc_offset", c_double),
     ]
 
 
@@ -211,6 +213,11 @@ class OracleModel:
         else:
             s.lv_alpha = 0.0
             s.lv_offset = POINTER(c_double)()
+        # the model's forward curve (analytic adjustment of interpolated forwards)
+        self._keep += [f64(tables.node_t), f64(tables.node_f)]
+        s.fc_n = self._keep[-1].size
+        s.fc_t, s.fc_f = _dp(self._keep[-2]), _dp(self._keep[-1])
+        s.fc_offset = float(wl.period)
         self.spec = s
         self.h = lib().orc_lmm_create(ctypes.byref(s))
         if not self.h:
@@ -237,6 +244,13 @@ class OracleModel:
     def numeraire(self, T):
         out = np.empty(self.P)
         if lib().orc_lmm_numeraire(self.h, T, _dp(out)) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return out
+
+    def forward_rate(self, time, start, end):
+        """getForwardRate(time, start, end), off-grid ends interpolated (LOG_LINEAR_UNCORRECTED)."""
+        out = np.empty(self.P)
+        if lib().orc_lmm_forward_rate(self.h, time, start, end, _dp(out)) != 0:
             raise RuntimeError(lib().orc_last_error().decode())
         return out
 

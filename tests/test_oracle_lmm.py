@@ -284,3 +284,115 @@ def test_payer_swaption_dominates_intrinsic_value():
     s = wl.products[0]
     v, _, pp = om.swaption(s, per_path=True)
     assert v >= 0 and np.all(pp >= 0)
+
+
+# ---- off-grid forwards and numeraires (LOG_LINEAR_UNCORRECTED) --------------------------
+
+class FineGrid(W.Workload):
+    """Tenor every `period`, simulation every period / 4 (the terminal numeraire off the tenor
+    grid needs the time on the simulation grid)."""
+
+    @property
+    def num_time_steps(self) -> int:
+        return self.num_libors * 4
+
+    def times(self):
+        return W.round_ticks(np.arange(self.num_time_steps + 1) * self.period / 4)
+
+
+def short_forward(t, T, period):
+    """ForwardCurveInterpolation.getForward(model, T, dt) of the tables' curve: the node forward
+    interpolated at T, converted from the curve's payment offset to dt."""
+    f = O.forward_curve(t.node_t, t.node_f, T)
+    return f if period == t.wl.period else (math.exp(math.log(1 + f * t.wl.period) * period / t.wl.period) - 1.0) / period
+
+
+def zero_vol_bond(t, T):
+    """P(0; T) at zero volatility without discount curve: the tenor bonds up to the next tenor
+    date T_u, times the curve's forward bond from T to T_u (the interpolated long forward and
+    the analytic interpolation cancel when the forwards are the curve's)."""
+    tenor = t.tenor
+    u = int(np.searchsorted(tenor, T - 1e-12))
+    if abs(tenor[u] - T) < 1e-9:
+        return float(np.prod(1.0 / (1.0 + 0.5 * t.L0[:u])))
+    short = tenor[u] - T
+    return float(np.prod(1.0 / (1.0 + 0.5 * t.L0[:u]))) * (1 + short_forward(t, T, short) * short)
+
+
+@pytest.mark.parametrize("measure", [0, 1])
+def test_off_grid_bonds_zero_volatility_closed_form(measure):
+    wl = FineGrid("fine", 8, 0.5, 2, 5, 0.2, np.array([0.02, 0.025, 0.03, 0.034, 0.037, 0.039, 0.04, 0.041]))
+    t = O.ModelTables(wl, vol_params=(0.0, 0.0, 0.0, 0.0))
+    om = O.OracleModel(t, 5, measure=measure, with_discount=False)
+    N0 = om.numeraire(0.0)
+    for T in np.arange(1, 31) * 0.125:
+        bond = N0 / om.numeraire(T)
+        assert np.allclose(bond, zero_vol_bond(t, T), rtol=2e-14, atol=0), T
+    # with the discount curve the bond is its log-linear interpolation between tenor dates
+    om = O.OracleModel(t, 5, measure=measure)
+    N0 = om.numeraire(0.0)
+    for T in np.arange(1, 31) * 0.125:
+        k = int(np.floor(T / 0.5 + 1e-9))
+        k = min(k, wl.num_libors - 1)
+        frac = (T - t.tenor[k]) / 0.5
+        expect = t.pd[k] * (t.pd[k + 1] / t.pd[k]) ** frac
+        assert np.allclose(N0 / om.numeraire(T), expect, rtol=2e-14, atol=0), T
+
+
+def test_off_grid_forwards_zero_volatility_are_bond_ratios():
+    # at zero volatility every (interpolated) forward is the forward of the zero-vol bonds
+    wl = W.Workload("fwd", 8, 0.5, 2, 5, 0.2, np.array([0.02, 0.025, 0.03, 0.034, 0.037, 0.039, 0.04, 0.041]))
+    t = O.ModelTables(wl, vol_params=(0.0, 0.0, 0.0, 0.0))
+    om = O.OracleModel(t, 5, with_discount=False)
+    cases = [(S, S + 0.5) for S in np.arange(1, 25) * 0.125] + [(0.75, 2.0), (1.0, 2.625), (0.375, 1.0), (1.125, 1.375)]
+    for S, E in cases:
+        for time in (0.0, S, S + 0.3):
+            F = om.forward_rate(time, S, E)
+            expect = (zero_vol_bond(t, S) / zero_vol_bond(t, E) - 1.0) / (E - S)
+            assert np.allclose(F, expect, rtol=1e-12, atol=0), (time, S, E)
+
+
+def test_interpolated_forward_reads_the_long_forward_at_the_right_time_index():
+    # S = 0.875 inside the tenor period [0.5, 1.0): the evaluation time is rounded to the
+    # nearest simulation time (LMM:1244-1252) and the long forward L_1 read there, at most at its
+    # fixing T_1 (LMM:1328-1334; L_1 is frozen afterwards); its one-plus is raised to
+    # alpha = (1.0 - S) / 0.5 and scaled by the curve's short / interpolated one-plus (LMM:1341-1380)
+    wl = tiny_workload(n=4, P=64)
+    t = O.ModelTables(wl)
+    om = O.OracleModel(t, 64, with_discount=False)
+    ratio = (1 + short_forward(t, 0.875, 0.125) * 0.125) / math.exp(math.log(1 + short_forward(t, 0.5, 0.5) * 0.5) * 0.25)
+
+    def expect(L):
+        return (np.exp(np.log(L * 0.5 + 1.0) * 0.25) * ratio - 1.0) / 0.125
+
+    F = om.forward_rate(0.875, 0.875, 1.0)          # nearest time 1.0, after T_1: read at index 1
+    assert np.allclose(F, expect(om.process_value(1, 1)), rtol=1e-13, atol=0)
+    assert np.ptp(F) > 0
+    assert np.array_equal(om.forward_rate(0.375, 0.875, 1.0), F)   # nearest time 0.5 = T_1 (not 0.0)
+    F0 = om.forward_rate(0.125, 0.875, 1.0)         # nearest time 0.0: the time-0 forward
+    assert np.ptp(F0) == 0 and np.allclose(F0, expect(t.L0[1]), rtol=1e-13)
+
+
+def test_reference_bond_and_fra_assertions_on_the_oracle():
+    # LIBORMarketModelValuationTest.testBond (|bond - P^d(T)| < 8e-3 at 0.125 steps) and
+    # testFRA (|par FRA| < 2e-3), on a smaller model: 16 forwards, 3 factors, 5,000 paths
+    from finmath_hip import _finmath_host as H
+    wl = W.Workload("val", 16, 0.5, 3, 5000, 0.1, np.full(16, 0.05), vol_params=(0.2, 0.0, 0.25, 0.3))
+    t = O.ModelTables(wl)
+    om = O.OracleModel(t, wl.num_paths, scheme=1)
+    N0 = om.numeraire(0.0)
+    worst = 0.0
+    for T in np.arange(0, 8 * 8 + 1) * 0.125:
+        bond = float(np.mean(N0 / om.numeraire(T)))
+        k = min(int(np.floor(T / 0.5 + 1e-9)), wl.num_libors - 1)
+        pd = t.pd[k] * (t.pd[k + 1] / t.pd[k]) ** ((T - t.tenor[k]) / 0.5)
+        worst = max(worst, abs(bond - pd))
+    assert worst < 8e-3
+    fc = H.ForwardCurveInterpolation.createForwardCurveFromForwards("fc", list(t.node_t), list(t.node_f), 0.5)
+    worst = 0.0
+    for S in np.arange(0, 8 * 8 - 3) * 0.125:
+        E = S + 0.5
+        par = H.getForwardSwapRate(H.TimeDiscretizationFromArray([S, E]), H.TimeDiscretizationFromArray([S, E]), fc)
+        v = (om.forward_rate(S, S, E) - par) * 0.5 / om.numeraire(E) * N0
+        worst = max(worst, abs(float(np.mean(v))))
+    assert worst < 2e-3
