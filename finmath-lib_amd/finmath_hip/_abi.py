@@ -275,6 +275,8 @@ class LmmContext:
         check(self.lib.fm_lmm_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
         self.discount_factors = None
+        self.factor_loading = None       # [nT][n][F], as set
+        self.local_volatility = None     # (alpha, offsets[nT][n]), as set
         self.n = cfg.num_libors
         self.nT = cfg.num_time_steps
         self.F = cfg.num_factors
@@ -307,6 +309,7 @@ class LmmContext:
         if lam.size != self.nT * self.n * self.F or var.size != self.nT * self.n:
             raise IllegalArgumentException("covariance table shapes do not match the configuration")
         check(self.lib.fm_lmm_set_covariance(self.h, _dptr(lam), _dptr(var)))
+        self.factor_loading = lam.reshape(self.nT, self.n, self.F).copy()
 
     def set_local_volatility(self, alpha, offsets):
         """Local-volatility factor f = (L - L alpha) + b_ij (L + b_ij for alpha = 0) on every
@@ -314,9 +317,11 @@ class LmmContext:
         broadcast), None removes the wrapper."""
         if offsets is None:
             check(self.lib.fm_lmm_set_local_volatility(self.h, 0.0, POINTER(c_double)()))
+            self.local_volatility = None
             return
         b = np.broadcast_to(as_f64(offsets), (self.nT, self.n)).copy()
         check(self.lib.fm_lmm_set_local_volatility(self.h, float(alpha), _dptr(b)))
+        self.local_volatility = (float(alpha), b)
 
     def set_discount_curve(self, df):
         if df is None:
@@ -383,6 +388,8 @@ class LmmContext:
         self.times = self.tenor = None
         self.cfg = None
         self.discount_factors = None
+        self.factor_loading = None
+        self.local_volatility = None
         return self
 
     def bermudan_exercise_times(self, bermudan: int = 0) -> np.ndarray:

@@ -392,7 +392,7 @@ class LIBORMonteCarloSimulation:
     LIBORMarketModelFromCovarianceModel for forwards and numeraires off the tenor grid
     (LIBORMarketModelFromCovarianceModel.java:883-906,968-1015,1236-1381) composed from the
     context's process values, with the interpolation LOG_LINEAR_UNCORRECTED (the reference's
-    default) or LINEAR and simulation times rounded ROUND_NEAREST.
+    default), LOG_LINEAR_CORRECTED or LINEAR and simulation times rounded ROUND_NEAREST.
 
     forward_curve: the model's forward curve (getForward(fixingTime, paymentOffset), as the host
     mirror's ForwardCurveInterpolation), needed for interpolated forwards. On-grid numeraires
@@ -404,7 +404,7 @@ class LIBORMonteCarloSimulation:
     def __init__(self, ctx, forward_curve=None, interpolation: str = "LOG_LINEAR_UNCORRECTED"):
         if ctx.times is None or ctx.cfg is None:
             raise _abi.IllegalArgumentException("the context must be created with its time grids")
-        if interpolation not in ("LOG_LINEAR_UNCORRECTED", "LINEAR"):
+        if interpolation not in ("LOG_LINEAR_UNCORRECTED", "LOG_LINEAR_CORRECTED", "LINEAR"):
             raise _abi.UnsupportedOperationException(f"interpolation {interpolation} is not supported")
         self.ctx = ctx
         self.forward_curve = forward_curve
@@ -412,6 +412,9 @@ class LIBORMonteCarloSimulation:
         self.times = np.array([_round_tick(t) for t in ctx.times])
         self.tenor = np.array([_round_tick(t) for t in ctx.tenor])
         self.terminal = ctx.cfg.measure == _abi.MEASURE["TERMINAL"]
+        self.lognormal = ctx.cfg.state_space == _abi.STATE_SPACE["LOGNORMAL"]
+        if interpolation == "LOG_LINEAR_CORRECTED" and ctx.factor_loading is None:
+            raise _abi.IllegalArgumentException("LOG_LINEAR_CORRECTED needs the context's factor loadings")
         self._numeraires = {}
         self._adjustments = None
 
@@ -437,6 +440,44 @@ class LIBORMonteCarloSimulation:
         fc = self.forward_curve
         return float(fc(fixing, period) if callable(fc) else fc.getForward(fixing, period))
 
+    def _interpolation_drift_adjustment(self, e: int, j: int) -> RandomVariable:
+        """getInterpolationDriftAdjustmentEvaluated (LMM:1432-1499): the trapezoid integral over
+        the simulation times up to index e of |lambda_j(t_s)|^2 / (1 + L_j(t_s) dT)^2 (times
+        L_j(t_s)^2 in the log-normal state space), L_j read at most at its fixing index."""
+        lam = self.ctx.factor_loading
+        lv = self.ctx.local_volatility
+        dT = float(self.tenor[j + 1] - self.tenor[j])
+        fix = _time_index(self.times, self.tenor[j])
+        fix = -fix - 2 if fix < 0 else fix
+        adj, prev = None, None
+        for s in range(e + 1):
+            if s >= lam.shape[0]:
+                raise _abi.IllegalArgumentException("factor loading beyond the simulation grid")
+            L = self.getLIBOR(min(s, fix), j)
+            if lv is None:
+                g = 0.0
+                for x in lam[s, j]:
+                    g = g + float(x) * float(x)
+                g = RandomVariable.constant(g)
+            else:
+                alpha, b = lv
+                f = L if alpha == 0.0 else L.sub(L.mult(alpha))
+                f = f.add(float(b[s, j]))
+                g = RandomVariable.constant(0.0)
+                for x in lam[s, j]:
+                    g = g.add(f.mult(float(x)).squared())
+            g = g.div(L.mult(dT).add(1.0).squared())
+            if self.lognormal:
+                g = g.mult(L.squared())
+            g = g.cache()
+            if s > 0:
+                step = 0.5 * float(self.times[s] - self.times[s - 1])
+                term = g.add(prev).mult(step)
+                adj = term if adj is None else adj.add(term)
+                adj = adj.cache()
+            prev = g
+        return RandomVariable.constant(0.0) if adj is None else adj
+
     def _one_plus_interpolated(self, ti: int, start: float, j: int) -> RandomVariable:
         """getOnePlusInterpolatedLIBORDt (LMM:1324-1381)."""
         ts, te = self.tenor[j], self.tenor[j + 1]
@@ -451,8 +492,12 @@ class LIBORMonteCarloSimulation:
         alpha = small / tdt
         if self.interpolation == "LINEAR":
             interp = long_.mult(alpha).add(1 - alpha)
-        else:
+        elif self.interpolation == "LOG_LINEAR_UNCORRECTED":
             interp = long_.log().mult(alpha).exp()
+        else:
+            coef = 0.5 * small * (ts - start)
+            a = self._interpolation_drift_adjustment(ti, j).mult(coef)
+            interp = long_.log().mult(alpha).sub(a).exp()
         a_long = 1 + self._curve_forward(ts, tdt) * tdt
         a_short = 1 + self._curve_forward(start, small) * small
         if self.interpolation == "LINEAR":

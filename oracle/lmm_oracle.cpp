@@ -557,6 +557,7 @@ struct Model {
     bool has_disc;
     std::vector<double> fc_t, fc_f;               // forward curve nodes (analytic interpolation adjustment)
     double fc_offset = 0.0;                       // the curve's payment offset
+    int interp = 1;                               // InterpolationMethod ordinal: LINEAR, LOG_LINEAR_UNCORRECTED, LOG_LINEAR_CORRECTED
     bool has_lv = false;                          // local-volatility wrapper of the covariance model
     double lv_alpha = 0.0;
     std::vector<double> lvb;                      // [nT][n] offsets
@@ -719,9 +720,36 @@ struct Model {
         return f;
     }
 
-    // getOnePlusInterpolatedLIBORDt (LMM:1324-1381), LOG_LINEAR_UNCORRECTED: the long forward
-    // of tenor period j at time index ti (its fixing time index once fixed), raised to the
-    // power alpha = (T_{j+1} - S) / (T_{j+1} - T_j), times the curve's short/interpolated ratio.
+    // getInterpolationDriftAdjustmentEvaluated (LMM:1432-1499): trapezoid integral over the
+    // simulation times up to index e of |lambda_j(t_s)|^2 / (1 + L_j(t_s) dT)^2 (times L_j^2
+    // log-normal), with L_j read at most at its fixing index.
+    RV interpolation_drift_adjustment(int e, int j) const {
+        const double tdt = tenor[j + 1] - tenor[j];
+        int fix = process_time_index(tenor[j]);
+        if (fix < 0) fix = -fix - 2;
+        RV adj = scalar(0.0);
+        RV prev;
+        std::vector<RV> real(n);
+        for (int s_ = 0; s_ <= e; s_++) {
+            if (s_ >= nT) throw std::runtime_error("loading beyond the simulation grid");
+            real[j] = libor_at(s_ == 0 ? 0 : std::min(s_, fix), j);
+            const std::vector<RV> l = loadings(s_, j, real);
+            RV g = scalar(0.0);
+            for (int k = 0; k < F; k++) g = add(g, mult(l[k], l[k]));
+            RV den = add(mult(real[j], tdt), 1.0);
+            g = div(g, mult(den, den));
+            if (ss == SS_LOGNORMAL) g = mult(g, mult(real[j], real[j]));
+            if (s_ > 0) adj = add(adj, mult(add(g, prev), 0.5 * (times[s_] - times[s_ - 1])));
+            prev = g;
+        }
+        return adj;
+    }
+
+    // getOnePlusInterpolatedLIBORDt (LMM:1324-1381): the long forward of tenor period j at time
+    // index ti (its fixing time index once fixed); LOG_LINEAR_*: its one-plus raised to the
+    // power alpha = (T_{j+1} - S) / (T_{j+1} - T_j) (CORRECTED: minus the drift adjustment times
+    // S dT (T_j - S) / 2 inside the exponential), LINEAR: the linear blend; times the curve's
+    // short/interpolated ratio.
     RV one_plus_interpolated(int ti, double S, int j) const {
         const double ts = tenor[j], te = tenor[j + 1], tdt = te - ts;
         if (ts < times[ti]) {
@@ -731,10 +759,17 @@ struct Model {
         }
         RV long_ = add(mult(libor_at(ti, j), tdt), 1.0);
         const double small = te - S, alpha = small / tdt;
-        RV r = expv(mult(logv(long_), alpha));
+        RV r;
+        if (interp == 0) r = add(mult(long_, alpha), 1 - alpha);
+        else if (interp == 1) r = expv(mult(logv(long_), alpha));
+        else {
+            const double coef = 0.5 * small * (ts - S);
+            RV a = mult(interpolation_drift_adjustment(ti, j), coef);
+            r = expv(sub(mult(logv(long_), alpha), a));
+        }
         const double a_long = 1 + curve_forward(ts, tdt) * tdt;
         const double a_short = 1 + curve_forward(S, small) * small;
-        const double a_interp = std::exp(std::log(a_long) * alpha);
+        const double a_interp = interp == 0 ? a_long * alpha + (1 - alpha) : std::exp(std::log(a_long) * alpha);
         return mult(r, a_short / a_interp);
     }
 
@@ -864,6 +899,7 @@ typedef struct orc_lmm_spec {
     const double* fc_t;
     const double* fc_f;
     double fc_offset;                  // the curve's payment offset
+    int interpolation;                 // InterpolationMethod ordinal (LMM:168)
 } orc_lmm_spec;
 
 static std::string g_err;
@@ -886,6 +922,7 @@ void* orc_lmm_create(const orc_lmm_spec* s) {
             m->fc_f.assign(s->fc_f, s->fc_f + s->fc_n);
             m->fc_offset = s->fc_offset;
         }
+        m->interp = s->interpolation;
         m->has_lv = s->lv_offset != nullptr;
         if (m->has_lv) {
             m->lv_alpha = s->lv_alpha;

@@ -155,6 +155,7 @@ class Spec(ctypes.Structure):
         ("state_space", c_int), ("pd", POINTER(c_double)), ("threads", c_int),
         ("lv_alpha", c_double), ("lv_offset", POINTER(c_double)),
         ("fc_n", c_int), ("fc_t", POINTER(c_double)), ("fc_f", POINTER(c_double)), ("fc_offset", c_double),
+        ("interpolation", c_int),
     ]
 
 
@@ -183,7 +184,7 @@ class ModelTables:
 
 class OracleModel:
     def __init__(self, tables: ModelTables, num_paths=None, *, cap=1e5, scheme=2, measure=0, state_space=1,
-                 with_discount=True, threads=8, seed=None, local_vol=None):
+                 with_discount=True, threads=8, seed=None, local_vol=None, interpolation=1):
         """local_vol: (alpha, offsets) of a local-volatility wrapper, offsets [nT][n] or a scalar."""
         wl = tables.wl
         self.t = tables
@@ -218,6 +219,7 @@ class OracleModel:
         s.fc_n = self._keep[-1].size
         s.fc_t, s.fc_f = _dp(self._keep[-2]), _dp(self._keep[-1])
         s.fc_offset = float(wl.period)
+        s.interpolation = interpolation          # LINEAR 0, LOG_LINEAR_UNCORRECTED 1, LOG_LINEAR_CORRECTED 2
         self.spec = s
         self.h = lib().orc_lmm_create(ctypes.byref(s))
         if not self.h:

@@ -4,7 +4,8 @@
 (:968-1015) and the log-linear discount-curve adjustment (:883-906).
 
 Bars: the oracle's off-grid forwards and numeraires within 1e-12 relative (both measures,
-with and without discount curve); the reference's LIBORMarketModelValuationTest.testBond
+with and without discount curve, LINEAR, LOG_LINEAR_UNCORRECTED and LOG_LINEAR_CORRECTED, the
+last also under a displaced local-volatility wrapper); the reference's LIBORMarketModelValuationTest.testBond
 (|bond - P^d(T)| < 8e-3 for maturities every 0.125 to 20y) and testFRA (|par FRA| < 2e-3 for
 starts every 0.125) on the reference's model, 20,000 paths, PREDICTOR_CORRECTOR.
 """
@@ -22,12 +23,17 @@ from test_oracle_lmm import FineGrid
 pytestmark = pytest.mark.gpu
 
 
-def context(wl, t, P, measure, scheme="EULER_FUNCTIONAL", with_discount=True):
+INTERPOLATION = ["LINEAR", "LOG_LINEAR_UNCORRECTED", "LOG_LINEAR_CORRECTED"]
+
+
+def context(wl, t, P, measure, scheme="EULER_FUNCTIONAL", with_discount=True, local_vol=None, state_space="LOGNORMAL"):
     ctx = _abi.LmmContext(t.times, t.tenor, wl.num_factors, P, wl.seed, measure=measure, scheme=scheme,
-                          store_paths=True)
+                          store_paths=True, state_space=state_space)
     ctx.set_initial_forwards(t.L0)
     ctx.set_covariance(t.lam, t.var)
     ctx.set_discount_curve(t.pd if with_discount else None)
+    if local_vol is not None:
+        ctx.set_local_volatility(*local_vol)
     ctx.simulate()
     return ctx
 
@@ -41,9 +47,10 @@ def rel(g, r):
     return float(np.max(np.abs(g - r) / np.maximum(np.abs(r), 1e-3)))
 
 
+@pytest.mark.parametrize("interpolation", INTERPOLATION)
 @pytest.mark.parametrize("with_discount", [True, False])
 @pytest.mark.parametrize("measure", ["SPOT", "TERMINAL"])
-def test_off_grid_numeraire_and_forwards_match_oracle(measure, with_discount):
+def test_off_grid_numeraire_and_forwards_match_oracle(measure, with_discount, interpolation):
     # terminal: the simulation grid every 0.125 holds the off-grid times; spot: the 0.5 grid
     forwards = np.array([0.02 + 0.0015 * j for j in range(12)])
     if measure == "TERMINAL":
@@ -52,10 +59,11 @@ def test_off_grid_numeraire_and_forwards_match_oracle(measure, with_discount):
         wl = W.Workload("coarse", 12, 0.5, 3, 3001, 0.1, forwards)
     t = O.ModelTables(wl)
     P = wl.num_paths
-    say(f"case off-grid numeraire/forwards {measure} discount={with_discount} P={P}")
+    say(f"case off-grid numeraire/forwards {measure} discount={with_discount} {interpolation} P={P}")
     ctx = context(wl, t, P, measure, with_discount=with_discount)
-    sim = RV.LIBORMonteCarloSimulation(ctx, curve(t))
-    om = O.OracleModel(t, P, measure=1 if measure == "TERMINAL" else 0, scheme=2, with_discount=with_discount)
+    sim = RV.LIBORMonteCarloSimulation(ctx, curve(t), interpolation)
+    om = O.OracleModel(t, P, measure=1 if measure == "TERMINAL" else 0, scheme=2, with_discount=with_discount,
+                       interpolation=INTERPOLATION.index(interpolation))
     worst = 0.0
     for T in np.arange(0, 12 * 4 + 1) * 0.125:
         worst = max(worst, rel(sim.getNumeraire(T).getRealizations(), om.numeraire(T)))
@@ -67,6 +75,27 @@ def test_off_grid_numeraire_and_forwards_match_oracle(measure, with_discount):
     for time, S, E in cases:
         worst = max(worst, rel(sim.getForwardRate(time, S, E).getRealizations(), om.forward_rate(time, S, E)))
     say(f"  worst relative forward difference {worst:.3e}")
+    assert worst < 1e-12
+
+
+@pytest.mark.parametrize("state_space", ["NORMAL", "LOGNORMAL"])
+def test_corrected_interpolation_under_a_displaced_wrapper(state_space):
+    # the drift adjustment integrates the wrapped loadings lambda (L + d)
+    wl = FineGrid("fine", 10, 0.5, 3, 2001, 0.1, np.array([0.02 + 0.002 * j for j in range(10)]))
+    t = O.ModelTables(wl, vol_params=(0.005, 0.0, 0.25, 0.002) if state_space == "NORMAL" else (0.05, 0.0, 0.25, 0.02))
+    lv = (0.0, np.full((wl.num_time_steps, wl.num_libors), 4.0))
+    P = wl.num_paths
+    say(f"case off-grid corrected forwards, displaced wrapper, {state_space} P={P}")
+    ctx = context(wl, t, P, "SPOT", local_vol=lv, state_space=state_space)
+    sim = RV.LIBORMonteCarloSimulation(ctx, curve(t), "LOG_LINEAR_CORRECTED")
+    om = O.OracleModel(t, P, scheme=2, state_space=1 if state_space == "LOGNORMAL" else 0, local_vol=lv,
+                       interpolation=2)
+    worst = 0.0
+    for time, S, E in [(S, S, S + 0.5) for S in np.arange(1, 30) * 0.125] + [(0.5, 1.875, 3.125)]:
+        worst = max(worst, rel(sim.getForwardRate(time, S, E).getRealizations(), om.forward_rate(time, S, E)))
+    for T in np.arange(1, 40) * 0.125:
+        worst = max(worst, rel(sim.getNumeraire(T).getRealizations(), om.numeraire(T)))
+    say(f"  worst relative difference {worst:.3e}")
     assert worst < 1e-12
 
 

@@ -319,11 +319,14 @@ def zero_vol_bond(t, T):
     return float(np.prod(1.0 / (1.0 + 0.5 * t.L0[:u]))) * (1 + short_forward(t, T, short) * short)
 
 
+@pytest.mark.parametrize("interpolation", [0, 1, 2])
 @pytest.mark.parametrize("measure", [0, 1])
-def test_off_grid_bonds_zero_volatility_closed_form(measure):
+def test_off_grid_bonds_zero_volatility_closed_form(measure, interpolation):
+    # every interpolation method (LINEAR, LOG_LINEAR_UNCORRECTED, LOG_LINEAR_CORRECTED, whose
+    # drift adjustment vanishes without volatility) gives the curve's short forward bond
     wl = FineGrid("fine", 8, 0.5, 2, 5, 0.2, np.array([0.02, 0.025, 0.03, 0.034, 0.037, 0.039, 0.04, 0.041]))
     t = O.ModelTables(wl, vol_params=(0.0, 0.0, 0.0, 0.0))
-    om = O.OracleModel(t, 5, measure=measure, with_discount=False)
+    om = O.OracleModel(t, 5, measure=measure, with_discount=False, interpolation=interpolation)
     N0 = om.numeraire(0.0)
     for T in np.arange(1, 31) * 0.125:
         bond = N0 / om.numeraire(T)
@@ -371,6 +374,37 @@ def test_interpolated_forward_reads_the_long_forward_at_the_right_time_index():
     assert np.array_equal(om.forward_rate(0.375, 0.875, 1.0), F)   # nearest time 0.5 = T_1 (not 0.0)
     F0 = om.forward_rate(0.125, 0.875, 1.0)         # nearest time 0.0: the time-0 forward
     assert np.ptp(F0) == 0 and np.allclose(F0, expect(t.L0[1]), rtol=1e-13)
+
+
+@pytest.mark.parametrize("state_space", [1, 0])
+def test_log_linear_corrected_forward_against_the_integral(state_space):
+    # LOG_LINEAR_CORRECTED (LMM:1355-1360,1432-1499): the log-linear one-plus of the long forward
+    # L_j is corrected by exp(-A dt_s (T_j - S) / 2), A the trapezoid integral over the simulation
+    # times up to the fixing of sigma_j(t)^2 / (1 + L_j(t) dT)^2, times L_j(t)^2 in the log-normal
+    # state space; written here from that formula on the process values
+    wl = FineGrid("fine", 6, 0.5, 2, 16, 0.2, np.array([0.02, 0.025, 0.03, 0.034, 0.037, 0.039]))
+    t = O.ModelTables(wl, vol_params=(0.3, 0.0, 0.25, 0.1) if state_space == 1 else (0.008, 0.0, 0.25, 0.003))
+    P = 16
+    corr = O.OracleModel(t, P, with_discount=False, state_space=state_space, interpolation=2)
+    unc = O.OracleModel(t, P, with_discount=False, state_space=state_space, interpolation=1)
+    times = t.times
+    for j, S in ((2, 1.125), (3, 1.875), (4, 2.25)):
+        Tj, Te = t.tenor[j], t.tenor[j + 1]
+        fix = int(np.searchsorted(times, Tj - 1e-12))
+        L = [corr.process_value(min(s, fix), j) for s in range(fix + 1)]
+        g = []
+        for s in range(fix + 1):
+            x = t.var[s, j] / (1 + L[s] * 0.5) ** 2
+            g.append(x * L[s] ** 2 if state_space == 1 else x)
+        A = sum(0.5 * (times[s] - times[s - 1]) * (g[s] + g[s - 1]) for s in range(1, fix + 1))
+        small = Te - S
+        ratio = (1 + short_forward(t, S, small) * small) / math.exp(math.log(1 + short_forward(t, Tj, 0.5) * 0.5) * small / 0.5)
+        one_plus = np.exp(np.log(L[fix] * 0.5 + 1) * (small / 0.5) - A * (0.5 * small * (Tj - S))) * ratio
+        expect = (one_plus - 1) / small
+        got = corr.forward_rate(S, S, Te)
+        assert np.allclose(got, expect, rtol=1e-12, atol=0), (j, S)
+        # the correction is small but present
+        assert 0 < np.max(np.abs(got - unc.forward_rate(S, S, Te))) < 1e-3
 
 
 def test_reference_bond_and_fra_assertions_on_the_oracle():
