@@ -44,6 +44,7 @@ SOURCES = {
     "lmm_step_pc_fp32.hip": [],
     "lmm_step_lv_fp64.hip": NO_CONTRACT,
     "lmm_step_lv_fp32.hip": [],
+    "lmm_stochvol.hip": NO_CONTRACT,
     "lmm_reduce.hip": NO_CONTRACT,
     "lmm_bermudan.hip": NO_CONTRACT,
     "rv.hip": NO_CONTRACT,

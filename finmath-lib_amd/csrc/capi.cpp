@@ -84,6 +84,8 @@ struct fm_lmm {
     std::vector<double> lvb;                        // local-volatility offsets b_ij [nT][n]
     double lv_alpha = 0.0;
     bool has_lv = false;
+    bool has_sv = false;                            // stochastic volatility scaling in d_svs [nT][P]
+    bool steps_ready = false;                       // d_times, d_sqrt_dt uploaded
     bool tables_dirty = true;
 
     // derived host tables
@@ -101,7 +103,7 @@ struct fm_lmm {
     bool uniform_dt = false;            // all simulation steps have the same length
     std::vector<std::vector<double>> berm_fixing;   // fixing dates of the Bermudans of the last pass
 
-    DevBuf d_times, d_first, d_tenor_ti, d_rec, d_delta, d_L0, d_Y0, d_Y0cap, d_sqrt_dt, d_lvb;
+    DevBuf d_times, d_first, d_tenor_ti, d_rec, d_delta, d_L0, d_Y0, d_Y0cap, d_sqrt_dt, d_lvb, d_svs;
     DevBuf d_states, d_poly, d_dw, d_row_of, d_paths, d_num;
     DevBuf d_products, d_periods, d_wave_sum, d_wave_m2, d_contrib, d_out, d_swapmeta, d_scale;
     DevBuf d_ckpt, d_adj_num, d_kcur, d_fsig, d_sig, d_cdiag, d_wave_grad, d_grad_out;
@@ -124,7 +126,7 @@ struct fm_lmm {
     }
     // spot measure, functional (or equivalent) Euler, loadings without local volatility:
     // the kernel the adjoint differentiates
-    bool plain_euler() const { return !terminal() && !pc() && !carry_matters() && !has_lv; }
+    bool plain_euler() const { return !terminal() && !pc() && !carry_matters() && !has_lv && !has_sv; }
     size_t real_size() const { return fp32() ? sizeof(float) : sizeof(double); }
     int process_time_index(double t) const { return jsearch(times, round_tick(t)); }
     int libor_index(double t) const { return jsearch(tenor, round_tick(t)); }
@@ -297,7 +299,21 @@ void ensure_states(fm_lmm* c) {
     c->states_ready = true;
 }
 
+// simulation times and sqrt(dt) on the device (also for a context used only as a Brownian driver)
+void ensure_time_steps(fm_lmm* c) {
+    if (c->steps_ready) return;
+    const int nT = c->nT();
+    std::vector<double> sq((size_t)nT);
+    for (int i = 0; i < nT; i++) sq[(size_t)i] = std::sqrt(c->times[(size_t)i + 1] - c->times[(size_t)i]);
+    c->uniform_dt = std::all_of(sq.begin(), sq.end(), [&](double v) { return v == sq[0]; });
+    upload(c->d_times, c->times, c->stream);
+    upload(c->d_sqrt_dt, sq, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->steps_ready = true;
+}
+
 void ensure_brownian(fm_lmm* c) {
+    ensure_time_steps(c);
     ensure_states(c);
     if (c->dw_ready) return;
     const size_t count = (size_t)c->P() * c->nT() * c->F();
@@ -734,13 +750,14 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     // EULER with a finite cap caps only the value and runs on the general kernel
     const bool logstate_kernel = c->cfg.precision == FM_PRECISION_FP64_LOGSTATE &&
         c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL && !c->terminal() && !c->pc() &&
-        (!c->carry() || std::isinf(c->cfg.libor_cap)) && !c->has_lv;
+        (!c->carry() || std::isinf(c->cfg.libor_cap)) && !c->has_lv && !c->has_sv;
     if (logstate_kernel) {
         a.Y0 = c->d_Y0cap.as<double>();
         launch_lmm_step_fp64_logstate(a, s);
-    } else if (c->has_lv) {
+    } else if (c->has_lv || c->has_sv) {
         a.lv_alpha = c->lv_alpha;
-        a.lvb = c->d_lvb.p;
+        a.lvb = c->has_lv ? c->d_lvb.p : nullptr;
+        a.svs = c->has_sv ? c->d_svs.p : nullptr;
         if (c->fp32()) launch_lmm_step_lv_fp32(a, s);
         else launch_lmm_step_lv_fp64(a, s);
     } else if (c->plain_euler()) {
@@ -893,6 +910,46 @@ int fm_lmm_set_local_volatility(fm_lmm* c, double alpha, const double* offsets) 
         c->lv_alpha = 0.0;
         c->has_lv = false;
     }
+    c->tables_dirty = true; c->simulated = false;
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_set_stochastic_volatility(fm_lmm* c, double nu, double rho, fm_lmm* driver, int32_t factor_a,
+                                     int32_t factor_b) {
+    FM_TRY
+    require(c != nullptr, FM_ERR_INVALID_ARGUMENT, "NULL context");
+    if (!driver) {
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->has_sv = false;
+        c->tables_dirty = true; c->simulated = false;
+        return FM_OK;
+    }
+    std::unique_lock<std::mutex> lk_c(c->mu, std::defer_lock), lk_d(driver->mu, std::defer_lock);
+    if (driver == c) lk_c.lock(); else std::lock(lk_c, lk_d);
+    require(std::isfinite(nu) && std::isfinite(rho) && rho >= -1.0 && rho <= 1.0, FM_ERR_INVALID_ARGUMENT,
+            "nu must be finite and rho in [-1, 1]");
+    require(driver->device == c->device, FM_ERR_INVALID_ARGUMENT, "the driver lives on another device");
+    require(driver->P() == c->P() && driver->cfg.path_offset == c->cfg.path_offset, FM_ERR_INVALID_ARGUMENT,
+            "the driver must simulate the same paths");
+    require(driver->times == c->times, FM_ERR_INVALID_ARGUMENT, "the driver must have the same time discretization");
+    require(!driver->fp32(), FM_ERR_UNSUPPORTED, "the driver's increments must be FP64");
+    require(factor_a >= 0 && factor_a < driver->F() && factor_b >= 0 && factor_b < driver->F(),
+            FM_ERR_INVALID_ARGUMENT, "driver factor out of range");
+    DeviceGuard g(c->device);
+    ensure_brownian(driver);
+    HIP_CHECK(hipStreamSynchronize(driver->stream));
+    ensure_time_steps(c);
+    // drift nu.squared().mult(-0.5), loadings rho.mult(nu) and rho.squared().sub(1).mult(-1).sqrt().mult(nu)
+    // (LIBORCovarianceModelStochasticVolatility.java:206-213)
+    const double drift = (nu * nu) * -0.5;
+    const double l0 = rho * nu;
+    const double l1 = std::sqrt((rho * rho - 1.0) * -1.0) * nu;
+    void* out = c->d_svs.get((size_t)c->nT() * c->P() * c->real_size());
+    launch_stochastic_volatility(driver->d_dw.as<double>(), c->nT(), driver->F(), factor_a, factor_b, c->P(),
+                                 c->d_times.as<double>(), drift, l0, l1, out, c->fp32() ? 1 : 0, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->has_sv = true;
     c->tables_dirty = true; c->simulated = false;
     return FM_OK;
     FM_CATCH
@@ -1298,6 +1355,8 @@ int fm_lmm_serialize(fm_lmm* c, void* buffer, int64_t* size) {
     FM_TRY
     require(c && size, FM_ERR_INVALID_ARGUMENT, "NULL argument");
     std::lock_guard<std::mutex> lk(c->mu);
+    require(!c->has_sv, FM_ERR_UNSUPPORTED,
+            "a stochastic volatility scaling depends on another context's Brownian motion and is not serialised");
     Writer w;
     w.b.insert(w.b.end(), kSerMagic, kSerMagic + 8);
     w.put<int32_t>(kSerVersion);

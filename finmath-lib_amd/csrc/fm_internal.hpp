@@ -103,7 +103,8 @@ struct StepArgs {
     int carry;                   // EULER / PREDICTOR_CORRECTOR: the state is carried (general kernel)
     int pc;                      // predictor-corrector (general kernel)
     double lv_alpha;             // local-volatility factor (L - L alpha) + b_ij, or L + b_ij for alpha = 0
-    const void* lvb;             // device [nT][n] b_ij (double or float), local-volatility kernels only
+    const void* lvb;             // device [nT][n] b_ij (double or float), or NULL; local-volatility kernels only
+    const void* svs;             // device [nT][P] stochastic volatility scaling (double or float), or NULL
     const void* dw;              // device dW [P][nT][F]
     double cap;
     int32_t scheme;
@@ -134,6 +135,11 @@ void launch_lmm_step_pc_fp32(const StepArgs& a, void* stream);
 // General variant with a local-volatility covariance wrapper (a.pc selects the corrector).
 void launch_lmm_step_lv_fp64(const StepArgs& a, void* stream);
 void launch_lmm_step_lv_fp32(const StepArgs& a, void* stream);
+// Stochastic volatility scaling S(t_i), i < nT, of every path: S(0) = 1 and
+// log S(t_{i+1}) = log S(t_i) + drift dt_i + l0 dW_a + l1 dW_b from the driver's FP64 increments
+// [P][nT][Fd] (factors a, b), written as [nT][P] (double, or float when out_fp32).
+void launch_stochastic_volatility(const double* dw, int nT, int Fd, int fa, int fb, int P, const double* times,
+                                  double drift, double l0, double l1, void* out, int out_fp32, void* stream);
 
 // Reverse sweep (lmm_adjoint.hpp): adjoint arguments and launchers.
 struct AdjArgs {

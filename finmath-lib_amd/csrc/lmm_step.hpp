@@ -187,7 +187,8 @@ struct Variant {
                        // (else the *_FUNCTIONAL schemes re-derive it from the value, Euler:234-237)
     real cap;
     real lv_alpha;     // local-volatility factor f = (L - L alpha) + b_ij, or L + b_ij for alpha = 0
-    const real* lvb;   // b_ij, [nT][n] (local-volatility kernels only)
+    const real* lvb;   // b_ij, [nT][n], or NULL: no local-volatility factor (LV kernels only)
+    const real* svs;   // stochastic volatility scaling S(t_i) [nT][P], or NULL (LV kernels only)
 };
 
 // Local-volatility factor of component j at time index i from its value L at that time
@@ -196,6 +197,7 @@ struct Variant {
 // every factor loading of the component is multiplied by it.
 template <typename real>
 __device__ __forceinline__ real local_vol_factor(real L, const Variant<real>& v, int i, int j, int n) {
+    if (!v.lvb) return (real)1;
     real x = L;
     if (v.lv_alpha != (real)0) x = x - x * v.lv_alpha;
     return x + v.lvb[(size_t)i * n + j];
@@ -206,13 +208,14 @@ __device__ __forceinline__ real local_vol_factor(real L, const Variant<real>& v,
 // TERMINAL takes mu_j = S . lambda_j of the later components first and then adds
 // w lambda_j. The record's d/d and 1/d hold -1 and -1/d under TERMINAL
 // (Scalar.of(-d).discount(L, d), stochastic/Scalar.java:320-328), so w carries the sign.
-// LV: the loadings are lambda_jk * f; the log-normal variance is then the sum of their
-// squares (AbstractLIBORCovarianceModel.getCovariance, covariance/AbstractLIBORCovarianceModel.java:79-92)
-// instead of the record's sigma_j sigma_j rho_jj.
+// LV: the loadings are (lambda_jk * f) * s, f the local-volatility factor and s the stochastic
+// volatility scaling (each 1 when absent, which multiplies exactly); the log-normal variance is
+// then the sum of their squares (AbstractLIBORCovarianceModel.getCovariance,
+// covariance/AbstractLIBORCovarianceModel.java:79-92) instead of the record's sigma_j sigma_j rho_jj.
 template <int FCAP, typename real, bool TERMINAL, bool LV = false>
 __device__ __forceinline__ real drift_component(real (&S)[FCAP], const Rec<FCAP, real>& R, real Lj, bool lognormal,
-                                                real f = (real)1) {
-    auto lam = [&](int k) -> real { return LV ? R.lam(k) * f : R.lam(k); };
+                                                real f = (real)1, real sv = (real)1) {
+    auto lam = [&](int k) -> real { return LV ? (R.lam(k) * f) * sv : R.lam(k); };
     real w = dev_div<real>((real)1.0, Lj * R.c1() + R.c2());
     if (lognormal) w = w * Lj;
     real mu;
@@ -243,7 +246,8 @@ __device__ __forceinline__ real drift_component(real (&S)[FCAP], const Rec<FCAP,
 // the corrector re-evaluates the drift on the predicted values and adds
 // ((mu_pred - mu) / 2) dt to the state (Euler:292-314). MU keeps the predictor's drift.
 // LV: the loadings of the drift and the diffusion carry the local-volatility factor of the
-// value the drift is evaluated on (the start value; the predicted value in the corrector).
+// value the drift is evaluated on (the start value; the predicted value in the corrector) and
+// the path's stochastic volatility scaling at t_i.
 template <int NCAP, int FCAP, typename real, bool TERMINAL, bool PC, bool LV>
 __device__ __forceinline__ void general_step(real (&X)[NCAP], real (&MU)[NCAP], int n, int i, int first, real dt,
                                              const real (&dW)[FCAP], const real* __restrict__ rec,
@@ -252,6 +256,8 @@ __device__ __forceinline__ void general_step(real (&X)[NCAP], real (&MU)[NCAP], 
     real S[FCAP];
 #pragma unroll
     for (int k = 0; k < FCAP; k++) S[k] = (real)0;
+    // stochastic volatility scaling of this path at time index i (1 without)
+    const real sv = (LV && v.svs) ? v.svs[(size_t)i * P + (active ? p : (int64_t)P - 1)] : (real)1;
 #pragma unroll
     for (int jj = 0; jj < NCAP; jj++) {
         const int j = TERMINAL ? NCAP - 1 - jj : jj;
@@ -259,11 +265,11 @@ __device__ __forceinline__ void general_step(real (&X)[NCAP], real (&MU)[NCAP], 
         const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, i, j);
         const real Lj = v.carry ? to_value<real>(X[j], v.lognormal, v.capped, v.cap) : X[j];
         const real f = LV ? local_vol_factor<real>(Lj, v, i, j, n) : (real)1;
-        const real mu = drift_component<FCAP, real, TERMINAL, LV>(S, R, Lj, v.lognormal, f);
+        const real mu = drift_component<FCAP, real, TERMINAL, LV>(S, R, Lj, v.lognormal, f, sv);
         real Y = v.carry ? X[j] : (v.lognormal ? dev_log<real>(Lj) : Lj);
         Y = Y + mu * dt;
 #pragma unroll
-        for (int k = 0; k < FCAP; k++) Y = Y + dW[k] * (LV ? R.lam(k) * f : R.lam(k));
+        for (int k = 0; k < FCAP; k++) Y = Y + dW[k] * (LV ? (R.lam(k) * f) * sv : R.lam(k));
         if (PC) {
             MU[j] = mu;
             X[j] = Y;
@@ -283,7 +289,7 @@ __device__ __forceinline__ void general_step(real (&X)[NCAP], real (&MU)[NCAP], 
             const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, i, j);
             const real Lp = to_value<real>(X[j], v.lognormal, v.capped, v.cap);    // predicted value
             const real f = LV ? local_vol_factor<real>(Lp, v, i, j, n) : (real)1;
-            const real mup = drift_component<FCAP, real, TERMINAL, LV>(S, R, Lp, v.lognormal, f);
+            const real mup = drift_component<FCAP, real, TERMINAL, LV>(S, R, Lp, v.lognormal, f, sv);
             const real Y = X[j] + ((mup - MU[j]) * (real)0.5) * dt;
             const real Ln = to_value<real>(Y, v.lognormal, v.capped, v.cap);
             X[j] = v.carry ? Y : Ln;
@@ -336,7 +342,8 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
     const bool lognormal = a.lognormal != 0;
     const bool capped = a.capped != 0;
     const real cap = (real)a.cap;
-    const Variant<real> var{lognormal, capped, GENERAL && a.carry != 0, cap, (real)a.lv_alpha, (const real*)a.lvb};
+    const Variant<real> var{lognormal, capped, GENERAL && a.carry != 0, cap, (real)a.lv_alpha, (const real*)a.lvb,
+                            (const real*)a.svs};
     const bool terminal = GENERAL && a.terminal != 0;
     // the array holds states (carried schemes) or values; val gives the value
     const real logcap = (real)a.logcap;

@@ -36,7 +36,8 @@ BASIS_REFERENCE, BASIS_SWAP_RATE = 0, 1
 # Every symbol include/finmath_lmm.h declares.
 EXPORTED_SYMBOLS = (
     "fm_lmm_create", "fm_lmm_destroy", "fm_lmm_set_initial_forwards", "fm_lmm_set_covariance",
-    "fm_lmm_set_discount_curve", "fm_lmm_set_local_volatility", "fm_lmm_simulate", "fm_lmm_get_process_value",
+    "fm_lmm_set_discount_curve", "fm_lmm_set_local_volatility", "fm_lmm_set_stochastic_volatility",
+    "fm_lmm_simulate", "fm_lmm_get_process_value",
     "fm_lmm_process_value_device_ptr", "fm_lmm_get_numeraire", "fm_lmm_get_brownian_increment",
     "fm_lmm_value_products", "fm_lmm_set_volatility_factorization", "fm_lmm_adjoint",
     "fm_lmm_bermudan_exercise_times", "fm_lmm_serialize", "fm_lmm_deserialize",
@@ -126,6 +127,7 @@ def load() -> ctypes.CDLL:
         "fm_lmm_set_covariance": ([c_void_p, dp, dp], c_int32),
         "fm_lmm_set_discount_curve": ([c_void_p, dp], c_int32),
         "fm_lmm_set_local_volatility": ([c_void_p, c_double, dp], c_int32),
+        "fm_lmm_set_stochastic_volatility": ([c_void_p, c_double, c_double, c_void_p, c_int32, c_int32], c_int32),
         "fm_lmm_simulate": ([c_void_p], c_int32),
         "fm_lmm_get_process_value": ([c_void_p, c_int32, c_int32, dp], c_int32),
         "fm_lmm_process_value_device_ptr": ([c_void_p, c_int32, c_int32, POINTER(c_void_p)], c_int32),
@@ -322,6 +324,15 @@ class LmmContext:
         b = np.broadcast_to(as_f64(offsets), (self.nT, self.n)).copy()
         check(self.lib.fm_lmm_set_local_volatility(self.h, float(alpha), _dptr(b)))
         self.local_volatility = (float(alpha), b)
+
+    def set_stochastic_volatility(self, nu, rho, driver, factor_a=0, factor_b=1):
+        """Scale every loading by S(t_i) = exp(X), X an Euler process with drift -nu^2/2 and loadings
+        (rho nu, sqrt(1 - rho^2) nu) on factors (factor_a, factor_b) of the context `driver`'s
+        Brownian motion (None removes the scaling)."""
+        h = None if driver is None else driver.h
+        check(self.lib.fm_lmm_set_stochastic_volatility(self.h, float(nu), float(rho), h, int(factor_a),
+                                                         int(factor_b)))
+        self._sv_driver = driver
 
     def set_discount_curve(self, df):
         if df is None:

@@ -188,6 +188,20 @@ PYBIND11_MODULE(_finmath_host, m) {
         })
         .def("getDisplacement", &BlendedLocalVolatilityModel::getDisplacement);
 
+    py::class_<LIBORCovarianceModelStochasticVolatility, LIBORCovarianceModelFromVolatilityAndCorrelation,
+               std::shared_ptr<LIBORCovarianceModelStochasticVolatility>>(m, "LIBORCovarianceModelStochasticVolatility")
+        .def(py::init([](std::shared_ptr<LIBORCovarianceModelFromVolatilityAndCorrelation> cov,
+                         std::shared_ptr<BrownianMotionFromMersenneRandomNumbers> bm, double nu, double rho,
+                         bool isCalibrateable) {
+            return std::make_shared<LIBORCovarianceModelStochasticVolatility>(cov, bm, nu, rho, isCalibrateable);
+        }), py::arg("covarianceModel"), py::arg("brownianMotion"), py::arg("nu"), py::arg("rho"),
+           py::arg("isCalibrateable"))
+        .def("getBaseCovarianceModel", [](const LIBORCovarianceModelStochasticVolatility& c) {
+            return std::const_pointer_cast<LIBORCovarianceModelFromVolatilityAndCorrelation>(c.getBaseCovarianceModel());
+        })
+        .def("getNu", &LIBORCovarianceModelStochasticVolatility::getNu)
+        .def("getRho", &LIBORCovarianceModelStochasticVolatility::getRho);
+
     py::class_<LevenbergMarquardt>(m, "LevenbergMarquardt")
         .def(py::init<std::vector<double>, std::vector<double>, int>(), py::arg("initialParameters"),
              py::arg("targetValues"), py::arg("maxIteration"))

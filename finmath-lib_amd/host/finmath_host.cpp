@@ -344,11 +344,70 @@ void LIBORCovarianceModelFromVolatilityAndCorrelation::tables(const TimeDiscreti
 // ---- local-volatility wrappers -------------------------------------------------------------
 static const LIBORCovarianceModelFromVolatilityAndCorrelation& unwrapped(const CovarianceModelPtr& base) {
     if (!base) throw IllegalArgumentException("covariance model must be given");
-    double a;
+    double a, r;
     std::vector<double> o;
+    BrownianMotionPtr bm;
     if (base->localVolatility(*base->getTimeDiscretization(), a, o))
         throw UnsupportedOperationException("local-volatility wrappers do not nest");
+    if (base->stochasticVolatility(a, r, bm))
+        throw UnsupportedOperationException("a stochastic volatility wrapper goes outside a local-volatility wrapper");
     return *base;
+}
+
+// the base of a stochastic volatility wrapper: a plain model or one local-volatility wrapper
+static const LIBORCovarianceModelFromVolatilityAndCorrelation& stochasticBase(const CovarianceModelPtr& base) {
+    if (!base) throw IllegalArgumentException("covariance model must be given");
+    double a, r;
+    BrownianMotionPtr bm;
+    if (base->stochasticVolatility(a, r, bm))
+        throw UnsupportedOperationException("stochastic volatility wrappers do not nest");
+    return *base;
+}
+
+LIBORCovarianceModelStochasticVolatility::LIBORCovarianceModelStochasticVolatility(CovarianceModelPtr covarianceModel,
+                                                                                   BrownianMotionPtr brownianMotion,
+                                                                                   double nu, double rho,
+                                                                                   bool isCalibrateable)
+    : LIBORCovarianceModelFromVolatilityAndCorrelation(stochasticBase(covarianceModel)),
+      base_(std::move(covarianceModel)), bm_(std::move(brownianMotion)), nu_(nu), rho_(rho),
+      calibrateable_(isCalibrateable) {
+    if (!bm_ || bm_->getNumberOfFactors() < 2)
+        throw IllegalArgumentException("the stochastic volatility needs a Brownian motion with at least two factors");
+    if (!std::isfinite(nu) || !(rho >= -1.0 && rho <= 1.0))
+        throw IllegalArgumentException("nu must be finite and rho in [-1, 1]");
+}
+
+// The scaling process of a stochastic volatility wrapper on the device: a context used only as
+// the wrapper's Brownian motion drives it (factors 0 and 1); without a wrapper the scaling is
+// removed.
+static void applyStochasticVolatility(fm_lmm* c, const LIBORCovarianceModelFromVolatilityAndCorrelation& cov,
+                                      const std::vector<double>& tenor, int paths) {
+    double nu = 0.0, rho = 0.0;
+    BrownianMotionPtr bm;
+    if (!cov.stochasticVolatility(nu, rho, bm)) {
+        check(fm_lmm_set_stochastic_volatility(c, 0.0, 0.0, nullptr, 0, 1));
+        return;
+    }
+    if (bm->getNumberOfPaths() != paths)
+        throw IllegalArgumentException("the stochastic volatility's Brownian motion has another number of paths");
+    const auto& times = bm->getTimeDiscretization()->getAsDoubleArray();
+    fm_lmm_config cfg{};
+    cfg.num_time_steps = (int32_t)times.size() - 1;
+    cfg.times = times.data();
+    cfg.num_libors = (int32_t)tenor.size() - 1;
+    cfg.tenor = tenor.data();
+    cfg.num_factors = bm->getNumberOfFactors();
+    cfg.num_paths = paths;
+    cfg.seed = bm->getSeed();
+    cfg.measure = FM_MEASURE_SPOT;
+    cfg.state_space = FM_STATE_SPACE_LOGNORMAL;
+    cfg.scheme = FM_SCHEME_EULER_FUNCTIONAL;
+    cfg.precision = FM_PRECISION_FP64;
+    cfg.libor_cap = 1e5;
+    fm_lmm* driver = nullptr;
+    check(fm_lmm_create(&cfg, &driver));
+    std::unique_ptr<fm_lmm, void (*)(fm_lmm*)> guard(driver, fm_lmm_destroy);
+    check(fm_lmm_set_stochastic_volatility(c, nu, rho, driver, 0, 1));
 }
 
 DisplacedLocalVolatilityModel::DisplacedLocalVolatilityModel(CovarianceModelPtr covarianceModel, double displacement,
@@ -506,6 +565,7 @@ void EulerSchemeFromProcessModel::ensureContext(bool storePaths) {
         std::vector<double> lvOffsets;
         if (model_->getCovarianceModel()->localVolatility(*driver_->getTimeDiscretization(), lvAlpha, lvOffsets))
             check(fm_lmm_set_local_volatility(c, lvAlpha, lvOffsets.data()));
+        applyStochasticVolatility(c, *model_->getCovarianceModel(), tenor, driver_->getNumberOfPaths());
         if (model_->getDiscountCurve()) {
             std::vector<double> pd(tenor.size());
             for (size_t k = 0; k < tenor.size(); k++) pd[k] = model_->getDiscountCurve()->getDiscountFactor(tenor[k]);
@@ -857,6 +917,11 @@ void LevenbergMarquardt::run(const ObjectiveFunction& setValues) {              
 
 // ---- calibration of the covariance model -----------------------------------------------------
 std::vector<double> getParameter(const LIBORCovarianceModelFromVolatilityAndCorrelation& cov) {
+    if (auto* w = dynamic_cast<const LIBORCovarianceModelStochasticVolatility*>(&cov)) {   // StochasticVolatility.java:103-121
+        std::vector<double> p = getParameter(*w->getBaseCovarianceModel());
+        if (w->isCalibrateable()) { p.push_back(w->getNu()); p.push_back(w->getRho()); }
+        return p;
+    }
     if (auto* w = dynamic_cast<const DisplacedLocalVolatilityModel*>(&cov)) {   // DisplacedLocalVolatilityModel.java:121-137
         std::vector<double> p = getParameter(*w->getBaseCovarianceModel());
         if (w->isCalibrateable()) p.push_back(w->getDisplacement());
@@ -882,6 +947,19 @@ CovarianceModelPtr getCloneWithModifiedParameters(const LIBORCovarianceModelFrom
         base.assign(parameters.begin(), parameters.end() - 1);
         return parameters.back();
     };
+    if (auto* w = dynamic_cast<const LIBORCovarianceModelStochasticVolatility*>(&cov)) {   // :123-160
+        std::vector<double> base = parameters;
+        double nu = w->getNu(), rho = w->getRho();
+        if (w->isCalibrateable()) {
+            if (parameters.size() < 2) throw IllegalArgumentException("nu and rho are missing");
+            base.assign(parameters.begin(), parameters.end() - 2);
+            nu = parameters[parameters.size() - 2];
+            rho = parameters[parameters.size() - 1];
+        }
+        return std::make_shared<LIBORCovarianceModelStochasticVolatility>(
+            getCloneWithModifiedParameters(*w->getBaseCovarianceModel(), base), w->getBrownianMotion(), nu, rho,
+            w->isCalibrateable());
+    }
     if (auto* w = dynamic_cast<const DisplacedLocalVolatilityModel*>(&cov)) {
         std::vector<double> base;
         const double d = split(w->isCalibrateable(), w->getDisplacement(), base);
@@ -979,6 +1057,7 @@ CalibrationResult getCloneCalibrated(const LIBORCovarianceModelFromVolatilityAnd
             double lvAlpha = 0.0;
             const bool lv = c->localVolatility(*td, lvAlpha, lvOffsets);
             check(fm_lmm_set_local_volatility(ctx, lvAlpha, lv ? lvOffsets.data() : nullptr));
+            applyStochasticVolatility(ctx, *c, tenor, numberOfPaths);
             check(fm_lmm_value_products(ctx, lowered.data(), (int32_t)lowered.size(), vals.data(), nullptr));
             valuations++;
             for (size_t q = 0; q < products.size(); q++)

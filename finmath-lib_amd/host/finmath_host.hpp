@@ -143,6 +143,8 @@ using CorrelationModelPtr = std::shared_ptr<const LIBORCorrelationModelExponenti
 // as an n x m row-major factor matrix (functions/LinearAlgebra.java:392-482).
 std::vector<double> factorReduction(const std::vector<double>& correlation, int n, int numberOfFactors);
 
+struct BrownianMotionFromMersenneRandomNumbers;
+
 // lambda_jk(t_i) = sigma_j(t_i) f_jk; covariance sigma_j sigma_l rho_jl
 // (covariance/LIBORCovarianceModelFromVolatilityAndCorrelation.java:48-93).
 class LIBORCovarianceModelFromVolatilityAndCorrelation {
@@ -167,6 +169,12 @@ public:
     // for a model without wrapper.
     virtual bool localVolatility(const TimeDiscretizationFromArray& simulationTimes, double& alpha,
                                  std::vector<double>& offsets) const;
+    // Stochastic volatility scaling of a wrapper: its parameters and Brownian motion. False for
+    // a model without scaling.
+    virtual bool stochasticVolatility(double& nu, double& rho,
+                                      std::shared_ptr<const BrownianMotionFromMersenneRandomNumbers>& brownianMotion) const {
+        return false;
+    }
 protected:
     // round-down index of a simulation time on the covariance model's grid
     // (AbstractLIBORCovarianceModel.getFactorLoading(double time, ...), :54-60)
@@ -256,6 +264,35 @@ struct BrownianMotionFromMersenneRandomNumbers {
     int numberOfFactors, numberOfPaths, seed;
 };
 using BrownianMotionPtr = std::shared_ptr<const BrownianMotionFromMersenneRandomNumbers>;
+
+// Factor loadings of the base model (plain or with a local-volatility wrapper) times S(t), the
+// exponential of a functional Euler process with drift -nu^2/2 and loadings (rho nu,
+// sqrt(1 - rho^2) nu) on the first two factors of the given Brownian motion
+// (covariance/LIBORCovarianceModelStochasticVolatility.java:72-99,174-256); the parameters are the
+// base model's, followed by nu and rho when calibrateable (:103-160).
+class LIBORCovarianceModelStochasticVolatility : public LIBORCovarianceModelFromVolatilityAndCorrelation {
+public:
+    LIBORCovarianceModelStochasticVolatility(CovarianceModelPtr covarianceModel, BrownianMotionPtr brownianMotion,
+                                             double nu, double rho, bool isCalibrateable);
+    CovarianceModelPtr getBaseCovarianceModel() const { return base_; }
+    BrownianMotionPtr getBrownianMotion() const { return bm_; }
+    double getNu() const { return nu_; }
+    double getRho() const { return rho_; }
+    bool isCalibrateable() const { return calibrateable_; }
+    bool localVolatility(const TimeDiscretizationFromArray& simulationTimes, double& alpha,
+                         std::vector<double>& offsets) const override {
+        return base_->localVolatility(simulationTimes, alpha, offsets);
+    }
+    bool stochasticVolatility(double& nu, double& rho, BrownianMotionPtr& brownianMotion) const override {
+        nu = nu_; rho = rho_; brownianMotion = bm_;
+        return true;
+    }
+private:
+    CovarianceModelPtr base_;
+    BrownianMotionPtr bm_;
+    double nu_, rho_;
+    bool calibrateable_;
+};
 
 // ---- a RandomVariable holding realisations on the host ------------------------------
 // getAverage is the Kahan mean (montecarlo/RandomVariableFromDoubleArray.java:287-307).

@@ -147,6 +147,22 @@ int fm_lmm_set_covariance(fm_lmm* ctx, const double* factor_loading, const doubl
  * AbstractLIBORCovarianceModel.getCovariance (models/covariance/AbstractLIBORCovarianceModel.java:79-92). */
 int fm_lmm_set_local_volatility(fm_lmm* ctx, double alpha, const double* offsets);
 
+/* Stochastic volatility scaling of the covariance model: every factor loading at simulation
+ * time index i is multiplied (after a local-volatility factor) by the path's S(t_i) of the
+ * scalar process
+ *   S(t_0) = 1,  log S(t_{i+1}) = log S(t_i) - (nu^2 / 2) dt_i + rho nu dB_a
+ *                                + sqrt(1 - rho^2) nu dB_b,
+ * with dB_a, dB_b the increments of factors factor_a and factor_b of the Brownian motion of the
+ * context `driver` (same simulation times, paths and path offset, FP64; it may be ctx itself,
+ * which shares those factors with the forwards as a BrownianMotionView would). S is computed
+ * on the device when this is called. driver == NULL removes the scaling. A context with the
+ * scaling is not serialised and the adjoint refuses it.
+ * Replaces LIBORCovarianceModelStochasticVolatility.getFactorLoading and its scaling process
+ * (models/covariance/LIBORCovarianceModelStochasticVolatility.java:174-256), run by
+ * EulerSchemeFromProcessModel (montecarlo/process/EulerSchemeFromProcessModel.java:232-256). */
+int fm_lmm_set_stochastic_volatility(fm_lmm* ctx, double nu, double rho, fm_lmm* driver, int32_t factor_a,
+                                     int32_t factor_b);
+
 /* Discount factors P^d(T_k), k <= n, of the model's discount curve, or NULL for a
  * model without discount curve (no numeraire adjustment).
  * Replaces the deterministic numeraire adjustment of

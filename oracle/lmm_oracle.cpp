@@ -561,6 +561,11 @@ struct Model {
     bool has_lv = false;                          // local-volatility wrapper of the covariance model
     double lv_alpha = 0.0;
     std::vector<double> lvb;                      // [nT][n] offsets
+    bool has_sv = false;                          // stochastic volatility scaling of the loadings
+    double sv_nu = 0.0, sv_rho = 0.0;
+    int32_t sv_seed = 0;
+    int sv_F = 2, sv_a = 0, sv_b = 1;
+    std::vector<RV> svs;                          // S(t_i), i < nT
     std::vector<std::vector<RV>> proc;            // [nT+1][n]
     std::vector<std::vector<RV>> dW;              // [nT][F]
     std::map<int, RV> numeraires;                 // unadjusted at tenor index
@@ -586,13 +591,15 @@ struct Model {
             f = add(f, lvb[(size_t)ti * n + j]);
             for (int k = 0; k < F; k++) l[k] = mult(l[k], f);
         }
+        if (has_sv)                                                              // StochasticVolatility.java:247-252
+            for (int k = 0; k < F; k++) l[k] = mult(l[k], svs[ti]);
         return l;
     }
     // Variance of the log-coordinate drift adjustment: sigma_j sigma_j rho_jj of the
     // covariance model, or, under a local-volatility wrapper, the sum of the squared
     // loadings (AbstractLIBORCovarianceModel.getCovariance, AbstractLIBORCovarianceModel.java:79-92).
     RV variance_of(int ti, int j, const std::vector<RV>& l) const {
-        if (!has_lv) return variance_(ti, j);
+        if (!has_lv && !has_sv) return variance_(ti, j);
         RV c = mult(l[0], l[0]);
         for (int k = 1; k < F; k++) c = addProduct(c, l[k], l[k]);
         return c;
@@ -650,7 +657,33 @@ struct Model {
         return out;
     }
 
+    // The scaling process of LIBORCovarianceModelStochasticVolatility (:174-236): S = exp(X),
+    // functional Euler (Euler:234-256) with drift nu^2 (-0.5) and loadings (rho nu,
+    // sqrt((rho^2 - 1)(-1)) nu) on factors a, b of its own Brownian motion; S(0) = exp(0.0).
+    void simulate_scaling() {
+        std::vector<double> bm((size_t)nT * sv_F * P);
+        orc_brownian(times.data(), nT, sv_F, P, sv_seed, bm.data());
+        auto inc = [&](int t, int k) {
+            std::vector<double> v(bm.begin() + ((size_t)t * sv_F + k) * P, bm.begin() + ((size_t)t * sv_F + k + 1) * P);
+            return rvarr(std::move(v));
+        };
+        const RV drift = mult(mult(scalar(sv_nu), scalar(sv_nu)), -0.5);
+        const RV l0 = mult(scalar(sv_rho), scalar(sv_nu));
+        const RV l1 = mult(scalar(std::sqrt((sv_rho * sv_rho - 1.0) * -1.0)), scalar(sv_nu));
+        svs.assign(nT, RV());
+        RV S = expv(scalar(0.0));
+        for (int t = 0; t < nT; t++) {
+            svs[t] = S;
+            RV X = logv(S);
+            X = addProduct(X, drift, times[t + 1] - times[t]);
+            X = addProduct(X, l0, inc(t, sv_a));
+            X = addProduct(X, l1, inc(t, sv_b));
+            S = expv(X);
+        }
+    }
+
     void simulate(int32_t seed, int threads) {
+        if (has_sv) simulate_scaling();
         // Brownian increments (BrownianMotionFromMersenneRandomNumbers.java:141-191)
         std::vector<double> bm((size_t)nT * F * P);
         orc_brownian(times.data(), nT, F, P, seed, bm.data());
@@ -900,6 +933,10 @@ typedef struct orc_lmm_spec {
     const double* fc_f;
     double fc_offset;                  // the curve's payment offset
     int interpolation;                 // InterpolationMethod ordinal (LMM:168)
+    int sv_on;                         // stochastic volatility scaling
+    double sv_nu, sv_rho;
+    int32_t sv_seed;                   // its Brownian motion: seed, factors, the two factors used
+    int sv_F, sv_a, sv_b;
 } orc_lmm_spec;
 
 static std::string g_err;
@@ -923,6 +960,9 @@ void* orc_lmm_create(const orc_lmm_spec* s) {
             m->fc_offset = s->fc_offset;
         }
         m->interp = s->interpolation;
+        m->has_sv = s->sv_on != 0;
+        m->sv_nu = s->sv_nu; m->sv_rho = s->sv_rho; m->sv_seed = s->sv_seed;
+        m->sv_F = s->sv_F; m->sv_a = s->sv_a; m->sv_b = s->sv_b;
         m->has_lv = s->lv_offset != nullptr;
         if (m->has_lv) {
             m->lv_alpha = s->lv_alpha;

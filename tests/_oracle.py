@@ -156,6 +156,8 @@ class Spec(ctypes.Structure):
         ("lv_alpha", c_double), ("lv_offset", POINTER(c_double)),
         ("fc_n", c_int), ("fc_t", POINTER(c_double)), ("fc_f", POINTER(c_double)), ("fc_offset", c_double),
         ("interpolation", c_int),
+        ("sv_on", c_int), ("sv_nu", c_double), ("sv_rho", c_double), ("sv_seed", c_int32),
+        ("sv_F", c_int), ("sv_a", c_int), ("sv_b", c_int),
     ]
 
 
@@ -184,8 +186,10 @@ class ModelTables:
 
 class OracleModel:
     def __init__(self, tables: ModelTables, num_paths=None, *, cap=1e5, scheme=2, measure=0, state_space=1,
-                 with_discount=True, threads=8, seed=None, local_vol=None, interpolation=1):
-        """local_vol: (alpha, offsets) of a local-volatility wrapper, offsets [nT][n] or a scalar."""
+                 with_discount=True, threads=8, seed=None, local_vol=None, interpolation=1, stochastic_vol=None):
+        """local_vol: (alpha, offsets) of a local-volatility wrapper, offsets [nT][n] or a scalar;
+        stochastic_vol: (nu, rho, seed, F, a, b) of a stochastic volatility scaling driven by
+        factors a, b of an F-factor Brownian motion with that seed."""
         wl = tables.wl
         self.t = tables
         self.P = int(num_paths or wl.num_paths)
@@ -220,6 +224,11 @@ class OracleModel:
         s.fc_t, s.fc_f = _dp(self._keep[-2]), _dp(self._keep[-1])
         s.fc_offset = float(wl.period)
         s.interpolation = interpolation          # LINEAR 0, LOG_LINEAR_UNCORRECTED 1, LOG_LINEAR_CORRECTED 2
+        if stochastic_vol is not None:
+            s.sv_on = 1
+            s.sv_nu, s.sv_rho, s.sv_seed, s.sv_F, s.sv_a, s.sv_b = stochastic_vol
+        else:
+            s.sv_on, s.sv_F, s.sv_a, s.sv_b = 0, 2, 0, 1
         self.spec = s
         self.h = lib().orc_lmm_create(ctypes.byref(s))
         if not self.h:

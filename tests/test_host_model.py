@@ -145,3 +145,31 @@ def test_local_volatility_wrapper_parameters_and_clones():
         free.getCloneWithModifiedParameters([0.3, 0.01, 0.2, 0.05])         # base model takes four
     with pytest.raises(Exception):
         H.DisplacedLocalVolatilityModel(free, 1.0, False)                  # wrappers do not nest
+
+
+def test_stochastic_volatility_wrapper_parameters_and_nesting():
+    wl = W.c0()
+    td, tenor, fc, _, corr, _, model = build_model(wl)
+    vol = H.LIBORVolatilityModelFourParameterExponentialForm(td, tenor, 0.2, 0.0, 0.25, 0.1, True)
+    cov = H.LIBORCovarianceModelFromVolatilityAndCorrelation(td, tenor, vol, corr)
+    bm = H.BrownianMotionFromMersenneRandomNumbers(td, 2, 1000, 7)
+    blended = H.BlendedLocalVolatilityModel(cov, 0.0, False)
+    sv = H.LIBORCovarianceModelStochasticVolatility(blended, bm, 0.01, -0.3, True)
+    assert list(sv.getParameter()) == [0.2, 0.0, 0.25, 0.1, 0.01, -0.3]
+    alpha, b = sv.localVolatility(td)                                       # the wrapped blend
+    assert alpha == 0.0 and np.all(b == 0.0)
+    c = sv.getCloneWithModifiedParameters([0.3, 0.01, 0.2, 0.05, 0.02, -0.5])
+    assert isinstance(c, H.LIBORCovarianceModelStochasticVolatility)
+    assert (c.getNu(), c.getRho()) == (0.02, -0.5)
+    assert isinstance(c.getBaseCovarianceModel(), H.BlendedLocalVolatilityModel)
+    fixed = H.LIBORCovarianceModelStochasticVolatility(cov, bm, 0.01, -0.3, False)
+    assert list(fixed.getParameter()) == [0.2, 0.0, 0.25, 0.1]
+    with pytest.raises(ValueError):
+        H.LIBORCovarianceModelStochasticVolatility(cov, bm, 0.01, 1.5, False)          # rho outside [-1, 1]
+    with pytest.raises(ValueError):
+        H.LIBORCovarianceModelStochasticVolatility(cov, H.BrownianMotionFromMersenneRandomNumbers(td, 1, 1000, 7),
+                                                   0.01, 0.0, False)                  # one factor
+    with pytest.raises(Exception):
+        H.LIBORCovarianceModelStochasticVolatility(sv, bm, 0.01, 0.0, False)          # no nesting
+    with pytest.raises(Exception):
+        H.DisplacedLocalVolatilityModel(sv, 1.0, False)                               # local vol goes inside

@@ -21,7 +21,7 @@ def tiny_workload(n=4, F=2, P=6):
     return w
 
 
-def brute_force_paths(tables, P, measure=0, scheme=2, state_space=1, cap=1e5, local_vol=None):
+def brute_force_paths(tables, P, measure=0, scheme=2, state_space=1, cap=1e5, local_vol=None, scaling=None):
     """Euler scheme per path, written from the SDE of the forward rates:
       log-normal: d log L_j = (mu_j - var_j / 2) dt + lambda_j . dW
       normal:     d L_j     =  mu_j dt             + lambda_j . dW
@@ -33,7 +33,8 @@ def brute_force_paths(tables, P, measure=0, scheme=2, state_space=1, cap=1e5, lo
     re-derive it from the value (they differ only when the cap binds).
     local_vol = (alpha, b[nT][n]): the loadings of L_j are scaled by ((1 - alpha) L_j + b_ij),
     with L_j the value the drift or the diffusion is evaluated on, and the log-normal
-    variance is the squared norm of the scaled loading."""
+    variance is the squared norm of the scaled loading. scaling[nT][P]: every loading of path p
+    at step i is also multiplied by scaling[i, p]."""
     times, tenor = tables.times, tables.tenor
     nT, n = times.size - 1, tenor.size - 1
     F = tables.factors.shape[1]
@@ -45,15 +46,17 @@ def brute_force_paths(tables, P, measure=0, scheme=2, state_space=1, cap=1e5, lo
     L[0] = np.array([to_value(y) for y in Y0])[:, None]
     state = np.repeat(Y0[:, None], P, axis=1)
 
-    def loading(i, Lv):
+    def loading(i, Lv, p):
         lam = tables.lam[i]                                         # [n][F]
-        if local_vol is None:
-            return lam
-        alpha, b = local_vol[0], np.broadcast_to(local_vol[1], (nT, n))
-        return lam * ((1.0 - alpha) * Lv + b[i])[:, None]
+        if local_vol is not None:
+            alpha, b = local_vol[0], np.broadcast_to(local_vol[1], (nT, n))
+            lam = lam * ((1.0 - alpha) * Lv + b[i])[:, None]
+        if scaling is not None:
+            lam = lam * scaling[i, p]
+        return lam
 
-    def drift(i, Lv, first):
-        lam = loading(i, Lv)
+    def drift(i, Lv, first, p):
+        lam = loading(i, Lv, p)
         mu = np.zeros(n)
         for j in range(first, n):
             ls = range(first, j + 1) if measure == 0 else range(j + 1, n)
@@ -63,7 +66,8 @@ def brute_force_paths(tables, P, measure=0, scheme=2, state_space=1, cap=1e5, lo
                 w = d * Lv[l] / (1 + d * Lv[l]) if lognormal else d / (1 + d * Lv[l])
                 mu[j] += sgn * w * float(lam[j] @ lam[l])
             if lognormal:
-                mu[j] -= 0.5 * (tables.var[i, j] if local_vol is None else float(lam[j] @ lam[j]))
+                plain = local_vol is None and scaling is None
+                mu[j] -= 0.5 * (tables.var[i, j] if plain else float(lam[j] @ lam[j]))
         return mu
 
     for i in range(nT):
@@ -72,11 +76,11 @@ def brute_force_paths(tables, P, measure=0, scheme=2, state_space=1, cap=1e5, lo
         L[i + 1] = L[i]
         for p in range(P):
             y0 = state[:, p] if scheme in (0, 1) else (np.log(L[i, :, p]) if lognormal else L[i, :, p].copy())
-            mu = drift(i, L[i, :, p], first)
-            diffusion = loading(i, L[i, :, p]) @ dW[i, :, p]
+            mu = drift(i, L[i, :, p], first, p)
+            diffusion = loading(i, L[i, :, p], p) @ dW[i, :, p]
             y = y0 + mu * dt + diffusion
             if scheme in (1, 3):
-                mup = drift(i, np.array([to_value(v) for v in y]), first)
+                mup = drift(i, np.array([to_value(v) for v in y]), first, p)
                 y = y0 + 0.5 * (mu + mup) * dt + diffusion
             for j in range(first, n):
                 state[j, p] = y[j]
@@ -139,6 +143,60 @@ def test_local_volatility_against_brute_force(measure, scheme, state_space, kind
     for i in range(wl.num_time_steps + 1):
         for j in range(wl.num_libors):
             assert np.allclose(om.process_value(i, j), ref[i, j], rtol=1e-12, atol=1e-16), (i, j)
+
+
+def sv_scaling(times, P, nu, rho, seed, F=2, a=0, b=1):
+    """exp of the Euler scheme of dX = -nu^2/2 dt + nu (rho dB_a + sqrt(1 - rho^2) dB_b), X(0) = 0,
+    on the increments of an F-factor Brownian motion, [nT][P]."""
+    dB = O.brownian(times, F, P, seed)
+    X = np.zeros(P)
+    out = np.empty((times.size - 1, P))
+    for i in range(times.size - 1):
+        out[i] = np.exp(X)
+        X = X - 0.5 * nu * nu * (times[i + 1] - times[i]) + nu * (rho * dB[i, a] + math.sqrt(1 - rho * rho) * dB[i, b])
+    return out
+
+
+@pytest.mark.parametrize("measure,scheme,state_space,lv", [(0, 2, 1, False), (1, 1, 0, True), (0, 3, 0, True),
+                                                           (1, 0, 1, False)])
+def test_stochastic_volatility_against_brute_force(measure, scheme, state_space, lv):
+    # LIBORCovarianceModelStochasticVolatility: loadings times S(t_i) of a log-Euler process on
+    # its own Brownian motion, over a plain or a blended local-volatility model
+    wl = tiny_workload()
+    t = O.ModelTables(wl, vol_params=(0.2, 0.0, 0.25, 0.1) if state_space == 1 else (0.006, 0.0, 0.25, 0.002))
+    nT, n = wl.num_time_steps, wl.num_libors
+    P = 6
+    local = (0.5, 0.5 * 0.03) if lv else None
+    if lv:
+        t = O.ModelTables(wl, vol_params=(0.2, 0.0, 0.25, 0.1))
+    sv = (0.4, -0.3, 777, 2, 0, 1)
+    scaling = sv_scaling(t.times, P, 0.4, -0.3, 777)
+    om = O.OracleModel(t, P, measure=measure, scheme=scheme, state_space=state_space,
+                       local_vol=None if local is None else (local[0], np.full((nT, n), local[1])), stochastic_vol=sv)
+    ref = brute_force_paths(t, P, measure, scheme, state_space, scaling=scaling,
+                            local_vol=None if local is None else (local[0], np.full((nT, n), local[1])))
+    assert np.ptp(scaling[1:]) > 0.1                         # the scaling moves the paths
+    for i in range(nT + 1):
+        for j in range(n):
+            assert np.allclose(om.process_value(i, j), ref[i, j], rtol=1e-12, atol=1e-16), (i, j)
+
+
+def test_stochastic_volatility_scaling_is_a_martingale_and_vanishes_without_vol_of_vol():
+    # E[S(t)] = 1 (log-normal martingale); nu = 0 gives S = 1 and the plain model bit for bit
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    P = 40_000
+    S = sv_scaling(t.times, P, 0.3, -0.5, 99)
+    for i in range(1, wl.num_time_steps):
+        assert abs(np.mean(S[i]) - 1.0) < 4 * np.std(S[i]) / math.sqrt(P)
+    small = tiny_workload()
+    ts = O.ModelTables(small)
+    a = O.OracleModel(ts, 6, stochastic_vol=(0.0, -0.3, 5, 2, 0, 1))
+    b = O.OracleModel(ts, 6)
+    # the log-normal drift sums the squared loadings instead of sigma^2: equal up to rounding
+    for i in range(small.num_time_steps + 1):
+        for j in range(small.num_libors):
+            assert np.allclose(a.process_value(i, j), b.process_value(i, j), rtol=1e-14, atol=0)
 
 
 def test_full_blend_with_unit_offset_is_the_plain_model():
