@@ -915,6 +915,24 @@ int fm_lmm_set_local_volatility(fm_lmm* c, double alpha, const double* offsets) 
     FM_CATCH
 }
 
+extern "C++" {
+namespace {
+// Checks of a scaling's driver context, under both contexts' locks; generates its increments.
+void prepare_driver(fm_lmm* c, fm_lmm* driver, std::initializer_list<int32_t> factors) {
+    require(driver->device == c->device, FM_ERR_INVALID_ARGUMENT, "the driver lives on another device");
+    require(driver->P() == c->P() && driver->cfg.path_offset == c->cfg.path_offset, FM_ERR_INVALID_ARGUMENT,
+            "the driver must simulate the same paths");
+    require(driver->times == c->times, FM_ERR_INVALID_ARGUMENT, "the driver must have the same time discretization");
+    require(!driver->fp32(), FM_ERR_UNSUPPORTED, "the driver's increments must be FP64");
+    for (int32_t f : factors)
+        require(f >= 0 && f < driver->F(), FM_ERR_INVALID_ARGUMENT, "driver factor out of range");
+    ensure_brownian(driver);
+    HIP_CHECK(hipStreamSynchronize(driver->stream));
+    ensure_time_steps(c);
+}
+} // namespace
+} // extern "C++"
+
 int fm_lmm_set_stochastic_volatility(fm_lmm* c, double nu, double rho, fm_lmm* driver, int32_t factor_a,
                                      int32_t factor_b) {
     FM_TRY
@@ -929,17 +947,8 @@ int fm_lmm_set_stochastic_volatility(fm_lmm* c, double nu, double rho, fm_lmm* d
     if (driver == c) lk_c.lock(); else std::lock(lk_c, lk_d);
     require(std::isfinite(nu) && std::isfinite(rho) && rho >= -1.0 && rho <= 1.0, FM_ERR_INVALID_ARGUMENT,
             "nu must be finite and rho in [-1, 1]");
-    require(driver->device == c->device, FM_ERR_INVALID_ARGUMENT, "the driver lives on another device");
-    require(driver->P() == c->P() && driver->cfg.path_offset == c->cfg.path_offset, FM_ERR_INVALID_ARGUMENT,
-            "the driver must simulate the same paths");
-    require(driver->times == c->times, FM_ERR_INVALID_ARGUMENT, "the driver must have the same time discretization");
-    require(!driver->fp32(), FM_ERR_UNSUPPORTED, "the driver's increments must be FP64");
-    require(factor_a >= 0 && factor_a < driver->F() && factor_b >= 0 && factor_b < driver->F(),
-            FM_ERR_INVALID_ARGUMENT, "driver factor out of range");
     DeviceGuard g(c->device);
-    ensure_brownian(driver);
-    HIP_CHECK(hipStreamSynchronize(driver->stream));
-    ensure_time_steps(c);
+    prepare_driver(c, driver, {factor_a, factor_b});
     // drift nu.squared().mult(-0.5), loadings rho.mult(nu) and rho.squared().sub(1).mult(-1).sqrt().mult(nu)
     // (LIBORCovarianceModelStochasticVolatility.java:206-213)
     const double drift = (nu * nu) * -0.5;
@@ -948,6 +957,26 @@ int fm_lmm_set_stochastic_volatility(fm_lmm* c, double nu, double rho, fm_lmm* d
     void* out = c->d_svs.get((size_t)c->nT() * c->P() * c->real_size());
     launch_stochastic_volatility(driver->d_dw.as<double>(), c->nT(), driver->F(), factor_a, factor_b, c->P(),
                                  c->d_times.as<double>(), drift, l0, l1, out, c->fp32() ? 1 : 0, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->has_sv = true;
+    c->tables_dirty = true; c->simulated = false;
+    return FM_OK;
+    FM_CATCH
+}
+
+int fm_lmm_set_heston_volatility(fm_lmm* c, double kappa, double theta, double xi, fm_lmm* driver, int32_t factor) {
+    FM_TRY
+    require(c != nullptr, FM_ERR_INVALID_ARGUMENT, "NULL context");
+    if (!driver) return fm_lmm_set_stochastic_volatility(c, 0.0, 0.0, nullptr, 0, 0);
+    std::unique_lock<std::mutex> lk_c(c->mu, std::defer_lock), lk_d(driver->mu, std::defer_lock);
+    if (driver == c) lk_c.lock(); else std::lock(lk_c, lk_d);
+    require(std::isfinite(kappa) && std::isfinite(theta) && std::isfinite(xi), FM_ERR_INVALID_ARGUMENT,
+            "kappa, theta and xi must be finite");
+    DeviceGuard g(c->device);
+    prepare_driver(c, driver, {factor});
+    void* out = c->d_svs.get((size_t)c->nT() * c->P() * c->real_size());
+    launch_heston_volatility(driver->d_dw.as<double>(), c->nT(), driver->F(), factor, c->P(), c->d_times.as<double>(),
+                             kappa, theta, xi, out, c->fp32() ? 1 : 0, c->stream);
     HIP_CHECK(hipStreamSynchronize(c->stream));
     c->has_sv = true;
     c->tables_dirty = true; c->simulated = false;

@@ -140,6 +140,9 @@ void launch_lmm_step_lv_fp32(const StepArgs& a, void* stream);
 // [P][nT][Fd] (factors a, b), written as [nT][P] (double, or float when out_fp32).
 void launch_stochastic_volatility(const double* dw, int nT, int Fd, int fa, int fb, int P, const double* times,
                                   double drift, double l0, double l1, void* out, int out_fp32, void* stream);
+// Heston scaling sqrt(max(V(t_i), 0)), V(0) = 1, V <- V + (V - theta)(-kappa) dt_i + sqrt(max(V, 0)) xi dW_a.
+void launch_heston_volatility(const double* dw, int nT, int Fd, int fa, int P, const double* times, double kappa,
+                              double theta, double xi, void* out, int out_fp32, void* stream);
 
 // Reverse sweep (lmm_adjoint.hpp): adjoint arguments and launchers.
 struct AdjArgs {

@@ -37,6 +37,7 @@ BASIS_REFERENCE, BASIS_SWAP_RATE = 0, 1
 EXPORTED_SYMBOLS = (
     "fm_lmm_create", "fm_lmm_destroy", "fm_lmm_set_initial_forwards", "fm_lmm_set_covariance",
     "fm_lmm_set_discount_curve", "fm_lmm_set_local_volatility", "fm_lmm_set_stochastic_volatility",
+    "fm_lmm_set_heston_volatility",
     "fm_lmm_simulate", "fm_lmm_get_process_value",
     "fm_lmm_process_value_device_ptr", "fm_lmm_get_numeraire", "fm_lmm_get_brownian_increment",
     "fm_lmm_value_products", "fm_lmm_set_volatility_factorization", "fm_lmm_adjoint",
@@ -128,6 +129,7 @@ def load() -> ctypes.CDLL:
         "fm_lmm_set_discount_curve": ([c_void_p, dp], c_int32),
         "fm_lmm_set_local_volatility": ([c_void_p, c_double, dp], c_int32),
         "fm_lmm_set_stochastic_volatility": ([c_void_p, c_double, c_double, c_void_p, c_int32, c_int32], c_int32),
+        "fm_lmm_set_heston_volatility": ([c_void_p, c_double, c_double, c_double, c_void_p, c_int32], c_int32),
         "fm_lmm_simulate": ([c_void_p], c_int32),
         "fm_lmm_get_process_value": ([c_void_p, c_int32, c_int32, dp], c_int32),
         "fm_lmm_process_value_device_ptr": ([c_void_p, c_int32, c_int32, POINTER(c_void_p)], c_int32),
@@ -332,6 +334,13 @@ class LmmContext:
         h = None if driver is None else driver.h
         check(self.lib.fm_lmm_set_stochastic_volatility(self.h, float(nu), float(rho), h, int(factor_a),
                                                          int(factor_b)))
+        self._sv_driver = driver
+
+    def set_heston_volatility(self, kappa, theta, xi, driver, factor=0):
+        """Scale every loading by sqrt(max(V(t_i), 0)), V a square-root process from 1 with mean
+        reversion kappa to theta and volatility xi on factor `factor` of `driver`'s Brownian motion."""
+        h = None if driver is None else driver.h
+        check(self.lib.fm_lmm_set_heston_volatility(self.h, float(kappa), float(theta), float(xi), h, int(factor)))
         self._sv_driver = driver
 
     def set_discount_curve(self, df):

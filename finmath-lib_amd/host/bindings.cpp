@@ -202,6 +202,23 @@ PYBIND11_MODULE(_finmath_host, m) {
         .def("getNu", &LIBORCovarianceModelStochasticVolatility::getNu)
         .def("getRho", &LIBORCovarianceModelStochasticVolatility::getRho);
 
+    py::class_<LIBORCovarianceModelStochasticHestonVolatility, LIBORCovarianceModelFromVolatilityAndCorrelation,
+               std::shared_ptr<LIBORCovarianceModelStochasticHestonVolatility>>(
+        m, "LIBORCovarianceModelStochasticHestonVolatility")
+        .def(py::init([](std::shared_ptr<LIBORCovarianceModelFromVolatilityAndCorrelation> cov,
+                         std::shared_ptr<BrownianMotionFromMersenneRandomNumbers> bm, double kappa, double theta,
+                         double xi, bool isCalibrateable) {
+            return std::make_shared<LIBORCovarianceModelStochasticHestonVolatility>(cov, bm, kappa, theta, xi,
+                                                                                    isCalibrateable);
+        }), py::arg("covarianceModel"), py::arg("brownianMotion"), py::arg("kappa"), py::arg("theta"), py::arg("xi"),
+           py::arg("isCalibrateable"))
+        .def("getBaseCovarianceModel", [](const LIBORCovarianceModelStochasticHestonVolatility& c) {
+            return std::const_pointer_cast<LIBORCovarianceModelFromVolatilityAndCorrelation>(c.getBaseCovarianceModel());
+        })
+        .def("getKappa", &LIBORCovarianceModelStochasticHestonVolatility::getKappa)
+        .def("getTheta", &LIBORCovarianceModelStochasticHestonVolatility::getTheta)
+        .def("getXi", &LIBORCovarianceModelStochasticHestonVolatility::getXi);
+
     py::class_<LevenbergMarquardt>(m, "LevenbergMarquardt")
         .def(py::init<std::vector<double>, std::vector<double>, int>(), py::arg("initialParameters"),
              py::arg("targetValues"), py::arg("maxIteration"))

@@ -342,14 +342,19 @@ void LIBORCovarianceModelFromVolatilityAndCorrelation::tables(const TimeDiscreti
 }
 
 // ---- local-volatility wrappers -------------------------------------------------------------
+static bool isStochastic(const LIBORCovarianceModelFromVolatilityAndCorrelation& cov) {
+    double a, b, c;
+    BrownianMotionPtr bm;
+    return cov.stochasticVolatility(a, b, bm) || cov.hestonVolatility(a, b, c, bm);
+}
+
 static const LIBORCovarianceModelFromVolatilityAndCorrelation& unwrapped(const CovarianceModelPtr& base) {
     if (!base) throw IllegalArgumentException("covariance model must be given");
-    double a, r;
+    double a;
     std::vector<double> o;
-    BrownianMotionPtr bm;
     if (base->localVolatility(*base->getTimeDiscretization(), a, o))
         throw UnsupportedOperationException("local-volatility wrappers do not nest");
-    if (base->stochasticVolatility(a, r, bm))
+    if (isStochastic(*base))
         throw UnsupportedOperationException("a stochastic volatility wrapper goes outside a local-volatility wrapper");
     return *base;
 }
@@ -357,11 +362,19 @@ static const LIBORCovarianceModelFromVolatilityAndCorrelation& unwrapped(const C
 // the base of a stochastic volatility wrapper: a plain model or one local-volatility wrapper
 static const LIBORCovarianceModelFromVolatilityAndCorrelation& stochasticBase(const CovarianceModelPtr& base) {
     if (!base) throw IllegalArgumentException("covariance model must be given");
-    double a, r;
-    BrownianMotionPtr bm;
-    if (base->stochasticVolatility(a, r, bm))
-        throw UnsupportedOperationException("stochastic volatility wrappers do not nest");
+    if (isStochastic(*base)) throw UnsupportedOperationException("stochastic volatility wrappers do not nest");
     return *base;
+}
+
+LIBORCovarianceModelStochasticHestonVolatility::LIBORCovarianceModelStochasticHestonVolatility(
+    CovarianceModelPtr covarianceModel, BrownianMotionPtr brownianMotion, double kappa, double theta, double xi,
+    bool isCalibrateable)
+    : LIBORCovarianceModelFromVolatilityAndCorrelation(stochasticBase(covarianceModel)),
+      base_(std::move(covarianceModel)), bm_(std::move(brownianMotion)), kappa_(kappa), theta_(theta), xi_(xi),
+      calibrateable_(isCalibrateable) {
+    if (!bm_) throw IllegalArgumentException("the Heston volatility needs a Brownian motion");
+    if (!std::isfinite(kappa) || !std::isfinite(theta) || !std::isfinite(xi))
+        throw IllegalArgumentException("kappa, theta and xi must be finite");
 }
 
 LIBORCovarianceModelStochasticVolatility::LIBORCovarianceModelStochasticVolatility(CovarianceModelPtr covarianceModel,
@@ -382,9 +395,10 @@ LIBORCovarianceModelStochasticVolatility::LIBORCovarianceModelStochasticVolatili
 // removed.
 static void applyStochasticVolatility(fm_lmm* c, const LIBORCovarianceModelFromVolatilityAndCorrelation& cov,
                                       const std::vector<double>& tenor, int paths) {
-    double nu = 0.0, rho = 0.0;
+    double nu = 0.0, rho = 0.0, xi = 0.0;
     BrownianMotionPtr bm;
-    if (!cov.stochasticVolatility(nu, rho, bm)) {
+    const bool heston = cov.hestonVolatility(nu, rho, xi, bm);     // kappa, theta, xi
+    if (!heston && !cov.stochasticVolatility(nu, rho, bm)) {
         check(fm_lmm_set_stochastic_volatility(c, 0.0, 0.0, nullptr, 0, 1));
         return;
     }
@@ -407,7 +421,8 @@ static void applyStochasticVolatility(fm_lmm* c, const LIBORCovarianceModelFromV
     fm_lmm* driver = nullptr;
     check(fm_lmm_create(&cfg, &driver));
     std::unique_ptr<fm_lmm, void (*)(fm_lmm*)> guard(driver, fm_lmm_destroy);
-    check(fm_lmm_set_stochastic_volatility(c, nu, rho, driver, 0, 1));
+    if (heston) check(fm_lmm_set_heston_volatility(c, nu, rho, xi, driver, 0));
+    else check(fm_lmm_set_stochastic_volatility(c, nu, rho, driver, 0, 1));
 }
 
 DisplacedLocalVolatilityModel::DisplacedLocalVolatilityModel(CovarianceModelPtr covarianceModel, double displacement,
@@ -917,6 +932,11 @@ void LevenbergMarquardt::run(const ObjectiveFunction& setValues) {              
 
 // ---- calibration of the covariance model -----------------------------------------------------
 std::vector<double> getParameter(const LIBORCovarianceModelFromVolatilityAndCorrelation& cov) {
+    if (auto* w = dynamic_cast<const LIBORCovarianceModelStochasticHestonVolatility*>(&cov)) {   // Heston:84-104
+        std::vector<double> p = getParameter(*w->getBaseCovarianceModel());
+        if (w->isCalibrateable()) { p.push_back(w->getKappa()); p.push_back(w->getTheta()); p.push_back(w->getXi()); }
+        return p;
+    }
     if (auto* w = dynamic_cast<const LIBORCovarianceModelStochasticVolatility*>(&cov)) {   // StochasticVolatility.java:103-121
         std::vector<double> p = getParameter(*w->getBaseCovarianceModel());
         if (w->isCalibrateable()) { p.push_back(w->getNu()); p.push_back(w->getRho()); }
@@ -947,6 +967,20 @@ CovarianceModelPtr getCloneWithModifiedParameters(const LIBORCovarianceModelFrom
         base.assign(parameters.begin(), parameters.end() - 1);
         return parameters.back();
     };
+    if (auto* w = dynamic_cast<const LIBORCovarianceModelStochasticHestonVolatility*>(&cov)) {   // Heston:107-164
+        std::vector<double> base = parameters;
+        double kappa = w->getKappa(), theta = w->getTheta(), xi = w->getXi();
+        if (w->isCalibrateable()) {
+            if (parameters.size() < 3) throw IllegalArgumentException("kappa, theta and xi are missing");
+            base.assign(parameters.begin(), parameters.end() - 3);
+            kappa = parameters[parameters.size() - 3];
+            theta = parameters[parameters.size() - 2];
+            xi = parameters[parameters.size() - 1];
+        }
+        return std::make_shared<LIBORCovarianceModelStochasticHestonVolatility>(
+            getCloneWithModifiedParameters(*w->getBaseCovarianceModel(), base), w->getBrownianMotion(), kappa, theta, xi,
+            w->isCalibrateable());
+    }
     if (auto* w = dynamic_cast<const LIBORCovarianceModelStochasticVolatility*>(&cov)) {   // :123-160
         std::vector<double> base = parameters;
         double nu = w->getNu(), rho = w->getRho();

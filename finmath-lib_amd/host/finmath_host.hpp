@@ -175,6 +175,11 @@ public:
                                       std::shared_ptr<const BrownianMotionFromMersenneRandomNumbers>& brownianMotion) const {
         return false;
     }
+    // Heston variance scaling of a wrapper: its parameters and Brownian motion. False without.
+    virtual bool hestonVolatility(double& kappa, double& theta, double& xi,
+                                  std::shared_ptr<const BrownianMotionFromMersenneRandomNumbers>& brownianMotion) const {
+        return false;
+    }
 protected:
     // round-down index of a simulation time on the covariance model's grid
     // (AbstractLIBORCovarianceModel.getFactorLoading(double time, ...), :54-60)
@@ -291,6 +296,36 @@ private:
     CovarianceModelPtr base_;
     BrownianMotionPtr bm_;
     double nu_, rho_;
+    bool calibrateable_;
+};
+
+// Factor loadings of the base model (plain or with a local-volatility wrapper) times
+// sqrt(max(V(t), 0)), V a square-root variance process from 1 with drift (V - theta)(-kappa)
+// and loading sqrt(max(V, 0)) xi on the first factor of the given Brownian motion
+// (covariance/LIBORCovarianceModelStochasticHestonVolatility.java:60-82,172-256); the parameters
+// are the base model's, followed by kappa, theta, xi when calibrateable (:84-164).
+class LIBORCovarianceModelStochasticHestonVolatility : public LIBORCovarianceModelFromVolatilityAndCorrelation {
+public:
+    LIBORCovarianceModelStochasticHestonVolatility(CovarianceModelPtr covarianceModel, BrownianMotionPtr brownianMotion,
+                                                   double kappa, double theta, double xi, bool isCalibrateable);
+    CovarianceModelPtr getBaseCovarianceModel() const { return base_; }
+    BrownianMotionPtr getBrownianMotion() const { return bm_; }
+    double getKappa() const { return kappa_; }
+    double getTheta() const { return theta_; }
+    double getXi() const { return xi_; }
+    bool isCalibrateable() const { return calibrateable_; }
+    bool localVolatility(const TimeDiscretizationFromArray& simulationTimes, double& alpha,
+                         std::vector<double>& offsets) const override {
+        return base_->localVolatility(simulationTimes, alpha, offsets);
+    }
+    bool hestonVolatility(double& kappa, double& theta, double& xi, BrownianMotionPtr& brownianMotion) const override {
+        kappa = kappa_; theta = theta_; xi = xi_; brownianMotion = bm_;
+        return true;
+    }
+private:
+    CovarianceModelPtr base_;
+    BrownianMotionPtr bm_;
+    double kappa_, theta_, xi_;
     bool calibrateable_;
 };
 

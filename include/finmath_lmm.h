@@ -163,6 +163,16 @@ int fm_lmm_set_local_volatility(fm_lmm* ctx, double alpha, const double* offsets
 int fm_lmm_set_stochastic_volatility(fm_lmm* ctx, double nu, double rho, fm_lmm* driver, int32_t factor_a,
                                      int32_t factor_b);
 
+/* Heston scaling of the covariance model: every factor loading at simulation time index i is
+ * multiplied (after a local-volatility factor) by sqrt(max(V(t_i), 0)) of the variance process
+ *   V(t_0) = 1,  V(t_{i+1}) = V(t_i) + (V(t_i) - theta)(-kappa) dt_i + sqrt(max(V(t_i), 0)) xi dB,
+ * with dB the increments of factor `factor` of the driver context's Brownian motion (the same
+ * conditions as fm_lmm_set_stochastic_volatility, whose scaling this replaces; driver == NULL
+ * removes either).
+ * Replaces LIBORCovarianceModelStochasticHestonVolatility.getFactorLoading and its variance
+ * process (models/covariance/LIBORCovarianceModelStochasticHestonVolatility.java:172-256). */
+int fm_lmm_set_heston_volatility(fm_lmm* ctx, double kappa, double theta, double xi, fm_lmm* driver, int32_t factor);
+
 /* Discount factors P^d(T_k), k <= n, of the model's discount curve, or NULL for a
  * model without discount curve (no numeraire adjustment).
  * Replaces the deterministic numeraire adjustment of

@@ -437,6 +437,7 @@ static RV expv(const RV& a) { return map1(a, [](double x) { return std::exp(x); 
 static RV logv(const RV& a) { return map1(a, [](double x) { return std::log(x); }); }
 static RV invert(const RV& a) { return map1(a, [](double x) { return 1.0 / x; }); }
 static RV powv(const RV& a, double e) { return map1(a, [e](double x) { return std::pow(x, e); }); }
+static RV sqrtv(const RV& a) { return map1(a, [](double x) { return std::sqrt(x); }); }
 
 // binary ops
 static RV add(const RV& a, const RV& b);
@@ -565,6 +566,8 @@ struct Model {
     double sv_nu = 0.0, sv_rho = 0.0;
     int32_t sv_seed = 0;
     int sv_F = 2, sv_a = 0, sv_b = 1;
+    int sv_kind = 0;                              // 0: log-normal scaling, 1: Heston variance
+    double sv_kappa = 0.0, sv_theta = 0.0, sv_xi = 0.0;
     std::vector<RV> svs;                          // S(t_i), i < nT
     std::vector<std::vector<RV>> proc;            // [nT+1][n]
     std::vector<std::vector<RV>> dW;              // [nT][F]
@@ -682,8 +685,32 @@ struct Model {
         }
     }
 
+    // The variance process of LIBORCovarianceModelStochasticHestonVolatility (:172-256): V(0) = 1,
+    // identity transform, drift (V - theta)(-kappa), loading sqrt(max(V, 0)) xi on factor a;
+    // the loadings are scaled by sqrt(max(V, 0)).
+    void simulate_heston_scaling() {
+        std::vector<double> bm((size_t)nT * sv_F * P);
+        orc_brownian(times.data(), nT, sv_F, P, sv_seed, bm.data());
+        const RV kappa = scalar(sv_kappa), theta = scalar(sv_theta), xi = scalar(sv_xi);
+        svs.assign(nT, RV());
+        RV V = scalar(1.0);
+        for (int t = 0; t < nT; t++) {
+            svs[t] = sqrtv(floor_(V, 0.0));
+            const RV drift = mult(sub(V, theta), mult(kappa, -1.0));
+            const RV load = mult(sqrtv(floor_(V, 0.0)), xi);
+            std::vector<double> v(bm.begin() + ((size_t)t * sv_F + sv_a) * P, bm.begin() + ((size_t)t * sv_F + sv_a + 1) * P);
+            RV X = V;
+            X = addProduct(X, drift, times[t + 1] - times[t]);
+            X = addProduct(X, load, rvarr(std::move(v)));
+            V = X;
+        }
+    }
+
     void simulate(int32_t seed, int threads) {
-        if (has_sv) simulate_scaling();
+        if (has_sv) {
+            if (sv_kind == 1) simulate_heston_scaling();
+            else simulate_scaling();
+        }
         // Brownian increments (BrownianMotionFromMersenneRandomNumbers.java:141-191)
         std::vector<double> bm((size_t)nT * F * P);
         orc_brownian(times.data(), nT, F, P, seed, bm.data());
@@ -937,6 +964,8 @@ typedef struct orc_lmm_spec {
     double sv_nu, sv_rho;
     int32_t sv_seed;                   // its Brownian motion: seed, factors, the two factors used
     int sv_F, sv_a, sv_b;
+    int sv_kind;                       // 0: log-normal scaling (nu, rho), 1: Heston (kappa, theta, xi)
+    double sv_kappa, sv_theta, sv_xi;
 } orc_lmm_spec;
 
 static std::string g_err;
@@ -963,6 +992,7 @@ void* orc_lmm_create(const orc_lmm_spec* s) {
         m->has_sv = s->sv_on != 0;
         m->sv_nu = s->sv_nu; m->sv_rho = s->sv_rho; m->sv_seed = s->sv_seed;
         m->sv_F = s->sv_F; m->sv_a = s->sv_a; m->sv_b = s->sv_b;
+        m->sv_kind = s->sv_kind; m->sv_kappa = s->sv_kappa; m->sv_theta = s->sv_theta; m->sv_xi = s->sv_xi;
         m->has_lv = s->lv_offset != nullptr;
         if (m->has_lv) {
             m->lv_alpha = s->lv_alpha;

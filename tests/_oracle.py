@@ -158,6 +158,7 @@ class Spec(ctypes.Structure):
         ("interpolation", c_int),
         ("sv_on", c_int), ("sv_nu", c_double), ("sv_rho", c_double), ("sv_seed", c_int32),
         ("sv_F", c_int), ("sv_a", c_int), ("sv_b", c_int),
+        ("sv_kind", c_int), ("sv_kappa", c_double), ("sv_theta", c_double), ("sv_xi", c_double),
     ]
 
 
@@ -186,10 +187,12 @@ class ModelTables:
 
 class OracleModel:
     def __init__(self, tables: ModelTables, num_paths=None, *, cap=1e5, scheme=2, measure=0, state_space=1,
-                 with_discount=True, threads=8, seed=None, local_vol=None, interpolation=1, stochastic_vol=None):
+                 with_discount=True, threads=8, seed=None, local_vol=None, interpolation=1, stochastic_vol=None,
+                 heston_vol=None):
         """local_vol: (alpha, offsets) of a local-volatility wrapper, offsets [nT][n] or a scalar;
         stochastic_vol: (nu, rho, seed, F, a, b) of a stochastic volatility scaling driven by
-        factors a, b of an F-factor Brownian motion with that seed."""
+        factors a, b of an F-factor Brownian motion with that seed; heston_vol: (kappa, theta, xi,
+        seed, F, a) of a Heston variance scaling driven by factor a."""
         wl = tables.wl
         self.t = tables
         self.P = int(num_paths or wl.num_paths)
@@ -224,9 +227,13 @@ class OracleModel:
         s.fc_t, s.fc_f = _dp(self._keep[-2]), _dp(self._keep[-1])
         s.fc_offset = float(wl.period)
         s.interpolation = interpolation          # LINEAR 0, LOG_LINEAR_UNCORRECTED 1, LOG_LINEAR_CORRECTED 2
+        s.sv_kind = 0
         if stochastic_vol is not None:
             s.sv_on = 1
             s.sv_nu, s.sv_rho, s.sv_seed, s.sv_F, s.sv_a, s.sv_b = stochastic_vol
+        elif heston_vol is not None:
+            s.sv_on, s.sv_kind, s.sv_b = 1, 1, 0
+            s.sv_kappa, s.sv_theta, s.sv_xi, s.sv_seed, s.sv_F, s.sv_a = heston_vol
         else:
             s.sv_on, s.sv_F, s.sv_a, s.sv_b = 0, 2, 0, 1
         self.spec = s

@@ -239,3 +239,83 @@ def test_calibration_with_a_stochastic_volatility_wrapper():
     assert isinstance(cov_cal, H.LIBORCovarianceModelStochasticVolatility)
     assert len(info["parameters"]) == 6
     assert info["rootMeanSquaredError"] < 0.01 * start
+
+
+@pytest.mark.parametrize("measure,scheme,state_space,blend,shared", [
+    ("SPOT", "EULER_FUNCTIONAL", "NORMAL", True, None),
+    ("TERMINAL", "PREDICTOR_CORRECTOR", "LOGNORMAL", False, None),
+    ("SPOT", "EULER", "LOGNORMAL", False, 2),
+])
+def test_heston_process_and_numeraire_match_oracle(measure, scheme, state_space, blend, shared):
+    # LIBORCovarianceModelStochasticHestonVolatility; xi = 1.2 lets V reach zero on many paths
+    wl = W.c0()
+    t = tables(wl, state_space, blend)
+    P = 4001
+    kappa, theta, xi = 0.8, 0.9, 1.2
+    say(f"case heston process c0 P={P} {measure} {scheme} {state_space} blend={blend} shared={shared}")
+    ctx = _abi.LmmContext(t.times, t.tenor, wl.num_factors, P, wl.seed, measure=measure, scheme=scheme,
+                          state_space=state_space, store_paths=True)
+    ctx.set_initial_forwards(t.L0)
+    ctx.set_covariance(t.lam, t.var)
+    ctx.set_discount_curve(t.pd)
+    lv = None
+    if blend:
+        lv = (0.0, np.zeros((wl.num_time_steps, wl.num_libors)))
+        ctx.set_local_volatility(*lv)
+    if shared is None:
+        driver = _abi.LmmContext(t.times, t.tenor, 1, P, DRIVER_SEED)
+        ctx.set_heston_volatility(kappa, theta, xi, driver, 0)
+        hv = (kappa, theta, xi, DRIVER_SEED, 1, 0)
+    else:
+        ctx.set_heston_volatility(kappa, theta, xi, ctx, shared)
+        hv = (kappa, theta, xi, wl.seed, wl.num_factors, shared)
+    ctx.simulate()
+    om = O.OracleModel(t, P, measure=MEASURES.index(measure), scheme=SCHEMES.index(scheme),
+                       state_space=1 if state_space == "LOGNORMAL" else 0, local_vol=lv, heston_vol=hv)
+    worst = 0.0
+    for ti in range(wl.num_time_steps + 1):
+        for j in range(wl.num_libors):
+            g, r = ctx.process_value(ti, j), om.process_value(ti, j)
+            worst = max(worst, float(np.max(np.abs(g - r) / np.maximum(np.abs(r), 1e-3))))
+    say(f"  worst relative process difference {worst:.3e}")
+    assert worst < 1e-12
+    for k in range(wl.num_libors + 1):
+        g, r = ctx.numeraire(t.tenor[k]), om.numeraire(t.tenor[k])
+        assert np.max(np.abs(g - r) / np.abs(r)) < 1e-12, k
+    prods = to_products(wl.products)
+    vals, _ = ctx.value_products(prods)
+    for q, s in enumerate(wl.products):
+        rv, _ = om.value(s)
+        assert abs(vals[q] - rv) <= 1e-10 * abs(rv)
+
+
+def test_heston_host_mirror_runs_the_same_simulation():
+    from finmath_hip import _finmath_host as H
+    wl = W.c0()
+    P = 3000
+    say(f"case heston host mirror c0 P={P}")
+    td = H.TimeDiscretizationFromArray(0.0, wl.num_time_steps, wl.period)
+    tenor = H.TimeDiscretizationFromArray(0.0, wl.num_libors, wl.period)
+    fc = H.ForwardCurveInterpolation.createForwardCurveFromForwards(
+        "forwardCurve", list(tenor.getAsDoubleArray()[:-1]), list(wl.forwards), wl.period)
+    vol = H.LIBORVolatilityModelFourParameterExponentialForm(td, tenor, 0.2, 0.0, 0.25, 0.1, False)
+    corr = H.LIBORCorrelationModelExponentialDecay(td, tenor, wl.num_factors, wl.corr_decay)
+    cov = H.LIBORCovarianceModelFromVolatilityAndCorrelation(td, tenor, vol, corr)
+    bm1 = H.BrownianMotionFromMersenneRandomNumbers(td, 1, P, DRIVER_SEED)
+    wrapped = H.LIBORCovarianceModelStochasticHestonVolatility(cov, bm1, 0.8, 0.9, 0.5, False)
+    model = H.LIBORMarketModelFromCovarianceModel.of(tenor, None, fc, H.DiscountCurveFromForwardCurve(fc), None,
+                                                     wrapped, [], {"measure": "SPOT", "stateSpace": "LOGNORMAL"})
+    bm = H.BrownianMotionFromMersenneRandomNumbers(td, wl.num_factors, P, wl.seed)
+    sim = H.LIBORMonteCarloSimulationFromLIBORModel(H.EulerSchemeFromProcessModel(model, bm))
+    s = wl.products[0]
+    v = sim.getValues([H.Swaption(s["exercise"], list(s["fixing"]), list(s["payment"]), list(s["strike"]))])[0]["value"]
+    lam, var = cov.tables(td)
+    ctx = _abi.LmmContext(td.getAsDoubleArray(), tenor.getAsDoubleArray(), wl.num_factors, P, wl.seed)
+    ctx.set_initial_forwards(model.getInitialForwards())
+    ctx.set_covariance(lam, var)
+    dc = model.getDiscountCurve()
+    ctx.set_discount_curve([dc.getDiscountFactor(T) for T in tenor.getAsDoubleArray()])
+    driver = _abi.LmmContext(td.getAsDoubleArray(), tenor.getAsDoubleArray(), 1, P, DRIVER_SEED)
+    ctx.set_heston_volatility(0.8, 0.9, 0.5, driver, 0)
+    ref, _ = ctx.value_products([_abi.Product.swaption(s["exercise"], s["fixing"], s["payment"], s["strike"])])
+    assert v == ref[0]

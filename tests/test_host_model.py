@@ -173,3 +173,21 @@ def test_stochastic_volatility_wrapper_parameters_and_nesting():
         H.LIBORCovarianceModelStochasticVolatility(sv, bm, 0.01, 0.0, False)          # no nesting
     with pytest.raises(Exception):
         H.DisplacedLocalVolatilityModel(sv, 1.0, False)                               # local vol goes inside
+
+
+def test_heston_wrapper_parameters():
+    wl = W.c0()
+    td, tenor, fc, _, corr, _, model = build_model(wl)
+    vol = H.LIBORVolatilityModelFourParameterExponentialForm(td, tenor, 0.2, 0.0, 0.25, 0.1, True)
+    cov = H.LIBORCovarianceModelFromVolatilityAndCorrelation(td, tenor, vol, corr)
+    bm = H.BrownianMotionFromMersenneRandomNumbers(td, 1, 1000, 7)
+    hv = H.LIBORCovarianceModelStochasticHestonVolatility(cov, bm, 0.5, 0.8, 0.3, True)
+    assert list(hv.getParameter()) == [0.2, 0.0, 0.25, 0.1, 0.5, 0.8, 0.3]
+    c = hv.getCloneWithModifiedParameters([0.3, 0.01, 0.2, 0.05, 1.0, 0.9, 0.4])
+    assert isinstance(c, H.LIBORCovarianceModelStochasticHestonVolatility)
+    assert (c.getKappa(), c.getTheta(), c.getXi()) == (1.0, 0.9, 0.4)
+    assert list(H.LIBORCovarianceModelStochasticHestonVolatility(cov, bm, 0.5, 0.8, 0.3, False).getParameter()) == \
+        [0.2, 0.0, 0.25, 0.1]
+    with pytest.raises(Exception):
+        H.LIBORCovarianceModelStochasticVolatility(hv, H.BrownianMotionFromMersenneRandomNumbers(td, 2, 1000, 7),
+                                                   0.1, 0.0, False)                    # no nesting

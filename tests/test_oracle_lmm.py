@@ -199,6 +199,49 @@ def test_stochastic_volatility_scaling_is_a_martingale_and_vanishes_without_vol_
             assert np.allclose(a.process_value(i, j), b.process_value(i, j), rtol=1e-14, atol=0)
 
 
+def heston_scaling(times, P, kappa, theta, xi, seed, F=1, a=0):
+    """sqrt(max(V, 0)) of the Euler scheme of dV = kappa (theta - V) dt + xi sqrt(max(V, 0)) dB,
+    V(0) = 1, on factor a of an F-factor Brownian motion, [nT][P]."""
+    dB = O.brownian(times, F, P, seed)
+    V = np.ones(P)
+    out = np.empty((times.size - 1, P))
+    for i in range(times.size - 1):
+        root = np.sqrt(np.maximum(V, 0.0))
+        out[i] = root
+        V = V + kappa * (theta - V) * (times[i + 1] - times[i]) + xi * root * dB[i, a]
+    return out
+
+
+@pytest.mark.parametrize("measure,scheme,state_space", [(0, 2, 0), (1, 1, 1)])
+def test_heston_volatility_against_brute_force(measure, scheme, state_space):
+    # LIBORCovarianceModelStochasticHestonVolatility: loadings times sqrt(max(V, 0)) of a
+    # square-root variance process on its own Brownian motion; xi = 1.5 makes V hit zero
+    wl = tiny_workload()
+    t = O.ModelTables(wl, vol_params=(0.2, 0.0, 0.25, 0.1) if state_space == 1 else (0.006, 0.0, 0.25, 0.002))
+    P = 8
+    hv = (0.8, 0.9, 1.5, 4242, 2, 1)
+    scaling = heston_scaling(t.times, P, 0.8, 0.9, 1.5, 4242, F=2, a=1)
+    assert np.min(scaling) == 0.0 and np.ptp(scaling) > 0.5
+    om = O.OracleModel(t, P, measure=measure, scheme=scheme, state_space=state_space, heston_vol=hv)
+    ref = brute_force_paths(t, P, measure, scheme, state_space, scaling=scaling)
+    for i in range(wl.num_time_steps + 1):
+        for j in range(wl.num_libors):
+            assert np.allclose(om.process_value(i, j), ref[i, j], rtol=1e-12, atol=1e-16), (i, j)
+
+
+def test_heston_variance_mean_reverts_like_its_expectation():
+    # E[V(t_{i+1})] = E[V(t_i)] + kappa (theta - E[V(t_i)]) dt for the Euler scheme (the increment
+    # has mean zero), so E[V] = theta + (1 - theta)(1 - kappa dt)^i
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    P = 40_000
+    S = heston_scaling(t.times, P, 0.5, 0.6, 0.3, 17)
+    for i in (4, 10, 19):
+        expect = 0.6 + 0.4 * (1 - 0.5 * 0.5) ** i
+        V = S[i] ** 2
+        assert abs(np.mean(V) - expect) < 4 * np.std(V) / math.sqrt(P) + 1e-3, i
+
+
 def test_full_blend_with_unit_offset_is_the_plain_model():
     # blend a = 1 with offset 1: f = (L - L) + 1 = 1 exactly, so in the normal state space
     # (no variance term, which the wrapper sums from the loadings) the paths are the plain
