@@ -32,6 +32,8 @@
 namespace fm {
 
 constexpr int kStepThreads = 256;
+// scalar-cache prefetch of each step's records in the spot-measure Euler kernels
+constexpr bool kPrefetchRecords = true;
 
 __device__ __forceinline__ double wave_reduce_sum(double v) {
 #pragma unroll
@@ -148,6 +150,19 @@ __device__ __forceinline__ void euler_component(real (&L)[NCAP], real (&S)[FCAP]
     }
 }
 
+// Warms the scalar data cache with the records of step i's alive components: one scalar load
+// per 64-byte line into a scratch SGPR, then one wait, so that the component loop's record
+// loads hit the cache instead of each waiting on L2 in turn.
+template <int FCAP, typename real>
+__device__ __forceinline__ void prefetch_step_records(const real* __restrict__ rec, int n, int i, int first) {
+    constexpr int rs = Rec<FCAP, real>::size;
+    const uintptr_t b = (uintptr_t)(rec + ((size_t)i * n + first) * rs) & ~(uintptr_t)63;
+    const uintptr_t e = (uintptr_t)(rec + ((size_t)i * n + n) * rs);
+    uint32_t sink = 0;
+    for (uintptr_t a = b; a < e; a += 64) asm volatile("s_load_dword %0, %1, 0x0" : "+s"(sink) : "s"(a));
+    asm volatile("s_waitcnt lgkmcnt(0)" : "+s"(sink) : : "memory");
+}
+
 // One Euler step t_i -> t_{i+1} of all alive components, in place. Shared by the
 // pricing pass and the adjoint recompute; both group sizes give bit-identical states.
 // GROUP 1: a uniform branch per component. GROUP 4: components in groups of four
@@ -155,12 +170,13 @@ __device__ __forceinline__ void euler_component(real (&L)[NCAP], real (&S)[FCAP]
 // components overlap; a component outside [first, n) of a partly alive group is
 // computed and discarded (w = 0 adds exact zeros to S).
 // LOGSTATE: the array holds the log-forwards (see euler_component).
-template <int NCAP, int FCAP, typename real, bool LOGSTATE = false, int GROUP = 1>
+template <int NCAP, int FCAP, typename real, bool LOGSTATE = false, int GROUP = 1, bool PREFETCH = false>
 __device__ __forceinline__ void euler_step(real (&L)[NCAP], int n, int i, int first, real dt, const real (&dW)[FCAP],
                                            const real* __restrict__ rec, bool lognormal, bool capped, real cap,
                                            real logcap, real* __restrict__ paths,
                                            const int32_t* __restrict__ row_next, int64_t p, int P, bool active) {
     static_assert(NCAP % GROUP == 0, "component capacity must be a multiple of the group size");
+    if (PREFETCH && first < n) prefetch_step_records<FCAP, real>(rec, n, i, first);
     real S[FCAP];
 #pragma unroll
     for (int k = 0; k < FCAP; k++) S[k] = (real)0;
@@ -500,7 +516,7 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
             else general_step<NCAP, FCAP, real, false, PC, LV>(L, MU, n, ti, first_alive[ti], dt, dW, rec, var,
                                                                paths, row_next, p, P, active);
         } else {
-            euler_step<NCAP, FCAP, real, LOGSTATE>(L, n, ti, first_alive[ti], dt, dW, rec, lognormal, capped, cap,
+            euler_step<NCAP, FCAP, real, LOGSTATE, 1, kPrefetchRecords>(L, n, ti, first_alive[ti], dt, dW, rec, lognormal, capped, cap,
                                                    logcap, paths, row_next, p, P, active);
         }
     }
