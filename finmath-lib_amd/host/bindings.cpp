@@ -298,6 +298,17 @@ PYBIND11_MODULE(_finmath_host, m) {
         .def(py::init([](double ex, const TimeDiscretizationFromArray& tenor, double k) {
             return std::make_shared<Swaption>(ex, tenor, k);
         }), py::arg("exerciseDate"), py::arg("swapTenor"), py::arg("swaprate"));
+    py::class_<SwaptionSimple, Product, std::shared_ptr<SwaptionSimple>>(m, "SwaptionSimple")
+        .def(py::init<double, std::vector<double>, const std::string&>(), py::arg("swaprate"), py::arg("swapTenor"),
+             py::arg("valueUnit") = "VALUE");
+    m.def("blackScholesGeneralizedOptionValue", &AnalyticFormulas::blackScholesGeneralizedOptionValue);
+    m.def("bachelierOptionValue", &AnalyticFormulas::bachelierOptionValue);
+    m.def("blackScholesOptionImpliedVolatility", &AnalyticFormulas::blackScholesOptionImpliedVolatility);
+    m.def("bachelierOptionImpliedVolatility", &AnalyticFormulas::bachelierOptionImpliedVolatility);
+    m.def("getSwapAnnuity", [](std::vector<double> tenor, std::shared_ptr<ForwardCurveInterpolation> fc) {
+        return getSwapAnnuity(tenor, *fc);
+    });
+
     py::class_<SimpleSwap, Product, std::shared_ptr<SimpleSwap>>(m, "SimpleSwap")
         .def(py::init([](std::vector<double> fix, std::vector<double> pay, std::vector<double> k, bool payFix,
                          std::vector<double> notional) {

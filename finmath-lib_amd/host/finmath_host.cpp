@@ -696,7 +696,10 @@ std::vector<std::pair<double, double>> LIBORMonteCarloSimulationFromLIBORModel::
     std::vector<double> v, e;
     process_->valueProducts(products, v, e);
     std::vector<std::pair<double, double>> out;
-    for (size_t q = 0; q < v.size(); q++) out.emplace_back(v[q], e[q]);
+    for (size_t q = 0; q < v.size(); q++) {
+        if (products[q]->isValueUnitValue()) out.emplace_back(v[q], e[q]);
+        else out.emplace_back(products[q]->toValueUnit(v[q], *process_->getModel()), 0.0);   // a constant
+    }
     return out;
 }
 
@@ -781,6 +784,121 @@ void Caplet::lower(std::deque<std::vector<double>>& arena, fm_product& o) const 
     o.notional = daycountFraction;
     o.is_floorlet = isFloorlet ? 1 : 0;
 }
+
+// ---- SwaptionSimple and the analytic formulas -------------------------------------------------
+SwaptionSimple::SwaptionSimple(double k, std::vector<double> tenor, const std::string& valueUnit)
+    : swaprate(k), swapTenor(std::move(tenor)) {
+    if (swapTenor.size() < 2) throw IllegalArgumentException("the swap tenor needs at least two dates");
+    static const std::map<std::string, int> units = {
+        {"VALUE", 0}, {"VOLATILITY", 1}, {"VOLATILITYLOGNORMAL", 1}, {"VOLATILITYNORMAL", 2},
+        {"INTEGRATEDVARIANCE", 3}, {"INTEGRATEDVARIANCELOGNORMAL", 3}, {"INTEGRATEDLOGNORMALVARIANCE", 3},
+        {"INTEGRATEDVARIANCENORMAL", 4}, {"INTEGRATEDNORMALVARIANCE", 4}};
+    const auto it = units.find(valueUnit);
+    if (it == units.end()) throw UnsupportedOperationException("Provided valueUnit not implemented.");
+    unit_ = it->second;
+}
+
+void SwaptionSimple::lower(std::deque<std::vector<double>>& arena, fm_product& o) const {
+    // the underlying Swaption(exercise T_0, fixings T_0..T_{n-1}, payments T_1..T_n, one rate) (:49-60)
+    std::memset(&o, 0, sizeof(o));
+    const size_t n = swapTenor.size() - 1;
+    o.type = FM_PRODUCT_SWAPTION;
+    o.num_periods = (int32_t)n;
+    o.exercise_date = swapTenor[0];
+    o.fixing_dates = keep(arena, std::vector<double>(swapTenor.begin(), swapTenor.end() - 1));
+    o.payment_dates = keep(arena, std::vector<double>(swapTenor.begin() + 1, swapTenor.end()));
+    o.swaprates = keep(arena, std::vector<double>(n, swaprate));
+    o.notional = 1.0;
+}
+
+double SwaptionSimple::toValueUnit(double value, const LIBORMarketModelFromCovarianceModel& model) const {
+    if (unit_ == 0) return value;
+    const ForwardCurvePtr fc = model.getForwardRateCurve();
+    if (!fc) throw IllegalArgumentException("volatility units need the model's forward curve");
+    const TimeDiscretizationFromArray t(swapTenor);
+    const double par = getForwardSwapRate(t, t, *fc);
+    const double T = swapTenor[0];
+    const double annuity = getSwapAnnuity(swapTenor, *fc);
+    const bool lognormal = unit_ == 1 || unit_ == 3;
+    const double vol = lognormal ? AnalyticFormulas::blackScholesOptionImpliedVolatility(par, T, swaprate, annuity, value)
+                                 : AnalyticFormulas::bachelierOptionImpliedVolatility(par, T, swaprate, annuity, value);
+    return unit_ <= 2 ? vol : vol * vol * T;
+}
+
+double getSwapAnnuity(const std::vector<double>& tenor, const ForwardCurveInterpolation& fc) {
+    auto fcp = std::shared_ptr<const ForwardCurveInterpolation>(&fc, [](const ForwardCurveInterpolation*) {});
+    DiscountCurveFromForwardCurve dc(fcp);
+    double value = 0.0;
+    for (size_t i = 0; i + 1 < tenor.size(); i++) {
+        const double pay = tenor[i + 1];
+        if (pay <= 0.0) continue;
+        value += (tenor[i + 1] - tenor[i]) * dc.getDiscountFactor(pay);
+    }
+    return value / dc.getDiscountFactor(0.0);
+}
+
+namespace AnalyticFormulas {
+namespace {
+double Phi(double x) { return 0.5 * std::erfc(-x / std::sqrt(2.0)); }
+double phi(double x) { return std::exp(-0.5 * x * x) / std::sqrt(2.0 * M_PI); }
+
+// root of the increasing function value(vol) = target on [lo, hi]: Newton steps with the
+// derivative, bisection whenever a step leaves the bracket, to a relative width of 1e-15
+template <class F>
+double invert(F value_and_vega, double target, double lo, double hi) {
+    double v = 0.5 * (lo + hi);
+    for (int it = 0; it < 500; it++) {
+        double val, vega;
+        value_and_vega(v, val, vega);
+        const double err = val - target;
+        if (err == 0.0) return v;
+        if (err > 0) hi = v; else lo = v;
+        double next = (vega > 0.0) ? v - err / vega : 0.5 * (lo + hi);
+        if (!(next > lo && next < hi)) next = 0.5 * (lo + hi);
+        if (std::fabs(next - v) <= 1e-15 * std::max(v, 1e-300) || hi - lo <= 1e-15 * std::max(hi, 1e-300)) return next;
+        v = next;
+    }
+    return v;
+}
+} // namespace
+
+double blackScholesGeneralizedOptionValue(double F, double vol, double T, double K, double pu) {
+    if (T < 0) return 0.0;
+    if (F <= 0 || K <= 0 || vol <= 0 || T == 0) return std::max(F - K, 0.0) * pu;
+    const double sd = vol * std::sqrt(T);
+    const double dp = (std::log(F / K) + 0.5 * vol * vol * T) / sd, dm = dp - sd;
+    return (F * Phi(dp) - K * Phi(dm)) * pu;
+}
+
+double bachelierOptionValue(double F, double vol, double T, double K, double pu) {
+    if (T < 0) return 0.0;
+    if (vol <= 0 || T == 0) return std::max(F - K, 0.0) * pu;
+    const double sd = vol * std::sqrt(T), d = (F - K) / sd;
+    return ((F - K) * Phi(d) + sd * phi(d)) * pu;
+}
+
+double blackScholesOptionImpliedVolatility(double F, double T, double K, double pu, double value) {
+    if (K <= 0.0) return 0.0;
+    if (!(value > std::max(F - K, 0.0) * pu) || !(value < F * pu) || T <= 0) return NAN;
+    double hi = 1.0;
+    while (blackScholesGeneralizedOptionValue(F, hi, T, K, pu) < value && hi < 1e6) hi *= 2.0;
+    return invert([&](double v, double& val, double& vega) {
+        val = blackScholesGeneralizedOptionValue(F, v, T, K, pu);
+        const double dp = (std::log(F / K) + 0.5 * v * v * T) / (v * std::sqrt(T));
+        vega = F * std::sqrt(T) * phi(dp) * pu;
+    }, value, 0.0, hi);
+}
+
+double bachelierOptionImpliedVolatility(double F, double T, double K, double pu, double value) {
+    if (!(value > std::max(F - K, 0.0) * pu) || T <= 0) return NAN;
+    double hi = 0.01;
+    while (bachelierOptionValue(F, hi, T, K, pu) < value && hi < 1e6) hi *= 2.0;
+    return invert([&](double v, double& val, double& vega) {
+        val = bachelierOptionValue(F, v, T, K, pu);
+        vega = std::sqrt(T) * phi((F - K) / (v * std::sqrt(T))) * pu;
+    }, value, 0.0, hi);
+}
+} // namespace AnalyticFormulas
 
 BermudanSwaption::BermudanSwaption(std::vector<bool> exercise, std::vector<double> fix, std::vector<double> pl,
                                    std::vector<double> pay, std::vector<double> notionals, std::vector<double> k,
@@ -1095,7 +1213,7 @@ CalibrationResult getCloneCalibrated(const LIBORCovarianceModelFromVolatilityAnd
             check(fm_lmm_value_products(ctx, lowered.data(), (int32_t)lowered.size(), vals.data(), nullptr));
             valuations++;
             for (size_t q = 0; q < products.size(); q++)
-                out[q] = (vals[q] - products[q].targetValue) * products[q].weight;
+                out[q] = (products[q].product->toValueUnit(vals[q], *model) - products[q].targetValue) * products[q].weight;
         } catch (const CalculationException&) {
             // a failed valuation (e.g. a non-finite price) is a rejected point
         } catch (const IllegalArgumentException&) {

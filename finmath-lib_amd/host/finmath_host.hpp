@@ -439,9 +439,14 @@ private:
 using SimulationPtr = std::shared_ptr<LIBORMonteCarloSimulationFromLIBORModel>;
 
 // ---- montecarlo/interestrate/products -------------------------------------------------
+class LIBORMarketModelFromCovarianceModel;
+
 class Product : public std::enable_shared_from_this<Product> {
 public:
     virtual ~Product() = default;
+    // The product's value unit applied to the average value of the pass (identity: VALUE).
+    virtual double toValueUnit(double value, const LIBORMarketModelFromCovarianceModel&) const { return value; }
+    virtual bool isValueUnitValue() const { return true; }
     // AbstractMonteCarloProduct.getValue(model) = getValue(0, model).getAverage() (AbstractMonteCarloProduct.java:82-84)
     double getValue(LIBORMonteCarloSimulationFromLIBORModel& model) const;
     // AbstractMonteCarloProduct.getValues: {"value", "error"} (AbstractMonteCarloProduct.java:87-104)
@@ -470,6 +475,45 @@ public:
     std::vector<double> fixingDates, paymentDates, swaprates, notional;
     bool isPayFix;
 };
+
+// SwaptionSimple (montecarlo/interestrate/products/SwaptionSimple.java:38-111): a payer swaption on
+// the tenor {T_0, ..., T_n} at one swap rate, exercised at T_0, valued in a value unit: VALUE, or
+// the implied volatility of the Monte Carlo price (VOLATILITYLOGNORMAL / VOLATILITY,
+// VOLATILITYNORMAL) or its integrated variance (INTEGRATEDVARIANCELOGNORMAL, ...NORMAL), with the
+// par swap rate and annuity of the model's forward curve (:68-104). A value in a volatility unit
+// is a constant: its error is 0.
+class SwaptionSimple : public Product {
+public:
+    SwaptionSimple(double swaprate, std::vector<double> swapTenor, const std::string& valueUnit = "VALUE");
+    void lower(std::deque<std::vector<double>>& arena, fm_product& out) const override;
+    double toValueUnit(double value, const LIBORMarketModelFromCovarianceModel& model) const override;
+    bool isValueUnitValue() const override { return unit_ == 0; }
+    double swaprate;
+    std::vector<double> swapTenor;
+private:
+    int unit_;       // 0 VALUE, 1 lognormal volatility, 2 normal volatility, 3, 4 the integrated variances
+};
+
+// AnalyticFormulas (functions/AnalyticFormulas.java): Black and Bachelier option values and their
+// implied volatilities, inverted by a Newton iteration safeguarded by bisection to full precision.
+namespace AnalyticFormulas {
+// (forward N(d+) - strike N(d-)) payoffUnit (:54-83); the intrinsic value for T <= 0 or vol <= 0
+double blackScholesGeneralizedOptionValue(double forward, double volatility, double optionMaturity, double optionStrike,
+                                          double payoffUnit);
+// ((forward - strike) N(d) + vol sqrt(T) phi(d)) payoffUnit, d = (forward - strike)/(vol sqrt(T)) (:1046-1076)
+double bachelierOptionValue(double forward, double volatility, double optionMaturity, double optionStrike,
+                            double payoffUnit);
+// blackScholesOptionImpliedVolatility (:667-713): 0 for a strike <= 0
+double blackScholesOptionImpliedVolatility(double forward, double optionMaturity, double optionStrike, double payoffUnit,
+                                           double optionValue);
+// bachelierOptionImpliedVolatility (:1102-1110)
+double bachelierOptionImpliedVolatility(double forward, double optionMaturity, double optionStrike, double payoffUnit,
+                                        double optionValue);
+} // namespace AnalyticFormulas
+
+// SwapAnnuity.getSwapAnnuity(tenor, forwardCurve) (marketdata/products/SwapAnnuity.java:70-129): sum of
+// period length times the forward curve's discount factor at the payment dates after 0.
+double getSwapAnnuity(const std::vector<double>& tenor, const ForwardCurveInterpolation& forwardCurve);
 
 // Caplet (montecarlo/interestrate/products/Caplet.java:115-140), value unit VALUE
 class Caplet : public Product {

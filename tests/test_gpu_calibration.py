@@ -117,3 +117,73 @@ def test_calibration_under_a_displaced_wrapper_in_the_normal_state_space():
     assert isinstance(cov_cal, H.DisplacedLocalVolatilityModel) and cov_cal.getDisplacement() == 4.0
     assert np.allclose(info["parameters"], true_vol, rtol=0, atol=2e-5)
     assert info["rootMeanSquaredError"] < 1e-8
+
+
+def simple_strip(fc, unit):
+    # ATM and out-of-the-money payers: an in-the-money Monte Carlo price can fall below the
+    # intrinsic value, where no implied volatility exists
+    products = []
+    for e in (1.0, 2.0, 3.0, 4.0, 5.0):
+        tenor = [e + 0.5 * i for i in range(11)]
+        t = H.TimeDiscretizationFromArray(tenor)
+        atm = H.getForwardSwapRate(t, t, fc)
+        for k in (0.0, 0.005, 0.01):
+            products.append(H.SwaptionSimple(atm + k, tenor, unit))
+    return products
+
+
+def test_swaption_simple_value_units():
+    # SwaptionSimple (SwaptionSimple.java:68-104): VALUE is the Swaption's price; the volatility
+    # units invert Black's and Bachelier's formulas at the curve's par rate and annuity
+    from scipy.optimize import brentq
+    from scipy.stats import norm
+    say("case SwaptionSimple value units")
+    td, tenor, fc, cov, model = build(TRUE)
+    bm = H.BrownianMotionFromMersenneRandomNumbers(td, 3, 5000, 31415)
+    sim = H.LIBORMonteCarloSimulationFromLIBORModel(H.EulerSchemeFromProcessModel(model, bm))
+    tenor_s = [2.0 + 0.5 * i for i in range(11)]
+    t = H.TimeDiscretizationFromArray(tenor_s)
+    par = H.getForwardSwapRate(t, t, fc)
+    K = par + 0.005
+    ref = sim.getValues([H.Swaption(2.0, tenor_s[:-1], tenor_s[1:], [K] * 10)])[0]["value"]
+    vals = sim.getValues([H.SwaptionSimple(K, tenor_s, u) for u in ("VALUE", "VOLATILITYLOGNORMAL", "VOLATILITYNORMAL",
+                                                                      "INTEGRATEDVARIANCELOGNORMAL")])
+    assert vals[0]["value"] == ref
+    A = H.getSwapAnnuity(tenor_s, fc)
+
+    def black(v):
+        sd = v * math.sqrt(2.0)
+        d1 = (math.log(par / K) + 0.5 * sd * sd) / sd
+        return (par * norm.cdf(d1) - K * norm.cdf(d1 - sd)) * A - ref
+
+    def bachelier(v):
+        sd = v * math.sqrt(2.0)
+        d = (par - K) / sd
+        return ((par - K) * norm.cdf(d) + sd * norm.pdf(d)) * A - ref
+
+    lv, nv = brentq(black, 1e-4, 5.0, xtol=1e-15), brentq(bachelier, 1e-6, 1.0, xtol=1e-16)
+    say(f"  price {ref:.6e} lognormal vol {vals[1]['value']:.10f} normal vol {vals[2]['value']:.10f}")
+    assert vals[1]["value"] == pytest.approx(lv, rel=1e-9) and vals[1]["error"] == 0.0
+    assert vals[2]["value"] == pytest.approx(nv, rel=1e-9)
+    assert vals[3]["value"] == pytest.approx(lv * lv * 2.0, rel=1e-9)
+
+
+import math  # noqa: E402
+
+
+def test_calibration_to_lognormal_volatilities():
+    # AbstractLIBORCovarianceModelParametric.getCloneCalibrated with SwaptionSimple products in
+    # VOLATILITYLOGNORMAL (the reference's calibration tests quote volatilities): targets are
+    # the implied volatilities of the known parameters on the same Brownian motion
+    say("case calibration to lognormal volatilities, 15 swaptions, 2000 paths")
+    td, tenor, fc, cov_true, model_true = build(TRUE)
+    products = simple_strip(fc, "VOLATILITYLOGNORMAL")
+    targets = prices(model_true, td, 3, products)
+    say(f"  target volatilities {np.round(targets, 4)}")
+    assert all(0.05 < v < 1.0 for v in targets)
+    td, tenor, fc, cov0, model0 = build([0.15, 0.0, 0.25, 0.15])
+    calib = [H.CalibrationProduct(p, v, 1.0) for p, v in zip(products, targets)]
+    cov_cal, info = cov0.getCloneCalibrated(model0, calib, {"accuracy": 1e-12, "maxIterations": 200})
+    say(f"  parameters {info['parameters']} iterations {info['iterations']} rms {info['rootMeanSquaredError']:.3e}")
+    assert np.allclose(info["parameters"], TRUE, rtol=0, atol=2e-4)
+    assert info["rootMeanSquaredError"] < 1e-7

@@ -191,3 +191,45 @@ def test_heston_wrapper_parameters():
     with pytest.raises(Exception):
         H.LIBORCovarianceModelStochasticVolatility(hv, H.BrownianMotionFromMersenneRandomNumbers(td, 2, 1000, 7),
                                                    0.1, 0.0, False)                    # no nesting
+
+
+@pytest.mark.parametrize("F,K,T,vol", [(0.03, 0.03, 1.0, 0.2), (0.03, 0.04, 5.0, 0.35), (0.05, 0.02, 10.0, 0.1),
+                                       (0.02, 0.021, 0.5, 0.6)])
+def test_black_value_and_implied_volatility(F, K, T, vol):
+    # AnalyticFormulas.blackScholesGeneralizedOptionValue against the closed form written with
+    # scipy's normal distribution, and the implied volatility as its inverse
+    from scipy.stats import norm
+    pu = 3.7
+    sd = vol * math.sqrt(T)
+    d1 = (math.log(F / K) + 0.5 * sd * sd) / sd
+    expect = (F * norm.cdf(d1) - K * norm.cdf(d1 - sd)) * pu
+    v = H.blackScholesGeneralizedOptionValue(F, vol, T, K, pu)
+    assert v == pytest.approx(expect, rel=1e-13)
+    assert H.blackScholesOptionImpliedVolatility(F, T, K, pu, v) == pytest.approx(vol, rel=1e-10)
+    assert H.blackScholesOptionImpliedVolatility(F, T, -0.01, pu, v) == 0.0        # strike <= 0: not an option
+
+
+@pytest.mark.parametrize("F,K,T,vol", [(0.03, 0.03, 1.0, 0.006), (0.03, 0.045, 5.0, 0.01), (0.01, -0.005, 10.0, 0.004)])
+def test_bachelier_value_and_implied_volatility(F, K, T, vol):
+    from scipy.stats import norm
+    pu = 2.5
+    sd = vol * math.sqrt(T)
+    d = (F - K) / sd
+    expect = ((F - K) * norm.cdf(d) + sd * norm.pdf(d)) * pu
+    v = H.bachelierOptionValue(F, vol, T, K, pu)
+    assert v == pytest.approx(expect, rel=1e-13)
+    assert H.bachelierOptionImpliedVolatility(F, T, K, pu, v) == pytest.approx(vol, rel=1e-10)
+
+
+def test_swap_annuity_and_swaption_simple_units():
+    wl = W.c0()
+    td, tenor, fc, vol, corr, cov, model = build_model(wl)
+    dc = H.DiscountCurveFromForwardCurve(fc)
+    t = [2.0, 2.5, 3.0, 3.5, 4.0]
+    expect = sum((t[i + 1] - t[i]) * dc.getDiscountFactor(t[i + 1]) for i in range(4)) / dc.getDiscountFactor(0.0)
+    assert H.getSwapAnnuity(t, fc) == pytest.approx(expect, rel=1e-15)
+    for unit in ("VALUE", "VOLATILITY", "VOLATILITYLOGNORMAL", "VOLATILITYNORMAL", "INTEGRATEDVARIANCELOGNORMAL",
+                 "INTEGRATEDNORMALVARIANCE"):
+        H.SwaptionSimple(0.03, t, unit)
+    with pytest.raises(Exception):
+        H.SwaptionSimple(0.03, t, "PRICE")
