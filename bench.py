@@ -40,7 +40,7 @@ HBM_PEAK_GBPS = 8000.0       # MI355X published HBM3E bandwidth
 # HBM bytes per launch of lmm_fused_kernel<40,5,double> on the 40 x 5 x 1,000,000-path strip:
 # FETCH_SIZE and WRITE_SIZE counter passes scaled by a plain 1 GiB copy
 # (profiles/lmm_fused_fp64_c1_hbm_traffic.md)
-PATH_KERNEL_TRAFFIC_BYTES = 1.968e9
+PATH_KERNEL_TRAFFIC_BYTES = 1.983e9
 FP32_PEAK_TFLOPS = 157.3     # MI355X published FP32 peak
 
 
