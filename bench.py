@@ -248,8 +248,8 @@ def main():
         args.no_fp32 = args.no_logstate = args.no_bermudan = args.no_greeks = args.no_cpu = args.no_rv = True
         args.no_calibration = True
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path axis would shard across GPUs "
-              "as described in DESIGN.md, but this build measures one MI355X.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); paths would shard across GPUs as "
+              "contiguous path ranges (path_offset), but this measures one MI355X.", file=sys.stderr)
         sys.exit(2)
 
     import torch
