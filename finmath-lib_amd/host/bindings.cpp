@@ -225,6 +225,7 @@ PYBIND11_MODULE(_finmath_host, m) {
         .def("setWeights", &LevenbergMarquardt::setWeights, py::return_value_policy::reference_internal)
         .def("setParameterSteps", &LevenbergMarquardt::setParameterSteps, py::return_value_policy::reference_internal)
         .def("setErrorTolerance", &LevenbergMarquardt::setErrorTolerance, py::return_value_policy::reference_internal)
+        .def("setNumberOfThreads", &LevenbergMarquardt::setNumberOfThreads, py::return_value_policy::reference_internal)
         .def("run", [](LevenbergMarquardt& lm, py::function f) {
             lm.run([&](const std::vector<double>& p, std::vector<double>& v) {
                 py::list out = f(p).cast<py::list>();

@@ -1,6 +1,10 @@
 // Implementation of the host-side mirror of the finmath-lib API (see finmath_host.hpp).
 #include "finmath_host.hpp"
 
+#include <atomic>
+#include <exception>
+#include <thread>
+
 #include <algorithm>
 #include <cstring>
 
@@ -966,7 +970,23 @@ double LevenbergMarquardt::meanSquaredError(const std::vector<double>& v) const 
     return e / (double)v.size();
 }
 
-void LevenbergMarquardt::run(const ObjectiveFunction& setValues) {                          // :640-731
+LevenbergMarquardt& LevenbergMarquardt::setNumberOfThreads(int numberOfThreads) {
+    if (numberOfThreads < 1) throw IllegalArgumentException("numberOfThreads must be positive");
+    threads_ = numberOfThreads;
+    return *this;
+}
+
+void LevenbergMarquardt::run(const ObjectiveFunction& setValues) {
+    const int threads = threads_;
+    threads_ = 1;
+    try {
+        run(WorkerObjectiveFunction([&](const std::vector<double>& p, std::vector<double>& v, int) { setValues(p, v); }));
+    } catch (...) { threads_ = threads; throw; }
+    threads_ = threads;
+}
+
+void LevenbergMarquardt::run(const WorkerObjectiveFunction& evaluate) {                    // :640-731
+    const ObjectiveFunction setValues = [&](const std::vector<double>& p, std::vector<double>& v) { evaluate(p, v, 0); };
     const int np = (int)initial_.size(), nv = (int)targets_.size();
     std::vector<double> parameterTest = initial_, valueTest((size_t)nv), valueCurrent((size_t)nv);
     std::vector<std::vector<double>> derivative((size_t)np, std::vector<double>((size_t)nv));
@@ -994,16 +1014,33 @@ void LevenbergMarquardt::run(const ObjectiveFunction& setValues) {              
         if (iteration_ > maxIteration_ || rmsChange <= tolerance_ || std::isinf(lambda_)) break;
         // updateParameterTest(): :749-815
         if (!derivativeValid) {                                     // forward differences (:550-620)
-            for (int i = 0; i < np; i++) {
+            auto column = [&](int i, int worker) {
                 std::vector<double> p = parameterCurrent_;
                 const double h = steps_.empty() ? (std::fabs(p[(size_t)i]) + 1) * 1e-8 : steps_[(size_t)i];
                 p[(size_t)i] += h;
                 std::vector<double>& d = derivative[(size_t)i];
-                setValues(p, d);
+                evaluate(p, d, worker);
                 for (int k = 0; k < nv; k++) {
                     d[(size_t)k] = (d[(size_t)k] - valueCurrent[(size_t)k]) / h;
                     if (std::isnan(d[(size_t)k])) d[(size_t)k] = 0.0;
                 }
+            };
+            const int workers = std::min(threads_, np);
+            if (workers <= 1) {
+                for (int i = 0; i < np; i++) column(i, 0);
+            } else {
+                // worker w takes the columns i = w, w + workers, ...; each column is independent
+                std::vector<std::thread> pool;
+                std::vector<std::exception_ptr> failed((size_t)workers);
+                for (int w = 0; w < workers; w++)
+                    pool.emplace_back([&, w] {
+                        try {
+                            for (int i = w; i < np; i += workers) column(i, w);
+                        } catch (...) { failed[(size_t)w] = std::current_exception(); }
+                    });
+                for (auto& t : pool) t.join();
+                for (auto& e : failed)
+                    if (e) std::rethrow_exception(e);
             }
             derivativeValid = true;
         }
@@ -1159,6 +1196,7 @@ CalibrationResult getCloneCalibrated(const LIBORCovarianceModelFromVolatilityAnd
     const double accuracy = opt("accuracy", 1e-7);
     const double parameterStep = opt("parameterStep", 1e-4);
     const int precision = (int)opt("precision", FM_PRECISION_FP64);
+    const int numberOfThreads = (int)opt("numberOfThreads", 4);
     if (!model) throw IllegalArgumentException("calibration model must be given");
     if (products.empty()) throw IllegalArgumentException("no calibration products");
     const std::vector<double> initial = getParameter(cov);
@@ -1182,15 +1220,23 @@ CalibrationResult getCloneCalibrated(const LIBORCovarianceModelFromVolatilityAnd
     cfg.precision = precision;
     cfg.libor_cap = model->liborCap;
     cfg.regenerate_increments = 0;
-    fm_lmm* ctx = nullptr;
-    check(fm_lmm_create(&cfg, &ctx));
-    std::unique_ptr<fm_lmm, void (*)(fm_lmm*)> guard(ctx, fm_lmm_destroy);
+    // one context per worker: the same seed gives every worker the same Brownian increments, and
+    // each context has its own stream, so the Jacobian's passes run concurrently on the device
+    if (numberOfThreads < 1) throw IllegalArgumentException("numberOfThreads must be positive");
+    const int workers = std::min(numberOfThreads, (int)initial.size());
+    std::vector<std::unique_ptr<fm_lmm, void (*)(fm_lmm*)>> contexts;
     const std::vector<double> L0 = model->getInitialForwards();
-    check(fm_lmm_set_initial_forwards(ctx, L0.data()));
+    std::vector<double> pd;
     if (model->getDiscountCurve()) {
-        std::vector<double> pd(tenor.size());
+        pd.resize(tenor.size());
         for (size_t k = 0; k < tenor.size(); k++) pd[k] = model->getDiscountCurve()->getDiscountFactor(tenor[k]);
-        check(fm_lmm_set_discount_curve(ctx, pd.data()));
+    }
+    for (int w = 0; w < workers; w++) {
+        fm_lmm* ctx = nullptr;
+        check(fm_lmm_create(&cfg, &ctx));
+        contexts.emplace_back(ctx, fm_lmm_destroy);
+        check(fm_lmm_set_initial_forwards(ctx, L0.data()));
+        if (!pd.empty()) check(fm_lmm_set_discount_curve(ctx, pd.data()));
     }
     std::deque<std::vector<double>> arena;
     std::vector<fm_product> lowered(products.size());
@@ -1198,9 +1244,10 @@ CalibrationResult getCloneCalibrated(const LIBORCovarianceModelFromVolatilityAnd
         if (!products[q].product) throw IllegalArgumentException("calibration product is null");
         products[q].product->lower(arena, lowered[q]);
     }
-    int valuations = 0;
-    std::vector<double> lam, var, lvOffsets, vals(products.size());
-    auto setValues = [&](const std::vector<double>& p, std::vector<double>& out) {
+    std::atomic<int> valuations{0};
+    auto setValues = [&](const std::vector<double>& p, std::vector<double>& out, int worker) {
+        fm_lmm* ctx = contexts[(size_t)worker].get();
+        std::vector<double> lam, var, lvOffsets, vals(products.size());
         out.assign(products.size(), NAN);
         try {
             const CovarianceModelPtr c = getCloneWithModifiedParameters(cov, p);
@@ -1222,13 +1269,14 @@ CalibrationResult getCloneCalibrated(const LIBORCovarianceModelFromVolatilityAnd
     };
     LevenbergMarquardt lm(initial, std::vector<double>(products.size(), 0.0), maxIterations);
     lm.setErrorTolerance(accuracy).setParameterSteps(std::vector<double>(initial.size(), parameterStep));
-    lm.run(setValues);
+    lm.setNumberOfThreads(workers);
+    lm.run(LevenbergMarquardt::WorkerObjectiveFunction(setValues));
     CalibrationResult r;
     r.parameters = lm.getBestFitParameters();
     r.covarianceModel = getCloneWithModifiedParameters(cov, r.parameters);
     r.iterations = lm.getIterations();
     r.rootMeanSquaredError = lm.getRootMeanSquaredError();
-    r.valuations = valuations;
+    r.valuations = valuations.load();
     return r;
 }
 

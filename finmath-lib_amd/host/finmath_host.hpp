@@ -570,11 +570,18 @@ private:
 class LevenbergMarquardt {
 public:
     using ObjectiveFunction = std::function<void(const std::vector<double>& parameters, std::vector<double>& values)>;
+    // objective evaluated by worker `worker` (0 .. numberOfThreads-1); workers run concurrently
+    using WorkerObjectiveFunction =
+        std::function<void(const std::vector<double>& parameters, std::vector<double>& values, int worker)>;
     LevenbergMarquardt(std::vector<double> initialParameters, std::vector<double> targetValues, int maxIteration);
     LevenbergMarquardt& setWeights(std::vector<double> weights);
     LevenbergMarquardt& setParameterSteps(std::vector<double> parameterSteps);
     LevenbergMarquardt& setErrorTolerance(double errorTolerance);
-    void run(const ObjectiveFunction& setValues);
+    // numberOfThreads: the Jacobian's columns are evaluated by that many workers at once
+    // (LevenbergMarquardt.java's numberOfThreads, :222-247,550-620)
+    LevenbergMarquardt& setNumberOfThreads(int numberOfThreads);
+    void run(const ObjectiveFunction& setValues);                // one worker
+    void run(const WorkerObjectiveFunction& setValues);
     std::vector<double> getBestFitParameters() const { return parameterCurrent_; }
     int getIterations() const { return iteration_; }
     double getRootMeanSquaredError() const { return std::sqrt(errorMeanSquaredCurrent_); }
@@ -582,6 +589,7 @@ private:
     double meanSquaredError(const std::vector<double>& values) const;
     std::vector<double> initial_, targets_, weights_, steps_;
     int maxIteration_;
+    int threads_ = 1;
     double tolerance_ = 0.0;
     double lambda_ = 0.001, lambdaDivisor_ = 3.0, lambdaMultiplicator_ = 2.0;
     std::vector<double> parameterCurrent_;
@@ -617,7 +625,8 @@ CovarianceModelPtr getCloneWithModifiedParameters(const LIBORCovarianceModelFrom
 // errors (value - target) * weight, every evaluation one fused pricing pass of all products on
 // the device, with one Brownian motion (its increments kept in device memory) for all
 // evaluations. calibrationParameters: numberOfPaths (2000), seed (31415), maxIterations (400),
-// accuracy (1e-7), parameterStep (1e-4), precision (FM_PRECISION_FP64).
+// accuracy (1e-7), parameterStep (1e-4), precision (FM_PRECISION_FP64), numberOfThreads (4: the
+// Jacobian's passes run on that many contexts at once; results do not depend on it).
 CalibrationResult getCloneCalibrated(const LIBORCovarianceModelFromVolatilityAndCorrelation& covarianceModel,
                                      LIBORMarketModelPtr calibrationModel,
                                      const std::vector<CalibrationProduct>& calibrationProducts,

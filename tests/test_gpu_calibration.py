@@ -187,3 +187,18 @@ def test_calibration_to_lognormal_volatilities():
     say(f"  parameters {info['parameters']} iterations {info['iterations']} rms {info['rootMeanSquaredError']:.3e}")
     assert np.allclose(info["parameters"], TRUE, rtol=0, atol=2e-4)
     assert info["rootMeanSquaredError"] < 1e-7
+
+
+def test_concurrent_jacobian_passes_give_the_same_calibration():
+    # numberOfThreads (LevenbergMarquardt.java's parallel Jacobian): the bumped parameter vectors
+    # are priced on separate contexts at once; the iterates are the sequential run's bit for bit
+    say("case calibration numberOfThreads 1 and 4")
+    td, tenor, fc, _, model_true = build(TRUE)
+    products = strip(fc)
+    targets = prices(model_true, td, 3, products)
+    td, tenor, fc, cov0, model0 = build([0.15, 0.0, 0.25, 0.15])
+    calib = [H.CalibrationProduct(p, v, 1.0) for p, v in zip(products, targets)]
+    runs = [cov0.getCloneCalibrated(model0, calib, {"maxIterations": 30, "numberOfThreads": k})[1] for k in (1, 4)]
+    say(f"  iterations {runs[0]['iterations']} / {runs[1]['iterations']}")
+    assert runs[0]["parameters"] == runs[1]["parameters"]
+    assert runs[0]["iterations"] == runs[1]["iterations"] and runs[0]["valuations"] == runs[1]["valuations"]

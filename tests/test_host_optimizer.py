@@ -95,3 +95,4 @@ def test_covariance_parameters_round_trip():
         td, tenor, H.LIBORVolatilityModelFourParameterExponentialForm(td, tenor, 0.2, 0.0, 0.25, 0.1, False),
         H.LIBORCorrelationModelExponentialDecay(td, tenor, 3, 0.1, False))
     assert fixed.getParameter() == []
+
