@@ -281,6 +281,7 @@ class LmmContext:
         self.discount_factors = None
         self.factor_loading = None       # [nT][n][F], as set
         self.local_volatility = None     # (alpha, offsets[nT][n]), as set
+        self._sv_driver = None           # driver of a stochastic volatility scaling, as set
         self.n = cfg.num_libors
         self.nT = cfg.num_time_steps
         self.F = cfg.num_factors

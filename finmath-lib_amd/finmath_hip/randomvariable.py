@@ -415,6 +415,9 @@ class LIBORMonteCarloSimulation:
         self.lognormal = ctx.cfg.state_space == _abi.STATE_SPACE["LOGNORMAL"]
         if interpolation == "LOG_LINEAR_CORRECTED" and ctx.factor_loading is None:
             raise _abi.IllegalArgumentException("LOG_LINEAR_CORRECTED needs the context's factor loadings")
+        if interpolation == "LOG_LINEAR_CORRECTED" and getattr(ctx, "_sv_driver", None) is not None:
+            # the drift adjustment would integrate the per-path scaling, which stays on the device
+            raise _abi.UnsupportedOperationException("LOG_LINEAR_CORRECTED under a stochastic volatility scaling")
         self._numeraires = {}
         self._adjustments = None
 

@@ -145,3 +145,14 @@ def test_reference_bond_and_fra_assertions():
         worst = max(worst, abs(value))
     say(f"  testFRA maximum abs deviation {worst:.3e}")
     assert worst < 2e-3
+
+
+def test_corrected_interpolation_refuses_a_stochastic_volatility_scaling():
+    wl = W.c0().with_paths(512)
+    t = O.ModelTables(wl)
+    ctx = context(wl, t, wl.num_paths, "SPOT")
+    driver = _abi.LmmContext(t.times, t.tenor, 2, wl.num_paths, 7)
+    ctx.set_stochastic_volatility(0.2, -0.3, driver)
+    RV.LIBORMonteCarloSimulation(ctx, curve(t), "LOG_LINEAR_UNCORRECTED")
+    with pytest.raises(_abi.UnsupportedOperationException):
+        RV.LIBORMonteCarloSimulation(ctx, curve(t), "LOG_LINEAR_CORRECTED")
