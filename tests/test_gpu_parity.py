@@ -202,3 +202,4 @@ def test_materialised_process_agrees_with_fused_prices():
     x = np.maximum(V, 0.0) / ctx.numeraire(s["exercise"])
     fused, _ = ctx.value_products(to_products([s]))
     assert abs(np.mean(x) - fused[0]) < 1e-12 * abs(fused[0])
+

@@ -152,3 +152,24 @@ def test_discount_curve_closed_form():
 def test_round_time_survives_half_years():
     for k in range(200):
         assert O.lib().orc_round_time(0.5 * k) == 0.5 * k
+
+
+
+def test_reference_quantile_test():
+    # RandomVariableTest.testGetQuantile (src/test/java/net/finmath/montecarlo/RandomVariableTest.java:144-172):
+    # commons-math3's MersenneTwister built from the int seed 3141 (init_genrand seeding, as
+    # numpy's legacy RandomState(3141) seeds), 26+26-bit nextDouble, the inverse normal, and the
+    # quantile rule of RandomVariableFromDoubleArray.getQuantile (:446-460)
+    n = 10_000_000
+    w = np.random.RandomState(3141).randint(0, 2**32, size=2 * n, dtype=np.uint64).astype(np.uint64)
+    u = (((w[0::2] >> np.uint64(6)) << np.uint64(26)) | (w[1::2] >> np.uint64(6))).astype(np.float64) * 2.0**-52
+    samples = np.sort(O.icdf(u))
+
+    def quantile(q):
+        return samples[min(max(int(math.floor((n + 1) * q - 1 + 0.5)), 0), n - 1)]
+
+    assert quantile(0.0) == samples[0]
+    assert abs(quantile(0.05) - (-1.645)) < 1e-3
+    assert abs(quantile(0.5)) < 2e-4
+    assert abs(quantile(0.95) - 1.645) < 1e-3
+    assert abs(quantile(0.99) - 2.33) < 1e-2
