@@ -93,6 +93,19 @@ PYBIND11_MODULE(_finmath_host, m) {
             return std::make_shared<LIBORVolatilityModelFromGivenMatrix>(td, tenor, v);
         }));
 
+    py::class_<LIBORVolatilityModelPiecewiseConstant, LIBORVolatilityModel,
+               std::shared_ptr<LIBORVolatilityModelPiecewiseConstant>>(m, "LIBORVolatilityModelPiecewiseConstant")
+        .def(py::init([](TimeDiscretizationPtr td, TimeDiscretizationPtr tenor, TimeDiscretizationPtr sim,
+                         TimeDiscretizationPtr ttm, py::object volatility, bool isCalibrateable) {
+            std::vector<double> v;
+            if (py::isinstance<py::float_>(volatility) || py::isinstance<py::int_>(volatility)) v = {volatility.cast<double>()};
+            else v = volatility.cast<std::vector<double>>();
+            return std::make_shared<LIBORVolatilityModelPiecewiseConstant>(td, tenor, sim, ttm, v, isCalibrateable);
+        }), py::arg("timeDiscretization"), py::arg("liborPeriodDiscretization"), py::arg("simulationTimeDiscretization"),
+           py::arg("timeToMaturityDiscretization"), py::arg("volatility"), py::arg("isCalibrateable") = true)
+        .def("getVolatility", &LIBORVolatilityModelPiecewiseConstant::getVolatility)
+        .def("getParameter", &LIBORVolatilityModelPiecewiseConstant::getParameter);
+
     py::class_<LIBORCorrelationModelExponentialDecay, std::shared_ptr<LIBORCorrelationModelExponentialDecay>>(
         m, "LIBORCorrelationModelExponentialDecay")
         .def(py::init<TimeDiscretizationPtr, TimeDiscretizationPtr, int, double, bool>(), py::arg("timeDiscretization"),

@@ -345,6 +345,36 @@ void LIBORCovarianceModelFromVolatilityAndCorrelation::tables(const TimeDiscreti
     }
 }
 
+// ---- piecewise-constant volatility ------------------------------------------------------------
+LIBORVolatilityModelPiecewiseConstant::LIBORVolatilityModelPiecewiseConstant(
+    TimeDiscretizationPtr td, TimeDiscretizationPtr tenor, TimeDiscretizationPtr simulationTimes,
+    TimeDiscretizationPtr timesToMaturity, std::vector<double> volatility, bool isCalibrateable)
+    : LIBORVolatilityModel(std::move(td), std::move(tenor)), sim_(std::move(simulationTimes)),
+      ttm_(std::move(timesToMaturity)), calibrateable_(isCalibrateable) {
+    const double maxMaturity = tenor_->getTime(tenor_->getNumberOfTimes() - 1);
+    int count = 0;
+    index_.assign((size_t)sim_->getNumberOfTimes(), std::vector<int>((size_t)ttm_->getNumberOfTimes(), -1));
+    for (int i = 0; i < sim_->getNumberOfTimes(); i++)
+        for (int k = 0; k < ttm_->getNumberOfTimes(); k++)
+            if (!(sim_->getTime(i) + ttm_->getTime(k) > maxMaturity)) index_[(size_t)i][(size_t)k] = count++;
+    if (volatility.size() == 1) vol_.assign((size_t)count, volatility[0]);
+    else if ((int)volatility.size() == count) vol_ = std::move(volatility);
+    else throw IllegalArgumentException("Volatility length does not match number of free parameters.");
+}
+
+double LIBORVolatilityModelPiecewiseConstant::getVolatility(int timeIndex, int liborIndex) const {
+    const double time = td_->getTime(timeIndex);
+    const double timeToMaturity = tenor_->getTime(liborIndex) - time;
+    if (timeToMaturity <= 0) return 0.0;                          // already fixed
+    int i = sim_->getTimeIndexNearestLessOrEqual(time);
+    i = std::min(std::max(i, 0), sim_->getNumberOfTimes() - 1);
+    int k = ttm_->getTimeIndexNearestLessOrEqual(timeToMaturity);
+    k = std::min(std::max(k, 0), ttm_->getNumberOfTimes() - 1);
+    const int p = index_[(size_t)i][(size_t)k];
+    if (p < 0) throw IllegalStateException("no volatility parameter for this bucket");
+    return vol_[(size_t)p];
+}
+
 // ---- local-volatility wrappers -------------------------------------------------------------
 static bool isStochastic(const LIBORCovarianceModelFromVolatilityAndCorrelation& cov) {
     double a, b, c;
@@ -1170,8 +1200,16 @@ CovarianceModelPtr getCloneWithModifiedParameters(const LIBORCovarianceModelFrom
             cov.getTimeDiscretization(), cov.getLiborPeriodDiscretization(), parameters[0], parameters[1], parameters[2],
             parameters[3], true);
         used = 4;
+    } else if (auto pw = std::dynamic_pointer_cast<const LIBORVolatilityModelPiecewiseConstant>(vol);
+               pw && pw->isCalibrateable()) {
+        const size_t np = pw->getParameter().size();
+        if (parameters.size() < np) throw IllegalArgumentException("piecewise-constant volatilities expected");
+        vol = std::make_shared<LIBORVolatilityModelPiecewiseConstant>(
+            cov.getTimeDiscretization(), cov.getLiborPeriodDiscretization(), pw->getSimulationTimeDiscretization(),
+            pw->getTimeToMaturityDiscretization(), std::vector<double>(parameters.begin(), parameters.begin() + np), true);
+        used = np;
     } else if (!vol->getParameter().empty()) {
-        throw UnsupportedOperationException("only the four-parameter volatility form is calibrated");
+        throw UnsupportedOperationException("only the four-parameter and piecewise-constant volatilities are calibrated");
     }
     CorrelationModelPtr corr = cov.getCorrelationModel();
     if (corr->isCalibrateable()) {

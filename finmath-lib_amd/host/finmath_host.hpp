@@ -119,6 +119,27 @@ private:
     std::vector<double> vol_;
 };
 
+// Volatility constant on buckets of (simulation time, time to maturity): the parameter of the
+// buckets starting at the nearest grid points at or below t_i and T_j - t_i; 0 once fixed; one
+// parameter per bucket pair whose start lies within the last tenor date
+// (covariance/LIBORVolatilityModelPiecewiseConstant.java:36-76,113-195).
+class LIBORVolatilityModelPiecewiseConstant : public LIBORVolatilityModel {
+public:
+    LIBORVolatilityModelPiecewiseConstant(TimeDiscretizationPtr td, TimeDiscretizationPtr tenor,
+                                          TimeDiscretizationPtr simulationTimes, TimeDiscretizationPtr timesToMaturity,
+                                          std::vector<double> volatility, bool isCalibrateable = true);
+    double getVolatility(int timeIndex, int liborIndex) const override;
+    std::vector<double> getParameter() const override { return calibrateable_ ? vol_ : std::vector<double>(); }
+    TimeDiscretizationPtr getSimulationTimeDiscretization() const { return sim_; }
+    TimeDiscretizationPtr getTimeToMaturityDiscretization() const { return ttm_; }
+    bool isCalibrateable() const { return calibrateable_; }
+private:
+    TimeDiscretizationPtr sim_, ttm_;
+    std::vector<std::vector<int>> index_;     // [simulation bucket][maturity bucket] -> parameter, -1 none
+    std::vector<double> vol_;
+    bool calibrateable_;
+};
+
 // rho_ij = exp(-a |T_i - T_j|), PCA-reduced to m factors
 // (covariance/LIBORCorrelationModelExponentialDecay.java:102-134, functions/LinearAlgebra.java:380-482).
 class LIBORCorrelationModelExponentialDecay {

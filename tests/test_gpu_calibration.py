@@ -202,3 +202,42 @@ def test_concurrent_jacobian_passes_give_the_same_calibration():
     say(f"  iterations {runs[0]['iterations']} / {runs[1]['iterations']}")
     assert runs[0]["parameters"] == runs[1]["parameters"]
     assert runs[0]["iterations"] == runs[1]["iterations"] and runs[0]["valuations"] == runs[1]["valuations"]
+
+
+def test_reference_displaced_calibration_setup():
+    # LIBORMarketModelCalibrationTest's DISPLACED model (src/test/java/net/finmath/montecarlo/
+    # interestrate/LIBORMarketModelCalibrationTest.java:270-300): piecewise-constant volatility on
+    # simulation-time x time-to-maturity buckets, exponential-decay correlation 0.05,
+    # DisplacedLocalVolatilityModel(1/0.25, not calibrateable), spot measure, normal state space;
+    # here on 20 semiannual forwards with buckets {0, 1, 2, 5, 10} (18 parameters) and targets
+    # from a known bucket pattern
+    say("case reference displaced calibration set-up, 18 piecewise-constant volatilities")
+    n = 20
+    td = H.TimeDiscretizationFromArray(0.0, n, 0.5)
+    tenor = H.TimeDiscretizationFromArray(0.0, n, 0.5)
+    fc = H.ForwardCurveInterpolation.createForwardCurveFromForwards(
+        "forwardCurve", list(tenor.getAsDoubleArray()[:-1]), list(0.02 + 0.001 * np.arange(n)), 0.5)
+    grid = H.TimeDiscretizationFromArray([0.0, 1.0, 2.0, 5.0, 10.0])
+
+    def displaced(params):
+        vol = H.LIBORVolatilityModelPiecewiseConstant(td, tenor, grid, grid, params, True)
+        corr = H.LIBORCorrelationModelExponentialDecay(td, tenor, 3, 0.05, False)
+        cov = H.DisplacedLocalVolatilityModel(H.LIBORCovarianceModelFromVolatilityAndCorrelation(td, tenor, vol, corr),
+                                              1.0 / 0.25, False)
+        model = H.LIBORMarketModelFromCovarianceModel.of(tenor, None, fc, H.DiscountCurveFromForwardCurve(fc), None,
+                                                         cov, [], {"measure": "SPOT", "stateSpace": "NORMAL"})
+        return cov, model
+
+    true = [0.005 + 0.0002 * ((7 * k) % 5) for k in range(18)]
+    _, model_true = displaced(true)
+    products = strip(fc)
+    targets = prices(model_true, td, 3, products)
+    cov0, model0 = displaced(0.005)
+    assert len(cov0.getParameter()) == 18
+    start = float(np.sqrt(np.mean((np.array(prices(model0, td, 3, products)) - np.array(targets)) ** 2)))
+    cov_cal, info = cov0.getCloneCalibrated(model0, [H.CalibrationProduct(p, v, 1.0) for p, v in zip(products, targets)],
+                                            {"maxIterations": 100})
+    say(f"  rms {start:.3e} -> {info['rootMeanSquaredError']:.3e} in {info['iterations']} iterations, "
+        f"{info['valuations']} passes")
+    assert isinstance(cov_cal, H.DisplacedLocalVolatilityModel)
+    assert info["rootMeanSquaredError"] < 0.02 * start

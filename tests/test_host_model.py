@@ -233,3 +233,39 @@ def test_swap_annuity_and_swaption_simple_units():
         H.SwaptionSimple(0.03, t, unit)
     with pytest.raises(Exception):
         H.SwaptionSimple(0.03, t, "PRICE")
+
+
+def test_piecewise_constant_volatility_buckets():
+    # LIBORVolatilityModelPiecewiseConstant: one parameter per (simulation time, time to
+    # maturity) bucket pair starting within the last tenor date, looked up at the nearest grid
+    # points at or below t_i and T_j - t_i; 0 once fixed
+    td = H.TimeDiscretizationFromArray(0.0, 20, 0.5)
+    tenor = H.TimeDiscretizationFromArray(0.0, 20, 0.5)
+    sg = [0.0, 1.0, 2.0, 5.0, 10.0]
+    mg = [0.0, 1.0, 2.0, 5.0, 10.0]
+    pairs = [(a, b) for a in range(5) for b in range(5) if sg[a] + mg[b] <= 10.0]
+    assert len(pairs) == 18
+    params = [0.004 + 0.0001 * k for k in range(len(pairs))]
+    vol = H.LIBORVolatilityModelPiecewiseConstant(td, tenor, H.TimeDiscretizationFromArray(sg),
+                                                  H.TimeDiscretizationFromArray(mg), params, True)
+    assert list(vol.getParameter()) == params
+    times, T = td.getAsDoubleArray(), tenor.getAsDoubleArray()
+    for i in range(20):
+        for j in range(20):
+            tau = T[j] - times[i]
+            if tau <= 0:
+                assert vol.getVolatility(i, j) == 0.0
+                continue
+            a = max(k for k in range(5) if sg[k] <= times[i])
+            b = max(k for k in range(5) if mg[k] <= tau)
+            assert vol.getVolatility(i, j) == params[pairs.index((a, b))], (i, j)
+    flat = H.LIBORVolatilityModelPiecewiseConstant(td, tenor, H.TimeDiscretizationFromArray(sg),
+                                                   H.TimeDiscretizationFromArray(mg), 0.005, False)
+    assert flat.getVolatility(3, 10) == 0.005 and list(flat.getParameter()) == []
+    with pytest.raises(ValueError):
+        H.LIBORVolatilityModelPiecewiseConstant(td, tenor, H.TimeDiscretizationFromArray(sg),
+                                                H.TimeDiscretizationFromArray(mg), [0.01, 0.02], True)
+    corr = H.LIBORCorrelationModelExponentialDecay(td, tenor, 3, 0.05)
+    cov = H.LIBORCovarianceModelFromVolatilityAndCorrelation(td, tenor, vol, corr)
+    c = cov.getCloneWithModifiedParameters([0.006] * 18)
+    assert list(c.getParameter()) == [0.006] * 18
