@@ -12,6 +12,9 @@ The headline line is the plain FP64 form at 40 semiannual forwards x 5 factors x
   fp32_mixed     the FP32-mixed form at 4,000,000 paths and its per-price difference to
                  the FP64 form on the same paths, in basis points
   fp64_logstate  FP64 carrying the log-forwards between steps
+  variants       the general path kernels on the same workload: terminal measure with the
+                 predictor-corrector, the displaced local-volatility and the stochastic
+                 volatility wrappers (normal state space)
   bermudan       the Bermudan swaption by regression at 80 forwards x 10 factors x
                  1,000,000 paths (FP32-mixed), its FP64 price, and its adjoint Greeks
   greeks         adjoint Greeks of the strip (1,000,000 paths, FP32 paths) over one
@@ -20,7 +23,7 @@ The headline line is the plain FP64 form at 40 semiannual forwards x 5 factors x
   cpu_baseline   the CPU restatement of the reference timed on the host cores
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-fp32]
-                    [--no-logstate] [--no-bermudan] [--no-greeks] [--only-fp64] [--copy-probe]
+                    [--no-logstate] [--no-variants] [--no-bermudan] [--no-greeks] [--only-fp64] [--copy-probe]
 """
 from __future__ import annotations
 
@@ -44,20 +47,25 @@ PATH_KERNEL_TRAFFIC_BYTES = 1.983e9
 FP32_PEAK_TFLOPS = 157.3     # MI355X published FP32 peak
 
 
-def build_simulation(wl, precision="FP64", regenerate=True):
+def build_simulation(wl, precision="FP64", regenerate=True, measure="SPOT", state_space="LOGNORMAL",
+                     scheme="EULER_FUNCTIONAL", vol_params=None, wrap=None):
+    """The workload's model through the host mirror. wrap(H, cov, td) may return a covariance
+    wrapper (local or stochastic volatility) around the plain model."""
     from finmath_hip import _finmath_host as H
     td = H.TimeDiscretizationFromArray(0.0, wl.num_time_steps, wl.period)
     tenor = H.TimeDiscretizationFromArray(0.0, wl.num_libors, wl.period)
     fc = H.ForwardCurveInterpolation.createForwardCurveFromForwards(
         "forwardCurve", list(tenor.getAsDoubleArray()[:-1]), list(wl.forwards), wl.period)
-    a, b, c, d = wl.vol_params
+    a, b, c, d = wl.vol_params if vol_params is None else vol_params
     vol = H.LIBORVolatilityModelFourParameterExponentialForm(td, tenor, a, b, c, d, False)
     corr = H.LIBORCorrelationModelExponentialDecay(td, tenor, wl.num_factors, wl.corr_decay)
     cov = H.LIBORCovarianceModelFromVolatilityAndCorrelation(td, tenor, vol, corr)
+    if wrap is not None:
+        cov = wrap(H, cov, td)
     model = H.LIBORMarketModelFromCovarianceModel.of(tenor, None, fc, H.DiscountCurveFromForwardCurve(fc), None, cov,
-                                                     [], {"measure": "SPOT", "stateSpace": "LOGNORMAL"})
+                                                     [], {"measure": measure, "stateSpace": state_space})
     bm = H.BrownianMotionFromMersenneRandomNumbers(td, wl.num_factors, wl.num_paths, wl.seed)
-    process = H.EulerSchemeFromProcessModel(model, bm, "EULER_FUNCTIONAL", precision)
+    process = H.EulerSchemeFromProcessModel(model, bm, scheme, precision)
     process.setRegenerateIncrements(regenerate)
     sim = H.LIBORMonteCarloSimulationFromLIBORModel(process)
     products = []
@@ -83,9 +91,9 @@ def step_kernel_flops_per_path(wl) -> int:
     return alive * (6 * F + 11) + 3 * n + 7 * periods
 
 
-def run_gpu(wl, precision, steps, warmup):
+def run_gpu(wl, precision, steps, warmup, **model):
     import torch
-    sim, process, products = build_simulation(wl, precision, regenerate=True)
+    sim, process, products = build_simulation(wl, precision, regenerate=True, **model)
     results = None
     for _ in range(warmup):
         results = sim.getValues(products)
@@ -237,6 +245,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-logstate", action="store_true")
+    ap.add_argument("--no-variants", action="store_true", help="skip the general-kernel variants leg")
     ap.add_argument("--no-bermudan", action="store_true")
     ap.add_argument("--no-rv", action="store_true", help="skip the device random-variable algebra leg")
     ap.add_argument("--no-calibration", action="store_true", help="skip the covariance calibration leg")
@@ -246,6 +255,7 @@ def main():
     args = ap.parse_args()
     if args.only_fp64:
         args.no_fp32 = args.no_logstate = args.no_bermudan = args.no_greeks = args.no_cpu = args.no_rv = True
+        args.no_variants = True
         args.no_calibration = True
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); paths would shard across GPUs as "
@@ -298,6 +308,24 @@ def main():
             "workload": "c1, FP64 arithmetic carrying the log-forwards between steps, fused multiply-add",
             "value": path_steps / (ms_ls * 1e-3), "unit": "path-steps/s", "ms_per_step": ms_ls,
             "path_kernel_ms": stage_ls[1], "max_rel_diff_vs_fp64": float(np.max(np.abs(v_ls - v64) / np.abs(v64)))}
+    if not args.no_variants:
+        # the general path kernels on the headline workload: terminal measure with the
+        # predictor-corrector, and the local / stochastic volatility wrappers (normal state space)
+        variants = {
+            "terminal_predictor_corrector": dict(measure="TERMINAL", scheme="PREDICTOR_CORRECTOR"),
+            "displaced_normal": dict(state_space="NORMAL", vol_params=(0.005, 0.0, 0.25, 0.002),
+                                     wrap=lambda H, cov, td: H.DisplacedLocalVolatilityModel(cov, 4.0, False)),
+            "stochastic_volatility_normal": dict(
+                state_space="NORMAL",
+                wrap=lambda H, cov, td: H.LIBORCovarianceModelStochasticVolatility(
+                    H.BlendedLocalVolatilityModel(cov, 0.0, False),
+                    H.BrownianMotionFromMersenneRandomNumbers(td, 2, wl.num_paths, 31415), 0.3, -0.3, False)),
+        }
+        line["variants"] = {"workload": "c1 strip, 1,000,000 paths, FP64, general path kernels"}
+        for name, model in variants.items():
+            msv, stv, _, _ = run_gpu(wl, "FP64", 3, 1, **model)
+            line["variants"][name] = {"ms_per_step": msv, "value": path_steps / (msv * 1e-3),
+                                      "unit": "path-steps/s", "path_kernel_ms": stv[1]}
     if not args.no_bermudan:
         wl4 = W.c4()
         ms4, st4, v4, e4 = run_gpu(wl4, "FP32_MIXED", max(2, args.steps // 2), 1)
