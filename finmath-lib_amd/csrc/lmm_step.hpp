@@ -333,7 +333,7 @@ constexpr int kVarFunctional = 0, kVarLogState = 1, kVarGeneral = 2, kVarGeneral
 
 template <int NCAP, int FCAP, typename real, int VAR = kVarFunctional>
 __global__ void __launch_bounds__(kStepThreads)
-__attribute__((amdgpu_waves_per_eu((VAR == kVarLogState && NCAP == 40) ? 3 : 1)))
+__attribute__((amdgpu_waves_per_eu((VAR == kVarLogState && NCAP == 40) ? 3 : ((VAR >= kVarGeneral && NCAP == 40) ? 2 : 1))))
 lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* __restrict__ delta,
                  const double* __restrict__ times, const int32_t* __restrict__ first_alive,
                  const int32_t* __restrict__ tenor_ti, const KProduct* __restrict__ products,
