@@ -890,6 +890,7 @@ int fm_lmm_set_covariance(fm_lmm* c, const double* factor_loading, const double*
     for (double v : c->lam) require(std::isfinite(v), FM_ERR_INVALID_ARGUMENT, "factor loadings must be finite");
     for (double v : c->var) require(std::isfinite(v), FM_ERR_INVALID_ARGUMENT, "variances must be finite");
     c->has_cov = true; c->tables_dirty = true; c->simulated = false;
+    c->has_vf = false;      // a factorization set before belongs to the previous tables
     return FM_OK;
     FM_CATCH
 }
@@ -1205,6 +1206,24 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
             "the adjoint runs on the FP64 and FP32_MIXED path kernels");
     require(c->plain_euler(), FM_ERR_UNSUPPORTED,
             "the adjoint differentiates the spot-measure Euler scheme (EULER_FUNCTIONAL)");
+    {
+        // the factorization must be the one of the tables that drive the path:
+        // lambda_jk(t_i) = sigma_j(t_i) f_jk and variance_j(t_i) = sigma_j(t_i)^2 c_j
+        const int n = c->n(), nT = c->nT(), F = c->F();
+        double lscale = 0.0, vscale = 0.0, lworst = 0.0, vworst = 0.0;
+        for (double v : c->lam) lscale = std::max(lscale, std::fabs(v));
+        for (double v : c->var) vscale = std::max(vscale, std::fabs(v));
+        for (int i = 0; i < nT; i++)
+            for (int j = 0; j < n; j++) {
+                const double sg = c->vf_sig[(size_t)i * n + j];
+                for (int k = 0; k < F; k++)
+                    lworst = std::max(lworst, std::fabs(c->lam[((size_t)i * n + j) * F + k] - sg * c->vf_fac[(size_t)j * F + k]));
+                vworst = std::max(vworst, std::fabs(c->var[(size_t)i * n + j] - sg * sg * c->vf_c[(size_t)j]));
+            }
+        require(lworst <= 1e-12 * lscale && vworst <= 1e-12 * vscale, FM_ERR_STATE,
+                "the volatility factorization does not reproduce the covariance tables "
+                "(lambda = sigma f, variance = sigma^2 c)");
+    }
     DeviceGuard g(c->device);
     prepare_model(c);
     const int n = c->n(), nT = c->nT(), P = c->P();

@@ -13,7 +13,7 @@
 //
 // Conventions of the reference's AAD (RandomVariableDifferentiableAAD.java:165-219):
 // averages over paths (the numeraire-adjustment factor) are constants, the floor of
-// the swaption payoff has derivative 1 above and 0 below the strike.
+// the swaption payoff has derivative 1 at and above zero and 0 below.
 #pragma once
 
 #include "lmm_step.hpp"
@@ -218,7 +218,7 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
                             Pc = Pc * D;
                         }
                     }
-                    if (V > (real)0) {
+                    if (V >= (real)0) {     // floor(0.0): derivative 1 at and above the floor
                         const double Vb = wq / Nsel;
                         const double nb = -wq * (double)V / (Nsel * Nsel);
                         if (top) Nb_top += nb; else Nb_low += nb;
@@ -243,7 +243,9 @@ lmm_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ r
                     const KPeriod pe = periods[pr.param_off];
                     const real Lj = pick<NCAP, real>(L, pr.first_j);
                     const real v = Lj - (real)pe.strike;
-                    const bool itm = pr.is_floorlet ? (v < (real)0) : (v > (real)0);
+                    // floor(0.0) has derivative 1 for v >= 0, cap(0.0) has derivative 1 for v < 0
+                    // (RandomVariableDifferentiableAAD.java:333-348)
+                    const bool itm = pr.is_floorlet ? (v < (real)0) : (v >= (real)0);
                     if (itm) {
                         const double dcf = pe.notional;
                         const double x = pr.is_floorlet ? -(double)v * dcf : (double)v * dcf;

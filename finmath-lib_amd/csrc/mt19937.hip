@@ -252,7 +252,7 @@ brownian_kernel(const uint32_t* __restrict__ states, int64_t num_sub, uint64_t t
     __shared__ uint32_t st[kGenWaves][N];
     __shared__ uint64_t tq_bits[kGenWaves][kTailQ];    // 52 uniform bits of a tail draw
     __shared__ uint32_t tq_off[kGenWaves][kTailQ];     // double index relative to the sub-stream's first
-    __shared__ uint16_t tq_ti[kGenWaves][kTailQ];
+    __shared__ uint32_t tq_ti[kGenWaves][kTailQ];     // time index of the draw (any nT)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t s = (int64_t)blockIdx.x * kGenWaves + wv;
     if (s >= num_sub) return;                          // the whole wavefront leaves
@@ -323,7 +323,7 @@ brownian_kernel(const uint32_t* __restrict__ states, int64_t num_sub, uint64_t t
                 const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
                 tq_bits[wv][pos] = bits;
                 tq_off[wv][pos] = (uint32_t)((wd >> 1) - d_begin);
-                tq_ti[wv][pos] = (uint16_t)ti;
+                tq_ti[wv][pos] = (uint32_t)ti;
             }
             count += __popcll(m);
             if (count >= 64) { drain(count); count = 0; }

@@ -59,7 +59,7 @@ class SimpleSwap(AbstractMonteCarloProduct):
     def getValueRandomVariable(self, evaluationTime, model):
         values = RandomVariable.constant(0.0)
         for i, (fix, pay, k) in enumerate(zip(self.fixingDates, self.paymentDates, self.swaprates)):
-            if pay <= evaluationTime:
+            if pay < evaluationTime:     # a period paying at the evaluation time still counts
                 continue
             libor = model.getForwardRate(fix, fix, pay)
             payoff = libor.sub(k).mult(pay - fix)

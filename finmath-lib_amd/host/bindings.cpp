@@ -385,6 +385,10 @@ PYBIND11_MODULE(_finmath_host, m) {
             d["volatility"] = vol;
             d["forwardVega"] = to_array(g.forwardVega);
             d["parameterVega"] = to_array(g.parameterVega);
+            const int nTc = n ? (int)(g.volatilityModel.size() / n) : 0;
+            py::array_t<double> volm({nTc, n});
+            std::copy(g.volatilityModel.begin(), g.volatilityModel.end(), volm.mutable_data());
+            d["volatilityModel"] = volm;
             return d;
         }, py::arg("products"), py::arg("weights") = std::vector<double>{}, py::arg("checkpointStride") = 4);
 

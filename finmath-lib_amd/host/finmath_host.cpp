@@ -659,16 +659,32 @@ Gradient EulerSchemeFromProcessModel::getGradient(const std::vector<ProductPtr>&
     g.forwardVega.assign((size_t)n, 0.0);
     for (int i = 0; i < nT; i++)
         for (int j = 0; j < n; j++) g.forwardVega[(size_t)j] += g.volatility[(size_t)i * n + j];
-    // chain rule to (a, b, c, d) of sigma = (a + b tau) exp(-c tau) + d, tau = T_j - t_i > 0
-    auto fp = std::dynamic_pointer_cast<const LIBORVolatilityModelFourParameterExponentialForm>(
-        model_->getCovarianceModel()->getVolatilityModel());
+    // sigma_j(t_i) is the volatility model's entry at the covariance model's time index of t_i,
+    // rounded down (AbstractLIBORCovarianceModel.java:45-60): the gradient with respect to the
+    // model's own matrix entries (the leaves of LIBORVolatilityModelFromGivenMatrix) sums the
+    // simulation steps that read each entry
+    const auto cov = model_->getCovarianceModel();
+    const auto vm = cov->getVolatilityModel();
+    const auto& sim = *getTimeDiscretization();
+    const int nTc = vm->getTimeDiscretization()->getNumberOfTimeSteps() + 1;
+    g.volatilityModel.assign((size_t)nTc * n, 0.0);
+    std::vector<int> ci_of((size_t)nT);
+    for (int i = 0; i < nT; i++) {
+        int ci = cov->getTimeDiscretization()->getTimeIndex(sim.getTime(i));
+        if (ci < 0) ci = -ci - 2;
+        ci_of[(size_t)i] = ci;
+        for (int j = 0; j < n; j++) g.volatilityModel[(size_t)ci * n + j] += g.volatility[(size_t)i * n + j];
+    }
+    // chain rule to (a, b, c, d) of sigma = (a + b tau) exp(-c tau) + d with tau = T_j - t_ci > 0,
+    // the volatility model's own time of the entry (LIBORVolatilityModelFourParameterExponentialForm.java:168-195)
+    auto fp = std::dynamic_pointer_cast<const LIBORVolatilityModelFourParameterExponentialForm>(vm);
     if (fp) {
         g.parameterVega.assign(4, 0.0);
-        const auto& sim = *getTimeDiscretization();
-        const auto& tenor = *model_->getLiborPeriodDiscretization();
+        const auto& vtd = *vm->getTimeDiscretization();
+        const auto& vtenor = *vm->getLiborPeriodDiscretization();
         for (int i = 0; i < nT; i++)
             for (int j = 0; j < n; j++) {
-                const double tau = tenor.getTime(j) - sim.getTime(i);
+                const double tau = vtenor.getTime(j) - vtd.getTime(ci_of[(size_t)i]);
                 if (tau <= 0) continue;
                 const double e = std::exp(-fp->c * tau), gs = g.volatility[(size_t)i * n + j];
                 g.parameterVega[0] += gs * e;

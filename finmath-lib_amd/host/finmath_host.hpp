@@ -385,6 +385,7 @@ struct Gradient {
     std::vector<double> delta;            // dV/dL_j(0), j < n
     std::vector<double> volatility;       // dV/dsigma_j(t_i), [nT][n]
     std::vector<double> forwardVega;      // sum_i dV/dsigma_j(t_i) (parallel bump of sigma_j)
+    std::vector<double> volatilityModel;  // dV/d(volatility model entry [c][j]), [nTc + 1][n] on its own time grid
     std::vector<double> parameterVega;    // dV/d(a,b,c,d) for the four-parameter form, else empty
 };
 

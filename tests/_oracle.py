@@ -328,3 +328,148 @@ class OracleModel:
         if spec["type"] == "bermudan":
             return self.bermudan(spec)
         return self.swaption(spec) if spec["type"] == "swaption" else self.swap(spec)
+
+
+# ---- tape AAD of the oracle (oracle/lmm_aad.cpp) -------------------------------------------
+
+class AadSpec(ctypes.Structure):
+    _fields_ = [
+        ("nT", c_int), ("times", POINTER(c_double)), ("n", c_int), ("tenor", POINTER(c_double)),
+        ("F", c_int), ("P", c_int), ("seed", c_int32), ("L0", POINTER(c_double)), ("factors", POINTER(c_double)),
+        ("cap", c_double), ("scheme", c_int), ("state_space", c_int), ("pd", POINTER(c_double)),
+        ("vol_mode", c_int), ("vol", POINTER(c_double)), ("abcd", POINTER(c_double)), ("vol_time", POINTER(c_double)),
+        ("dirac", c_int), ("dirac_width", c_double),
+    ]
+
+
+AAD_OPS = {"add": 1, "sub": 2, "mult": 3, "div": 4, "exp": 5, "log": 6, "invert": 7, "cap": 8, "floor": 9,
+           "addProduct": 10, "discount": 11, "accrue": 12, "choose": 13, "condexp": 14}
+DIRAC = {"ONE": 0, "ZERO": 1, "DISCRETE_DELTA": 2}
+
+
+def _aad_lib():
+    L = lib()
+    if not hasattr(L, "_aad_ready"):
+        dp, ip = POINTER(c_double), POINTER(c_int)
+        L.orc_aad_swaptions.argtypes = [c_void_p, c_int, dp, ip, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.orc_aad_bermudan.argtypes = [c_void_p, c_int, ip, dp, dp, dp, dp, c_int, c_int, dp, dp, dp, dp]
+        L.orc_aad_last_error.restype = ctypes.c_char_p
+        L.orc_aad_tape_new.argtypes = [c_int, c_double]
+        L.orc_aad_tape_new.restype = c_void_p
+        L.orc_aad_tape_free.argtypes = [c_void_p]
+        L.orc_aad_input.argtypes = [c_void_p, c_int, c_int, dp]
+        L.orc_aad_estimator.argtypes = [c_void_p, c_int, c_int, dp]
+        L.orc_aad_op.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_double]
+        L.orc_aad_value.argtypes = [c_void_p, c_int, c_int, dp]
+        L.orc_aad_derivative.argtypes = [c_void_p, c_int, c_int, c_int, dp]
+        L._aad_ready = True
+    return L
+
+
+class AadTape:
+    """A bare tape of the oracle's AAD: inputs, operations and per-path derivatives."""
+
+    def __init__(self, size, dirac="DISCRETE_DELTA", width=0.05):
+        self.size = int(size)
+        self.h = _aad_lib().orc_aad_tape_new(DIRAC[dirac], width)
+
+    def close(self):
+        if self.h:
+            _aad_lib().orc_aad_tape_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def input(self, values, differentiable=True):
+        v = f64(np.atleast_1d(values))
+        return _aad_lib().orc_aad_input(self.h, 1 if differentiable else 0, v.size, _dp(v))
+
+    def estimator(self, basis):
+        b = f64(basis)
+        return _aad_lib().orc_aad_estimator(self.h, b.shape[0], b.shape[1], _dp(b))
+
+    def op(self, name, a, b=-1, c=-1, imm=0.0):
+        r = _aad_lib().orc_aad_op(self.h, AAD_OPS[name], a, b, c, float(imm))
+        if r < 0:
+            raise RuntimeError(_aad_lib().orc_aad_last_error().decode())
+        return r
+
+    def value(self, h):
+        out = np.empty(self.size)
+        _aad_lib().orc_aad_value(self.h, h, self.size, _dp(out))
+        return out
+
+    def derivative(self, result, wrt):
+        out = np.empty(self.size)
+        _aad_lib().orc_aad_derivative(self.h, result, wrt, self.size, _dp(out))
+        return out
+
+
+def _aad_spec(tables, P, *, with_discount, vol_mode, abcd, vol_time, vol, state_space, scheme, cap, dirac, width,
+              seed, L0, keep):
+    wl = tables.wl
+    s = AadSpec()
+    times, tenor = f64(tables.times), f64(tables.tenor)
+    L0 = f64(tables.L0 if L0 is None else L0)
+    fac = f64(tables.factors)
+    volt = f64(tables.vol if vol is None else vol)
+    keep += [times, tenor, L0, fac, volt, f64(tables.pd)]
+    s.nT, s.times, s.n, s.tenor = times.size - 1, _dp(times), wl.num_libors, _dp(tenor)
+    s.F, s.P, s.seed = wl.num_factors, int(P), wl.seed if seed is None else seed
+    s.L0, s.factors = _dp(L0), _dp(fac)
+    s.cap = math.inf if cap is None else cap
+    s.scheme, s.state_space = scheme, state_space
+    s.pd = _dp(keep[5]) if with_discount else POINTER(c_double)()
+    s.vol_mode = vol_mode
+    s.vol = _dp(volt)
+    if vol_mode == 1:
+        ab = f64(abcd if abcd is not None else wl.vol_params)
+        vt = f64(times[:-1] if vol_time is None else vol_time)
+        keep += [ab, vt]
+        s.abcd, s.vol_time = _dp(ab), _dp(vt)
+    s.dirac, s.dirac_width = DIRAC[dirac], width
+    return s
+
+
+def aad_swaptions(tables, P, specs, weights=None, *, with_discount=True, vol_mode=0, abcd=None, vol_time=None,
+                  vol=None, state_space=1, scheme=2, cap=1e5, seed=None, L0=None):
+    """Sum_q w_q Swaption_q.getValue and its tape gradient: (value, dY0[n], dvol[nT][n] | None,
+    dabcd[4] | None). vol_mode 0: the volatility entries are the leaves; 1: a, b, c, d."""
+    keep = []
+    s = _aad_spec(tables, P, with_discount=with_discount, vol_mode=vol_mode, abcd=abcd, vol_time=vol_time, vol=vol,
+                  state_space=state_space, scheme=scheme, cap=cap, dirac="ZERO", width=0.05, seed=seed, L0=L0,
+                  keep=keep)
+    ex = f64([sp["exercise"] for sp in specs])
+    nper = np.ascontiguousarray([len(sp["fixing"]) for sp in specs], dtype=np.int32)
+    fix = f64(np.concatenate([sp["fixing"] for sp in specs]))
+    pay = f64(np.concatenate([sp["payment"] for sp in specs]))
+    K = f64(np.concatenate([sp["strike"] for sp in specs]))
+    w = f64(np.ones(len(specs)) if weights is None else weights)
+    n, nT = tables.wl.num_libors, tables.times.size - 1
+    v, dY, dv, dp_ = np.zeros(1), np.zeros(n), np.zeros((nT, n)), np.zeros(4)
+    rc = _aad_lib().orc_aad_swaptions(ctypes.byref(s), len(specs), _dp(ex), nper.ctypes.data_as(POINTER(c_int)),
+                                      _dp(fix), _dp(pay), _dp(K), _dp(w), _dp(v), _dp(dY), _dp(dv), _dp(dp_))
+    if rc != 0:
+        raise RuntimeError(_aad_lib().orc_aad_last_error().decode())
+    return v[0], dY, (dv if vol_mode == 0 else None), (dp_ if vol_mode == 1 else None)
+
+
+def aad_bermudan(tables, P, spec, *, with_discount=True, vol_mode=0, abcd=None, vol_time=None, vol=None,
+                 state_space=1, scheme=2, cap=1e5, dirac="DISCRETE_DELTA", width=0.05, seed=None, L0=None):
+    """BermudanSwaption.getValues and its tape gradient (outputs as aad_swaptions)."""
+    keep = []
+    s = _aad_spec(tables, P, with_discount=with_discount, vol_mode=vol_mode, abcd=abcd, vol_time=vol_time, vol=vol,
+                  state_space=state_space, scheme=scheme, cap=cap, dirac=dirac, width=width, seed=seed, L0=L0,
+                  keep=keep)
+    fix, pay, K = f64(spec["fixing"]), f64(spec["payment"]), f64(spec["strike"])
+    pl = f64(spec.get("period_lengths", pay - fix))
+    flags = np.ascontiguousarray(np.asarray(spec["exercise_flags"], dtype=np.int32))
+    n, nT = tables.wl.num_libors, tables.times.size - 1
+    v, dY, dv, dp_ = np.zeros(1), np.zeros(n), np.zeros((nT, n)), np.zeros(4)
+    rc = _aad_lib().orc_aad_bermudan(ctypes.byref(s), fix.size, flags.ctypes.data_as(POINTER(c_int)), _dp(fix),
+                                     _dp(pl), _dp(pay), _dp(K), 1 if spec.get("callable", True) else 0,
+                                     int(spec.get("basis", 0)), _dp(v), _dp(dY), _dp(dv), _dp(dp_))
+    if rc != 0:
+        raise RuntimeError(_aad_lib().orc_aad_last_error().decode())
+    return v[0], dY, (dv if vol_mode == 0 else None), (dp_ if vol_mode == 1 else None)
