@@ -48,6 +48,7 @@ SOURCES = {
     "lmm_reduce.hip": NO_CONTRACT,
     "lmm_bermudan.hip": NO_CONTRACT,
     "rv.hip": NO_CONTRACT,
+    "lmm_noalloc.hip": NO_CONTRACT,
     "capi.cpp": NO_CONTRACT,
 }
 

@@ -19,6 +19,15 @@ namespace fm {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
+// Set while an entry point that must not allocate runs on this thread: a context buffer
+// that would have to grow is an error instead of a hipMalloc.
+static thread_local bool g_no_alloc = false;
+
+struct NoAllocScope {
+    NoAllocScope() { g_no_alloc = true; }
+    ~NoAllocScope() { g_no_alloc = false; }
+};
+
 // Grow-only device buffer.
 struct DevBuf {
     void* p = nullptr;
@@ -30,6 +39,9 @@ struct DevBuf {
     void* get(size_t need) {
         if (need == 0) need = 8;
         if (need > bytes) {
+            if (g_no_alloc)
+                throw FmError(FM_ERR_STATE, "the context would have to allocate device memory (" + std::to_string(need) +
+                                            " bytes): call the allocating entry point once with this model first");
             if (p) { HIP_CHECK(hipFree(p)); p = nullptr; bytes = 0; }
             HIP_CHECK(hipMalloc(&p, need));
             bytes = need;
@@ -45,6 +57,26 @@ static T* upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
     if (!v.empty()) HIP_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
     return d;
 }
+
+// Device memory handed in by the caller for the per-call buffers of one pass (wave
+// partials, product tables, reduction output), taken in 256-byte aligned pieces.
+struct ExternalScratch {
+    char* base = nullptr;
+    size_t bytes = 0, used = 0;
+    static size_t piece(size_t n) { return (std::max<size_t>(n, 8) + 255) & ~(size_t)255; }
+    void* take(size_t n) {
+        const size_t need = piece(n);
+        if (used + need > bytes) throw FmError(FM_ERR_INTERNAL, "scratch carve-up exceeds its checked size");
+        void* p = base + used;
+        used += need;
+        return p;
+    }
+    template <class T> T* upload(const std::vector<T>& v, hipStream_t s) {
+        T* d = (T*)take(v.size() * sizeof(T));
+        if (!v.empty()) HIP_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        return d;
+    }
+};
 
 // TimeDiscretizationFromArray tick rounding (time/TimeDiscretizationFromArray.java:387-389).
 static double round_tick(double t) {
@@ -692,12 +724,25 @@ void run_bermudan(fm_lmm* c, const BermPlan& b, int bi, const std::vector<double
     HIP_CHECK(hipStreamSynchronize(s));      // cols and scales are host sources of the uploads above
 }
 
+// Scratch bytes of a pass over single-date products with the per-call buffers in caller memory
+// (run_pass with an ExternalScratch).
+size_t external_scratch_bytes(const fm_lmm* c, const Lowered& L) {
+    const size_t num_waves = ((size_t)c->P() + 63) / 64, stride = (size_t)L.num_slots + c->n() + 1;
+    return 2 * ExternalScratch::piece(num_waves * stride * sizeof(double)) +
+           ExternalScratch::piece(L.kp.size() * sizeof(KProduct)) + ExternalScratch::piece(L.per.size() * sizeof(KPeriod)) +
+           ExternalScratch::piece(stride * 2 * sizeof(double));
+}
+
 // Run generator + fused step (+ reductions). Returns per-column sums and M2 (host).
+// With xs, the wave partials, product tables and reduction output live in that caller memory
+// and are cleared by a fill kernel; such a pass takes single-date products only.
 void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums, std::vector<double>& m2s,
-              void* ckpt = nullptr, int ckpt_stride = 1, double* num_out = nullptr) {
+              void* ckpt = nullptr, int ckpt_stride = 1, double* num_out = nullptr, ExternalScratch* xs = nullptr) {
     prepare_model(c);
     hipStream_t s = c->stream;
     const int n = c->n(), P = c->P();
+    if (xs) require(L.num_contrib == 0 && L.berm.empty() && !store && !ckpt, FM_ERR_INTERNAL,
+                    "a pass in caller scratch takes single-date products only");
     Timer tm(c);
     tm.mark(0);
     if (c->cfg.regenerate_increments) c->dw_ready = false;
@@ -706,10 +751,16 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     tm.mark(1);
     const int64_t num_waves = ((int64_t)P + 63) / 64;
     const int stride = L.num_slots + n + 1;
-    double* wsum = (double*)c->d_wave_sum.get((size_t)num_waves * stride * sizeof(double));
-    double* wm2 = (double*)c->d_wave_m2.get((size_t)num_waves * stride * sizeof(double));
-    HIP_CHECK(hipMemsetAsync(wsum, 0, (size_t)num_waves * stride * sizeof(double), s));
-    HIP_CHECK(hipMemsetAsync(wm2, 0, (size_t)num_waves * stride * sizeof(double), s));
+    const size_t wbytes = (size_t)num_waves * stride * sizeof(double);
+    double* wsum = (double*)(xs ? xs->take(wbytes) : c->d_wave_sum.get(wbytes));
+    double* wm2 = (double*)(xs ? xs->take(wbytes) : c->d_wave_m2.get(wbytes));
+    if (xs) {
+        launch_fill_zero(wsum, (size_t)num_waves * stride, s);
+        launch_fill_zero(wm2, (size_t)num_waves * stride, s);
+    } else {
+        HIP_CHECK(hipMemsetAsync(wsum, 0, wbytes, s));
+        HIP_CHECK(hipMemsetAsync(wm2, 0, wbytes, s));
+    }
     StepArgs a{};
     a.nT = c->nT(); a.n = n; a.F = c->F(); a.P = P;
     a.precision = c->cfg.precision;
@@ -732,8 +783,8 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     a.cap = c->cfg.libor_cap;
     a.scheme = c->cfg.scheme;
     a.num_products = (int)L.kp.size();
-    a.products = upload(c->d_products, L.kp, s);
-    a.periods = upload(c->d_periods, L.per, s);
+    a.products = xs ? xs->upload(L.kp, s) : upload(c->d_products, L.kp, s);
+    a.periods = xs ? xs->upload(L.per, s) : upload(c->d_periods, L.per, s);
     a.num_slots = L.num_slots;
     a.num_contrib = L.num_contrib;
     a.contrib = L.num_contrib ? (double*)c->d_contrib.get((size_t)L.num_contrib * P * sizeof(double)) : nullptr;
@@ -773,7 +824,7 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     tm.mark(3);
     // numeraire-adjustment averages first (the swap contributions need them)
     const int cols = L.num_slots + n + 1;
-    double* dout = (double*)c->d_out.get((size_t)cols * 2 * sizeof(double));
+    double* dout = (double*)(xs ? xs->take((size_t)cols * 2 * sizeof(double)) : c->d_out.get((size_t)cols * 2 * sizeof(double)));
     launch_reduce_partials(wsum, wm2, (int)num_waves, stride, L.num_slots, n + 1, P, dout, dout + cols, s);
     sums.assign((size_t)cols, 0.0); m2s.assign((size_t)cols, 0.0);
     HIP_CHECK(hipMemcpyAsync(sums.data(), dout, cols * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1155,16 +1206,41 @@ int fm_lmm_get_brownian_increment(fm_lmm* c, int32_t t, int32_t k, double* out) 
 }
 
 int fm_lmm_value_products(fm_lmm* c, const fm_product* products, int32_t count, double* values, double* errors) {
+    return fm::value_products_in(c, products, count, values, errors, nullptr, 0);
+}
+
+} // extern "C"
+
+namespace fm {
+
+int context_device(const fm_lmm* c) { return c->device; }
+
+int value_products_in(fm_lmm* c, const fm_product* products, int32_t count, double* values, double* errors,
+                      void* scratch, size_t scratch_bytes) {
     FM_TRY
     require(c != nullptr, FM_ERR_INVALID_ARGUMENT, "NULL context");
     require(count >= 0, FM_ERR_INVALID_ARGUMENT, "negative product count");
     require(count == 0 || values != nullptr, FM_ERR_INVALID_ARGUMENT, "values is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
     DeviceGuard g(c->device);
+    std::unique_ptr<NoAllocScope> no_alloc;
+    if (scratch) no_alloc.reset(new NoAllocScope());
     prepare_model(c);
     Lowered L = lower_products(c, products, count);
+    ExternalScratch xs;
+    if (scratch) {
+        require(L.num_contrib == 0 && L.berm.empty(), FM_ERR_UNSUPPORTED,
+                "without allocation only single-date products (swaptions, caplets) are valued: swaps and "
+                "Bermudans need per-path columns");
+        const size_t need = external_scratch_bytes(c, L);
+        require(need <= scratch_bytes, FM_ERR_INVALID_ARGUMENT,
+                "the pass needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                std::to_string(scratch_bytes) + " bytes of the no-allocation entry point (fewer paths or products)");
+        xs.base = (char*)scratch;
+        xs.bytes = scratch_bytes;
+    }
     std::vector<double> sums, m2s;
-    run_pass(c, L, false, sums, m2s);
+    run_pass(c, L, false, sums, m2s, nullptr, 1, nullptr, scratch ? &xs : nullptr);
     const std::vector<double> adj = adjustment_chain(c);
     const double P = (double)c->P();
     for (int q = 0; q < count; q++) {
@@ -1180,6 +1256,10 @@ int fm_lmm_value_products(fm_lmm* c, const fm_product* products, int32_t count, 
     return FM_OK;
     FM_CATCH
 }
+
+} // namespace fm
+
+extern "C" {
 
 int fm_lmm_set_volatility_factorization(fm_lmm* c, const double* volatility, const double* factors,
                                         const double* variance_factor) {

@@ -158,6 +158,8 @@ void launch_lmm_adjoint_fp64(const StepArgs& a, const AdjArgs& g, void* stream);
 void launch_lmm_adjoint_fp32(const StepArgs& a, const AdjArgs& g, void* stream);
 
 // Deterministic reduction of per-wave partials: out_sum[col], out_m2[col] (Chan merge).
+// Zero count doubles with a kernel (no memset node: safe inside a captured stream).
+void launch_fill_zero(double* p, size_t count, void* stream);
 void launch_reduce_partials(const double* wave_sum, const double* wave_m2, int num_waves, int stride,
                             int num_slots, int num_sums, int P, double* out_sum, double* out_m2, void* stream);
 
@@ -228,5 +230,13 @@ fm_rv* rv_from_device(int device, const void* src, bool is_float, int64_t size, 
 fm_rv* rv_constant(double time, double value);
 // Numeraire adjustment (x zb) / adj, or (1/adj) (x zb) in the Scalar form (new random variable).
 fm_rv* rv_scale_numeraire(fm_rv* x, double zb, double adj, bool scalar_form);
+
+// ---- entry points over caller scratch (capi.cpp) -------------------------------------------
+int context_device(const fm_lmm* c);
+// fm_lmm_value_products; with scratch != NULL the per-call device buffers (wave partials, product
+// tables, reduction output) are taken from [scratch, scratch + scratch_bytes) and no context
+// buffer may grow (FM_ERR_STATE), so the call allocates no device memory.
+int value_products_in(fm_lmm* c, const fm_product* products, int32_t count, double* values, double* errors,
+                      void* scratch, size_t scratch_bytes);
 
 } // namespace fm

@@ -8,6 +8,7 @@
 // result is deterministic run to run.
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "fm_internal.hpp"
 #include "hip_check.hpp"
 #include "lmm_step.hpp"
@@ -144,6 +145,18 @@ void launch_reduce_columns(const double* rows, int num_rows, int stride, int col
 }
 
 int reduce_columns_scratch(int cols) { return kColChunks * cols; }
+
+static __global__ void __launch_bounds__(256) fill_zero_kernel(double* __restrict__ p, size_t count) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+void launch_fill_zero(double* p, size_t count, void* stream) {
+    if (count == 0) return;
+    const size_t blocks = std::min<size_t>((count + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, p, count);
+    HIP_CHECK(hipGetLastError());
+}
 
 void launch_reduce_partials(const double* wave_sum, const double* wave_m2, int num_waves, int stride,
                             int num_slots, int num_sums, int P, double* out_sum, double* out_m2, void* stream) {

@@ -40,7 +40,7 @@ EXPORTED_SYMBOLS = (
     "fm_lmm_set_heston_volatility",
     "fm_lmm_simulate", "fm_lmm_get_process_value",
     "fm_lmm_process_value_device_ptr", "fm_lmm_get_numeraire", "fm_lmm_get_brownian_increment",
-    "fm_lmm_value_products", "fm_lmm_set_volatility_factorization", "fm_lmm_adjoint",
+    "fm_lmm_value_products", "fm_lmm_value_products_noalloc", "fm_lmm_set_volatility_factorization", "fm_lmm_adjoint",
     "fm_lmm_bermudan_exercise_times", "fm_lmm_serialize", "fm_lmm_deserialize",
     "fm_mt19937_words", "fm_mt19937_doubles",
     "fm_last_error", "fm_lmm_last_timings", "fm_device_info",
@@ -136,6 +136,7 @@ def load() -> ctypes.CDLL:
         "fm_lmm_get_numeraire": ([c_void_p, c_double, dp], c_int32),
         "fm_lmm_get_brownian_increment": ([c_void_p, c_int32, c_int32, dp], c_int32),
         "fm_lmm_value_products": ([c_void_p, POINTER(FmProduct), c_int32, dp, dp], c_int32),
+        "fm_lmm_value_products_noalloc": ([c_void_p, POINTER(FmProduct), c_int32, dp, dp], c_int32),
         "fm_lmm_set_volatility_factorization": ([c_void_p, dp, dp, dp], c_int32),
         "fm_lmm_adjoint": ([c_void_p, POINTER(FmProduct), c_int32, dp, dp, dp, dp], c_int32),
         "fm_lmm_bermudan_exercise_times": ([c_void_p, c_int32, dp], c_int32),
@@ -376,6 +377,15 @@ class LmmContext:
         vals = np.empty(len(products))
         errs = np.empty(len(products))
         check(self.lib.fm_lmm_value_products(self.h, arr, len(products), _dptr(vals), _dptr(errs)))
+        return vals, errs
+
+    def value_products_noalloc(self, products):
+        """value_products without device allocation (single-date products; the context must
+        have been used by value_products once)."""
+        arr = product_array(products)
+        vals = np.empty(len(products))
+        errs = np.empty(len(products))
+        check(self.lib.fm_lmm_value_products_noalloc(self.h, arr, len(products), _dptr(vals), _dptr(errs)))
         return vals, errs
 
     def set_volatility_factorization(self, volatility, factors, variance_factor=None):

@@ -258,6 +258,21 @@ typedef struct fm_product {
 int fm_lmm_value_products(fm_lmm* ctx, const fm_product* products, int32_t count,
                           double* values, double* errors);
 
+/* fm_lmm_value_products without device allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (tight pricing loops, harnesses that forbid
+ * hipMalloc). Same inputs, outputs and results as fm_lmm_value_products. The per-call device
+ * buffers (wave partials, product tables, reduction output) come from a 64 MiB array that
+ * every GPU holds from the library's first use until the process ends; a pass needing more
+ * (about 16 B x (products + forwards + 1) per 64 paths) fails with FM_ERR_INVALID_ARGUMENT.
+ * Only single-date products (swaptions, caplets): swaps and Bermudans need per-path columns
+ * (FM_ERR_UNSUPPORTED). The context's own buffers (model tables, generator states, Brownian
+ * increments) must already have their size: call fm_lmm_value_products once on the context
+ * first, and again after a change that adds a table (a first local-volatility offset table);
+ * a call that would have to grow them fails with FM_ERR_STATE. Calls on one GPU share the
+ * array and are serialised by the library; each returns after its kernels have finished. */
+int fm_lmm_value_products_noalloc(fm_lmm* ctx, const fm_product* products, int32_t count,
+                                  double* values, double* errors);
+
 /* Exercise time of every path of Bermudan swaption number `bermudan` (counting only the
  * Bermudan products, in order) of the last fm_lmm_value_products / fm_lmm_adjoint call:
  * the fixing date of the period at which the backward induction exercised, +infinity if
