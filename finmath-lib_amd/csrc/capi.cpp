@@ -812,6 +812,11 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
         if (c->fp32()) launch_lmm_step_lv_fp32(a, s);
         else launch_lmm_step_lv_fp64(a, s);
     } else if (c->plain_euler()) {
+        // the spot kernel's initial states: FP64 log-normal the log-forwards capped at log(cap),
+        // otherwise the capped initial forwards
+        if (!ckpt)
+            a.Y0 = (!c->fp32() && c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL) ? c->d_Y0cap.as<double>()
+                                                                                   : c->d_L0.as<double>();
         if (c->fp32()) launch_lmm_step_fp32(a, s);
         else launch_lmm_step_fp64(a, s);
     } else if (c->pc()) {

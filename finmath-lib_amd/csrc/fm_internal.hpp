@@ -97,7 +97,8 @@ struct StepArgs {
     const double* delta;         // device n: T_{j+1}-T_j
     const double* L0;            // device n: initial forwards in value space
     const double* Y0;            // device n: initial states (log(max(L0, 0)) or L0) of the carried schemes;
-                                 // for the log-forward kernel kept below log(cap)
+                                 // for the log-forward and spot kernels kept below log(cap)
+                                 // (the spot kernel in the normal state space: the capped forwards)
     double logcap;               // log(cap) for the log-forward kernel
     int terminal;                // TERMINAL measure (general kernel)
     int carry;                   // EULER / PREDICTOR_CORRECTOR: the state is carried (general kernel)
