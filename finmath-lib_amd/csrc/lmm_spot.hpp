@@ -89,10 +89,12 @@ template <> __device__ __forceinline__ float spot_div<float>(float a, float b) {
 template <typename real> constexpr bool carries_log() { return std::is_same<real, double>::value; }
 
 // One Euler step of all alive components, in place. Y holds log-forwards (FP64, log-normal) or
-// forwards; ycap is log(cap) for log-forwards, cap otherwise.
-template <int NCH, int FCAP, typename real>
+// forwards; ycap is log(cap) for log-forwards, cap otherwise. The state space is a template
+// argument: a run-time branch around the exponential would split the component's block and
+// expose the latency of its record load.
+template <int NCH, int FCAP, typename real, bool lognormal>
 __device__ __forceinline__ void spot_step(ChunkedForwards<NCH, real>& Y, int n, int i, int first, real dt,
-                                          const real (&dW)[FCAP], const real* __restrict__ rec, bool lognormal,
+                                          const real (&dW)[FCAP], const real* __restrict__ rec,
                                           bool capped, real ycap, real* __restrict__ paths,
                                           const int32_t* __restrict__ row_next, int64_t p, int P, bool active) {
     constexpr bool LOGCARRY = carries_log<real>();
@@ -134,7 +136,7 @@ __device__ __forceinline__ void spot_step(ChunkedForwards<NCH, real>& Y, int n, 
     });
 }
 
-template <int NCH, int FCAP, typename real>
+template <int NCH, int FCAP, typename real, bool lognormal>
 __global__ void __launch_bounds__(kStepThreads)
 lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __restrict__ delta,
                 const double* __restrict__ times, const int32_t* __restrict__ first_alive,
@@ -152,7 +154,6 @@ lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __
     const int64_t pc = active ? p : (int64_t)P - 1;
     const int cnt = (int)((int64_t)P - wave * 64 > 64 ? 64 : ((int64_t)P - wave * 64));
     const int stride = a.num_slots + n + 1;
-    const bool lognormal = a.lognormal != 0;
     const bool capped = a.capped != 0;
     constexpr bool LOGCARRY = carries_log<real>();
     const real ycap = (LOGCARRY && lognormal) ? (real)a.logcap : (real)a.cap;
@@ -267,18 +268,25 @@ lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __
         load_increments<FCAP, real>(dW, dw, pc, nT, ti, F);
         const real dt = (real)(times[ti + 1] - times[ti]);
         const int32_t* row_next = row_of + (size_t)(ti + 1) * n;
-        spot_step<NCH, FCAP, real>(Y, n, ti, first_alive[ti], dt, dW, rec, lognormal, capped, ycap, paths, row_next,
-                                   p, P, active);
+        spot_step<NCH, FCAP, real, lognormal>(Y, n, ti, first_alive[ti], dt, dW, rec, capped, ycap, paths, row_next,
+                                              p, P, active);
     }
+}
+
+template <int NCH, int FCAP, typename real, bool lognormal>
+void launch_spot_ss(const StepArgs& a, hipStream_t stream) {
+    const int64_t blocks = ((int64_t)a.P + kStepThreads - 1) / kStepThreads;
+    hipLaunchKernelGGL((lmm_spot_kernel<NCH, FCAP, real, lognormal>), dim3((unsigned)blocks), dim3(kStepThreads), 0,
+                       stream, a, (const real*)a.rec, a.delta, a.times, a.first_alive, a.tenor_time_index, a.products,
+                       a.periods, (const real*)a.dw, a.wave_sum, a.wave_m2, a.contrib, (real*)a.paths, a.row_of,
+                       a.numeraire);
+    HIP_CHECK(hipGetLastError());
 }
 
 template <int NCH, int FCAP, typename real>
 void launch_spot_cap(const StepArgs& a, hipStream_t stream) {
-    const int64_t blocks = ((int64_t)a.P + kStepThreads - 1) / kStepThreads;
-    hipLaunchKernelGGL((lmm_spot_kernel<NCH, FCAP, real>), dim3((unsigned)blocks), dim3(kStepThreads), 0, stream, a,
-                       (const real*)a.rec, a.delta, a.times, a.first_alive, a.tenor_time_index, a.products, a.periods,
-                       (const real*)a.dw, a.wave_sum, a.wave_m2, a.contrib, (real*)a.paths, a.row_of, a.numeraire);
-    HIP_CHECK(hipGetLastError());
+    if (a.lognormal) launch_spot_ss<NCH, FCAP, real, true>(a, stream);
+    else launch_spot_ss<NCH, FCAP, real, false>(a, stream);
 }
 
 template <int NCH, typename real>
