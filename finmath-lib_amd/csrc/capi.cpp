@@ -814,9 +814,8 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     } else if (c->plain_euler()) {
         // the spot kernel's initial states: FP64 log-normal the log-forwards capped at log(cap),
         // otherwise the capped initial forwards
-        if (!ckpt)
-            a.Y0 = (!c->fp32() && c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL) ? c->d_Y0cap.as<double>()
-                                                                                   : c->d_L0.as<double>();
+        a.Y0 = (!c->fp32() && c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL) ? c->d_Y0cap.as<double>()
+                                                                               : c->d_L0.as<double>();
         if (c->fp32()) launch_lmm_step_fp32(a, s);
         else launch_lmm_step_fp64(a, s);
     } else if (c->pc()) {
@@ -1388,6 +1387,7 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     ga.kcur_at = upload(c->d_kcur, kcur, s);
     ga.weight = upload(c->d_scale, kweights, s);
     ga.berm_ex = L.berm.empty() ? nullptr : c->d_berm_ex.as<int32_t>();
+    ga.berm_weight = nullptr;
     // factor matrix padded to the factor capacity and -sigma_j(t_i) c_j, in the path precision
     std::vector<double> fpad((size_t)n * fcap_of(c->F()), 0.0), vterm((size_t)nT * n, 0.0);
     for (int j = 0; j < n; j++)

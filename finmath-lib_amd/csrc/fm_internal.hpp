@@ -154,6 +154,8 @@ struct AdjArgs {
     const void* vterm;               // [nT][n] -sigma_j(t_i) c_j, d(variance * -1/2)/dsigma (path precision)
     double* wave_grad;               // [num_waves][nT*n + n] partials of dV/dsigma and dV/dL(0)
     const int32_t* berm_ex;          // [num Bermudans][P] exercise period per path (-1: never), or NULL
+    const double* berm_weight;       // [contribution column][P] weight of each Bermudan period's payoff on
+                                     // each path (the induction's adjoint), or NULL: 1 or 0 by berm_ex
 };
 void launch_lmm_adjoint_fp64(const StepArgs& a, const AdjArgs& g, void* stream);
 void launch_lmm_adjoint_fp32(const StepArgs& a, const AdjArgs& g, void* stream);

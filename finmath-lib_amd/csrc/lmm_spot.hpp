@@ -79,6 +79,22 @@ struct ChunkedForwards {
         });
         return r;
     }
+    // Element j += x for a uniform run-time j: one indexed register write in the chunk of j. The
+    // asm takes the chunk as an input and output, which keeps it a register value.
+    __device__ __forceinline__ void add(int j, real x) {
+        const int e = j & 7, qj = j >> 3;
+        static_for<0, NCH>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            if (qj == q) {
+                V v = ch<q>();
+#if defined(__HIP_DEVICE_COMPILE__)
+                asm volatile("" : "+v"(v));
+#endif
+                v[e] = v[e] + x;
+                ch<q>() = v;
+            }
+        });
+    }
 };
 
 template <typename real> __device__ __forceinline__ real spot_div(real a, real b);
@@ -144,7 +160,8 @@ lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __
                 const KPeriod* __restrict__ periods, const real* __restrict__ dw,
                 double* __restrict__ wave_sum, double* __restrict__ wave_m2,
                 double* __restrict__ contrib, real* __restrict__ paths,
-                const int32_t* __restrict__ row_of, double* __restrict__ numeraire_out)
+                const int32_t* __restrict__ row_of, double* __restrict__ numeraire_out,
+                real* __restrict__ ckpt)
 {
     const int64_t p = (int64_t)blockIdx.x * kStepThreads + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -170,6 +187,15 @@ lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __
     int qp = 0;
 
     for (int ti = 0; ti <= nT; ti++) {
+        // ---- the carried states every ckpt_stride steps, for the adjoint's reverse sweep:
+        // [ti / stride][j][P]
+        if (ckpt && active && (ti % a.ckpt_stride) == 0) {
+            const size_t base = (size_t)(ti / a.ckpt_stride) * n;
+            static_for<0, 8 * NCH>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                if (j < n) ckpt[(base + j) * P + p] = Y.template at<j>();
+            });
+        }
         // ---- N(T_{k+1}) = N(T_k).accrue(L_k(T_k), d_k) once L_k is final (LMM:1050-1066)
         while (kcur < n && tenor_ti[kcur] <= ti) {
             const double Lk = (double)val(Y.get(kcur));
@@ -279,7 +305,7 @@ void launch_spot_ss(const StepArgs& a, hipStream_t stream) {
     hipLaunchKernelGGL((lmm_spot_kernel<NCH, FCAP, real, lognormal>), dim3((unsigned)blocks), dim3(kStepThreads), 0,
                        stream, a, (const real*)a.rec, a.delta, a.times, a.first_alive, a.tenor_time_index, a.products,
                        a.periods, (const real*)a.dw, a.wave_sum, a.wave_m2, a.contrib, (real*)a.paths, a.row_of,
-                       a.numeraire);
+                       a.numeraire, (real*)a.ckpt);
     HIP_CHECK(hipGetLastError());
 }
 

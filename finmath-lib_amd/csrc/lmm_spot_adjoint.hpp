@@ -1,0 +1,325 @@
+// Pathwise adjoint (reverse sweep) of the spot Euler pricing pass.
+//
+// The pricing pass (lmm_spot_kernel) stores the carried state of every path every `stride`
+// time steps and the numeraire N(T_k). The reverse sweep walks the time indices backwards;
+// for each index t it reloads the nearest checkpoint from device memory and rebuilds the state
+// at t with the pricing pass's own step (spot_step), so the sweep differentiates the paths that
+// were priced, then
+//   1. reverses the Euler step t -> t+1 (adjoint of L(t+1) -> adjoint of L(t)),
+//   2. adds the direct sensitivities of the products evaluated at t,
+//   3. reverses the numeraire updates N(T_{k+1}) = N(T_k)(1 + L_k d_k) made at t.
+// Per alive forward-step it reduces dV/dsigma_j(t_i) over the wavefront, where the factor
+// loading is lambda_jk = sigma_j f_jk and the variance sigma_j^2 c_j
+// (LIBORCovarianceModelFromVolatilityAndCorrelation.java:48-93); at the end it reduces dV/dL_j(0).
+//
+// Per path the sweep holds two chunked register arrays, the state at t and the adjoint of the
+// forwards, and processes components in rolled loops (lmm_spot.hpp), so 80 forwards x 10 factors
+// fit in registers at two waves per SIMD without scratch.
+//
+// Conventions of the reference's AAD (RandomVariableDifferentiableAAD.java:165-219): averages
+// over paths (the numeraire-adjustment factor) are constants, the floor of the swaption payoff
+// has derivative 1 at and above zero and 0 below. A Bermudan period's payoff enters with a
+// per-path weight: 1 or 0 by the recorded exercise (the exercise decisions held fixed), or the
+// weights of the induction's own adjoint (lmm_bermudan.hip) when given.
+#pragma once
+
+#include "lmm_spot.hpp"
+#include "lmm_adjoint.hpp"
+
+namespace fm {
+
+template <int NCH, int FCAP, typename real, bool lognormal>
+__global__ void __launch_bounds__(kStepThreads)
+lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restrict__ rec,
+                        const double* __restrict__ delta, const double* __restrict__ times,
+                        const int32_t* __restrict__ first_alive, const int32_t* __restrict__ kcur_at,
+                        const KProduct* __restrict__ products, const KPeriod* __restrict__ periods,
+                        const real* __restrict__ dw, const real* __restrict__ ckpt,
+                        const double* __restrict__ numeraire, const double* __restrict__ weight,
+                        const real* __restrict__ fsig, const real* __restrict__ vterm,
+                        double* __restrict__ wave_grad)
+{
+    constexpr bool LOGCARRY = carries_log<real>() && lognormal;   // the state is the log-forward
+    const int64_t p = (int64_t)blockIdx.x * kStepThreads + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = p >> 6;
+    const int P = a.P, n = a.n, F = a.F, nT = a.nT;
+    const bool active = p < P;
+    const int64_t pc = active ? p : (int64_t)P - 1;
+    const int cnt = (int)((int64_t)P - wave * 64 > 64 ? 64 : ((int64_t)P - wave * 64));
+    const int gstride = nT * n + n;
+    const bool capped = a.capped != 0;
+    const real ycap = LOGCARRY ? (real)a.logcap : (real)a.cap;
+    const int stride = g.stride;
+    auto val = [&](real x) -> real { return LOGCARRY ? dev_exp<real>(x) : x; };
+
+    ChunkedForwards<NCH, real> X;     // state at t
+    ChunkedForwards<NCH, real> Lb;    // adjoint of the forwards L_j(t)
+    static_for<0, 8 * NCH>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        Lb.template put<j>((real)0);
+    });
+    double Nb_top = 0.0, Nb_low = 0.0;      // adjoints of N(T_K) and N(T_{K-1})
+    int K = kcur_at[nT];
+    int qp = a.num_products - 1;            // products sorted by evaluation index
+
+    for (int c = (nT / stride) * stride; c >= 0; c -= stride) {
+        const size_t base = (size_t)(c / stride) * n;
+        const int hi = (c + stride - 1 < nT) ? c + stride - 1 : nT;
+        for (int t = hi; t >= c; t--) {
+            // ---- the state at t: the checkpoint at c, then the pricing pass's steps c .. t-1.
+            // The checkpoint is read again for every t (coalesced, from L2): the empty asm makes
+            // its offset opaque, so the compiler does not hoist the loads out of the t loop and
+            // keep a third register array of forwards live across the segment.
+            size_t row = base;
+#if defined(__HIP_DEVICE_COMPILE__)
+            asm volatile("" : "+s"(row));
+#endif
+            static_for<0, 8 * NCH>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                X.template put<j>((j < n) ? ckpt[(row + j) * P + pc] : (real)0);
+            });
+            for (int s = c; s < t; s++) {
+                real dWs[FCAP];
+                load_increments<FCAP, real>(dWs, dw, pc, nT, s, F);
+                spot_step<NCH, FCAP, real, lognormal>(X, n, s, first_alive[s], (real)(times[s + 1] - times[s]), dWs,
+                                                      rec, capped, ycap, (real*)nullptr, (const int32_t*)nullptr, p, P,
+                                                      active);
+            }
+
+            // ---- 1. reverse the Euler step t -> t+1
+            if (t < nT) {
+                const int first = first_alive[t];
+                const real dt = (real)(times[t + 1] - times[t]);
+                real dW[FCAP], S[FCAP], Sb[FCAP];
+                load_increments<FCAP, real>(dW, dw, pc, nT, t, F);
+#pragma unroll
+                for (int k = 0; k < FCAP; k++) { S[k] = (real)0; Sb[k] = (real)0; }
+                // pass A (ascending): recompute the step; adjoint through exp, log and the cap
+                static_for<0, NCH>([&](auto Q) {
+                    constexpr int q = decltype(Q)::value;
+                    const int j0 = first > 8 * q ? first : 8 * q;
+                    const int j1 = n < 8 * q + 8 ? n : 8 * q + 8;
+#pragma unroll 1
+                    for (int j = j0; j < j1; j++) {
+                        const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, t, j);
+                        const real x = X.template ch<q>()[j - 8 * q];
+                        const real Lj = val(x);
+                        real w = spot_div<real>((real)1.0, Lj * R.c1() + R.c2());
+                        if (lognormal) w = w * Lj;
+#pragma unroll
+                        for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * R.lam(k);
+                        real mu = S[0] * R.lam(0);
+#pragma unroll
+                        for (int k = 1; k < FCAP; k++) mu = mu + S[k] * R.lam(k);
+                        mu = mu + R.hv();
+                        real y = (!LOGCARRY && lognormal) ? dev_log<real>(Lj) : x;
+                        y = y + mu * dt;
+#pragma unroll
+                        for (int k = 0; k < FCAP; k++) y = y + dW[k] * R.lam(k);
+                        // adjoint of the new state from the adjoint of the new value (zero where
+                        // the cap binds: cap has derivative 0 above its level)
+                        const real lb = Lb.template ch<q>()[j - 8 * q];
+                        real yb;
+                        if (LOGCARRY) {
+                            yb = (capped && y > ycap) ? (real)0 : lb * dev_exp<real>(y);
+                        } else {
+                            const real E = lognormal ? dev_exp<real>(y) : y;
+                            yb = (capped && E > ycap) ? (real)0 : (lognormal ? lb * E : lb);
+                        }
+                        // direct part of the adjoint of L_j(t): y = log(L_j(t)) + ...
+                        Lb.template ch<q>()[j - 8 * q] = lognormal ? yb * adj_rcp<real>(Lj) : yb;
+                    }
+                });
+                // pass B (descending): the drift chain S_k = sum_{l<=j} w_l lambda_lk; the per-path
+                // dV/dsigma_j(t) of groups of four j are summed over the wavefront together
+                double gq0 = 0.0, gq1 = 0.0, gq2 = 0.0, gq3 = 0.0;
+                static_for<0, NCH>([&](auto QR) {
+                    constexpr int q = NCH - 1 - decltype(QR)::value;
+                    const int j0 = first > 8 * q ? first : 8 * q;
+                    const int j1 = n < 8 * q + 8 ? n : 8 * q + 8;
+#pragma unroll 1
+                    for (int j = j1 - 1; j >= j0; j--) {
+                        const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, t, j);
+                        const real Lj = val(X.template ch<q>()[j - 8 * q]);
+                        const real r = spot_div<real>((real)1.0, Lj * R.c1() + R.c2());
+                        const real w = lognormal ? r * Lj : r;
+                        // adjoint of Y_j(t+1) from the direct part left by pass A
+                        const real lb = Lb.template ch<q>()[j - 8 * q];
+                        const real Yb = lognormal ? lb * Lj : lb;
+                        const real mb = Yb * dt;
+                        real wb = (real)0;
+                        real sgb = (real)0;             // dV/dsigma_j(t) on this path
+#pragma unroll
+                        for (int k = 0; k < FCAP; k++) {
+                            Sb[k] = Sb[k] + mb * R.lam(k);
+                            wb = wb + Sb[k] * R.lam(k);
+                            const real lamb = Yb * dW[k] + mb * S[k] + Sb[k] * w;
+                            sgb = sgb + lamb * fsig[j * FCAP + k];
+                            S[k] = S[k] - w * R.lam(k);    // S^(j) -> S^(j-1)
+                        }
+                        // variance sigma^2 c in the drift: d(-var/2)/dsigma = -sigma c
+                        if (lognormal) sgb = sgb + mb * vterm[(size_t)t * n + j];
+                        const real dwdL = lognormal ? R.c2() * r * r : -R.c1() * r * r;
+                        Lb.template ch<q>()[j - 8 * q] = lb + wb * dwdL;
+                        const double gs = active ? (double)sgb : 0.0;
+                        const int m = j & 3;
+                        gq0 = (m == 0) ? gs : gq0;
+                        gq1 = (m == 1) ? gs : gq1;
+                        gq2 = (m == 2) ? gs : gq2;
+                        gq3 = (m == 3) ? gs : gq3;
+                        if (m == 0 || j == first) {     // uniform: the group of four is complete
+                            const double v4[4] = {gq0, gq1, gq2, gq3};
+                            const double s = wave_sum4(v4, lane);
+                            const int jw = (j & ~3) + 2 * (lane & 1) + ((lane >> 1) & 1);
+                            if (lane < 4 && cnt > 0 && jw < n && jw >= first)
+                                wave_grad[(size_t)wave * gstride + (size_t)t * n + jw] = s;
+                            gq0 = gq1 = gq2 = gq3 = 0.0;
+                        }
+                    }
+                });
+            }
+
+            // ---- 2. products evaluated at t (reverse order of evaluation)
+            const int Kt = kcur_at[t];
+            while (qp >= 0 && products[qp].eval_index == t) {
+                const KProduct pr = products[qp];
+                const double wq = weight[qp];
+                qp--;
+                const bool top = (pr.num_k == Kt);
+                const double Nsel = (pr.num_k == 0) ? 1.0 : numeraire[(size_t)pr.num_k * P + pc];
+                if (pr.type == FM_PRODUCT_SWAPTION) {
+                    const int j0 = pr.first_j, j1 = pr.first_j + pr.num_periods;
+                    // V = sum_j p_j prod_{l<=j} D_l, D_l = adj_l / (1 + L_l tau_l)
+                    real V = (real)0, Pc = (real)1;
+#pragma unroll 1
+                    for (int j = j1 - 1; j >= j0; j--) {
+                        const KPeriod pe = periods[pr.param_off + (j - j0)];
+                        const real Lj = val(X.get(j));
+                        const real D = (real)pe.adj * adj_rcp<real>((real)1.0 + Lj * (real)pe.tau);
+                        V = (V + ((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pr.notional) * D;
+                        Pc = Pc * D;
+                    }
+                    if (V >= (real)0) {     // floor(0.0): derivative 1 at and above the floor
+                        const double Vb = wq / Nsel;
+                        const double nb = -wq * (double)V / (Nsel * Nsel);
+                        if (top) Nb_top += nb; else Nb_low += nb;
+                        // descending over j: P_j = prod_{l<=j} D_l, U_j = sum_{m>=j} p_m P_m
+                        real U = (real)0;
+#pragma unroll 1
+                        for (int j = j1 - 1; j >= j0; j--) {
+                            const KPeriod pe = periods[pr.param_off + (j - j0)];
+                            const real Lj = val(X.get(j));
+                            const real one = (real)1.0 + Lj * (real)pe.tau;
+                            const real inv = adj_rcp<real>(one);
+                            const real pj = ((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pr.notional;
+                            U = U + pj * Pc;
+                            const real dV = (real)pe.period_length * (real)pr.notional * Pc - ((real)pe.tau * inv) * U;
+                            Lb.add(j, (real)Vb * dV);
+                            Pc = Pc * (one * (real)pe.inv_adj);      // P_{j-1} = P_j / D_j
+                        }
+                    }
+                } else if (pr.type == FM_PRODUCT_CAPLET) {
+                    const KPeriod pe = periods[pr.param_off];
+                    const real Lj = val(X.get(pr.first_j));
+                    const real v = Lj - (real)pe.strike;
+                    // floor(0.0) has derivative 1 for v >= 0, cap(0.0) has derivative 1 for v < 0
+                    // (RandomVariableDifferentiableAAD.java:333-348)
+                    const bool itm = pr.is_floorlet ? (v < (real)0) : (v >= (real)0);
+                    if (itm) {
+                        const double dcf = pe.notional;
+                        const double x = pr.is_floorlet ? -(double)v * dcf : (double)v * dcf;
+                        const double dLj = (pr.is_floorlet ? -dcf : dcf) * wq / Nsel;
+                        const double nb = -wq * x / (Nsel * Nsel);
+                        if (top) Nb_top += nb; else Nb_low += nb;
+                        Lb.add(pr.first_j, (real)dLj);
+                    }
+                } else if (pr.type == K_BERM_BASIS) {
+                    // regression basis: not differentiated (the reference's conditional
+                    // expectation treats its basis functions as constants)
+                } else {
+                    double pw = 1.0;       // the payoff's weight on this path
+                    if (pr.type == K_BERM_PERIOD) {
+                        if (g.berm_weight) {
+                            pw = g.berm_weight[(size_t)pr.out_slot * P + pc];
+                        } else {
+                            // the period pays on this path if the exercise (callable: entered at or
+                            // before it; cancelable: terminated after it) keeps it in the value
+                            const int ex = g.berm_ex[(size_t)pr.pad2 * P + pc];
+                            pw = ((pr.flags & 2) ? (ex >= 0 && pr.is_floorlet >= ex)
+                                                 : (ex < 0 || pr.is_floorlet < ex)) ? 1.0 : 0.0;
+                        }
+                        if (pw == 0.0) continue;
+                    }
+                    const KPeriod pe = periods[pr.param_off];
+                    const real Lj = val(X.get(pr.first_j));
+                    const double sgn = (pr.flags & 1) ? 1.0 : -1.0;
+                    const double pay = sgn * (double)(((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pe.notional);
+                    const double dLj = pw * sgn * pe.period_length * pe.notional * wq / Nsel;
+                    const double nb = -pw * wq * pay / (Nsel * Nsel);
+                    if (top) Nb_top += nb; else Nb_low += nb;
+                    Lb.add(pr.first_j, (real)dLj);
+                }
+            }
+
+            // ---- 3. numeraire updates made at t: N_{k+1} = N_k (1 + L_k d_k), k = K-1 down to kcur_at[t-1]
+            const int Kprev = (t > 0) ? kcur_at[t - 1] : 0;
+            while (K > Kprev) {
+                const int k = K - 1;
+                const double Nk = (k == 0) ? 1.0 : numeraire[(size_t)k * P + pc];
+                const double Lk = (double)val(X.get(k));
+                const double dk = delta[k];
+                Lb.add(k, (real)(Nb_top * dk * Nk));
+                Nb_low += Nb_top * (1.0 + Lk * dk);
+                Nb_top = Nb_low;
+                Nb_low = 0.0;
+                K--;
+            }
+        }
+    }
+    // ---- dV/dL_j(0)
+    static_for<0, 8 * NCH>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if (j < n) {
+            const double s = wave_sum_dpp(active ? (double)Lb.template at<j>() : 0.0);
+            if (lane == 63 && cnt > 0) wave_grad[(size_t)wave * gstride + (size_t)nT * n + j] = s;
+        }
+    });
+}
+
+template <int NCH, int FCAP, typename real, bool lognormal>
+void launch_spot_adjoint_ss(const StepArgs& a, const AdjArgs& g, hipStream_t stream) {
+    const int64_t blocks = ((int64_t)a.P + kStepThreads - 1) / kStepThreads;
+    hipLaunchKernelGGL((lmm_spot_adjoint_kernel<NCH, FCAP, real, lognormal>), dim3((unsigned)blocks),
+                       dim3(kStepThreads), 0, stream, a, g, (const real*)a.rec, a.delta, a.times, a.first_alive,
+                       g.kcur_at, a.products, a.periods, (const real*)a.dw, (const real*)a.ckpt, a.numeraire,
+                       g.weight, (const real*)g.fsig, (const real*)g.vterm, g.wave_grad);
+    HIP_CHECK(hipGetLastError());
+}
+
+template <int NCH, int FCAP, typename real>
+void launch_spot_adjoint_cap(const StepArgs& a, const AdjArgs& g, hipStream_t stream) {
+    if (a.lognormal) launch_spot_adjoint_ss<NCH, FCAP, real, true>(a, g, stream);
+    else launch_spot_adjoint_ss<NCH, FCAP, real, false>(a, g, stream);
+}
+
+template <int NCH, typename real>
+void launch_spot_adjoint_n(const StepArgs& a, const AdjArgs& g, hipStream_t stream) {
+    if (a.F <= 1) launch_spot_adjoint_cap<NCH, 1, real>(a, g, stream);
+    else if (a.F <= 3) launch_spot_adjoint_cap<NCH, 3, real>(a, g, stream);
+    else if (a.F <= 5) launch_spot_adjoint_cap<NCH, 5, real>(a, g, stream);
+    else if (a.F <= 10) launch_spot_adjoint_cap<NCH, 10, real>(a, g, stream);
+    else throw FmError(FM_ERR_UNSUPPORTED, "number of factors above 10 is not supported by the adjoint kernel");
+}
+
+// The reverse sweep of the spot pricing pass; a.ckpt holds its checkpointed states.
+template <typename real>
+void launch_spot_adjoint(const StepArgs& a, const AdjArgs& g, hipStream_t stream) {
+    if (a.n <= 8) launch_spot_adjoint_n<1, real>(a, g, stream);
+    else if (a.n <= 24) launch_spot_adjoint_n<3, real>(a, g, stream);
+    else if (a.n <= 40) launch_spot_adjoint_n<5, real>(a, g, stream);
+    else if (a.n <= 80) launch_spot_adjoint_n<10, real>(a, g, stream);
+    else throw FmError(FM_ERR_UNSUPPORTED, "more than 80 forward rates are not supported by the adjoint kernel");
+}
+
+} // namespace fm

@@ -76,12 +76,8 @@ def test_c3_value_is_the_price_with_discount_curve(c3):
     wl, t, out = c3
     for prec, g in out.items():
         say(f"case c3 adjoint value {prec} P={P}: {g['value']:.15e}")
-        # FP64: the adjoint's forward pass and the pricing pass give the same paths to the last
-        # digits. FP32-mixed: the two passes run different kernels whose float arithmetic the
-        # compiler contracts into fused multiply-adds differently, so the prices agree to float
-        # rounding (measured 1.6e-7 relative on the strip)
-        tol = 1e-12 if prec == "FP64" else 2e-6
-        assert g["value"] == pytest.approx(float(np.sum(g["disc"])), rel=tol)
+        # the adjoint's forward pass is the pricing pass (the same kernel, with checkpoints)
+        assert g["value"] == pytest.approx(float(np.sum(g["disc"])), rel=1e-12)
         assert np.all(np.isfinite(g["dL"])) and np.all(np.isfinite(g["dsig"]))
     # the strip is worth more than its MC error and the discount curve moves it
     assert np.all(out["FP64"]["disc"] > 0) and np.max(np.abs(out["FP64"]["ratio"] - 1.0)) > 1e-6
