@@ -138,7 +138,7 @@ struct fm_lmm {
     DevBuf d_times, d_first, d_tenor_ti, d_rec, d_delta, d_L0, d_Y0, d_Y0cap, d_sqrt_dt, d_lvb, d_svs;
     DevBuf d_states, d_poly, d_dw, d_row_of, d_paths, d_num;
     DevBuf d_products, d_periods, d_wave_sum, d_wave_m2, d_contrib, d_out, d_swapmeta, d_scale;
-    DevBuf d_ckpt, d_adj_num, d_kcur, d_fsig, d_sig, d_cdiag, d_wave_grad, d_grad_out;
+    DevBuf d_ckpt, d_seg, d_adj_num, d_kcur, d_fsig, d_sig, d_cdiag, d_wave_grad, d_grad_out;
     DevBuf d_berm_vals, d_berm_under, d_berm_ex, d_berm_part, d_berm_coef, d_berm_col, d_berm_scale;
 
     int n() const { return cfg.num_libors; }
@@ -1388,6 +1388,7 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     ga.weight = upload(c->d_scale, kweights, s);
     ga.berm_ex = L.berm.empty() ? nullptr : c->d_berm_ex.as<int32_t>();
     ga.berm_weight = nullptr;
+    ga.seg = c->d_seg.get((size_t)std::max(stride - 2, 1) * n * P * c->real_size());
     // factor matrix padded to the factor capacity and -sigma_j(t_i) c_j, in the path precision
     std::vector<double> fpad((size_t)n * fcap_of(c->F()), 0.0), vterm((size_t)nT * n, 0.0);
     for (int j = 0; j < n; j++)

@@ -152,6 +152,7 @@ struct AdjArgs {
     const double* weight;            // [num kernel products] objective weight of each lowered product
     const void* fsig;                // [n][fcap_of(F)] factor matrix f_jk, zero padded (path precision)
     const void* vterm;               // [nT][n] -sigma_j(t_i) c_j, d(variance * -1/2)/dsigma (path precision)
+    void* seg;                       // [max(stride-2,1)][n][P] states inside a checkpoint segment (path precision)
     double* wave_grad;               // [num_waves][nT*n + n] partials of dV/dsigma and dV/dL(0)
     const int32_t* berm_ex;          // [num Bermudans][P] exercise period per path (-1: never), or NULL
     const double* berm_weight;       // [contribution column][P] weight of each Bermudan period's payoff on

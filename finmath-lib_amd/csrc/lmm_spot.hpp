@@ -43,15 +43,20 @@ __device__ __forceinline__ void static_for(Fn&& fn) {
     }
 }
 
-template <typename real> struct Vec8;
-template <> struct Vec8<double> { typedef double t __attribute__((ext_vector_type(8))); };
-template <> struct Vec8<float> { typedef float t __attribute__((ext_vector_type(8))); };
+// Forwards per chunk: a chunk is sixteen VGPRs, eight FP64 or sixteen FP32 forwards. At this
+// width the compiler reads and writes an element at a uniform run-time index with one indexed
+// move (s_set_gpr_idx_on); narrower FP32 chunks are lowered to chains of compares and selects.
+template <typename real> constexpr int chunk_width() { return 64 / (int)sizeof(real); }
+template <typename real> struct VecCh;
+template <> struct VecCh<double> { typedef double t __attribute__((ext_vector_type(8))); };
+template <> struct VecCh<float> { typedef float t __attribute__((ext_vector_type(16))); };
 
-// NCH chunks of eight forwards in registers (NCH <= 10).
+// NCH chunks of forwards in registers (NCH <= 10).
 template <int NCH, typename real>
 struct ChunkedForwards {
-    using V = typename Vec8<real>::t;
-    static_assert(NCH >= 1 && NCH <= 10, "one to ten chunks of eight forwards");
+    using V = typename VecCh<real>::t;
+    static constexpr int W = chunk_width<real>();
+    static_assert(NCH >= 1 && NCH <= 10, "one to ten chunks of forwards");
     V c0, c1, c2, c3, c4, c5, c6, c7, c8, c9;
     template <int Q> __device__ __forceinline__ V& ch() {
         if constexpr (Q == 0) return c0; else if constexpr (Q == 1) return c1; else if constexpr (Q == 2) return c2;
@@ -59,13 +64,13 @@ struct ChunkedForwards {
         else if constexpr (Q == 6) return c6; else if constexpr (Q == 7) return c7; else if constexpr (Q == 8) return c8;
         else return c9;
     }
-    template <int J> __device__ __forceinline__ real at() { return ch<J / 8>()[J % 8]; }
-    template <int J> __device__ __forceinline__ void put(real v) { ch<J / 8>()[J % 8] = v; }
+    template <int J> __device__ __forceinline__ real at() { return ch<J / W>()[J % W]; }
+    template <int J> __device__ __forceinline__ void put(real v) { ch<J / W>()[J % W] = v; }
     // Element j for a uniform run-time j: one indexed register read in the chunk of j. The
     // empty asm keeps each chunk a register value, so the per-chunk reads are not merged into
     // a load from a memory copy of the forwards.
     __device__ __forceinline__ real get(int j) {
-        const int e = j & 7, qj = j >> 3;
+        const int e = j % W, qj = j / W;
         real r = (real)0;
         static_for<0, NCH>([&](auto Q) {
             constexpr int q = decltype(Q)::value;
@@ -82,7 +87,7 @@ struct ChunkedForwards {
     // Element j += x for a uniform run-time j: one indexed register write in the chunk of j. The
     // asm takes the chunk as an input and output, which keeps it a register value.
     __device__ __forceinline__ void add(int j, real x) {
-        const int e = j & 7, qj = j >> 3;
+        const int e = j % W, qj = j / W;
         static_for<0, NCH>([&](auto Q) {
             constexpr int q = decltype(Q)::value;
             if (qj == q) {
@@ -114,17 +119,18 @@ __device__ __forceinline__ void spot_step(ChunkedForwards<NCH, real>& Y, int n, 
                                           bool capped, real ycap, real* __restrict__ paths,
                                           const int32_t* __restrict__ row_next, int64_t p, int P, bool active) {
     constexpr bool LOGCARRY = carries_log<real>();
+    constexpr int W = chunk_width<real>();
     real S[FCAP];
 #pragma unroll
     for (int k = 0; k < FCAP; k++) S[k] = (real)0;
     static_for<0, NCH>([&](auto Q) {
         constexpr int q = decltype(Q)::value;
-        const int j0 = first > 8 * q ? first : 8 * q;
-        const int j1 = n < 8 * q + 8 ? n : 8 * q + 8;
+        const int j0 = first > W * q ? first : W * q;
+        const int j1 = n < W * q + W ? n : W * q + W;
 #pragma unroll 1
         for (int j = j0; j < j1; j++) {
             const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, i, j);
-            const real x = Y.template ch<q>()[j - 8 * q];
+            const real x = Y.template ch<q>()[j - W * q];
             const real Lj = (LOGCARRY && lognormal) ? dev_exp<real>(x) : x;
             real w = spot_div<real>((real)1.0, Lj * R.c1() + R.c2());
             if (lognormal) w = w * Lj;
@@ -140,12 +146,12 @@ __device__ __forceinline__ void spot_step(ChunkedForwards<NCH, real>& Y, int n, 
             for (int k = 0; k < FCAP; k++) y = y + dW[k] * R.lam(k);
             if (LOGCARRY) {
                 if (capped) y = jmin(y, ycap);
-                Y.template ch<q>()[j - 8 * q] = y;
+                Y.template ch<q>()[j - W * q] = y;
                 if (paths && active) paths[(size_t)row_next[j] * P + p] = lognormal ? dev_exp<real>(y) : y;
             } else {
                 real v = lognormal ? dev_exp<real>(y) : y;
                 if (capped) v = jmin(v, ycap);
-                Y.template ch<q>()[j - 8 * q] = v;
+                Y.template ch<q>()[j - W * q] = v;
                 if (paths && active) paths[(size_t)row_next[j] * P + p] = v;
             }
         }
@@ -173,11 +179,12 @@ lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __
     const int stride = a.num_slots + n + 1;
     const bool capped = a.capped != 0;
     constexpr bool LOGCARRY = carries_log<real>();
+    constexpr int W = chunk_width<real>();
     const real ycap = (LOGCARRY && lognormal) ? (real)a.logcap : (real)a.cap;
     auto val = [&](real y) -> real { return (LOGCARRY && lognormal) ? dev_exp<real>(y) : y; };
 
     ChunkedForwards<NCH, real> Y;
-    static_for<0, 8 * NCH>([&](auto J) {
+    static_for<0, W * NCH>([&](auto J) {
         constexpr int j = decltype(J)::value;
         Y.template put<j>((j < n) ? (real)a.Y0[j] : (real)0);
     });
@@ -191,7 +198,7 @@ lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __
         // [ti / stride][j][P]
         if (ckpt && active && (ti % a.ckpt_stride) == 0) {
             const size_t base = (size_t)(ti / a.ckpt_stride) * n;
-            static_for<0, 8 * NCH>([&](auto J) {
+            static_for<0, W * NCH>([&](auto J) {
                 constexpr int j = decltype(J)::value;
                 if (j < n) ckpt[(base + j) * P + p] = Y.template at<j>();
             });
@@ -329,10 +336,13 @@ void launch_spot_n(const StepArgs& a, hipStream_t stream) {
 // capped initial forwards.
 template <typename real>
 void launch_spot(const StepArgs& a, hipStream_t stream) {
-    if (a.n <= 8) launch_spot_n<1, real>(a, stream);
-    else if (a.n <= 24) launch_spot_n<3, real>(a, stream);
-    else if (a.n <= 40) launch_spot_n<5, real>(a, stream);
-    else if (a.n <= 80) launch_spot_n<10, real>(a, stream);
+    // 1, 3, 5, 10 chunks of eight FP64 forwards; 1, 2, 3, 5 chunks of sixteen FP32 forwards
+    constexpr int W = chunk_width<real>();
+    constexpr int N2 = W == 8 ? 3 : 2, N3 = W == 8 ? 5 : 3, N4 = W == 8 ? 10 : 5;
+    if (a.n <= W) launch_spot_n<1, real>(a, stream);
+    else if (a.n <= W * N2) launch_spot_n<N2, real>(a, stream);
+    else if (a.n <= W * N3) launch_spot_n<N3, real>(a, stream);
+    else if (a.n <= 80) launch_spot_n<N4, real>(a, stream);
     else throw FmError(FM_ERR_UNSUPPORTED, "more than 80 forward rates are not supported by the path kernel");
 }
 

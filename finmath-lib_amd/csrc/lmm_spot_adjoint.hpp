@@ -1,20 +1,22 @@
 // Pathwise adjoint (reverse sweep) of the spot Euler pricing pass.
 //
 // The pricing pass (lmm_spot_kernel) stores the carried state of every path every `stride`
-// time steps and the numeraire N(T_k). The reverse sweep walks the time indices backwards;
-// for each index t it reloads the nearest checkpoint from device memory and rebuilds the state
-// at t with the pricing pass's own step (spot_step), so the sweep differentiates the paths that
-// were priced, then
-//   1. reverses the Euler step t -> t+1 (adjoint of L(t+1) -> adjoint of L(t)),
+// time steps and the numeraire N(T_k). The reverse sweep walks the segments between
+// checkpoints backwards. At the top index of a segment it reloads the segment's checkpoint and
+// rebuilds the state with the pricing pass's own step (spot_step), so the sweep differentiates
+// the paths that were priced; the states in between go to a per-path segment buffer
+// ([stride-2][n][P]), so each step is recomputed once. For each index t it then
+//   1. reverses the Euler step t -> t+1 (adjoint of y(t+1) -> adjoint of L(t)),
 //   2. adds the direct sensitivities of the products evaluated at t,
-//   3. reverses the numeraire updates N(T_{k+1}) = N(T_k)(1 + L_k d_k) made at t.
+//   3. reverses the numeraire updates N(T_{k+1}) = N(T_k)(1 + L_k d_k) made at t,
+//   4. turns the adjoint of L(t) into that of the state y(t) of the step t-1 -> t while the
+//      state at t is at hand, so step 1 at t-1 needs only the drift sums of its step.
 // Per alive forward-step it reduces dV/dsigma_j(t_i) over the wavefront, where the factor
 // loading is lambda_jk = sigma_j f_jk and the variance sigma_j^2 c_j
 // (LIBORCovarianceModelFromVolatilityAndCorrelation.java:48-93); at the end it reduces dV/dL_j(0).
 //
 // Per path the sweep holds two chunked register arrays, the state at t and the adjoint of the
-// forwards, and processes components in rolled loops (lmm_spot.hpp), so 80 forwards x 10 factors
-// fit in registers at two waves per SIMD without scratch.
+// forwards, and processes components in rolled loops (lmm_spot.hpp).
 //
 // Conventions of the reference's AAD (RandomVariableDifferentiableAAD.java:165-219): averages
 // over paths (the numeraire-adjustment factor) are constants, the floor of the swaption payoff
@@ -37,9 +39,10 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
                         const real* __restrict__ dw, const real* __restrict__ ckpt,
                         const double* __restrict__ numeraire, const double* __restrict__ weight,
                         const real* __restrict__ fsig, const real* __restrict__ vterm,
-                        double* __restrict__ wave_grad)
+                        real* __restrict__ seg, double* __restrict__ wave_grad)
 {
     constexpr bool LOGCARRY = carries_log<real>() && lognormal;   // the state is the log-forward
+    constexpr int W = chunk_width<real>();
     const int64_t p = (int64_t)blockIdx.x * kStepThreads + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int64_t wave = p >> 6;
@@ -55,7 +58,7 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
 
     ChunkedForwards<NCH, real> X;     // state at t
     ChunkedForwards<NCH, real> Lb;    // adjoint of the forwards L_j(t)
-    static_for<0, 8 * NCH>([&](auto J) {
+    static_for<0, W * NCH>([&](auto J) {
         constexpr int j = decltype(J)::value;
         Lb.template put<j>((real)0);
     });
@@ -64,30 +67,41 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
     int qp = a.num_products - 1;            // products sorted by evaluation index
 
     for (int c = (nT / stride) * stride; c >= 0; c -= stride) {
-        const size_t base = (size_t)(c / stride) * n;
         const int hi = (c + stride - 1 < nT) ? c + stride - 1 : nT;
         for (int t = hi; t >= c; t--) {
-            // ---- the state at t: the checkpoint at c, then the pricing pass's steps c .. t-1.
-            // The checkpoint is read again for every t (coalesced, from L2): the empty asm makes
-            // its offset opaque, so the compiler does not hoist the loads out of the t loop and
-            // keep a third register array of forwards live across the segment.
-            size_t row = base;
+            // ---- the state at t: the checkpoint at c for t = c and t = hi, the segment's stored
+            // states in between. At t = hi the pricing pass's steps c .. hi-1 rebuild the state
+            // and store the states c+1 .. hi-1 for the later t of the segment. The empty asm
+            // makes the source's offset opaque, so the compiler does not hoist the loads out of
+            // the t loop and keep another register array of forwards live across the segment.
+            size_t row = (t > c && t < hi) ? (size_t)(t - c - 1) * n : (size_t)(c / stride) * n;
+            const real* src = (t > c && t < hi) ? seg : ckpt;
 #if defined(__HIP_DEVICE_COMPILE__)
             asm volatile("" : "+s"(row));
 #endif
-            static_for<0, 8 * NCH>([&](auto J) {
+            static_for<0, W * NCH>([&](auto J) {
                 constexpr int j = decltype(J)::value;
-                X.template put<j>((j < n) ? ckpt[(row + j) * P + pc] : (real)0);
+                X.template put<j>((j < n) ? src[(row + j) * P + pc] : (real)0);
             });
-            for (int s = c; s < t; s++) {
-                real dWs[FCAP];
-                load_increments<FCAP, real>(dWs, dw, pc, nT, s, F);
-                spot_step<NCH, FCAP, real, lognormal>(X, n, s, first_alive[s], (real)(times[s + 1] - times[s]), dWs,
-                                                      rec, capped, ycap, (real*)nullptr, (const int32_t*)nullptr, p, P,
-                                                      active);
+            if (t == hi) {
+                for (int s = c; s < t; s++) {
+                    real dWs[FCAP];
+                    load_increments<FCAP, real>(dWs, dw, pc, nT, s, F);
+                    spot_step<NCH, FCAP, real, lognormal>(X, n, s, first_alive[s], (real)(times[s + 1] - times[s]),
+                                                          dWs, rec, capped, ycap, (real*)nullptr,
+                                                          (const int32_t*)nullptr, p, P, active);
+                    if (s + 1 < hi && active) {
+                        const size_t srow = (size_t)(s - c) * n;
+                        static_for<0, W * NCH>([&](auto J) {
+                            constexpr int j = decltype(J)::value;
+                            if (j < n) seg[(srow + j) * P + p] = X.template at<j>();
+                        });
+                    }
+                }
             }
 
-            // ---- 1. reverse the Euler step t -> t+1
+            // ---- 1. reverse the Euler step t -> t+1. Lb holds the adjoint of the new state
+            // y_j(t+1) for the components the step moves (converted at the end of t+1).
             if (t < nT) {
                 const int first = first_alive[t];
                 const real dt = (real)(times[t + 1] - times[t]);
@@ -95,58 +109,35 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
                 load_increments<FCAP, real>(dW, dw, pc, nT, t, F);
 #pragma unroll
                 for (int k = 0; k < FCAP; k++) { S[k] = (real)0; Sb[k] = (real)0; }
-                // pass A (ascending): recompute the step; adjoint through exp, log and the cap
+                // pass A (ascending): the drift sums S_k = sum_j w_j lambda_jk of the step
                 static_for<0, NCH>([&](auto Q) {
                     constexpr int q = decltype(Q)::value;
-                    const int j0 = first > 8 * q ? first : 8 * q;
-                    const int j1 = n < 8 * q + 8 ? n : 8 * q + 8;
+                    const int j0 = first > W * q ? first : W * q;
+                    const int j1 = n < W * q + W ? n : W * q + W;
 #pragma unroll 1
                     for (int j = j0; j < j1; j++) {
                         const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, t, j);
-                        const real x = X.template ch<q>()[j - 8 * q];
-                        const real Lj = val(x);
+                        const real Lj = val(X.template ch<q>()[j - W * q]);
                         real w = spot_div<real>((real)1.0, Lj * R.c1() + R.c2());
                         if (lognormal) w = w * Lj;
 #pragma unroll
                         for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * R.lam(k);
-                        real mu = S[0] * R.lam(0);
-#pragma unroll
-                        for (int k = 1; k < FCAP; k++) mu = mu + S[k] * R.lam(k);
-                        mu = mu + R.hv();
-                        real y = (!LOGCARRY && lognormal) ? dev_log<real>(Lj) : x;
-                        y = y + mu * dt;
-#pragma unroll
-                        for (int k = 0; k < FCAP; k++) y = y + dW[k] * R.lam(k);
-                        // adjoint of the new state from the adjoint of the new value (zero where
-                        // the cap binds: cap has derivative 0 above its level)
-                        const real lb = Lb.template ch<q>()[j - 8 * q];
-                        real yb;
-                        if (LOGCARRY) {
-                            yb = (capped && y > ycap) ? (real)0 : lb * dev_exp<real>(y);
-                        } else {
-                            const real E = lognormal ? dev_exp<real>(y) : y;
-                            yb = (capped && E > ycap) ? (real)0 : (lognormal ? lb * E : lb);
-                        }
-                        // direct part of the adjoint of L_j(t): y = log(L_j(t)) + ...
-                        Lb.template ch<q>()[j - 8 * q] = lognormal ? yb * adj_rcp<real>(Lj) : yb;
                     }
                 });
-                // pass B (descending): the drift chain S_k = sum_{l<=j} w_l lambda_lk; the per-path
-                // dV/dsigma_j(t) of groups of four j are summed over the wavefront together
+                // pass B (descending): the drift chain back from S^(n-1), S^(j-1) = S^(j) - w_j lambda_j;
+                // the per-path dV/dsigma_j(t) of groups of four j are summed over the wavefront together
                 double gq0 = 0.0, gq1 = 0.0, gq2 = 0.0, gq3 = 0.0;
                 static_for<0, NCH>([&](auto QR) {
                     constexpr int q = NCH - 1 - decltype(QR)::value;
-                    const int j0 = first > 8 * q ? first : 8 * q;
-                    const int j1 = n < 8 * q + 8 ? n : 8 * q + 8;
+                    const int j0 = first > W * q ? first : W * q;
+                    const int j1 = n < W * q + W ? n : W * q + W;
 #pragma unroll 1
                     for (int j = j1 - 1; j >= j0; j--) {
                         const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, t, j);
-                        const real Lj = val(X.template ch<q>()[j - 8 * q]);
+                        const real Lj = val(X.template ch<q>()[j - W * q]);
                         const real r = spot_div<real>((real)1.0, Lj * R.c1() + R.c2());
                         const real w = lognormal ? r * Lj : r;
-                        // adjoint of Y_j(t+1) from the direct part left by pass A
-                        const real lb = Lb.template ch<q>()[j - 8 * q];
-                        const real Yb = lognormal ? lb * Lj : lb;
+                        const real Yb = Lb.template ch<q>()[j - W * q];     // adjoint of y_j(t+1)
                         const real mb = Yb * dt;
                         real wb = (real)0;
                         real sgb = (real)0;             // dV/dsigma_j(t) on this path
@@ -160,8 +151,10 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
                         }
                         // variance sigma^2 c in the drift: d(-var/2)/dsigma = -sigma c
                         if (lognormal) sgb = sgb + mb * vterm[(size_t)t * n + j];
+                        // y_j(t+1) = log L_j(t) + ... (or L_j(t) + ...), and w_j depends on L_j(t)
                         const real dwdL = lognormal ? R.c2() * r * r : -R.c1() * r * r;
-                        Lb.template ch<q>()[j - 8 * q] = lb + wb * dwdL;
+                        const real direct = lognormal ? Yb * adj_rcp<real>(Lj) : Yb;
+                        Lb.template ch<q>()[j - W * q] = direct + wb * dwdL;
                         const double gs = active ? (double)sgb : 0.0;
                         const int m = j & 3;
                         gq0 = (m == 0) ? gs : gq0;
@@ -180,7 +173,10 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
                 });
             }
 
-            // ---- 2. products evaluated at t (reverse order of evaluation)
+            // ---- 2. products evaluated at t (reverse order of evaluation). The per-path conditions
+            // (in the money, exercised) scale the contributions instead of branching around them, and
+            // the swaption walks the chunks of its periods in place: the adjoint forwards then stay
+            // one register array through the product loop.
             const int Kt = kcur_at[t];
             while (qp >= 0 && products[qp].eval_index == t) {
                 const KProduct pr = products[qp];
@@ -188,37 +184,47 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
                 qp--;
                 const bool top = (pr.num_k == Kt);
                 const double Nsel = (pr.num_k == 0) ? 1.0 : numeraire[(size_t)pr.num_k * P + pc];
+                double nb = 0.0;            // adjoint of the selected numeraire
+                real xadd = (real)0;        // adjoint of L_{first_j}(t) (caplets and swap periods)
                 if (pr.type == FM_PRODUCT_SWAPTION) {
                     const int j0 = pr.first_j, j1 = pr.first_j + pr.num_periods;
                     // V = sum_j p_j prod_{l<=j} D_l, D_l = adj_l / (1 + L_l tau_l)
                     real V = (real)0, Pc = (real)1;
+                    static_for<0, NCH>([&](auto QR) {
+                        constexpr int q = NCH - 1 - decltype(QR)::value;
+                        const int ja = j0 > W * q ? j0 : W * q;
+                        const int jb = j1 < W * q + W ? j1 : W * q + W;
 #pragma unroll 1
-                    for (int j = j1 - 1; j >= j0; j--) {
-                        const KPeriod pe = periods[pr.param_off + (j - j0)];
-                        const real Lj = val(X.get(j));
-                        const real D = (real)pe.adj * adj_rcp<real>((real)1.0 + Lj * (real)pe.tau);
-                        V = (V + ((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pr.notional) * D;
-                        Pc = Pc * D;
-                    }
-                    if (V >= (real)0) {     // floor(0.0): derivative 1 at and above the floor
-                        const double Vb = wq / Nsel;
-                        const double nb = -wq * (double)V / (Nsel * Nsel);
-                        if (top) Nb_top += nb; else Nb_low += nb;
-                        // descending over j: P_j = prod_{l<=j} D_l, U_j = sum_{m>=j} p_m P_m
-                        real U = (real)0;
-#pragma unroll 1
-                        for (int j = j1 - 1; j >= j0; j--) {
+                        for (int j = jb - 1; j >= ja; j--) {
                             const KPeriod pe = periods[pr.param_off + (j - j0)];
-                            const real Lj = val(X.get(j));
+                            const real Lj = val(X.template ch<q>()[j - W * q]);
+                            const real D = (real)pe.adj * adj_rcp<real>((real)1.0 + Lj * (real)pe.tau);
+                            V = (V + ((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pr.notional) * D;
+                            Pc = Pc * D;
+                        }
+                    });
+                    // floor(0.0): derivative 1 at and above the floor, 0 below
+                    const double Vb = (V >= (real)0) ? wq / Nsel : 0.0;
+                    nb = (V >= (real)0) ? -wq * (double)V / (Nsel * Nsel) : 0.0;
+                    // descending over j: P_j = prod_{l<=j} D_l, U_j = sum_{m>=j} p_m P_m
+                    real U = (real)0;
+                    static_for<0, NCH>([&](auto QR) {
+                        constexpr int q = NCH - 1 - decltype(QR)::value;
+                        const int ja = j0 > W * q ? j0 : W * q;
+                        const int jb = j1 < W * q + W ? j1 : W * q + W;
+#pragma unroll 1
+                        for (int j = jb - 1; j >= ja; j--) {
+                            const KPeriod pe = periods[pr.param_off + (j - j0)];
+                            const real Lj = val(X.template ch<q>()[j - W * q]);
                             const real one = (real)1.0 + Lj * (real)pe.tau;
                             const real inv = adj_rcp<real>(one);
                             const real pj = ((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pr.notional;
                             U = U + pj * Pc;
                             const real dV = (real)pe.period_length * (real)pr.notional * Pc - ((real)pe.tau * inv) * U;
-                            Lb.add(j, (real)Vb * dV);
+                            Lb.template ch<q>()[j - W * q] = Lb.template ch<q>()[j - W * q] + (real)Vb * dV;
                             Pc = Pc * (one * (real)pe.inv_adj);      // P_{j-1} = P_j / D_j
                         }
-                    }
+                    });
                 } else if (pr.type == FM_PRODUCT_CAPLET) {
                     const KPeriod pe = periods[pr.param_off];
                     const real Lj = val(X.get(pr.first_j));
@@ -226,14 +232,10 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
                     // floor(0.0) has derivative 1 for v >= 0, cap(0.0) has derivative 1 for v < 0
                     // (RandomVariableDifferentiableAAD.java:333-348)
                     const bool itm = pr.is_floorlet ? (v < (real)0) : (v >= (real)0);
-                    if (itm) {
-                        const double dcf = pe.notional;
-                        const double x = pr.is_floorlet ? -(double)v * dcf : (double)v * dcf;
-                        const double dLj = (pr.is_floorlet ? -dcf : dcf) * wq / Nsel;
-                        const double nb = -wq * x / (Nsel * Nsel);
-                        if (top) Nb_top += nb; else Nb_low += nb;
-                        Lb.add(pr.first_j, (real)dLj);
-                    }
+                    const double dcf = pe.notional;
+                    const double x = pr.is_floorlet ? -(double)v * dcf : (double)v * dcf;
+                    xadd = itm ? (real)((pr.is_floorlet ? -dcf : dcf) * wq / Nsel) : (real)0;
+                    nb = itm ? -wq * x / (Nsel * Nsel) : 0.0;
                 } else if (pr.type == K_BERM_BASIS) {
                     // regression basis: not differentiated (the reference's conditional
                     // expectation treats its basis functions as constants)
@@ -249,17 +251,16 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
                             pw = ((pr.flags & 2) ? (ex >= 0 && pr.is_floorlet >= ex)
                                                  : (ex < 0 || pr.is_floorlet < ex)) ? 1.0 : 0.0;
                         }
-                        if (pw == 0.0) continue;
                     }
                     const KPeriod pe = periods[pr.param_off];
                     const real Lj = val(X.get(pr.first_j));
                     const double sgn = (pr.flags & 1) ? 1.0 : -1.0;
                     const double pay = sgn * (double)(((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pe.notional);
-                    const double dLj = pw * sgn * pe.period_length * pe.notional * wq / Nsel;
-                    const double nb = -pw * wq * pay / (Nsel * Nsel);
-                    if (top) Nb_top += nb; else Nb_low += nb;
-                    Lb.add(pr.first_j, (real)dLj);
+                    xadd = (real)(pw * sgn * pe.period_length * pe.notional * wq / Nsel);
+                    nb = -pw * wq * pay / (Nsel * Nsel);
                 }
+                if (top) Nb_top += nb; else Nb_low += nb;
+                Lb.add(pr.first_j, xadd);
             }
 
             // ---- 3. numeraire updates made at t: N_{k+1} = N_k (1 + L_k d_k), k = K-1 down to kcur_at[t-1]
@@ -275,10 +276,31 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
                 Nb_low = 0.0;
                 K--;
             }
+
+            // ---- 4. Lb now holds the full adjoint of L(t). For the components the step t-1 -> t
+            // moved, turn it into the adjoint of the new state y_j(t): times L_j(t) = exp(y_j(t))
+            // (log-normal), and zero where the cap binds (cap(x, c) has derivative 0 for x >= c,
+            // RandomVariableDifferentiableAAD.java:316-323; x >= c exactly where the capped state
+            // equals the cap).
+            if (t > 0) {
+                const int first = first_alive[t - 1];
+                static_for<0, NCH>([&](auto Q) {
+                    constexpr int q = decltype(Q)::value;
+                    const int j0 = first > W * q ? first : W * q;
+                    const int j1 = n < W * q + W ? n : W * q + W;
+#pragma unroll 1
+                    for (int j = j0; j < j1; j++) {
+                        const real x = X.template ch<q>()[j - W * q];
+                        const real lb = Lb.template ch<q>()[j - W * q];
+                        const real yb = lognormal ? lb * val(x) : lb;
+                        Lb.template ch<q>()[j - W * q] = (capped && x >= ycap) ? (real)0 : yb;
+                    }
+                });
+            }
         }
     }
     // ---- dV/dL_j(0)
-    static_for<0, 8 * NCH>([&](auto J) {
+    static_for<0, W * NCH>([&](auto J) {
         constexpr int j = decltype(J)::value;
         if (j < n) {
             const double s = wave_sum_dpp(active ? (double)Lb.template at<j>() : 0.0);
@@ -293,7 +315,7 @@ void launch_spot_adjoint_ss(const StepArgs& a, const AdjArgs& g, hipStream_t str
     hipLaunchKernelGGL((lmm_spot_adjoint_kernel<NCH, FCAP, real, lognormal>), dim3((unsigned)blocks),
                        dim3(kStepThreads), 0, stream, a, g, (const real*)a.rec, a.delta, a.times, a.first_alive,
                        g.kcur_at, a.products, a.periods, (const real*)a.dw, (const real*)a.ckpt, a.numeraire,
-                       g.weight, (const real*)g.fsig, (const real*)g.vterm, g.wave_grad);
+                       g.weight, (const real*)g.fsig, (const real*)g.vterm, (real*)g.seg, g.wave_grad);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -315,10 +337,13 @@ void launch_spot_adjoint_n(const StepArgs& a, const AdjArgs& g, hipStream_t stre
 // The reverse sweep of the spot pricing pass; a.ckpt holds its checkpointed states.
 template <typename real>
 void launch_spot_adjoint(const StepArgs& a, const AdjArgs& g, hipStream_t stream) {
-    if (a.n <= 8) launch_spot_adjoint_n<1, real>(a, g, stream);
-    else if (a.n <= 24) launch_spot_adjoint_n<3, real>(a, g, stream);
-    else if (a.n <= 40) launch_spot_adjoint_n<5, real>(a, g, stream);
-    else if (a.n <= 80) launch_spot_adjoint_n<10, real>(a, g, stream);
+    // the chunk counts of the pricing pass (lmm_spot.hpp)
+    constexpr int W = chunk_width<real>();
+    constexpr int N2 = W == 8 ? 3 : 2, N3 = W == 8 ? 5 : 3, N4 = W == 8 ? 10 : 5;
+    if (a.n <= W) launch_spot_adjoint_n<1, real>(a, g, stream);
+    else if (a.n <= W * N2) launch_spot_adjoint_n<N2, real>(a, g, stream);
+    else if (a.n <= W * N3) launch_spot_adjoint_n<N3, real>(a, g, stream);
+    else if (a.n <= 80) launch_spot_adjoint_n<N4, real>(a, g, stream);
     else throw FmError(FM_ERR_UNSUPPORTED, "more than 80 forward rates are not supported by the adjoint kernel");
 }
 
