@@ -818,12 +818,18 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
                                                                                : c->d_L0.as<double>();
         if (c->fp32()) launch_lmm_step_fp32(a, s);
         else launch_lmm_step_fp64(a, s);
-    } else if (c->pc()) {
-        if (c->fp32()) launch_lmm_step_pc_fp32(a, s);
-        else launch_lmm_step_pc_fp64(a, s);
     } else {
-        if (c->fp32()) launch_lmm_step_general_fp32(a, s);
-        else launch_lmm_step_general_fp64(a, s);
+        // the general kernel: FP64 log-normal functional schemes carry the log-forwards, capped
+        // at log(cap), like the spot kernel
+        if (!c->fp32() && c->cfg.state_space == FM_STATE_SPACE_LOGNORMAL && !c->carry())
+            a.Y0 = c->d_Y0cap.as<double>();
+        if (c->pc()) {
+            if (c->fp32()) launch_lmm_step_pc_fp32(a, s);
+            else launch_lmm_step_pc_fp64(a, s);
+        } else {
+            if (c->fp32()) launch_lmm_step_general_fp32(a, s);
+            else launch_lmm_step_general_fp64(a, s);
+        }
     }
     tm.mark(3);
     // numeraire-adjustment averages first (the swap contributions need them)

@@ -158,6 +158,94 @@ __device__ __forceinline__ void spot_step(ChunkedForwards<NCH, real>& Y, int n, 
     });
 }
 
+// Products evaluated at time index ti, in the order they were lowered (qp advances over them):
+// swaptions and caplets are reduced into their wave partials, swap periods and Bermudan
+// regression bases go to per-path contribution columns. value_of(j) gives L_j(t_ti); N and
+// Nprev are the numeraires N(T_knum) and the one before it.
+template <typename real, typename ValueOf>
+__device__ __forceinline__ void eval_products_at(int ti, int& qp, int num_products, const KProduct* __restrict__ products,
+                                                 const KPeriod* __restrict__ periods, const double* __restrict__ delta,
+                                                 double N, double Nprev, int knum, ValueOf&& value_of, bool active,
+                                                 int cnt, int lane, int64_t wave, double* __restrict__ wave_sum,
+                                                 double* __restrict__ wave_m2, int stride, double* __restrict__ contrib,
+                                                 int64_t p, int P) {
+    while (qp < num_products && products[qp].eval_index == ti) {
+        const KProduct pr = products[qp];
+        qp++;
+        const double Nsel = (pr.num_k == knum) ? N : Nprev;
+        if (pr.type == FM_PRODUCT_SWAPTION) {
+            // Swaption.java:150-190: the periods backwards, value = (value + payoff) / (1 + L tau) * adj
+            real V = (real)0;
+#pragma unroll 1
+            for (int j = pr.first_j + pr.num_periods - 1; j >= pr.first_j; j--) {
+                const KPeriod pe = periods[pr.param_off + (j - pr.first_j)];
+                const real Lj = value_of(j);
+                const real payoff = ((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pr.notional;
+                V = V + payoff;
+                V = spot_div<real>(V, (real)1.0 + Lj * (real)pe.tau);
+                V = V * (real)pe.adj;
+            }
+            const double x = (double)jmax0(V) / Nsel;
+            reduce_slot(x, active, cnt, lane, wave, wave_sum, wave_m2, stride, pr.out_slot);
+        } else if (pr.type == FM_PRODUCT_CAPLET) {
+            const KPeriod pe = periods[pr.param_off];
+            const real Lj = value_of(pr.first_j);
+            real v = Lj - (real)pe.strike;
+            if (!pr.is_floorlet) v = jmax0(v) * (real)pe.notional;
+            else v = ((v != v) ? v : (v > (real)0 ? (real)0 : v)) * ((real)(-1.0) * (real)pe.notional);
+            const double x = (double)v / Nsel;
+            reduce_slot(x, active, cnt, lane, wave, wave_sum, wave_m2, stride, pr.out_slot);
+        } else if (pr.type == K_BERM_BASIS) {
+            // raw regression basis of a Bermudan exercise date T_e (BermudanSwaption.java:215-252)
+            if (pr.flags == FM_BASIS_REFERENCE) {
+                // L_short over tenor periods [first_j, pe.j), L_long over [first_j, pe.pay_k):
+                // a single period is the forward itself, several accrue (LMM:1290-1305)
+                const KPeriod pe = periods[pr.param_off];
+                real accS = (real)0, accL = (real)0, rs = (real)0, rl = (real)0;
+#pragma unroll 1
+                for (int j = pr.first_j; j < pe.pay_k; j++) {
+                    const real Lj = value_of(j);
+                    const real d = (real)delta[j];
+                    if (j == pr.first_j) { rs = Lj; rl = Lj; }
+                    accL = (j == pr.first_j) ? Lj * d + (real)1.0 : accL * ((real)1 + Lj * d);
+                    if (j < pe.j) accS = (j == pr.first_j) ? Lj * d + (real)1.0 : accS * ((real)1 + Lj * d);
+                }
+                if (pe.j > pr.first_j + 1) rs = (accS - (real)1.0) / (real)pe.tau;
+                if (pe.pay_k > pr.first_j + 1) rl = (accL - (real)1.0) / (real)pe.period_length;
+                const real ds = (real)1.0 / (rs * (real)pe.tau + (real)1.0);
+                const real dl = (real)1.0 / (rl * (real)pe.period_length + (real)1.0);
+                if (active) {
+                    contrib[(size_t)pr.out_slot * P + p] = (double)ds;
+                    contrib[(size_t)(pr.out_slot + 1) * P + p] = (double)dl;
+                    contrib[(size_t)(pr.out_slot + 2) * P + p] = Nsel;
+                }
+            } else {
+                // par rate of the remaining periods: P = 1/(L pl + 1), A = P pl, then
+                // P = P/(L pl + 1), A = A + P pl; S = (1 - P)/A
+                real Pd = (real)1, A = (real)0;
+#pragma unroll 1
+                for (int j = pr.first_j; j < pr.first_j + pr.num_periods; j++) {
+                    const real pl = (real)periods[pr.param_off + (j - pr.first_j)].period_length;
+                    const real Lj = value_of(j);
+                    if (j == pr.first_j) { Pd = (real)1.0 / (Lj * pl + (real)1.0); A = Pd * pl; }
+                    else { Pd = Pd / (Lj * pl + (real)1.0); A = A + Pd * pl; }
+                }
+                const real S = (Pd * (real)(-1.0) + (real)1.0) / A;
+                if (active) {
+                    contrib[(size_t)pr.out_slot * P + p] = (double)S;
+                    contrib[(size_t)(pr.out_slot + 1) * P + p] = Nsel;
+                }
+            }
+        } else {   // one period of a swap: contribution column, combined after the adjustment is known
+            const KPeriod pe = periods[pr.param_off];
+            const real Lj = value_of(pr.first_j);
+            double payoff = (double)(((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pe.notional);
+            if (!(pr.flags & 1)) payoff = payoff * -1.0;
+            if (active) contrib[(size_t)pr.out_slot * P + p] = payoff / Nsel;
+        }
+    }
+}
+
 template <int NCH, int FCAP, typename real, bool lognormal>
 __global__ void __launch_bounds__(kStepThreads)
 lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __restrict__ delta,
@@ -218,81 +306,9 @@ lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __
             if (numeraire_out && active) numeraire_out[(size_t)kcur * P + p] = N;
         }
         // ---- products evaluated at this time index
-        while (qp < a.num_products && products[qp].eval_index == ti) {
-            const KProduct pr = products[qp];
-            qp++;
-            const double Nsel = (pr.num_k == knum) ? N : Nprev;
-            if (pr.type == FM_PRODUCT_SWAPTION) {
-                // Swaption.java:150-190: the periods backwards, value = (value + payoff) / (1 + L tau) * adj
-                real V = (real)0;
-#pragma unroll 1
-                for (int j = pr.first_j + pr.num_periods - 1; j >= pr.first_j; j--) {
-                    const KPeriod pe = periods[pr.param_off + (j - pr.first_j)];
-                    const real Lj = val(Y.get(j));
-                    const real payoff = ((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pr.notional;
-                    V = V + payoff;
-                    V = spot_div<real>(V, (real)1.0 + Lj * (real)pe.tau);
-                    V = V * (real)pe.adj;
-                }
-                const double x = (double)jmax0(V) / Nsel;
-                reduce_slot(x, active, cnt, lane, wave, wave_sum, wave_m2, stride, pr.out_slot);
-            } else if (pr.type == FM_PRODUCT_CAPLET) {
-                const KPeriod pe = periods[pr.param_off];
-                const real Lj = val(Y.get(pr.first_j));
-                real v = Lj - (real)pe.strike;
-                if (!pr.is_floorlet) v = jmax0(v) * (real)pe.notional;
-                else v = ((v != v) ? v : (v > (real)0 ? (real)0 : v)) * ((real)(-1.0) * (real)pe.notional);
-                const double x = (double)v / Nsel;
-                reduce_slot(x, active, cnt, lane, wave, wave_sum, wave_m2, stride, pr.out_slot);
-            } else if (pr.type == K_BERM_BASIS) {
-                // raw regression basis of a Bermudan exercise date T_e (BermudanSwaption.java:215-252)
-                if (pr.flags == FM_BASIS_REFERENCE) {
-                    // L_short over tenor periods [first_j, pe.j), L_long over [first_j, pe.pay_k):
-                    // a single period is the forward itself, several accrue (LMM:1290-1305)
-                    const KPeriod pe = periods[pr.param_off];
-                    real accS = (real)0, accL = (real)0, rs = (real)0, rl = (real)0;
-#pragma unroll 1
-                    for (int j = pr.first_j; j < pe.pay_k; j++) {
-                        const real Lj = val(Y.get(j));
-                        const real d = (real)delta[j];
-                        if (j == pr.first_j) { rs = Lj; rl = Lj; }
-                        accL = (j == pr.first_j) ? Lj * d + (real)1.0 : accL * ((real)1 + Lj * d);
-                        if (j < pe.j) accS = (j == pr.first_j) ? Lj * d + (real)1.0 : accS * ((real)1 + Lj * d);
-                    }
-                    if (pe.j > pr.first_j + 1) rs = (accS - (real)1.0) / (real)pe.tau;
-                    if (pe.pay_k > pr.first_j + 1) rl = (accL - (real)1.0) / (real)pe.period_length;
-                    const real ds = (real)1.0 / (rs * (real)pe.tau + (real)1.0);
-                    const real dl = (real)1.0 / (rl * (real)pe.period_length + (real)1.0);
-                    if (active) {
-                        contrib[(size_t)pr.out_slot * P + p] = (double)ds;
-                        contrib[(size_t)(pr.out_slot + 1) * P + p] = (double)dl;
-                        contrib[(size_t)(pr.out_slot + 2) * P + p] = Nsel;
-                    }
-                } else {
-                    // par rate of the remaining periods: P = 1/(L pl + 1), A = P pl, then
-                    // P = P/(L pl + 1), A = A + P pl; S = (1 - P)/A
-                    real Pd = (real)1, A = (real)0;
-#pragma unroll 1
-                    for (int j = pr.first_j; j < pr.first_j + pr.num_periods; j++) {
-                        const real pl = (real)periods[pr.param_off + (j - pr.first_j)].period_length;
-                        const real Lj = val(Y.get(j));
-                        if (j == pr.first_j) { Pd = (real)1.0 / (Lj * pl + (real)1.0); A = Pd * pl; }
-                        else { Pd = Pd / (Lj * pl + (real)1.0); A = A + Pd * pl; }
-                    }
-                    const real S = (Pd * (real)(-1.0) + (real)1.0) / A;
-                    if (active) {
-                        contrib[(size_t)pr.out_slot * P + p] = (double)S;
-                        contrib[(size_t)(pr.out_slot + 1) * P + p] = Nsel;
-                    }
-                }
-            } else {   // one period of a swap: contribution column, combined after the adjustment is known
-                const KPeriod pe = periods[pr.param_off];
-                const real Lj = val(Y.get(pr.first_j));
-                double payoff = (double)(((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pe.notional);
-                if (!(pr.flags & 1)) payoff = payoff * -1.0;
-                if (active) contrib[(size_t)pr.out_slot * P + p] = payoff / Nsel;
-            }
-        }
+        eval_products_at<real>(ti, qp, a.num_products, products, periods, delta, N, Nprev, knum,
+                               [&](int j) { return val(Y.get(j)); }, active, cnt, lane, wave, wave_sum, wave_m2,
+                               stride, contrib, p, P);
 
         if (ti == nT) break;
 

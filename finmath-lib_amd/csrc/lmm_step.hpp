@@ -14,10 +14,6 @@
 // Products are evaluated on the fly at their exercise/fixing time index and reduced
 // per wavefront (sum and centred second moment); the numeraire is accumulated per
 // path in double precision (SPOT measure, LMM:1050-1066).
-// The general variant (kVarGeneral*) adds the terminal measure (components in
-// descending order, mu_j from the later components, N(T_k) = prod_{l>=k} 1/(1+L_l d_l)),
-// the carried state of the EULER / PREDICTOR_CORRECTOR schemes, the corrector step and
-// local-volatility factors on the loadings.
 //
 // Model data layout: one record of rec_size_of(FCAP) reals per (time step,
 // component): the factor loadings padded with zeros to FCAP, then variance*(-0.5),
@@ -194,126 +190,6 @@ __device__ __forceinline__ void euler_step(real (&L)[NCAP], int n, int i, int fi
     }
 }
 
-// ---- general variant: terminal measure, carried state, predictor-corrector -------------
-// Run-time choices of the general path kernel (uniform over the launch).
-template <typename real>
-struct Variant {
-    bool lognormal, capped;
-    bool carry;        // EULER / PREDICTOR_CORRECTOR: the state is carried between steps
-                       // (else the *_FUNCTIONAL schemes re-derive it from the value, Euler:234-237)
-    real cap;
-    real lv_alpha;     // local-volatility factor f = (L - L alpha) + b_ij, or L + b_ij for alpha = 0
-    const real* lvb;   // b_ij, [nT][n], or NULL: no local-volatility factor (LV kernels only)
-    const real* svs;   // stochastic volatility scaling S(t_i) [nT][P], or NULL (LV kernels only)
-};
-
-// Local-volatility factor of component j at time index i from its value L at that time
-// index: DisplacedLocalVolatilityModel L + d (covariance/DisplacedLocalVolatilityModel.java:175-186),
-// BlendedLocalVolatilityModel (L - L a) + a f0 (covariance/BlendedLocalVolatilityModel.java:265-283);
-// every factor loading of the component is multiplied by it.
-template <typename real>
-__device__ __forceinline__ real local_vol_factor(real L, const Variant<real>& v, int i, int j, int n) {
-    if (!v.lvb) return (real)1;
-    real x = L;
-    if (v.lv_alpha != (real)0) x = x - x * v.lv_alpha;
-    return x + v.lvb[(size_t)i * n + j];
-}
-
-// Drift of one component and the update of the factor sums S, in the measure's order
-// (LMM:1132-1181): SPOT adds w lambda_j to S and then takes mu_j = S . lambda_j;
-// TERMINAL takes mu_j = S . lambda_j of the later components first and then adds
-// w lambda_j. The record's d/d and 1/d hold -1 and -1/d under TERMINAL
-// (Scalar.of(-d).discount(L, d), stochastic/Scalar.java:320-328), so w carries the sign.
-// LV: the loadings are (lambda_jk * f) * s, f the local-volatility factor and s the stochastic
-// volatility scaling (each 1 when absent, which multiplies exactly); the log-normal variance is
-// then the sum of their squares (AbstractLIBORCovarianceModel.getCovariance,
-// covariance/AbstractLIBORCovarianceModel.java:79-92) instead of the record's sigma_j sigma_j rho_jj.
-template <int FCAP, typename real, bool TERMINAL, bool LV = false>
-__device__ __forceinline__ real drift_component(real (&S)[FCAP], const Rec<FCAP, real>& R, real Lj, bool lognormal,
-                                                real f = (real)1, real sv = (real)1) {
-    auto lam = [&](int k) -> real { return LV ? (R.lam(k) * f) * sv : R.lam(k); };
-    real w = dev_div<real>((real)1.0, Lj * R.c1() + R.c2());
-    if (lognormal) w = w * Lj;
-    real mu;
-    if (TERMINAL) {
-        mu = S[0] * lam(0);
-#pragma unroll
-        for (int k = 1; k < FCAP; k++) mu = mu + S[k] * lam(k);
-#pragma unroll
-        for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * lam(k);
-    } else {
-#pragma unroll
-        for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * lam(k);
-        mu = S[0] * lam(0);
-#pragma unroll
-        for (int k = 1; k < FCAP; k++) mu = mu + S[k] * lam(k);
-    }
-    if (LV && lognormal) {
-        real c = lam(0) * lam(0);
-#pragma unroll
-        for (int k = 1; k < FCAP; k++) c = c + lam(k) * lam(k);
-        return mu + c * (real)(-0.5);
-    }
-    return mu + R.hv();
-}
-
-// One step of the general variant. X holds the value (functional schemes) or the state
-// (carried schemes) of each component. The predictor pass follows Euler:232-276; with PC
-// the corrector re-evaluates the drift on the predicted values and adds
-// ((mu_pred - mu) / 2) dt to the state (Euler:292-314). MU keeps the predictor's drift.
-// LV: the loadings of the drift and the diffusion carry the local-volatility factor of the
-// value the drift is evaluated on (the start value; the predicted value in the corrector) and
-// the path's stochastic volatility scaling at t_i.
-template <int NCAP, int FCAP, typename real, bool TERMINAL, bool PC, bool LV>
-__device__ __forceinline__ void general_step(real (&X)[NCAP], real (&MU)[NCAP], int n, int i, int first, real dt,
-                                             const real (&dW)[FCAP], const real* __restrict__ rec,
-                                             const Variant<real>& v, real* __restrict__ paths,
-                                             const int32_t* __restrict__ row_next, int64_t p, int P, bool active) {
-    real S[FCAP];
-#pragma unroll
-    for (int k = 0; k < FCAP; k++) S[k] = (real)0;
-    // stochastic volatility scaling of this path at time index i (1 without)
-    const real sv = (LV && v.svs) ? v.svs[(size_t)i * P + (active ? p : (int64_t)P - 1)] : (real)1;
-#pragma unroll
-    for (int jj = 0; jj < NCAP; jj++) {
-        const int j = TERMINAL ? NCAP - 1 - jj : jj;
-        if (j < first || j >= n) continue;                  // uniform
-        const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, i, j);
-        const real Lj = v.carry ? to_value<real>(X[j], v.lognormal, v.capped, v.cap) : X[j];
-        const real f = LV ? local_vol_factor<real>(Lj, v, i, j, n) : (real)1;
-        const real mu = drift_component<FCAP, real, TERMINAL, LV>(S, R, Lj, v.lognormal, f, sv);
-        real Y = v.carry ? X[j] : (v.lognormal ? dev_log<real>(Lj) : Lj);
-        Y = Y + mu * dt;
-#pragma unroll
-        for (int k = 0; k < FCAP; k++) Y = Y + dW[k] * (LV ? (R.lam(k) * f) * sv : R.lam(k));
-        if (PC) {
-            MU[j] = mu;
-            X[j] = Y;
-        } else {
-            const real Ln = to_value<real>(Y, v.lognormal, v.capped, v.cap);
-            X[j] = v.carry ? Y : Ln;
-            if (paths && active) paths[(size_t)row_next[j] * P + p] = Ln;
-        }
-    }
-    if (PC) {
-#pragma unroll
-        for (int k = 0; k < FCAP; k++) S[k] = (real)0;
-#pragma unroll
-        for (int jj = 0; jj < NCAP; jj++) {
-            const int j = TERMINAL ? NCAP - 1 - jj : jj;
-            if (j < first || j >= n) continue;
-            const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, i, j);
-            const real Lp = to_value<real>(X[j], v.lognormal, v.capped, v.cap);    // predicted value
-            const real f = LV ? local_vol_factor<real>(Lp, v, i, j, n) : (real)1;
-            const real mup = drift_component<FCAP, real, TERMINAL, LV>(S, R, Lp, v.lognormal, f, sv);
-            const real Y = X[j] + ((mup - MU[j]) * (real)0.5) * dt;
-            const real Ln = to_value<real>(Y, v.lognormal, v.capped, v.cap);
-            X[j] = v.carry ? Y : Ln;
-            if (paths && active) paths[(size_t)row_next[j] * P + p] = Ln;
-        }
-    }
-}
-
 template <int FCAP, typename real>
 __device__ __forceinline__ void load_increments(real (&dW)[FCAP], const real* __restrict__ dw, int64_t pc, int nT,
                                                 int i, int F) {
@@ -321,19 +197,17 @@ __device__ __forceinline__ void load_increments(real (&dW)[FCAP], const real* __
     for (int k = 0; k < FCAP; k++) dW[k] = (k < F) ? dw[((size_t)pc * nT + i) * F + k] : (real)0;
 }
 
-// Variants of the path kernel (template argument VAR):
-//   kVarFunctional  spot measure, EULER_FUNCTIONAL (the array holds the values)
-//   kVarLogState    spot measure, log-normal, the log-forwards carried between steps and
-//                   capped as log-states, contraction allowed
-//   kVarGeneral     spot or terminal measure, any state space, functional or carried state
-//   kVarGeneralPC   kVarGeneral with the predictor-corrector
-//   kVarGeneralLV, kVarGeneralPCLV  the same with a local-volatility covariance wrapper
-constexpr int kVarFunctional = 0, kVarLogState = 1, kVarGeneral = 2, kVarGeneralPC = 3, kVarGeneralLV = 4,
-              kVarGeneralPCLV = 5;
+// Variants of this path kernel (template argument VAR), both spot measure, Euler:
+//   kVarFunctional  EULER_FUNCTIONAL (the array holds the values)
+//   kVarLogState    log-normal, the log-forwards carried between steps and capped as
+//                   log-states, contraction allowed
+// The spot-measure pricing kernels are in lmm_spot.hpp, the terminal measure, the carried
+// schemes, the predictor-corrector and the volatility wrappers in lmm_general.hpp.
+constexpr int kVarFunctional = 0, kVarLogState = 1;
 
 template <int NCAP, int FCAP, typename real, int VAR = kVarFunctional>
 __global__ void __launch_bounds__(kStepThreads)
-__attribute__((amdgpu_waves_per_eu((VAR == kVarLogState && NCAP == 40) ? 3 : ((VAR >= kVarGeneral && NCAP == 40) ? 2 : 1))))
+__attribute__((amdgpu_waves_per_eu((VAR == kVarLogState && NCAP == 40) ? 3 : 1)))
 lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* __restrict__ delta,
                  const double* __restrict__ times, const int32_t* __restrict__ first_alive,
                  const int32_t* __restrict__ tenor_ti, const KProduct* __restrict__ products,
@@ -344,9 +218,6 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
                  real* __restrict__ ckpt)
 {
     constexpr bool LOGSTATE = VAR == kVarLogState;
-    constexpr bool GENERAL = VAR >= kVarGeneral;
-    constexpr bool PC = VAR == kVarGeneralPC || VAR == kVarGeneralPCLV;
-    constexpr bool LV = VAR >= kVarGeneralLV;
     const int64_t p = (int64_t)blockIdx.x * kStepThreads + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int64_t wave = p >> 6;
@@ -358,22 +229,15 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
     const bool lognormal = a.lognormal != 0;
     const bool capped = a.capped != 0;
     const real cap = (real)a.cap;
-    const Variant<real> var{lognormal, capped, GENERAL && a.carry != 0, cap, (real)a.lv_alpha, (const real*)a.lvb,
-                            (const real*)a.svs};
-    const bool terminal = GENERAL && a.terminal != 0;
-    // the array holds states (carried schemes) or values; val gives the value
+    // the array holds log-forwards (LOGSTATE) or values; val gives the value
     const real logcap = (real)a.logcap;
-    auto val = [&](real x) -> real {
-        return LOGSTATE ? dev_exp<real>(x) : (var.carry ? to_value<real>(x, lognormal, capped, cap) : x);
-    };
+    auto val = [&](real x) -> real { return LOGSTATE ? dev_exp<real>(x) : x; };
 
     real L[NCAP];
 #pragma unroll
-    for (int j = 0; j < NCAP; j++) L[j] = (j < n) ? (real)((LOGSTATE || var.carry) ? a.Y0[j] : a.L0[j]) : (real)0;
-    real MU[NCAP];      // predictor drift (PC only)
+    for (int j = 0; j < NCAP; j++) L[j] = (j < n) ? (real)(LOGSTATE ? a.Y0[j] : a.L0[j]) : (real)0;
 
     double N = 1.0, Nprev = 1.0;   // unadjusted numeraire N(T_knum) and the one before it
-    double N0 = 1.0;               // N(T_0) (terminal measure)
     int kcur = 0, knum = 0;
     int qp = 0;
 
@@ -384,30 +248,8 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
 #pragma unroll
             for (int j = 0; j < NCAP; j++) if (j < n) ckpt[(base + j) * P + p] = val(L[j]);
         }
-        // ---- terminal measure: N(T_k) = prod_{l >= k} 1/(1 + L_l(T_k) d_l) at the time index
-        // of T_k (LMM:1020-1042); the time-0 forwards are Scalars, whose discount is
-        // 1/(L (d/v) + 1/v) (stochastic/Scalar.java:320-328); later ones are random
-        // variables: 1/(L d + 1), then v / (1 + L d) (RandomVariableFromDoubleArray.java:1308-1333)
-        while (terminal && kcur <= n && tenor_ti[kcur] <= ti) {
-            double Nn = 1.0;
-#pragma unroll
-            for (int l = 0; l < NCAP; l++) {
-                if (l < kcur || l >= n) continue;
-                const double Ll = (double)val(L[l]);
-                const double dl = delta[l];
-                if (ti == 0) Nn = 1.0 / (Ll * (dl / Nn) + 1.0 / Nn);
-                else if (l == kcur) Nn = 1.0 / (Ll * dl + 1.0);
-                else Nn = Nn / (1.0 + Ll * dl);
-            }
-            if (kcur == 0) N0 = Nn;
-            Nprev = N; N = Nn; knum = kcur; kcur++;
-            // sum of N(T_0)/N(T_k) over paths (numeraire adjustment average)
-            const double s = wave_reduce_sum(active ? (1.0 / N) * N0 : 0.0);
-            if (lane == 0 && cnt > 0) wave_sum[(size_t)wave * stride + a.num_slots + knum] = s;
-            if (numeraire_out && active) numeraire_out[(size_t)knum * P + p] = N;
-        }
         // ---- spot measure: N(T_{k+1}) = N(T_k).accrue(L_k(T_k), d_k) once L_k is final
-        while (!terminal && kcur < n && tenor_ti[kcur] <= ti) {
+        while (kcur < n && tenor_ti[kcur] <= ti) {
             const double Lk = (double)val(pick<NCAP, real>(L, kcur));
             const double dk = delta[kcur];
             double Nn;
@@ -510,15 +352,8 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
         load_increments<FCAP, real>(dW, dw, pc, nT, ti, F);
         const real dt = (real)(times[ti + 1] - times[ti]);
         const int32_t* row_next = row_of + (size_t)(ti + 1) * n;
-        if constexpr (GENERAL) {
-            if (terminal) general_step<NCAP, FCAP, real, true, PC, LV>(L, MU, n, ti, first_alive[ti], dt, dW, rec,
-                                                                       var, paths, row_next, p, P, active);
-            else general_step<NCAP, FCAP, real, false, PC, LV>(L, MU, n, ti, first_alive[ti], dt, dW, rec, var,
-                                                               paths, row_next, p, P, active);
-        } else {
-            euler_step<NCAP, FCAP, real, LOGSTATE, 1, kPrefetchRecords>(L, n, ti, first_alive[ti], dt, dW, rec, lognormal, capped, cap,
-                                                   logcap, paths, row_next, p, P, active);
-        }
+        euler_step<NCAP, FCAP, real, LOGSTATE, 1, kPrefetchRecords>(L, n, ti, first_alive[ti], dt, dW, rec, lognormal,
+                                                                    capped, cap, logcap, paths, row_next, p, P, active);
     }
 }
 
