@@ -1,30 +1,10 @@
-// Wavefront reductions and the reciprocal of the adjoint sweep (lmm_spot_adjoint.hpp).
+// Wavefront reductions and the reciprocal of the adjoint sweep (lmm_spot_adjoint.hpp); the
+// single-value DPP sum is in lmm_step.hpp.
 #pragma once
 
 #include "lmm_step.hpp"
 
 namespace fm {
-
-// Wavefront sum by DPP row operations (quad_perm [1,0,3,2], [2,3,0,1], row_ror 4 and 8,
-// row_bcast 15 and 31): the total is valid in lane 63 only. Each step moves the two
-// 32-bit halves of the double; lanes without a source read 0.
-template <int CTRL>
-__device__ __forceinline__ double dpp_move(double x) {
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
-__device__ __forceinline__ double wave_sum_dpp(double v) {
-    v = v + dpp_move<0xb1>(v);
-    v = v + dpp_move<0x4e>(v);
-    v = v + dpp_move<0x124>(v);
-    v = v + dpp_move<0x128>(v);
-    v = v + dpp_move<0x142>(v);
-    v = v + dpp_move<0x143>(v);
-    return v;
-}
 
 // Wavefront sums of four per-lane values at once (a transposed butterfly): two exchange
 // steps halve the values a lane carries, four more sum across the wavefront. Lane l < 4

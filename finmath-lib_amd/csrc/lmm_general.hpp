@@ -92,9 +92,12 @@ __device__ __forceinline__ real drift_component(real (&S)[FCAP], const Rec<FCAP,
 // LV: the loadings of the drift and the diffusion carry the local-volatility factor of the
 // value the drift is evaluated on (the start value; the predicted value in the corrector) and
 // the path's stochastic volatility scaling at t_i.
-template <int NCH, int FCAP, typename real, bool TERMINAL, bool PC, bool LV, bool LOGCARRY>
+// STAGE: the step's increments are read from this thread's LDS slots sdw[k * kStepThreads]
+// instead of registers (the FP64 predictor-corrector: it frees the registers a third wave per
+// SIMD needs).
+template <int NCH, int FCAP, typename real, bool TERMINAL, bool PC, bool LV, bool LOGCARRY, bool STAGE>
 __device__ __forceinline__ void general_step(ChunkedForwards<NCH, real>& X, int n, int i, int first, real dt,
-                                             const real (&dW)[FCAP], const real* __restrict__ rec,
+                                             const real (&dW)[FCAP], const real* sdw, const real* __restrict__ rec,
                                              const Variant<real>& v, real* __restrict__ paths,
                                              const int32_t* __restrict__ row_next, int64_t p, int P, bool active) {
     constexpr int W = chunk_width<real>();
@@ -118,7 +121,18 @@ __device__ __forceinline__ void general_step(ChunkedForwards<NCH, real>& X, int 
             real Y = (LOGCARRY || v.carry) ? x : (v.lognormal ? dev_log<real>(Lj) : Lj);
             Y = Y + mu * dt;
 #pragma unroll
-            for (int k = 0; k < FCAP; k++) Y = Y + dW[k] * (LV ? (R.lam(k) * f) * sv : R.lam(k));
+            for (int k = 0; k < FCAP; k++) {
+                real d = dW[k];
+                if constexpr (STAGE) {
+                    // read from LDS in the component loop: the empty asm keeps the loads there
+                    const real* q = sdw;
+#if defined(__HIP_DEVICE_COMPILE__)
+                    asm volatile("" : "+v"(q));
+#endif
+                    d = q[k * kStepThreads];
+                }
+                Y = Y + d * (LV ? (R.lam(k) * f) * sv : R.lam(k));
+            }
             if constexpr (PC) {
                 const real Lp = to_value<real>(Y, v.lognormal, v.capped, v.cap);    // predicted value
                 const real fp = LV ? local_vol_factor<real>(Lp, v, i, j, n) : (real)1;
@@ -144,6 +158,7 @@ __device__ __forceinline__ void general_step(ChunkedForwards<NCH, real>& X, int 
 // initial log-forwards capped at log(cap); a.L0 the initial forwards.
 template <int NCH, int FCAP, typename real, bool TERMINAL, bool PC, bool LV, int SS>
 __global__ void __launch_bounds__(kStepThreads)
+__attribute__((amdgpu_waves_per_eu((PC && !LV && sizeof(real) == 8 && NCH <= 5) ? 3 : 1)))
 lmm_general_kernel(const StepArgs a, const real* __restrict__ rec, const double* __restrict__ delta,
                    const double* __restrict__ times, const int32_t* __restrict__ first_alive,
                    const int32_t* __restrict__ tenor_ti, const KProduct* __restrict__ products,
@@ -156,7 +171,8 @@ lmm_general_kernel(const StepArgs a, const real* __restrict__ rec, const double*
     constexpr int W = chunk_width<real>();
     const int64_t p = (int64_t)blockIdx.x * kStepThreads + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const int64_t wave = p >> 6;
+    // the wavefront index, uniform: kept in a scalar register
+    const int64_t wave = (int64_t)blockIdx.x * (kStepThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int P = a.P, n = a.n, F = a.F, nT = a.nT;
     const bool active = p < P;
     const int64_t pc = active ? p : (int64_t)P - 1;
@@ -165,7 +181,9 @@ lmm_general_kernel(const StepArgs a, const real* __restrict__ rec, const double*
     const bool lognormal = SS >= 0 ? (SS & 1) != 0 : a.lognormal != 0;
     const bool carry = SS >= 0 ? (SS & 2) != 0 : a.carry != 0;
     constexpr bool LOGCARRY = carries_log<real>() && SS == 1;
-    const bool capped = a.capped != 0;
+    // the cap is applied always: an absent cap is +inf (log-cap +inf), and jmin(x, +inf) = x
+    // for every x, NaN included
+    constexpr bool capped = true;
     const real cap = (real)a.cap;
     const Variant<real> var{lognormal, capped, carry, cap, (real)a.logcap, (real)a.lv_alpha,
                             LV ? (const real*)a.lvb : nullptr, LV ? (const real*)a.svs : nullptr};
@@ -174,6 +192,10 @@ lmm_general_kernel(const StepArgs a, const real* __restrict__ rec, const double*
         return LOGCARRY ? dev_exp<real>(x) : (carry ? to_value<real>(x, lognormal, capped, cap) : x);
     };
 
+    // FP64 predictor-corrector: the increments in LDS, [k][thread]; each thread reads only its
+    // own slots, after writing them, so no barrier is needed
+    constexpr bool STAGE = PC && !LV && sizeof(real) == 8;
+    __shared__ real sdw[STAGE ? FCAP * kStepThreads : 1];
     ChunkedForwards<NCH, real> X;
     static_for<0, W * NCH>([&](auto J) {
         constexpr int j = decltype(J)::value;
@@ -245,11 +267,23 @@ lmm_general_kernel(const StepArgs a, const real* __restrict__ rec, const double*
         if (ti == nT) break;
 
         // ---- step ti -> ti+1
+        // the path index made opaque per step: its products with the increments' strides are
+        // recomputed instead of being hoisted into registers held across the whole path
+        int64_t pcs = p;
+#if defined(__HIP_DEVICE_COMPILE__)
+        asm volatile("" : "+v"(pcs));
+#endif
+        pcs = pcs < P ? pcs : (int64_t)P - 1;
         real dW[FCAP];
-        load_increments<FCAP, real>(dW, dw, pc, nT, ti, F);
+        load_increments<FCAP, real>(dW, dw, pcs, nT, ti, F);
+        if constexpr (STAGE) {
+#pragma unroll
+            for (int k = 0; k < FCAP; k++) sdw[k * kStepThreads + threadIdx.x] = dW[k];
+        }
         const real dt = (real)(times[ti + 1] - times[ti]);
         const int32_t* row_next = row_of + (size_t)(ti + 1) * n;
-        general_step<NCH, FCAP, real, TERMINAL, PC, LV, LOGCARRY>(X, n, ti, first_alive[ti], dt, dW, rec, var, paths, row_next,
+        general_step<NCH, FCAP, real, TERMINAL, PC, LV, LOGCARRY, STAGE>(X, n, ti, first_alive[ti], dt, dW,
+                                                               sdw + threadIdx.x, rec, var, paths, row_next,
                                                         p, P, active);
     }
 }

@@ -259,7 +259,8 @@ lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __
 {
     const int64_t p = (int64_t)blockIdx.x * kStepThreads + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const int64_t wave = p >> 6;
+    // the wavefront index, uniform: kept in a scalar register
+    const int64_t wave = (int64_t)blockIdx.x * (kStepThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int P = a.P, n = a.n, F = a.F, nT = a.nT;
     const bool active = p < P;
     const int64_t pc = active ? p : (int64_t)P - 1;

@@ -45,7 +45,8 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
     constexpr int W = chunk_width<real>();
     const int64_t p = (int64_t)blockIdx.x * kStepThreads + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const int64_t wave = p >> 6;
+    // the wavefront index, uniform: kept in a scalar register
+    const int64_t wave = (int64_t)blockIdx.x * (kStepThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int P = a.P, n = a.n, F = a.F, nT = a.nT;
     const bool active = p < P;
     const int64_t pc = active ? p : (int64_t)P - 1;

@@ -31,10 +31,35 @@ constexpr int kStepThreads = 256;
 // scalar-cache prefetch of each step's records in the spot-measure Euler kernels
 constexpr bool kPrefetchRecords = true;
 
-__device__ __forceinline__ double wave_reduce_sum(double v) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+// Wavefront sum by DPP row operations (quad_perm [1,0,3,2], [2,3,0,1], row_ror 4 and 8,
+// row_bcast 15 and 31): the total is valid in lane 63 only. Each step moves the two
+// 32-bit halves of the double; lanes without a source read 0. Call with every lane of the
+// wavefront active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v = v + dpp_move<0xb1>(v);
+    v = v + dpp_move<0x4e>(v);
+    v = v + dpp_move<0x124>(v);
+    v = v + dpp_move<0x128>(v);
+    v = v + dpp_move<0x142>(v);
+    v = v + dpp_move<0x143>(v);
     return v;
+}
+
+// Wavefront sum, returned in every lane: the DPP sum read from lane 63. Unlike a butterfly of
+// lane permutes it keeps no per-lane permute addresses live across the kernel.
+__device__ __forceinline__ double wave_reduce_sum(double v) {
+    const long long b = __double_as_longlong(wave_sum_dpp(v));
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 template <typename real> __device__ __forceinline__ real dev_exp(real x);
@@ -220,7 +245,8 @@ lmm_fused_kernel(const StepArgs a, const real* __restrict__ rec, const double* _
     constexpr bool LOGSTATE = VAR == kVarLogState;
     const int64_t p = (int64_t)blockIdx.x * kStepThreads + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const int64_t wave = p >> 6;
+    // the wavefront index, uniform: kept in a scalar register
+    const int64_t wave = (int64_t)blockIdx.x * (kStepThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int P = a.P, n = a.n, F = a.F, nT = a.nT;
     const bool active = p < P;
     const int64_t pc = active ? p : (int64_t)P - 1;
