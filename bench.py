@@ -318,7 +318,7 @@ def main():
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": PATH_KERNEL_TRAFFIC_BYTES,
                      "alu": "FP64 vector (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
                      "hbm_gbps_at_traffic": PATH_KERNEL_TRAFFIC_BYTES / (step_ms * 1e-3) / 1e9,
-                     "kernel": "lmm_fused_kernel<40,5,double>", "flops_per_launch": flops},
+                     "kernel": "lmm_spot_kernel<5,5,double,true>", "flops_per_launch": flops},
     }
     if not args.no_fp32:
         wl2 = W.c2()

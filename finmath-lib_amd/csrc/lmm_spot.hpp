@@ -17,15 +17,14 @@
 // log-forward is at least 2 in magnitude, that is for forwards below 13.5 %); the cap is
 // applied to the state as Y <= log(cap). Products read L_j = exp(Y_j) where they are evaluated.
 // FP32-mixed: the value is carried and the state re-derived each step with the hardware
-// logarithm, as the fused kernel the adjoint's forward pass runs does it, so that the pricing
-// pass and the adjoint see the same float paths.
+// logarithm. The adjoint's forward pass is this kernel, so it sees the priced float paths.
 //
-// Layout: the forwards of a path live in registers as chunks of eight (one vector register
-// block per chunk). Components are processed by a rolled loop per chunk that reads and writes
-// its element with an indexed register move, so the kernel body holds one copy of the
-// component update per chunk instead of one per forward: a small instruction footprint and
-// about 156 VGPRs at 40 forwards x 5 factors in FP64 (three waves per SIMD). The step's model
-// records are read through the scalar cache, one 64-byte record per component.
+// Layout: the forwards of a path live in registers as chunks of sixteen VGPRs (eight FP64 or
+// sixteen FP32 forwards). Components are processed by a rolled loop per chunk that reads and
+// writes its element with an indexed register move (s_set_gpr_idx_on), so the kernel body holds
+// one copy of the component update per chunk instead of one per forward: a small instruction
+// footprint and about 160 VGPRs at 40 forwards x 5 factors in FP64 (three waves per SIMD). The
+// step's model records are read through the scalar cache, one 64-byte record per component.
 #pragma once
 
 #include <type_traits>
@@ -246,7 +245,11 @@ __device__ __forceinline__ void eval_products_at(int ti, int& qp, int num_produc
     }
 }
 
-template <int NCH, int FCAP, typename real, bool lognormal>
+// BUILD tells apart instantiations compiled with different floating-point flags (0: without
+// contraction, the FP64 and FP32-mixed builds; 1: with fused multiply-add, FP64_LOGSTATE). Two
+// translation units that instantiated one kernel under one name with different flags would
+// give one definition to both: the linker keeps a single copy of a template instantiation.
+template <int NCH, int FCAP, typename real, bool lognormal, int BUILD = 0>
 __global__ void __launch_bounds__(kStepThreads)
 lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __restrict__ delta,
                 const double* __restrict__ times, const int32_t* __restrict__ first_alive,
@@ -323,43 +326,43 @@ lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __
     }
 }
 
-template <int NCH, int FCAP, typename real, bool lognormal>
+template <int NCH, int FCAP, typename real, bool lognormal, int BUILD>
 void launch_spot_ss(const StepArgs& a, hipStream_t stream) {
     const int64_t blocks = ((int64_t)a.P + kStepThreads - 1) / kStepThreads;
-    hipLaunchKernelGGL((lmm_spot_kernel<NCH, FCAP, real, lognormal>), dim3((unsigned)blocks), dim3(kStepThreads), 0,
+    hipLaunchKernelGGL((lmm_spot_kernel<NCH, FCAP, real, lognormal, BUILD>), dim3((unsigned)blocks), dim3(kStepThreads), 0,
                        stream, a, (const real*)a.rec, a.delta, a.times, a.first_alive, a.tenor_time_index, a.products,
                        a.periods, (const real*)a.dw, a.wave_sum, a.wave_m2, a.contrib, (real*)a.paths, a.row_of,
                        a.numeraire, (real*)a.ckpt);
     HIP_CHECK(hipGetLastError());
 }
 
-template <int NCH, int FCAP, typename real>
+template <int NCH, int FCAP, typename real, int BUILD>
 void launch_spot_cap(const StepArgs& a, hipStream_t stream) {
-    if (a.lognormal) launch_spot_ss<NCH, FCAP, real, true>(a, stream);
-    else launch_spot_ss<NCH, FCAP, real, false>(a, stream);
+    if (a.lognormal) launch_spot_ss<NCH, FCAP, real, true, BUILD>(a, stream);
+    else launch_spot_ss<NCH, FCAP, real, false, BUILD>(a, stream);
 }
 
-template <int NCH, typename real>
+template <int NCH, typename real, int BUILD>
 void launch_spot_n(const StepArgs& a, hipStream_t stream) {
-    if (a.F <= 1) launch_spot_cap<NCH, 1, real>(a, stream);
-    else if (a.F <= 3) launch_spot_cap<NCH, 3, real>(a, stream);
-    else if (a.F <= 5) launch_spot_cap<NCH, 5, real>(a, stream);
-    else if (a.F <= 10) launch_spot_cap<NCH, 10, real>(a, stream);
+    if (a.F <= 1) launch_spot_cap<NCH, 1, real, BUILD>(a, stream);
+    else if (a.F <= 3) launch_spot_cap<NCH, 3, real, BUILD>(a, stream);
+    else if (a.F <= 5) launch_spot_cap<NCH, 5, real, BUILD>(a, stream);
+    else if (a.F <= 10) launch_spot_cap<NCH, 10, real, BUILD>(a, stream);
     else throw FmError(FM_ERR_UNSUPPORTED, "number of factors above 10 is not supported by the path kernel");
 }
 
 // The spot Euler kernel of the pricing and simulation passes. a.Y0 holds the initial states:
 // FP64 log-normal the log-forwards capped at log(cap) (a.logcap = log(cap)), otherwise the
 // capped initial forwards.
-template <typename real>
+template <typename real, int BUILD = 0>
 void launch_spot(const StepArgs& a, hipStream_t stream) {
     // 1, 3, 5, 10 chunks of eight FP64 forwards; 1, 2, 3, 5 chunks of sixteen FP32 forwards
     constexpr int W = chunk_width<real>();
     constexpr int N2 = W == 8 ? 3 : 2, N3 = W == 8 ? 5 : 3, N4 = W == 8 ? 10 : 5;
-    if (a.n <= W) launch_spot_n<1, real>(a, stream);
-    else if (a.n <= W * N2) launch_spot_n<N2, real>(a, stream);
-    else if (a.n <= W * N3) launch_spot_n<N3, real>(a, stream);
-    else if (a.n <= 80) launch_spot_n<N4, real>(a, stream);
+    if (a.n <= W) launch_spot_n<1, real, BUILD>(a, stream);
+    else if (a.n <= W * N2) launch_spot_n<N2, real, BUILD>(a, stream);
+    else if (a.n <= W * N3) launch_spot_n<N3, real, BUILD>(a, stream);
+    else if (a.n <= 80) launch_spot_n<N4, real, BUILD>(a, stream);
     else throw FmError(FM_ERR_UNSUPPORTED, "more than 80 forward rates are not supported by the path kernel");
 }
 

@@ -184,9 +184,10 @@ __device__ __forceinline__ void neumaier_add(double& s, double& c, double x) {
 // weight). Each thread visits its elements in increasing order (grid-stride) and the
 // block tree is fixed, so the partials depend only on the size and the grid.
 // Two elements per thread share each instruction's dispatch; the light instantiation is
-// held to 168 VGPRs (3 waves per SIMD, no spills), the full one takes what it needs.
+// held to 256 VGPRs (2 waves per SIMD; at 168 for three it spills), the full one takes what
+// it needs.
 template <bool FULL>
-__global__ void __launch_bounds__(kRvThreads) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 3))) rv_eval_kernel(const RvProg g, double* __restrict__ out,
+__global__ void __launch_bounds__(kRvThreads) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 2))) rv_eval_kernel(const RvProg g, double* __restrict__ out,
                                                                double* __restrict__ partials) {
     constexpr int E = kRvPerThread;
     const int64_t span = (int64_t)gridDim.x * kRvThreads * E;
