@@ -158,7 +158,7 @@ __device__ __forceinline__ void general_step(ChunkedForwards<NCH, real>& X, int 
 // initial log-forwards capped at log(cap); a.L0 the initial forwards.
 template <int NCH, int FCAP, typename real, bool TERMINAL, bool PC, bool LV, int SS>
 __global__ void __launch_bounds__(kStepThreads)
-__attribute__((amdgpu_waves_per_eu((PC && !LV && sizeof(real) == 8 && NCH <= 5 && FCAP <= 5) ? 3 : 1)))
+__attribute__((amdgpu_waves_per_eu((!LV && sizeof(real) == 8 && NCH <= 5 && FCAP <= 5) ? 3 : 1)))
 lmm_general_kernel(const StepArgs a, const real* __restrict__ rec, const double* __restrict__ delta,
                    const double* __restrict__ times, const int32_t* __restrict__ first_alive,
                    const int32_t* __restrict__ tenor_ti, const KProduct* __restrict__ products,

@@ -145,6 +145,42 @@ def test_prices_without_discount_curve_and_without_cap():
         assert abs(vals[q] - rv) <= 1e-10 * abs(rv)
 
 
+def test_products_sharing_an_exercise_date_match_oracle():
+    # several products evaluated at one time index: five swaptions over the same periods (flat and
+    # per-period strikes, other notionals), interleaved with a caplet, a swaption over fewer
+    # periods and a swap
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    P = 6000
+    say(f"case products at one exercise date c0 P={P}")
+    fix, pay = [2.0, 2.5, 3.0, 3.5], [2.5, 3.0, 3.5, 4.0]
+    sw = [dict(type="swaption", exercise=2.0, fixing=fix, payment=pay, strike=[0.03] * 4),
+          dict(type="swaption", exercise=2.0, fixing=fix, payment=pay, strike=[0.02, 0.025, 0.03, 0.035], notional=2.0),
+          dict(type="swaption", exercise=2.0, fixing=fix[:3], payment=pay[:3], strike=[0.03] * 3),
+          dict(type="swaption", exercise=2.0, fixing=fix, payment=pay, strike=[0.04] * 4),
+          dict(type="swaption", exercise=2.0, fixing=fix, payment=pay, strike=[0.05] * 4),
+          dict(type="swaption", exercise=2.0, fixing=fix, payment=pay, strike=[0.01] * 4, notional=0.5)]
+    swap = dict(type="swap", fixing=[1.0, 1.5, 2.0], payment=[1.5, 2.0, 2.5], strike=[0.03] * 3)
+    ctx = make_context(wl, t, P)
+    prods = []
+    for s in sw[:2]:
+        prods.append(_abi.Product.swaption(s["exercise"], s["fixing"], s["payment"], s["strike"],
+                                           notional=s.get("notional", 1.0)))
+    prods.append(_abi.Product.caplet(2.0, 0.5, 0.03))
+    for s in sw[2:]:
+        prods.append(_abi.Product.swaption(s["exercise"], s["fixing"], s["payment"], s["strike"],
+                                           notional=s.get("notional", 1.0)))
+    prods.append(_abi.Product.swap(swap["fixing"], swap["payment"], swap["strike"]))
+    vals, errs = ctx.value_products(prods)
+    om = O.OracleModel(t, P)
+    ref = [om.swaption(s) for s in sw[:2]] + [om.caplet(2.0, 0.5, 0.03)] + [om.swaption(s) for s in sw[2:]] + \
+          [om.swap(swap)]
+    for q, (rv, re_) in enumerate(ref):
+        say(f"  q={q}: gpu {vals[q]:.15e} oracle {rv:.15e}")
+        assert abs(vals[q] - rv) <= 1e-10 * abs(rv)
+        assert abs(errs[q] - re_) <= 1e-8 * abs(re_)
+
+
 def test_repeated_pricing_is_deterministic():
     wl = W.c0()
     t = O.ModelTables(wl)
