@@ -40,10 +40,13 @@ import numpy as np  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X published FP64 peak (vector and matrix)
 HBM_PEAK_GBPS = 8000.0       # MI355X published HBM3E bandwidth
-# HBM bytes per launch of lmm_fused_kernel<40,5,double> on the 40 x 5 x 1,000,000-path strip:
+# HBM bytes per launch of lmm_spot_kernel<5,5,double,true,0> on the 40 x 5 x 1,000,000-path strip:
 # FETCH_SIZE and WRITE_SIZE counter passes scaled by a plain 1 GiB copy
-# (profiles/lmm_fused_fp64_c1_hbm_traffic.md)
-PATH_KERNEL_TRAFFIC_BYTES = 1.983e9
+# (profiles/lmm_spot_fp64_c1_hbm_traffic.md)
+PATH_KERNEL_TRAFFIC_BYTES = 2.672e9
+# VALU-busy fraction of the same kernel from its SQ counter pass: 4 SQ_ACTIVE_INST_VALU over
+# 1,024 SIMDs x GRBM_GUI_ACTIVE / 8 (profiles/lmm_spot_c1_sq_counters.md)
+PATH_KERNEL_VALU_BUSY = 0.746
 FP32_PEAK_TFLOPS = 157.3     # MI355X published FP32 peak
 
 
@@ -318,7 +321,9 @@ def main():
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": PATH_KERNEL_TRAFFIC_BYTES,
                      "alu": "FP64 vector (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
                      "hbm_gbps_at_traffic": PATH_KERNEL_TRAFFIC_BYTES / (step_ms * 1e-3) / 1e9,
-                     "kernel": "lmm_spot_kernel<5,5,double,true>", "flops_per_launch": flops},
+                     "kernel": "lmm_spot_kernel<5,5,double,true,0>", "flops_per_launch": flops,
+                     "valu_busy": PATH_KERNEL_VALU_BUSY,
+                     "valu_busy_source": "SQ counter pass, profiles/lmm_spot_c1_sq_counters.md"},
     }
     if not args.no_fp32:
         wl2 = W.c2()
