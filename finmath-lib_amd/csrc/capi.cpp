@@ -140,6 +140,14 @@ struct fm_lmm {
     DevBuf d_products, d_periods, d_wave_sum, d_wave_m2, d_contrib, d_out, d_swapmeta, d_scale;
     DevBuf d_ckpt, d_seg, d_adj_num, d_kcur, d_fsig, d_sig, d_cdiag, d_wave_grad, d_grad_out;
     DevBuf d_berm_vals, d_berm_under, d_berm_ex, d_berm_part, d_berm_coef, d_berm_col, d_berm_scale;
+    // the Bermudan induction's adjoint: triggers and choices recorded by the pricing pass
+    // ([exercise dates of all Bermudans][P]), the per-path adjoint state, the period weights
+    DevBuf d_berm_trig, d_berm_diff, d_berm_vb, d_berm_ub, d_berm_weight, d_berm_bcoef, d_berm_wcol;
+    DevBuf d_berm_wsum, d_berm_wm2, d_berm_stat;
+    bool berm_record = false;                 // the next pricing pass records triggers and choices
+    std::vector<int> berm_trig_off;           // first recorded date of each Bermudan
+    int dirac_method = FM_DIRAC_DISCRETE_DELTA;
+    double dirac_width = 0.05;
 
     int n() const { return cfg.num_libors; }
     int nT() const { return cfg.num_time_steps; }
@@ -707,6 +715,11 @@ void run_bermudan(fm_lmm* c, const BermPlan& b, int bi, const std::vector<double
             t.apply_basis = basis_ref(c, b, prev, adj);
             t.apply_coef = coef + (size_t)prev * kBermMaxBasis;
             t.apply_period = b.ex_period[(size_t)prev];
+            if (c->berm_record) {
+                const size_t row = (size_t)(c->berm_trig_off[(size_t)bi] + prev) * P;
+                t.rec_trig = c->d_berm_trig.as<double>() + row;
+                t.rec_diff = c->d_berm_diff.as<double>() + row;
+            }
         }
         t.per_col = dcol + (m - done);
         t.per_scale = dscale + (m - done);
@@ -722,6 +735,71 @@ void run_bermudan(fm_lmm* c, const BermPlan& b, int bi, const std::vector<double
         done = lo;
     }
     HIP_CHECK(hipStreamSynchronize(s));      // cols and scales are host sources of the uploads above
+}
+
+// The adjoint of Bermudan bi's induction (lmm_bermudan.hip): exercise dates ascending, from the
+// triggers and choices its pricing pass recorded; leaves dV/d(payoff) of every period per path in
+// weight[period column][P]. The DISCRETE_DELTA width of each date is width x sd(trigger).
+void run_bermudan_adjoint(fm_lmm* c, const BermPlan& b, int bi, const std::vector<double>& adj, const double* contrib,
+                          double* weight) {
+    hipStream_t s = c->stream;
+    const int P = c->P(), m = (int)b.per_col.size(), nex = (int)b.ex_period.size();
+    const size_t row0 = (size_t)c->berm_trig_off[(size_t)bi] * P;
+    const double* trig = c->d_berm_trig.as<double>() + row0;
+    const double* diff = c->d_berm_diff.as<double>() + row0;
+    std::vector<double> eps((size_t)std::max(nex, 1), 0.0);
+    if (c->dirac_method == FM_DIRAC_DISCRETE_DELTA && nex > 0) {
+        const int num_waves = (P + 63) / 64;
+        double* wsum = (double*)c->d_berm_wsum.get((size_t)num_waves * nex * sizeof(double));
+        double* wm2 = (double*)c->d_berm_wm2.get((size_t)num_waves * nex * sizeof(double));
+        double* out = (double*)c->d_berm_stat.get((size_t)2 * nex * sizeof(double));
+        launch_berm_trigger_stats(trig, nex, P, wsum, wm2, nex, s);
+        launch_reduce_partials(wsum, wm2, num_waves, nex, nex, 0, P, out, out + nex, s);
+        std::vector<double> host((size_t)2 * nex);
+        HIP_CHECK(hipMemcpyAsync(host.data(), out, host.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        // getStandardDeviation: the square root of the population variance
+        for (int e = 0; e < nex; e++) eps[(size_t)e] = c->dirac_width * std::sqrt(host[(size_t)(nex + e)] / (double)P);
+    }
+    const int32_t* wcol = upload(c->d_berm_wcol, b.per_col, s);       // ascending periods
+    double* vb = (double*)c->d_berm_vb.get((size_t)P * sizeof(double));
+    double* ub = (double*)c->d_berm_ub.get((size_t)P * sizeof(double));
+    const int nb = berm_num_blocks(P);
+    double* part = (double*)c->d_berm_part.get((size_t)nb * kBermParts * sizeof(double));
+    double* bcoef = (double*)c->d_berm_bcoef.get((size_t)std::max(nex, 1) * kBermMaxBasis * sizeof(double));
+    BermAdjStep a{};
+    a.P = P;
+    a.callable = b.callable;
+    a.method = c->dirac_method;
+    a.contrib = contrib;
+    a.weight = weight;
+    a.vb = vb; a.ub = ub;
+    a.partials = part;
+    int lo = 0;         // first period whose weight is still to be written
+    for (int e = 0; e <= nex; e++) {
+        BermAdjStep t = a;
+        t.first = e == 0 ? 1 : 0;
+        t.finish = (e > 0 && c->dirac_method != FM_DIRAC_ZERO) ? 1 : 0;
+        if (t.finish) {
+            t.fin_basis = basis_ref(c, b, e - 1, adj);
+            t.fin_coef = bcoef + (size_t)(e - 1) * kBermMaxBasis;
+        }
+        const int hi = e < nex ? b.ex_period[(size_t)e] : m;
+        t.w_col = wcol + lo;
+        t.num_w = hi - lo;
+        t.decide = e < nex ? 1 : 0;
+        if (t.decide) {
+            t.dec_basis = basis_ref(c, b, e, adj);
+            t.trig = trig + (size_t)e * P;
+            t.diff = diff + (size_t)e * P;
+            t.eps = eps[(size_t)e];
+        }
+        launch_berm_adjoint_step(t, b.basis, s);
+        if (t.decide && c->dirac_method != FM_DIRAC_ZERO)
+            launch_berm_solve(part, nb, b.num_basis, P, bcoef + (size_t)e * kBermMaxBasis, s);
+        lo = hi;
+    }
+    HIP_CHECK(hipStreamSynchronize(s));      // per_col is the host source of an upload above
 }
 
 // Scratch bytes of a pass over single-date products with the per-call buffers in caller memory
@@ -1285,6 +1363,22 @@ int fm_lmm_set_volatility_factorization(fm_lmm* c, const double* volatility, con
     FM_CATCH
 }
 
+int fm_lmm_set_dirac_delta(fm_lmm* c, int32_t method, double width) {
+    FM_TRY
+    require(c != nullptr, FM_ERR_INVALID_ARGUMENT, "NULL context");
+    require(method == FM_DIRAC_DISCRETE_DELTA || method == FM_DIRAC_ONE || method == FM_DIRAC_ZERO ||
+                method == FM_DIRAC_REGRESSION_ON_DENSITY || method == FM_DIRAC_REGRESSION_ON_DISTRIBUTION,
+            FM_ERR_INVALID_ARGUMENT, "unknown Dirac delta approximation method");
+    require(method != FM_DIRAC_REGRESSION_ON_DENSITY && method != FM_DIRAC_REGRESSION_ON_DISTRIBUTION,
+            FM_ERR_UNSUPPORTED, "the regression Dirac delta approximations are not implemented");
+    require(width > 0.0, FM_ERR_INVALID_ARGUMENT, "the Dirac delta width must be positive");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->dirac_method = method;
+    c->dirac_width = width;
+    return FM_OK;
+    FM_CATCH
+}
+
 int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const double* weights, double* value,
                    double* d_initial_forwards, double* d_volatility) {
     FM_TRY
@@ -1326,7 +1420,27 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     hipEvent_t e0 = c->ev[5];
     HIP_CHECK(hipEventRecord(e0, s));
     std::vector<double> sums, m2s;
-    run_pass(c, L, false, sums, m2s, ckpt, stride, num);
+    // Bermudans under ONE or DISCRETE_DELTA: the pricing pass records each decision's trigger and
+    // choice for the induction's adjoint
+    const bool berm_adjoint = !L.berm.empty() && c->dirac_method != FM_DIRAC_ZERO;
+    if (berm_adjoint) {
+        c->berm_trig_off.assign(L.berm.size(), 0);
+        int total = 0;
+        for (size_t bi = 0; bi < L.berm.size(); bi++) {
+            c->berm_trig_off[bi] = total;
+            total += (int)L.berm[bi].ex_period.size();
+        }
+        c->d_berm_trig.get((size_t)std::max(total, 1) * P * sizeof(double));
+        c->d_berm_diff.get((size_t)std::max(total, 1) * P * sizeof(double));
+        c->berm_record = true;
+    }
+    try {
+        run_pass(c, L, false, sums, m2s, ckpt, stride, num);
+    } catch (...) {
+        c->berm_record = false;
+        throw;
+    }
+    c->berm_record = false;
     const double fwd_ms = c->timings[4];
     // objective weights per lowered product: w_q * (adjustment scale) / P
     const std::vector<double> adj = adjustment_chain(c);
@@ -1394,6 +1508,13 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     ga.weight = upload(c->d_scale, kweights, s);
     ga.berm_ex = L.berm.empty() ? nullptr : c->d_berm_ex.as<int32_t>();
     ga.berm_weight = nullptr;
+    if (berm_adjoint) {
+        // dV/d(payoff) of every Bermudan period per path, [contribution column][P]
+        double* w = (double*)c->d_berm_weight.get((size_t)L.num_contrib * P * sizeof(double));
+        for (size_t bi = 0; bi < L.berm.size(); bi++)
+            run_bermudan_adjoint(c, L.berm[bi], (int)bi, adj, c->d_contrib.as<double>(), w);
+        ga.berm_weight = w;
+    }
     ga.seg = c->d_seg.get((size_t)std::max(stride - 2, 1) * n * P * c->real_size());
     // factor matrix padded to the factor capacity and -sigma_j(t_i) c_j, in the path precision
     std::vector<double> fpad((size_t)n * fcap_of(c->F()), 0.0), vterm((size_t)nT * n, 0.0);
@@ -1451,7 +1572,7 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
 extern "C++" {
 namespace {
 constexpr char kSerMagic[8] = {'F', 'M', 'L', 'M', 'M', 'S', 'E', 'R'};
-constexpr int32_t kSerVersion = 1;
+constexpr int32_t kSerVersion = 2;      // 2: the adjoint's Dirac delta method and width
 
 struct Writer {
     std::vector<unsigned char> b;
@@ -1515,6 +1636,8 @@ int fm_lmm_serialize(fm_lmm* c, void* buffer, int64_t* size) {
     if (c->has_pd) w.put_doubles(c->pd);
     if (c->has_vf) { w.put_doubles(c->vf_sig); w.put_doubles(c->vf_fac); w.put_doubles(c->vf_c); }
     if (c->has_lv) { w.put<double>(c->lv_alpha); w.put_doubles(c->lvb); }
+    w.put<int32_t>(c->dirac_method);
+    w.put<double>(c->dirac_width);
     const int64_t need = (int64_t)w.b.size();
     if (buffer) {
         require(*size >= need, FM_ERR_INVALID_ARGUMENT, "buffer too small for the serialised context");
@@ -1533,7 +1656,8 @@ int fm_lmm_deserialize(const void* buffer, int64_t size, fm_lmm** out) {
     char magic[8];
     for (char& m : magic) m = r.get<char>();
     require(std::memcmp(magic, kSerMagic, 8) == 0, FM_ERR_INVALID_ARGUMENT, "not a serialised context");
-    require(r.get<int32_t>() == kSerVersion, FM_ERR_INVALID_ARGUMENT, "unsupported serialisation version");
+    const int32_t version = r.get<int32_t>();
+    require(version == 1 || version == kSerVersion, FM_ERR_INVALID_ARGUMENT, "unsupported serialisation version");
     fm_lmm_config g{};
     g.num_time_steps = r.get<int32_t>(); g.num_libors = r.get<int32_t>(); g.num_factors = r.get<int32_t>();
     g.num_paths = r.get<int32_t>(); g.seed = r.get<int32_t>(); g.measure = r.get<int32_t>();
@@ -1556,6 +1680,9 @@ int fm_lmm_deserialize(const void* buffer, int64_t size, fm_lmm** out) {
     if (hpd) pd = r.get_doubles((int64_t)n + 1);
     if (hvf) { sig = r.get_doubles((int64_t)nT * n); fac = r.get_doubles((int64_t)n * F); cv = r.get_doubles(n); }
     if (hlv) { lv_alpha = r.get<double>(); lvb = r.get_doubles((int64_t)nT * n); }
+    int32_t dirac_method = FM_DIRAC_DISCRETE_DELTA;      // version 1: the defaults
+    double dirac_width = 0.05;
+    if (version >= 2) { dirac_method = r.get<int32_t>(); dirac_width = r.get<double>(); }
     require(r.left == 0, FM_ERR_INVALID_ARGUMENT, "trailing bytes after the serialised context");
     fm_lmm* c = nullptr;
     int rc = fm_lmm_create(&g, &c);
@@ -1566,6 +1693,7 @@ int fm_lmm_deserialize(const void* buffer, int64_t size, fm_lmm** out) {
     if (hpd && (rc = fm_lmm_set_discount_curve(c, pd.data())) != FM_OK) return rc;
     if (hvf && (rc = fm_lmm_set_volatility_factorization(c, sig.data(), fac.data(), cv.data())) != FM_OK) return rc;
     if (hlv && (rc = fm_lmm_set_local_volatility(c, lv_alpha, lvb.data())) != FM_OK) return rc;
+    if ((rc = fm_lmm_set_dirac_delta(c, dirac_method, dirac_width)) != FM_OK) return rc;
     *out = guard.release();
     return FM_OK;
     FM_CATCH

@@ -223,10 +223,41 @@ struct BermStep {
     double* wave_sum;
     double* wave_m2;
     int stride, slot;
+    double* rec_trig;            // device [P]: the applied decision's trigger per path, or NULL
+    double* rec_diff;            // device [P]: values - underlying it chose between, or NULL
 };
 int berm_num_blocks(int P);
 void launch_berm_step(const BermStep& s, int kind, void* stream);
 void launch_berm_solve(const double* partials, int num_rows, int K, int P, double* coef, void* stream);
+
+// One step of the Bermudan induction's adjoint, exercise dates ascending (the reverse of the
+// induction). Per path it carries the adjoints vb of the values and ub of the underlying.
+struct BermAdjStep {
+    int P;
+    int callable;
+    int method;                  // FM_DIRAC_*: derivative of the exercise indicator in its trigger
+    const double* contrib;       // device [columns][P] (the raw regression bases)
+    int first;                   // the state starts at vb = 1, ub = 0
+    int finish;                  // add the projection of the lower date's trigger adjoint:
+    BermBasisRef fin_basis;      //   its basis
+    const double* fin_coef;      //   device [kBermMaxBasis]: (X^T X)^+ X^T trigger-adjoint
+    const int32_t* w_col;        // device: payoff columns whose weight is the state after finish
+    int num_w;
+    double* weight;              // device [columns][P]: dV/d(payoff) of each period per path
+    int decide;                  // the exercise decision of this date:
+    BermBasisRef dec_basis;      //   its basis (regression partials of the trigger adjoint)
+    const double* trig;          //   device [P]: the trigger (the regressed continuation)
+    const double* diff;          //   device [P]: values - underlying the decision chose between
+    double eps;                  //   DISCRETE_DELTA width: width x sd(trigger)
+    double* partials;            // device [blocks][kBermParts]
+    double* vb;                  // device [P]
+    double* ub;                  // device [P]
+};
+void launch_berm_adjoint_step(const BermAdjStep& a, int kind, void* stream);
+// Per-date sums and centred second moments of trigger rows [num_dates][P] into the wave
+// partials [num_waves][stride] (columns 0 .. num_dates-1).
+void launch_berm_trigger_stats(const double* trig, int num_dates, int P, double* wave_sum, double* wave_m2, int stride,
+                               void* stream);
 
 // ---- device random variables (rv.hip) ---------------------------------------------------
 // Copy of P device realisations (double, or float widened to double) as a random variable.

@@ -17,6 +17,16 @@
 //                      symmetric positive semi-definite matrix).
 // The final step applies the first date's decision, adds the remaining periods and
 // reduces values * N(0) / w into the product's reduction slot.
+//
+// The adjoint of the induction (berm_adjoint_step_kernel) runs the exercise dates forwards with
+// the per-path adjoints of the values and of the underlying, and leaves the weight of every period
+// payoff per path for the reverse sweep of the paths. At an exercise date, choose(trigger, values,
+// underlying) passes the adjoint to the side it chose; the trigger's own derivative is the
+// DiracDeltaApproximationMethod of RandomVariableDifferentiableAADFactory.java:26-67 (AAD:396-424:
+// ONE (Y - Z), ZERO 0, DISCRETE_DELTA (Y - Z) 1{-eps/2 <= X < eps/2} / eps, eps = width x sd(X));
+// the trigger is a conditional expectation, self-adjoint (AAD:188-192), so its adjoint is regressed
+// on the same basis, whose functions are not differentiated, and added to values and subtracted
+// from the underlying.
 
 #include <hip/hip_runtime.h>
 #include "fm_internal.hpp"
@@ -53,6 +63,38 @@ __device__ __forceinline__ void berm_basis(const BermBasisRef& r, const double* 
     }
 }
 
+// Regression partials of one path: X^T y and the upper triangle of X^T X.
+template <int K>
+__device__ __forceinline__ void regression_accumulate(double (&acc)[K + K * (K + 1) / 2], double y,
+                                                      const double (&b)[K]) {
+    int e = 0;
+#pragma unroll
+    for (int i = 0; i < K; i++, e++) acc[e] += y * b[i];
+#pragma unroll
+    for (int i = 0; i < K; i++)
+#pragma unroll
+        for (int j = i; j < K; j++, e++) acc[e] += b[i] * b[j];
+}
+
+// One row of regression partials per workgroup (fixed-order wave and workgroup sums).
+template <int K>
+__device__ __forceinline__ void regression_partials(const double (&acc)[K + K * (K + 1) / 2],
+                                                    double* __restrict__ partials, int lane, int wid) {
+    constexpr int NP = K + K * (K + 1) / 2;
+    __shared__ double part[kBermThreads / 64][NP];
+#pragma unroll
+    for (int e = 0; e < NP; e++) {
+        const double r = wave_reduce_sum(acc[e]);
+        if (lane == 0) part[wid][e] = r;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < NP) {
+        double t = 0.0;
+        for (int w = 0; w < kBermThreads / 64; w++) t += part[w][threadIdx.x];
+        partials[(size_t)blockIdx.x * kBermParts + threadIdx.x] = t;
+    }
+}
+
 // Grid-stride over paths: every thread handles the paths p = g, g + G, ... (G threads in
 // the grid) and keeps its regression partials in registers; each workgroup writes one
 // row of partials (fixed-order wave and workgroup reductions).
@@ -85,6 +127,10 @@ berm_step_kernel(const BermStep s)
 #pragma unroll
             for (int k = 1; k < K; k++) t = t + b[k] * s.apply_coef[k];
             // choose (RandomVariableFromDoubleArray.java:1342-1363): trigger >= 0 keeps the values
+            if (s.rec_trig && active) {
+                s.rec_trig[p] = t;
+                s.rec_diff[p] = v - u;
+            }
             if (!(t >= 0.0)) {
                 v = u;
                 if (s.ex_index && active) s.ex_index[p] = s.apply_period;
@@ -98,14 +144,7 @@ berm_step_kernel(const BermStep s)
         if (s.regress && active) {
             double b[K];
             berm_basis<KIND>(s.reg_basis, s.contrib, P, pc, b);
-            const double y = v - u;
-            int e = 0;
-#pragma unroll
-            for (int i = 0; i < K; i++, e++) acc[e] += y * b[i];
-#pragma unroll
-            for (int i = 0; i < K; i++)
-#pragma unroll
-                for (int j = i; j < K; j++, e++) acc[e] += b[i] * b[j];
+            regression_accumulate<K>(acc, v - u, b);
         }
         if (s.final_) {
             const double val = (v * s.n0) / s.w;
@@ -115,20 +154,86 @@ berm_step_kernel(const BermStep s)
             s.under[p] = u;
         }
     }
-    if (s.regress) {
-        __shared__ double part[kBermThreads / 64][NP];
+    if (s.regress) regression_partials<K>(acc, s.partials, lane, wid);
+}
+
+// One step of the induction's adjoint (exercise dates ascending; the layout of berm_step_kernel):
+//   finish  add the projection ce = X_lower (X^T X)^+ X^T trigger-adjoint of the lower date:
+//           vb += ce, ub -= ce;
+//   weights the payoff periods from the lower date up to this one: dV/d(payoff) = ub (callable:
+//           the payoffs went to the underlying) or vb;
+//   decide  this date's choose(trigger, values, underlying): the trigger adjoint
+//           d(trigger) x vb, then vb to the chosen side (trigger >= 0 keeps the values); the
+//           regression partials of the trigger adjoint on this date's basis.
+template <int KIND>
+__global__ void __launch_bounds__(kBermThreads)
+berm_adjoint_step_kernel(const BermAdjStep a)
+{
+    constexpr int K = KIND == FM_BASIS_REFERENCE ? 6 : 4;
+    constexpr int NP = K + K * (K + 1) / 2;
+    const int P = a.P;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double acc[NP];
 #pragma unroll
-        for (int e = 0; e < NP; e++) {
-            const double r = wave_reduce_sum(acc[e]);
-            if (lane == 0) part[wid][e] = r;
+    for (int e = 0; e < NP; e++) acc[e] = 0.0;
+    const int64_t G = (int64_t)gridDim.x * kBermThreads;
+    for (int64_t p0 = (int64_t)blockIdx.x * kBermThreads; p0 < P; p0 += G) {
+        const int64_t p = p0 + threadIdx.x;
+        const bool active = p < P;
+        const int64_t pc = active ? p : (int64_t)P - 1;
+        double v = a.first ? 1.0 : a.vb[pc];
+        double u = a.first ? 0.0 : a.ub[pc];
+        if (a.finish) {
+            double b[K];
+            berm_basis<KIND>(a.fin_basis, a.contrib, P, pc, b);
+            double ce = b[0] * a.fin_coef[0];
+#pragma unroll
+            for (int k = 1; k < K; k++) ce = ce + b[k] * a.fin_coef[k];
+            v = v + ce;
+            u = u - ce;
         }
-        __syncthreads();
-        if ((int)threadIdx.x < NP) {
-            double t = 0.0;
-            for (int w = 0; w < kBermThreads / 64; w++) t += part[w][threadIdx.x];
-            s.partials[(size_t)blockIdx.x * kBermParts + threadIdx.x] = t;
+        if (active)
+            for (int q = 0; q < a.num_w; q++) a.weight[(size_t)a.w_col[q] * P + p] = a.callable ? u : v;
+        if (a.decide) {
+            const double t = a.trig[pc], d = a.diff[pc];
+            double tb = 0.0;                                   // ZERO
+            if (a.method == FM_DIRAC_ONE || (a.method == FM_DIRAC_DISCRETE_DELTA && isinf(a.eps))) {
+                tb = d * v;
+            } else if (a.method == FM_DIRAC_DISCRETE_DELTA && a.eps > 0.0) {
+                const bool in = (t + a.eps / 2 >= 0.0) && (t - a.eps / 2 < 0.0);
+                tb = in ? (d / a.eps) * v : 0.0;
+            }
+            if (!(t >= 0.0)) {      // exercised: the values were the underlying's
+                u = u + v;
+                v = 0.0;
+            }
+            if (active && a.method != FM_DIRAC_ZERO) {
+                double b[K];
+                berm_basis<KIND>(a.dec_basis, a.contrib, P, pc, b);
+                regression_accumulate<K>(acc, tb, b);
+            }
+        }
+        if (active) {
+            a.vb[p] = v;
+            a.ub[p] = u;
         }
     }
+    if (a.decide && a.method != FM_DIRAC_ZERO) regression_partials<K>(acc, a.partials, lane, wid);
+}
+
+// Wave partials (sum, centred second moment) of every trigger row, for sd(trigger).
+__global__ void __launch_bounds__(kBermThreads)
+berm_trigger_stats_kernel(const double* __restrict__ trig, int num_dates, int P, double* __restrict__ wave_sum,
+                          double* __restrict__ wave_m2, int stride)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBermThreads + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (kBermThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const bool active = p < P;
+    const int64_t pc = active ? p : (int64_t)P - 1;
+    const int cnt = (int)((int64_t)P - wave * 64 > 64 ? 64 : ((int64_t)P - wave * 64));
+    for (int e = 0; e < num_dates; e++)
+        reduce_slot(trig[(size_t)e * P + pc], active, cnt, lane, wave, wave_sum, wave_m2, stride, e);
 }
 
 // Pseudo-inverse solve of the K x K Gram system by cyclic Jacobi rotations
@@ -255,6 +360,25 @@ void launch_berm_step(const BermStep& s, int kind, void* stream) {
         hipLaunchKernelGGL(berm_step_kernel<FM_BASIS_REFERENCE>, grid, dim3(kBermThreads), 0, (hipStream_t)stream, s);
     else
         hipLaunchKernelGGL(berm_step_kernel<FM_BASIS_SWAP_RATE>, grid, dim3(kBermThreads), 0, (hipStream_t)stream, s);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_berm_adjoint_step(const BermAdjStep& a, int kind, void* stream) {
+    const dim3 grid((unsigned)berm_num_blocks(a.P));
+    if (kind == FM_BASIS_REFERENCE)
+        hipLaunchKernelGGL(berm_adjoint_step_kernel<FM_BASIS_REFERENCE>, grid, dim3(kBermThreads), 0,
+                           (hipStream_t)stream, a);
+    else
+        hipLaunchKernelGGL(berm_adjoint_step_kernel<FM_BASIS_SWAP_RATE>, grid, dim3(kBermThreads), 0,
+                           (hipStream_t)stream, a);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_berm_trigger_stats(const double* trig, int num_dates, int P, double* wave_sum, double* wave_m2, int stride,
+                               void* stream) {
+    const int64_t blocks = ((int64_t)P + kBermThreads - 1) / kBermThreads;
+    hipLaunchKernelGGL(berm_trigger_stats_kernel, dim3((unsigned)blocks), dim3(kBermThreads), 0, (hipStream_t)stream,
+                       trig, num_dates, P, wave_sum, wave_m2, stride);
     HIP_CHECK(hipGetLastError());
 }
 

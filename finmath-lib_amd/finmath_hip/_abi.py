@@ -30,6 +30,8 @@ MEASURE = {"SPOT": 0, "TERMINAL": 1}
 STATE_SPACE = {"NORMAL": 0, "LOGNORMAL": 1}
 SCHEME = {"EULER": 0, "PREDICTOR_CORRECTOR": 1, "EULER_FUNCTIONAL": 2, "PREDICTOR_CORRECTOR_FUNCTIONAL": 3}
 PRECISION = {"FP64": 0, "FP32_MIXED": 1, "FP64_LOGSTATE": 2}
+# RandomVariableDifferentiableAADFactory.DiracDeltaApproximationMethod ordinals
+DIRAC = {"DISCRETE_DELTA": 0, "REGRESSION_ON_DENSITY": 1, "REGRESSION_ON_DISTRIBUITON": 2, "ONE": 3, "ZERO": 4}
 PRODUCT_SWAPTION, PRODUCT_SWAP, PRODUCT_CAPLET, PRODUCT_BERMUDAN_SWAPTION = 1, 2, 3, 4
 BASIS_REFERENCE, BASIS_SWAP_RATE = 0, 1
 
@@ -41,6 +43,7 @@ EXPORTED_SYMBOLS = (
     "fm_lmm_simulate", "fm_lmm_get_process_value",
     "fm_lmm_process_value_device_ptr", "fm_lmm_get_numeraire", "fm_lmm_get_brownian_increment",
     "fm_lmm_value_products", "fm_lmm_value_products_noalloc", "fm_lmm_set_volatility_factorization", "fm_lmm_adjoint",
+    "fm_lmm_set_dirac_delta",
     "fm_lmm_bermudan_exercise_times", "fm_lmm_serialize", "fm_lmm_deserialize",
     "fm_mt19937_words", "fm_mt19937_doubles",
     "fm_last_error", "fm_lmm_last_timings", "fm_device_info",
@@ -139,6 +142,7 @@ def load() -> ctypes.CDLL:
         "fm_lmm_value_products_noalloc": ([c_void_p, POINTER(FmProduct), c_int32, dp, dp], c_int32),
         "fm_lmm_set_volatility_factorization": ([c_void_p, dp, dp, dp], c_int32),
         "fm_lmm_adjoint": ([c_void_p, POINTER(FmProduct), c_int32, dp, dp, dp, dp], c_int32),
+        "fm_lmm_set_dirac_delta": ([c_void_p, c_int32, c_double], c_int32),
         "fm_lmm_bermudan_exercise_times": ([c_void_p, c_int32, dp], c_int32),
         "fm_lmm_serialize": ([c_void_p, c_void_p, POINTER(c_int64)], c_int32),
         "fm_lmm_deserialize": ([c_void_p, c_int64, POINTER(c_void_p)], c_int32),
@@ -394,6 +398,12 @@ class LmmContext:
         c = None if variance_factor is None else as_f64(variance_factor)
         check(self.lib.fm_lmm_set_volatility_factorization(self.h, _dptr(v), _dptr(f),
                                                            POINTER(c_double)() if c is None else _dptr(c)))
+
+    def set_dirac_delta(self, method="DISCRETE_DELTA", width=0.05):
+        """Derivative of a Bermudan's exercise indicator in its trigger within adjoint()
+        (RandomVariableDifferentiableAADFactory's diracDeltaApproximationMethod and
+        diracDeltaApproximationWidthPerStdDev): DISCRETE_DELTA (default), ONE or ZERO."""
+        check(self.lib.fm_lmm_set_dirac_delta(self.h, DIRAC[method], float(width)))
 
     def serialize(self) -> bytes:
         """Configuration and tables as bytes (fm_lmm_serialize); the paths are regenerated."""

@@ -367,6 +367,8 @@ PYBIND11_MODULE(_finmath_host, m) {
         .def("getTimeIndex", &EulerSchemeFromProcessModel::getTimeIndex)
         .def("lastTimings", [](EulerSchemeFromProcessModel& p) { return to_array(p.lastTimings()); })
         .def("setRegenerateIncrements", &EulerSchemeFromProcessModel::setRegenerateIncrements)
+        .def("setDiracDeltaApproximation", &EulerSchemeFromProcessModel::setDiracDeltaApproximation,
+             py::arg("method"), py::arg("widthPerStdDev") = 0.05)
         .def("getGradient", [](EulerSchemeFromProcessModel& p, const std::vector<std::shared_ptr<Product>>& ps,
                                std::vector<double> weights, int stride) {
             std::vector<ProductPtr> v(ps.begin(), ps.end());

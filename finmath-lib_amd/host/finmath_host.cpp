@@ -640,6 +640,19 @@ void EulerSchemeFromProcessModel::ensureContext(bool storePaths) {
     ctxStores_ = storePaths;
 }
 
+void EulerSchemeFromProcessModel::setDiracDeltaApproximation(const std::string& method, double widthPerStdDev) {
+    if (!(widthPerStdDev > 0.0)) throw IllegalArgumentException("diracDeltaApproximationWidthPerStdDev must be positive");
+    int m;
+    if (method == "DISCRETE_DELTA") m = FM_DIRAC_DISCRETE_DELTA;
+    else if (method == "ONE") m = FM_DIRAC_ONE;
+    else if (method == "ZERO") m = FM_DIRAC_ZERO;
+    else if (method == "REGRESSION_ON_DENSITY" || method == "REGRESSION_ON_DISTRIBUITON")
+        throw UnsupportedOperationException("the regression Dirac delta approximations are not implemented");
+    else throw IllegalArgumentException("unknown diracDeltaApproximationMethod " + method);
+    diracMethod_ = m;
+    diracWidth_ = widthPerStdDev;
+}
+
 Gradient EulerSchemeFromProcessModel::getGradient(const std::vector<ProductPtr>& products, const std::vector<double>& weights,
                                                   int checkpointStride) {
     if (ctx_ && checkpointStride != ckptStride_) { fm_lmm_destroy(ctx_); ctx_ = nullptr; simulated_ = false; }
@@ -654,6 +667,7 @@ Gradient EulerSchemeFromProcessModel::getGradient(const std::vector<ProductPtr>&
     Gradient g;
     g.delta.assign((size_t)n, 0.0);
     g.volatility.assign((size_t)nT * n, 0.0);
+    check(fm_lmm_set_dirac_delta(ctx_, diracMethod_, diracWidth_));
     check(fm_lmm_adjoint(ctx_, lowered.data(), (int32_t)lowered.size(), weights.empty() ? nullptr : weights.data(),
                          &g.value, g.delta.data(), g.volatility.data()));
     g.forwardVega.assign((size_t)n, 0.0);

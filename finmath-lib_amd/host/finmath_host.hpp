@@ -424,6 +424,11 @@ public:
     std::vector<double> lastTimings();
     Gradient getGradient(const std::vector<std::shared_ptr<const Product>>& products, const std::vector<double>& weights,
                          int checkpointStride = 4);
+    // The derivative of a Bermudan's exercise indicator in getGradient: the properties
+    // "diracDeltaApproximationMethod" (DISCRETE_DELTA, ONE or ZERO) and
+    // "diracDeltaApproximationWidthPerStdDev" of the reference's AAD factory
+    // (RandomVariableDifferentiableAADFactory.java:60-67), defaults DISCRETE_DELTA and 0.05.
+    void setDiracDeltaApproximation(const std::string& method, double widthPerStdDev = 0.05);
     fm_lmm* handle();
 private:
     void ensureContext(bool storePaths);
@@ -436,6 +441,8 @@ private:
     bool simulated_ = false;
     bool regenerate_ = false;
     int ckptStride_ = 4;
+    int diracMethod_ = FM_DIRAC_DISCRETE_DELTA;
+    double diracWidth_ = 0.05;
 };
 using ProcessPtr = std::shared_ptr<EulerSchemeFromProcessModel>;
 

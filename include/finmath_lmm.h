@@ -311,6 +311,27 @@ int fm_lmm_set_volatility_factorization(fm_lmm* ctx, const double* volatility, c
 int fm_lmm_adjoint(fm_lmm* ctx, const fm_product* products, int32_t count, const double* weights,
                    double* value, double* d_initial_forwards, double* d_volatility);
 
+/* Derivative of a Bermudan's exercise indicator choose(trigger, values, underlying) in its
+ * trigger within fm_lmm_adjoint: the DiracDeltaApproximationMethod and
+ * diracDeltaApproximationWidthPerStdDev properties of the reference's AAD factory
+ * (montecarlo/automaticdifferentiation/backward/RandomVariableDifferentiableAADFactory.java:26-67,
+ * RandomVariableDifferentiableAAD.java:396-424), with the enum's ordinals:
+ *   FM_DIRAC_DISCRETE_DELTA (the default, width 0.05): (Y - Z) 1{-eps/2 <= X < eps/2} / eps,
+ *                           eps = width * sd(X) over the paths (eps infinite: Y - Z; eps <= 0: 0)
+ *   FM_DIRAC_ONE:           Y - Z
+ *   FM_DIRAC_ZERO:          0 (the exercise decisions held fixed)
+ * X is the trigger, the regression of values - underlying on the date's basis; Y - Z the values
+ * and underlying it chooses between. The trigger's adjoint is regressed on the same basis (the
+ * conditional expectation is self-adjoint, AAD:188-192; its basis functions are constants).
+ * The REGRESSION_ON_DENSITY and REGRESSION_ON_DISTRIBUITON methods give FM_ERR_UNSUPPORTED;
+ * width must be positive. Products other than Bermudans have no trigger. */
+#define FM_DIRAC_DISCRETE_DELTA 0
+#define FM_DIRAC_REGRESSION_ON_DENSITY 1
+#define FM_DIRAC_REGRESSION_ON_DISTRIBUTION 2
+#define FM_DIRAC_ONE 3
+#define FM_DIRAC_ZERO 4
+int fm_lmm_set_dirac_delta(fm_lmm* ctx, int32_t method, double width);
+
 /* ---- generator ----------------------------------------------------------------
  * The GPU MT19937 stream of commons-math3 MersenneTwister(long seed) as used by
  * the reference (randomnumbers/MersenneTwister.java:26-51): 32-bit output words

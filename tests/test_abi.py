@@ -68,7 +68,7 @@ def test_reference_validation_message():
         _ctx(num_factors=0)
 
 
-@pytest.mark.parametrize("blob", [b"", b"garbage-bytes-not-a-context", b"FMLMMSER\x02\x00\x00\x00" + bytes(64),
+@pytest.mark.parametrize("blob", [b"", b"garbage-bytes-not-a-context", b"FMLMMSER\x63\x00\x00\x00" + bytes(64),
                                   b"FMLMMSER\x01\x00\x00\x00" + bytes(8)])
 def test_malformed_serialised_context_is_rejected_before_device_use(blob):
     if not blob:

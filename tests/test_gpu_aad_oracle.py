@@ -5,7 +5,8 @@ of the reference's RandomVariableDifferentiableAAD (RandomVariableDifferentiable
 Both hold every path average constant (getAverage returns a double: the numeraire adjustment
 avg(N(0)/N(T)) of getNumeraire, LIBORMarketModelFromCovarianceModel.java:860-876), so the
 discount-curve case is compared as it is, without freezing anything by hand. The Bermudan is
-compared under the ZERO Dirac method (exercise decisions held fixed).
+compared under the ZERO Dirac method (exercise decisions held fixed) and under the reference's
+default DISCRETE_DELTA (and ONE), where the exercise indicator's trigger is differentiated.
 
 Bars: the FP64 kernels within 1e-10 of the tape's largest entry (value within 1e-12); the
 FP32-mixed kernels within 2e-3 of it (their paths round to float at every step).
@@ -28,13 +29,16 @@ def say(*a):
     print(*a, flush=True)
 
 
-def gpu_context(wl, t, P, *, precision="FP64", with_discount=True, stride=4, state_space="LOGNORMAL", cap=1e5):
+def gpu_context(wl, t, P, *, precision="FP64", with_discount=True, stride=4, state_space="LOGNORMAL", cap=1e5,
+                dirac=None, width=0.05):
     ctx = _abi.LmmContext(t.times, t.tenor, wl.num_factors, P, wl.seed, precision=precision,
                           checkpoint_stride=stride, state_space=state_space, libor_cap=cap)
     ctx.set_initial_forwards(t.L0)
     ctx.set_covariance(t.vol[:, :, None] * t.factors[None, :, :], t.vol * t.vol * 1.0)
     ctx.set_discount_curve(t.pd if with_discount else None)
     ctx.set_volatility_factorization(t.vol, t.factors)
+    if dirac is not None:
+        ctx.set_dirac_delta(dirac, width)
     return ctx
 
 
@@ -122,7 +126,7 @@ def test_bermudan_fp64_against_tape_zero_method(basis):
     P = 4000
     spec = W.bermudan_spec(2.0, 10.0, 0.5, 0.03, 1.0, True, basis=basis)
     say(f"case aad tape bermudan c0 basis={basis} P={P}")
-    v, dL, dsig = gpu_context(wl, t, P).adjoint([bermudan(spec)])
+    v, dL, dsig = gpu_context(wl, t, P, dirac="ZERO").adjoint([bermudan(spec)])
     vt, dY, dvol, _ = O.aad_bermudan(t, P, spec, with_discount=True, dirac="ZERO")
     assert v == pytest.approx(vt, rel=1e-11)
     close(dL * t.L0, dY, 1e-10, "deltas")
@@ -137,8 +141,48 @@ def test_bermudan_80_forwards_10_factors_against_tape(precision, tol):
     P = 2048
     spec = wl.products[0]
     say(f"case aad tape bermudan c4 {precision} P={P}")
-    v, dL, dsig = gpu_context(wl, t, P, precision=precision).adjoint([bermudan(spec)])
+    v, dL, dsig = gpu_context(wl, t, P, precision=precision, dirac="ZERO").adjoint([bermudan(spec)])
     vt, dY, dvol, _ = O.aad_bermudan(t, P, spec, with_discount=True, dirac="ZERO")
+    assert v == pytest.approx(vt, rel=1e-11 if precision == "FP64" else 1e-4)
+    close(dL * t.L0, dY, tol, "deltas")
+    close(dsig, dvol, tol, "volatility entries")
+
+
+@pytest.mark.parametrize("dirac,basis,callable_", [("DISCRETE_DELTA", 0, True), ("DISCRETE_DELTA", 1, True),
+                                                   ("DISCRETE_DELTA", 0, False), ("ONE", 0, True)])
+def test_bermudan_fp64_against_tape_trigger_methods(dirac, basis, callable_):
+    # the reference's default DISCRETE_DELTA (and ONE): the exercise indicator's derivative in its
+    # trigger, whose adjoint is regressed on the exercise date's basis
+    wl = W.c0()
+    wl.forwards = 0.02 + 0.001 * np.arange(wl.num_libors)
+    t = O.ModelTables(wl)
+    P = 4000
+    spec = W.bermudan_spec(2.0, 10.0, 0.5, 0.03, 1.0, callable_, basis=basis)
+    say(f"case aad tape bermudan c0 {dirac} basis={basis} callable={callable_} P={P}")
+    v, dL, dsig = gpu_context(wl, t, P, dirac=dirac).adjoint([bermudan(spec)])
+    vt, dY, dvol, _ = O.aad_bermudan(t, P, spec, with_discount=True, dirac=dirac)
+    _, dY0, dvol0, _ = O.aad_bermudan(t, P, spec, with_discount=True, dirac="ZERO")
+    # the trigger's term must matter for the comparison to pin it
+    say(f"  trigger term: max |tape - tape(ZERO)| / max |tape| = {np.max(np.abs(dY - dY0)) / np.max(np.abs(dY)):.3e}")
+    assert np.max(np.abs(dY - dY0)) > 1e-6 * np.max(np.abs(dY))
+    assert v == pytest.approx(vt, rel=1e-11)
+    # the trigger's adjoint is regressed on a nearly collinear basis (1, ds, ds^2, dl, dl^2, 1/N),
+    # so two pseudo-inverse solves of its Gram matrix (Jacobi here, the tape's eigen solve) agree to
+    # about 1e-8 of the gradient; the bar stays far below the trigger term (3e-2 of it here)
+    close(dL * t.L0, dY, 1e-7, "deltas")
+    close(dsig, dvol, 1e-7, "volatility entries")
+
+
+@pytest.mark.parametrize("precision,tol", [("FP64", 1e-7), ("FP32_MIXED", 5e-3)])
+def test_bermudan_80_forwards_10_factors_discrete_delta_against_tape(precision, tol):
+    # the c4 model and product under the reference's default DISCRETE_DELTA
+    wl = W.c4()
+    t = O.ModelTables(wl)
+    P = 2048
+    spec = wl.products[0]
+    say(f"case aad tape bermudan c4 DISCRETE_DELTA {precision} P={P}")
+    v, dL, dsig = gpu_context(wl, t, P, precision=precision).adjoint([bermudan(spec)])
+    vt, dY, dvol, _ = O.aad_bermudan(t, P, spec, with_discount=True, dirac="DISCRETE_DELTA")
     assert v == pytest.approx(vt, rel=1e-11 if precision == "FP64" else 1e-4)
     close(dL * t.L0, dY, tol, "deltas")
     close(dsig, dvol, tol, "volatility entries")

@@ -112,9 +112,12 @@ def test_bermudan_is_deterministic():
     assert np.array_equal(a, b) and np.array_equal(ea, eb)
 
 
-def _adjoint_context(wl, t, P, precision="FP64", with_discount=True):
+def _adjoint_context(wl, t, P, precision="FP64", with_discount=True, dirac="ZERO"):
+    # ZERO holds the exercise decisions fixed, which the finite differences below reproduce
+    # (the DISCRETE_DELTA default is compared with the oracle's tape in test_gpu_aad_oracle.py)
     ctx = make_context(wl, t, P, precision=precision, with_discount=with_discount)
     ctx.set_volatility_factorization(t.vol, t.factors)
+    ctx.set_dirac_delta(dirac)
     return ctx
 
 
