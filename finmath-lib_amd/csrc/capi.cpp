@@ -143,7 +143,7 @@ struct fm_lmm {
     // the Bermudan induction's adjoint: triggers and choices recorded by the pricing pass
     // ([exercise dates of all Bermudans][P]), the per-path adjoint state, the period weights
     DevBuf d_berm_trig, d_berm_diff, d_berm_vb, d_berm_ub, d_berm_weight, d_berm_bcoef, d_berm_wcol;
-    DevBuf d_berm_wsum, d_berm_wm2, d_berm_stat;
+    DevBuf d_berm_wsum, d_berm_wm2, d_berm_stat, d_berm_pinv;
     bool berm_record = false;                 // the next pricing pass records triggers and choices
     std::vector<int> berm_trig_off;           // first recorded date of each Bermudan
     int dirac_method = FM_DIRAC_DISCRETE_DELTA;
@@ -729,7 +729,11 @@ void run_bermudan(fm_lmm* c, const BermPlan& b, int bi, const std::vector<double
         t.final_ = e < 0 ? 1 : 0;
         launch_berm_step(t, b.basis, s);
         if (e >= 0) {
-            launch_berm_solve(part, nb, b.num_basis, P, coef + (size_t)e * kBermMaxBasis, s);
+            // recording for the adjoint: keep the date's pseudo-inverse, which projects the
+            // trigger's adjoint on the same basis
+            double* factors = c->berm_record
+                ? c->d_berm_pinv.as<double>() + (size_t)(c->berm_trig_off[(size_t)bi] + e) * kBermPinv : nullptr;
+            launch_berm_solve(part, nb, b.num_basis, P, coef + (size_t)e * kBermMaxBasis, s, factors);
             prev = e;
         }
         done = lo;
@@ -747,19 +751,15 @@ void run_bermudan_adjoint(fm_lmm* c, const BermPlan& b, int bi, const std::vecto
     const size_t row0 = (size_t)c->berm_trig_off[(size_t)bi] * P;
     const double* trig = c->d_berm_trig.as<double>() + row0;
     const double* diff = c->d_berm_diff.as<double>() + row0;
-    std::vector<double> eps((size_t)std::max(nex, 1), 0.0);
+    const double* pinv = c->d_berm_pinv.as<double>() + (size_t)c->berm_trig_off[(size_t)bi] * kBermPinv;
+    // the centred second moment of every date's trigger, for the DISCRETE_DELTA width
+    double* stat = (double*)c->d_berm_stat.get((size_t)2 * std::max(nex, 1) * sizeof(double));
     if (c->dirac_method == FM_DIRAC_DISCRETE_DELTA && nex > 0) {
         const int num_waves = (P + 63) / 64;
         double* wsum = (double*)c->d_berm_wsum.get((size_t)num_waves * nex * sizeof(double));
         double* wm2 = (double*)c->d_berm_wm2.get((size_t)num_waves * nex * sizeof(double));
-        double* out = (double*)c->d_berm_stat.get((size_t)2 * nex * sizeof(double));
         launch_berm_trigger_stats(trig, nex, P, wsum, wm2, nex, s);
-        launch_reduce_partials(wsum, wm2, num_waves, nex, nex, 0, P, out, out + nex, s);
-        std::vector<double> host((size_t)2 * nex);
-        HIP_CHECK(hipMemcpyAsync(host.data(), out, host.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
-        // getStandardDeviation: the square root of the population variance
-        for (int e = 0; e < nex; e++) eps[(size_t)e] = c->dirac_width * std::sqrt(host[(size_t)(nex + e)] / (double)P);
+        launch_reduce_partials(wsum, wm2, num_waves, nex, nex, 0, P, stat, stat + nex, s);
     }
     const int32_t* wcol = upload(c->d_berm_wcol, b.per_col, s);       // ascending periods
     double* vb = (double*)c->d_berm_vb.get((size_t)P * sizeof(double));
@@ -792,11 +792,13 @@ void run_bermudan_adjoint(fm_lmm* c, const BermPlan& b, int bi, const std::vecto
             t.dec_basis = basis_ref(c, b, e, adj);
             t.trig = trig + (size_t)e * P;
             t.diff = diff + (size_t)e * P;
-            t.eps = eps[(size_t)e];
+            t.trig_m2 = stat + nex + e;
+            t.width = c->dirac_width;
         }
         launch_berm_adjoint_step(t, b.basis, s);
         if (t.decide && c->dirac_method != FM_DIRAC_ZERO)
-            launch_berm_solve(part, nb, b.num_basis, P, bcoef + (size_t)e * kBermMaxBasis, s);
+            launch_berm_project(part, nb, b.num_basis, P, pinv + (size_t)e * kBermPinv,
+                                bcoef + (size_t)e * kBermMaxBasis, s);
         lo = hi;
     }
     HIP_CHECK(hipStreamSynchronize(s));      // per_col is the host source of an upload above
@@ -1432,6 +1434,7 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
         }
         c->d_berm_trig.get((size_t)std::max(total, 1) * P * sizeof(double));
         c->d_berm_diff.get((size_t)std::max(total, 1) * P * sizeof(double));
+        c->d_berm_pinv.get((size_t)std::max(total, 1) * kBermPinv * sizeof(double));
         c->berm_record = true;
     }
     try {

@@ -228,7 +228,15 @@ struct BermStep {
 };
 int berm_num_blocks(int P);
 void launch_berm_step(const BermStep& s, int kind, void* stream);
-void launch_berm_solve(const double* partials, int num_rows, int K, int P, double* coef, void* stream);
+// factors (may be NULL) receives the solve's pseudo-inverse: the eigenvectors V [K][K] (row r,
+// column i at r * kBermMaxBasis + i) and the inverted kept eigenvalues [K] (0 where dropped) at
+// kBermMaxBasis * kBermMaxBasis, so that another right-hand side projects the same way.
+constexpr int kBermPinv = kBermMaxBasis * kBermMaxBasis + kBermMaxBasis;
+void launch_berm_solve(const double* partials, int num_rows, int K, int P, double* coef, void* stream,
+                       double* factors = nullptr);
+// x = pinv (X^T y / P) of one date from its stored factors and the rows of X^T y partials.
+void launch_berm_project(const double* partials, int num_rows, int K, int P, const double* factors, double* coef,
+                         void* stream);
 
 // One step of the Bermudan induction's adjoint, exercise dates ascending (the reverse of the
 // induction). Per path it carries the adjoints vb of the values and ub of the underlying.
@@ -248,8 +256,9 @@ struct BermAdjStep {
     BermBasisRef dec_basis;      //   its basis (regression partials of the trigger adjoint)
     const double* trig;          //   device [P]: the trigger (the regressed continuation)
     const double* diff;          //   device [P]: values - underlying the decision chose between
-    double eps;                  //   DISCRETE_DELTA width: width x sd(trigger)
-    double* partials;            // device [blocks][kBermParts]
+    const double* trig_m2;       //   device: the trigger's centred second moment over the paths
+    double width;                //   DISCRETE_DELTA: eps = width x sqrt(m2 / P)
+    double* partials;            // device [blocks][kBermParts]: X^T trigger-adjoint
     double* vb;                  // device [P]
     double* ub;                  // device [P]
 };

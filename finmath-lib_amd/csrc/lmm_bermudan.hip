@@ -76,11 +76,10 @@ __device__ __forceinline__ void regression_accumulate(double (&acc)[K + K * (K +
         for (int j = i; j < K; j++, e++) acc[e] += b[i] * b[j];
 }
 
-// One row of regression partials per workgroup (fixed-order wave and workgroup sums).
-template <int K>
-__device__ __forceinline__ void regression_partials(const double (&acc)[K + K * (K + 1) / 2],
-                                                    double* __restrict__ partials, int lane, int wid) {
-    constexpr int NP = K + K * (K + 1) / 2;
+// One row of NP regression partials per workgroup (fixed-order wave and workgroup sums).
+template <int NP>
+__device__ __forceinline__ void regression_partials(const double (&acc)[NP], double* __restrict__ partials, int lane,
+                                                    int wid) {
     __shared__ double part[kBermThreads / 64][NP];
 #pragma unroll
     for (int e = 0; e < NP; e++) {
@@ -154,7 +153,7 @@ berm_step_kernel(const BermStep s)
             s.under[p] = u;
         }
     }
-    if (s.regress) regression_partials<K>(acc, s.partials, lane, wid);
+    if (s.regress) regression_partials<NP>(acc, s.partials, lane, wid);
 }
 
 // One step of the induction's adjoint (exercise dates ascending; the layout of berm_step_kernel):
@@ -170,12 +169,13 @@ __global__ void __launch_bounds__(kBermThreads)
 berm_adjoint_step_kernel(const BermAdjStep a)
 {
     constexpr int K = KIND == FM_BASIS_REFERENCE ? 6 : 4;
-    constexpr int NP = K + K * (K + 1) / 2;
     const int P = a.P;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    double acc[NP];
+    // eps = width x sd(trigger), sd the square root of the population variance (getStandardDeviation)
+    const double eps = (a.decide && a.method == FM_DIRAC_DISCRETE_DELTA) ? a.width * sqrt(*a.trig_m2 / (double)P) : 0.0;
+    double acc[K];
 #pragma unroll
-    for (int e = 0; e < NP; e++) acc[e] = 0.0;
+    for (int e = 0; e < K; e++) acc[e] = 0.0;
     const int64_t G = (int64_t)gridDim.x * kBermThreads;
     for (int64_t p0 = (int64_t)blockIdx.x * kBermThreads; p0 < P; p0 += G) {
         const int64_t p = p0 + threadIdx.x;
@@ -197,11 +197,11 @@ berm_adjoint_step_kernel(const BermAdjStep a)
         if (a.decide) {
             const double t = a.trig[pc], d = a.diff[pc];
             double tb = 0.0;                                   // ZERO
-            if (a.method == FM_DIRAC_ONE || (a.method == FM_DIRAC_DISCRETE_DELTA && isinf(a.eps))) {
+            if (a.method == FM_DIRAC_ONE || (a.method == FM_DIRAC_DISCRETE_DELTA && isinf(eps))) {
                 tb = d * v;
-            } else if (a.method == FM_DIRAC_DISCRETE_DELTA && a.eps > 0.0) {
-                const bool in = (t + a.eps / 2 >= 0.0) && (t - a.eps / 2 < 0.0);
-                tb = in ? (d / a.eps) * v : 0.0;
+            } else if (a.method == FM_DIRAC_DISCRETE_DELTA && eps > 0.0) {
+                const bool in = (t + eps / 2 >= 0.0) && (t - eps / 2 < 0.0);
+                tb = in ? (d / eps) * v : 0.0;
             }
             if (!(t >= 0.0)) {      // exercised: the values were the underlying's
                 u = u + v;
@@ -210,7 +210,8 @@ berm_adjoint_step_kernel(const BermAdjStep a)
             if (active && a.method != FM_DIRAC_ZERO) {
                 double b[K];
                 berm_basis<KIND>(a.dec_basis, a.contrib, P, pc, b);
-                regression_accumulate<K>(acc, tb, b);
+#pragma unroll
+                for (int i = 0; i < K; i++) acc[i] += tb * b[i];
             }
         }
         if (active) {
@@ -241,7 +242,8 @@ berm_trigger_stats_kernel(const double* __restrict__ trig, int num_dates, int P,
 // index is static). Singular values at or below the SVD solver's threshold
 // max(K sigma_max 2^-52, sqrt(2^-1022)) are dropped.
 template <int K>
-__device__ __forceinline__ void jacobi_pinv_solve(const double* tot, int P, double* __restrict__ coef) {
+__device__ __forceinline__ void jacobi_pinv_solve(const double* tot, int P, double* __restrict__ coef,
+                                                  double* __restrict__ factors) {
     double A[K][K], V[K][K], rhs[K], x[K];
     int q = 0;
 #pragma unroll
@@ -311,13 +313,45 @@ __device__ __forceinline__ void jacobi_pinv_solve(const double* tot, int P, doub
     }
 #pragma unroll
     for (int i = 0; i < kBermMaxBasis; i++) coef[i] = (i < K) ? x[i] : 0.0;
+    if (factors) {
+#pragma unroll
+        for (int r = 0; r < K; r++)
+#pragma unroll
+            for (int i = 0; i < K; i++) factors[r * kBermMaxBasis + i] = V[r][i];
+#pragma unroll
+        for (int i = 0; i < K; i++) factors[kBermMaxBasis * kBermMaxBasis + i] = (fabs(A[i][i]) > tol) ? A[i][i] : 0.0;
+    }
+}
+
+// x = pinv (rhs) from stored factors, with the solve's operations: for every kept eigenvalue a_i,
+// x += V[:, i] ((V[:, i] . rhs) / a_i).
+template <int K>
+__device__ __forceinline__ void pinv_apply(const double* rhs, const double* __restrict__ factors,
+                                           double* __restrict__ coef) {
+    double x[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) x[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+        const double a = factors[kBermMaxBasis * kBermMaxBasis + i];
+        if (a == 0.0) continue;
+        double d = 0.0;
+#pragma unroll
+        for (int r = 0; r < K; r++) d += factors[r * kBermMaxBasis + i] * rhs[r];
+        d /= a;
+#pragma unroll
+        for (int r = 0; r < K; r++) x[r] += factors[r * kBermMaxBasis + i] * d;
+    }
+#pragma unroll
+    for (int i = 0; i < kBermMaxBasis; i++) coef[i] = (i < K) ? x[i] : 0.0;
 }
 
 // One workgroup: fixed-order reduction of the workgroup rows (entry e = t % nparts,
 // rows c, c + C, ... with c = t / nparts), then (thread 0) the regression solve in FP64.
 template <int K>
 __global__ void __launch_bounds__(kBermThreads)
-berm_solve_kernel(const double* __restrict__ partials, int num_rows, int P, double* __restrict__ coef)
+berm_solve_kernel(const double* __restrict__ partials, int num_rows, int P, double* __restrict__ coef,
+                  double* __restrict__ factors)
 {
     constexpr int nparts = K + K * (K + 1) / 2;
     constexpr int C = kBermThreads / nparts;
@@ -343,7 +377,40 @@ berm_solve_kernel(const double* __restrict__ partials, int num_rows, int P, doub
         tot[t] = sum;
     }
     __syncthreads();
-    if (t == 0) jacobi_pinv_solve<K>(tot, P, coef);
+    if (t == 0) jacobi_pinv_solve<K>(tot, P, coef, factors);
+}
+
+// One workgroup: the fixed-order reduction of K partials (as berm_solve_kernel's), then
+// (thread 0) x = pinv (X^T y / P) from the date's stored pseudo-inverse factors.
+template <int K>
+__global__ void __launch_bounds__(kBermThreads)
+berm_project_kernel(const double* __restrict__ partials, int num_rows, int P, const double* __restrict__ factors,
+                    double* __restrict__ coef)
+{
+    constexpr int C = kBermThreads / K;
+    __shared__ double red[C * K];
+    __shared__ double tot[K];
+    const int t = threadIdx.x;
+    const int e = t % K, ch = t / K;
+    if (ch < C) {
+        double sum = 0.0, c = 0.0;
+#pragma unroll 8
+        for (int r = ch; r < num_rows; r += C) {
+            const double x = partials[(size_t)r * kBermParts + e];
+            const double tt = sum + x;
+            if (fabs(sum) >= fabs(x)) c += (sum - tt) + x; else c += (x - tt) + sum;
+            sum = tt;
+        }
+        red[ch * K + e] = sum + c;
+    }
+    __syncthreads();
+    if (t < K) {
+        double sum = 0.0;
+        for (int c2 = 0; c2 < C; c2++) sum += red[c2 * K + t];
+        tot[t] = sum / (double)P;
+    }
+    __syncthreads();
+    if (t == 0) pinv_apply<K>(tot, factors, coef);
 }
 
 } // namespace
@@ -382,13 +449,27 @@ void launch_berm_trigger_stats(const double* trig, int num_dates, int P, double*
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_berm_solve(const double* partials, int num_rows, int K, int P, double* coef, void* stream) {
+void launch_berm_project(const double* partials, int num_rows, int K, int P, const double* factors, double* coef,
+                         void* stream) {
+    if (K == 6)
+        hipLaunchKernelGGL(berm_project_kernel<6>, dim3(1), dim3(kBermThreads), 0, (hipStream_t)stream,
+                           partials, num_rows, P, factors, coef);
+    else if (K == 4)
+        hipLaunchKernelGGL(berm_project_kernel<4>, dim3(1), dim3(kBermThreads), 0, (hipStream_t)stream,
+                           partials, num_rows, P, factors, coef);
+    else
+        throw FmError(FM_ERR_INTERNAL, "regression basis size " + std::to_string(K) + " has no projection kernel");
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_berm_solve(const double* partials, int num_rows, int K, int P, double* coef, void* stream,
+                       double* factors) {
     if (K == 6)
         hipLaunchKernelGGL(berm_solve_kernel<6>, dim3(1), dim3(kBermThreads), 0, (hipStream_t)stream,
-                           partials, num_rows, P, coef);
+                           partials, num_rows, P, coef, factors);
     else if (K == 4)
         hipLaunchKernelGGL(berm_solve_kernel<4>, dim3(1), dim3(kBermThreads), 0, (hipStream_t)stream,
-                           partials, num_rows, P, coef);
+                           partials, num_rows, P, coef, factors);
     else
         throw FmError(FM_ERR_INTERNAL, "regression basis size " + std::to_string(K) + " has no solve kernel");
     HIP_CHECK(hipGetLastError());
