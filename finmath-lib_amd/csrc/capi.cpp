@@ -138,7 +138,7 @@ struct fm_lmm {
     DevBuf d_times, d_first, d_tenor_ti, d_rec, d_delta, d_L0, d_Y0, d_Y0cap, d_sqrt_dt, d_lvb, d_svs;
     DevBuf d_states, d_poly, d_dw, d_row_of, d_paths, d_num;
     DevBuf d_products, d_periods, d_wave_sum, d_wave_m2, d_contrib, d_out, d_swapmeta, d_scale;
-    DevBuf d_ckpt, d_seg, d_adj_num, d_kcur, d_fsig, d_sig, d_cdiag, d_wave_grad, d_grad_out;
+    DevBuf d_ckpt, d_ckpt_first, d_seg, d_adj_num, d_kcur, d_fsig, d_sig, d_cdiag, d_wave_grad, d_grad_out;
     DevBuf d_berm_vals, d_berm_under, d_berm_ex, d_berm_part, d_berm_coef, d_berm_col, d_berm_scale;
     // the Bermudan induction's adjoint: triggers and choices recorded by the pricing pass
     // ([exercise dates of all Bermudans][P]), the per-path adjoint state, the period weights
@@ -801,7 +801,8 @@ void run_bermudan_adjoint(fm_lmm* c, const BermPlan& b, int bi, const std::vecto
                                 bcoef + (size_t)e * kBermMaxBasis, s);
         lo = hi;
     }
-    HIP_CHECK(hipStreamSynchronize(s));      // per_col is the host source of an upload above
+    // no synchronisation: the upload's host source (the plan's per_col) lives until the caller's
+    // final synchronisation
 }
 
 // Scratch bytes of a pass over single-date products with the per-call buffers in caller memory
@@ -817,7 +818,8 @@ size_t external_scratch_bytes(const fm_lmm* c, const Lowered& L) {
 // With xs, the wave partials, product tables and reduction output live in that caller memory
 // and are cleared by a fill kernel; such a pass takes single-date products only.
 void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums, std::vector<double>& m2s,
-              void* ckpt = nullptr, int ckpt_stride = 1, double* num_out = nullptr, ExternalScratch* xs = nullptr) {
+              void* ckpt = nullptr, int ckpt_stride = 1, double* num_out = nullptr, ExternalScratch* xs = nullptr,
+              const int32_t* ckpt_first = nullptr) {
     prepare_model(c);
     hipStream_t s = c->stream;
     const int n = c->n(), P = c->P();
@@ -874,6 +876,7 @@ void run_pass(fm_lmm* c, const Lowered& L, bool store, std::vector<double>& sums
     a.numeraire = store ? (double*)c->d_num.get((size_t)(n + 1) * P * sizeof(double)) : num_out;
     a.ckpt = ckpt;
     a.ckpt_stride = ckpt ? ckpt_stride : 1;
+    a.ckpt_first = ckpt_first;
     tm.mark(2);
     // kernel variant: the spot-measure Euler kernels, else the general kernel (terminal
     // measure, carried state, predictor-corrector)
@@ -1437,8 +1440,23 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
         c->d_berm_pinv.get((size_t)std::max(total, 1) * kBermPinv * sizeof(double));
         c->berm_record = true;
     }
+    // the first component the reverse sweep reads from each checkpoint's segment [c, c + stride) on:
+    // the alive components of its steps (first_alive[c - 1] for the state the step into c left),
+    // the forwards of the products evaluated from c on, and the numeraire's forwards from
+    // kcur_at(c - 1) on; rows below are neither written nor read
+    std::vector<int32_t> cfirst((size_t)ncp, 0);
+    for (int cc = 0; cc < ncp; cc++) {
+        const int c0 = cc * stride;
+        int f = n;
+        if (c0 < nT) f = std::min(f, c->first_alive[(size_t)c0]);
+        if (c0 >= 1) f = std::min({f, c->first_alive[(size_t)c0 - 1], kcur_at(c, c0 - 1)});
+        for (const KProduct& k : L.kp)
+            if (k.eval_index >= c0) f = std::min(f, (int)k.first_j);
+        cfirst[(size_t)cc] = c0 == 0 ? 0 : f;
+    }
+    const int32_t* d_cfirst = upload(c->d_ckpt_first, cfirst, s);
     try {
-        run_pass(c, L, false, sums, m2s, ckpt, stride, num);
+        run_pass(c, L, false, sums, m2s, ckpt, stride, num, nullptr, d_cfirst);
     } catch (...) {
         c->berm_record = false;
         throw;
@@ -1504,7 +1522,7 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     a.num_products = (int)kp.size();
     a.products = upload(c->d_products, kp, s);
     a.periods = upload(c->d_periods, L.per, s);
-    a.numeraire = num; a.ckpt = ckpt; a.ckpt_stride = stride;
+    a.numeraire = num; a.ckpt = ckpt; a.ckpt_stride = stride; a.ckpt_first = d_cfirst;
     AdjArgs ga{};
     ga.stride = stride;
     ga.kcur_at = upload(c->d_kcur, kcur, s);

@@ -124,6 +124,8 @@ struct StepArgs {
     double* numeraire;           // device [(n+1)][P] unadjusted numeraire (store_paths or adjoint)
     void* ckpt;                  // device [ceil((nT+1)/stride)][n][P] forward states (adjoint), or NULL
     int ckpt_stride;
+    const int32_t* ckpt_first;   // device [checkpoints]: the first component the reverse sweep reads from
+                                 // each checkpoint's segment on (lower rows are neither written nor read)
 };
 void launch_lmm_step_fp64(const StepArgs& a, void* stream);
 void launch_lmm_step_fp32(const StepArgs& a, void* stream);

@@ -290,9 +290,10 @@ lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __
         // [ti / stride][j][P]
         if (ckpt && active && (ti % a.ckpt_stride) == 0) {
             const size_t base = (size_t)(ti / a.ckpt_stride) * n;
+            const int cfirst = a.ckpt_first[ti / a.ckpt_stride];
             static_for<0, W * NCH>([&](auto J) {
                 constexpr int j = decltype(J)::value;
-                if (j < n) ckpt[(base + j) * P + p] = Y.template at<j>();
+                if (j < n && j >= cfirst) ckpt[(base + j) * P + p] = Y.template at<j>();
             });
         }
         // ---- N(T_{k+1}) = N(T_k).accrue(L_k(T_k), d_k) once L_k is final (LMM:1050-1066)

@@ -233,6 +233,11 @@ JNIEXPORT jdouble JNICALL JNI_NAME(adjoint)(JNIEnv* env, jclass cls, jlong conte
     return value;
 }
 
+JNIEXPORT void JNICALL JNI_NAME(setDiracDelta)(JNIEnv* env, jclass cls, jlong context, jint method, jdouble width) {
+    (void)cls;
+    throw_status(env, fm_lmm_set_dirac_delta(handle(context), (int32_t)method, (double)width));
+}
+
 JNIEXPORT void JNICALL JNI_NAME(destroy)(JNIEnv* env, jclass cls, jlong context) {
     (void)env; (void)cls;
     fm_lmm_destroy(handle(context));

@@ -99,6 +99,8 @@ public final class FinmathHip {
 			FunctionDescriptor.of(JAVA_INT, ADDRESS, ADDRESS));
 	static final MethodHandle SET_VOLATILITY_FACTORIZATION = fn("fm_lmm_set_volatility_factorization",
 			FunctionDescriptor.of(JAVA_INT, ADDRESS, ADDRESS, ADDRESS, ADDRESS));
+	static final MethodHandle SET_DIRAC_DELTA = fn("fm_lmm_set_dirac_delta",
+			FunctionDescriptor.of(JAVA_INT, ADDRESS, JAVA_INT, JAVA_DOUBLE));
 	static final MethodHandle SIMULATE = fn("fm_lmm_simulate", FunctionDescriptor.of(JAVA_INT, ADDRESS));
 	static final MethodHandle GET_PROCESS_VALUE = fn("fm_lmm_get_process_value",
 			FunctionDescriptor.of(JAVA_INT, ADDRESS, JAVA_INT, JAVA_INT, ADDRESS));

@@ -43,6 +43,9 @@ public final class FinmathHipJNI {
 			int[] flags, double[][] fixing, double[][] payment, double[][] periodLength, double[][] swaprate,
 			double[][] periodNotional, int[][] exerciseFlags, double[] values, double[] errors);
 
+	/** fm_lmm_set_dirac_delta: method is the ordinal of RandomVariableDifferentiableAADFactory.DiracDeltaApproximationMethod. */
+	public static native void setDiracDelta(long context, int method, double widthPerStdDev);
+
 	/** fm_lmm_adjoint over products given as for valueProducts; returns the value. */
 	public static native double adjoint(long context, int[] type, double[] exerciseDate, double[] notional,
 			int[] flags, double[][] fixing, double[][] payment, double[][] periodLength, double[][] swaprate,

@@ -26,6 +26,7 @@ import java.util.List;
 import java.util.Map;
 
 import net.finmath.exception.CalculationException;
+import net.finmath.montecarlo.automaticdifferentiation.backward.RandomVariableDifferentiableAADFactory.DiracDeltaApproximationMethod;
 import net.finmath.montecarlo.interestrate.LIBORMonteCarloSimulationFromLIBORModel;
 import net.finmath.montecarlo.interestrate.models.covariance.LIBORCovarianceModel;
 import net.finmath.montecarlo.interestrate.models.covariance.LIBORCovarianceModelFromVolatilityAndCorrelation;
@@ -47,6 +48,10 @@ import static java.lang.foreign.ValueLayout.JAVA_INT;
 public class LIBORMonteCarloSimulationHIP extends LIBORMonteCarloSimulationFromLIBORModel {
 
 	private final EulerSchemeFromProcessModelHIP process;
+	// the AAD factory's diracDeltaApproximationMethod and diracDeltaApproximationWidthPerStdDev
+	// (RandomVariableDifferentiableAADFactory.java:60-67): the reference's defaults
+	private DiracDeltaApproximationMethod diracDeltaApproximationMethod = DiracDeltaApproximationMethod.DISCRETE_DELTA;
+	private double diracDeltaApproximationWidthPerStdDev = 0.05;
 
 	public LIBORMonteCarloSimulationHIP(EulerSchemeFromProcessModelHIP process) {
 		super(process.liborModel(), process);
@@ -196,6 +201,16 @@ public class LIBORMonteCarloSimulationHIP extends LIBORMonteCarloSimulationFromL
 	}
 
 	/**
+	 * The derivative of a Bermudan's exercise indicator in its trigger within getGradient, as the
+	 * properties of the reference's RandomVariableDifferentiableAADFactory choose it. The engine
+	 * takes the enum's ordinals (fm_lmm_set_dirac_delta); DISCRETE_DELTA, ONE and ZERO are supported.
+	 */
+	public void setDiracDeltaApproximation(DiracDeltaApproximationMethod method, double widthPerStdDev) {
+		diracDeltaApproximationMethod = method;
+		diracDeltaApproximationWidthPerStdDev = widthPerStdDev;
+	}
+
+	/**
 	 * Gradient of sum_q weights[q] value_q, keyed like getModelParameters(). Every product must be
 	 * one the fused kernel knows; averages over paths are constants, as in
 	 * RandomVariableDifferentiableAAD.getGradient (backward/RandomVariableDifferentiableAAD.java:165-219).
@@ -217,6 +232,8 @@ public class LIBORMonteCarloSimulationHIP extends LIBORMonteCarloSimulationFromL
 			final MemorySegment value = arena.allocate(JAVA_DOUBLE);
 			final MemorySegment dL0 = arena.allocate(JAVA_DOUBLE, n);
 			final MemorySegment dSigma = arena.allocate(JAVA_DOUBLE, (long) nT * n);
+			FinmathHip.check((int) FinmathHip.call(FinmathHip.SET_DIRAC_DELTA, process.context(),
+					diracDeltaApproximationMethod.ordinal(), diracDeltaApproximationWidthPerStdDev));
 			FinmathHip.check((int) FinmathHip.call(FinmathHip.ADJOINT, process.context(), array, products.size(),
 					FinmathHip.doubles(arena, weights), value, dL0, dSigma));
 			for(int j = 0; j < n; j++) {
