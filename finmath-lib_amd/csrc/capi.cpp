@@ -404,6 +404,56 @@ int forward_time_index(const fm_lmm* c, double time) {
     return ti;
 }
 
+// Swaptions evaluated at one time index over the same periods (first forward, count, period
+// lengths, discounting taus and adjustments) and divided by the same numeraire differ only in
+// strike and notional: the path kernel values up to kSwaptionGroup of them in one walk over the
+// periods. Each group is made consecutive (the order of products within a time index changes no
+// result: each has its own reduction slot) and its first member carries the group's size - 1.
+// The members' periods are copied back to back (member s at the first's offset + s * periods),
+// each period with its swaption's notional.
+static bool same_swaption_periods(const KProduct& a, const KProduct& b, const std::vector<KPeriod>& per) {
+    if (a.type != FM_PRODUCT_SWAPTION || b.type != FM_PRODUCT_SWAPTION) return false;
+    if (a.eval_index != b.eval_index || a.first_j != b.first_j || a.num_periods != b.num_periods ||
+        a.num_k != b.num_k)
+        return false;
+    for (int i = 0; i < a.num_periods; i++) {
+        const KPeriod& p = per[(size_t)(a.param_off + i)];
+        const KPeriod& q = per[(size_t)(b.param_off + i)];
+        if (p.period_length != q.period_length || p.tau != q.tau || p.adj != q.adj) return false;
+    }
+    return true;
+}
+
+static void group_swaptions(Lowered& L) {
+    std::vector<KProduct>& kp = L.kp;
+    for (size_t i = 0; i < kp.size(); i++) kp[i].pad2 = 0;
+    for (size_t i = 0; i < kp.size();) {
+        size_t e = i + 1;
+        if (kp[i].type == FM_PRODUCT_SWAPTION) {
+            for (size_t f = i + 1; f < kp.size() && kp[f].eval_index == kp[i].eval_index &&
+                                   e - i < (size_t)kSwaptionGroup; f++) {
+                if (!same_swaption_periods(kp[i], kp[f], L.per)) continue;
+                std::rotate(kp.begin() + (long)e, kp.begin() + (long)f, kp.begin() + (long)f + 1);
+                e++;
+            }
+            kp[i].pad2 = (int32_t)(e - i - 1);
+            if (e - i > 1) {
+                const int np = kp[i].num_periods;
+                const int base = (int)L.per.size();
+                for (size_t f = i; f < e; f++) {
+                    for (int q = 0; q < np; q++) {
+                        KPeriod pe = L.per[(size_t)(kp[f].param_off + q)];
+                        pe.notional = kp[f].notional;
+                        L.per.push_back(pe);
+                    }
+                    kp[f].param_off = base + (int)(f - i) * np;
+                }
+            }
+        }
+        i = e;
+    }
+}
+
 Lowered lower_products(fm_lmm* c, const fm_product* prods, int count) {
     Lowered L;
     require(count >= 0 && (count == 0 || prods != nullptr), FM_ERR_INVALID_ARGUMENT, "products array is NULL");
@@ -615,6 +665,7 @@ Lowered lower_products(fm_lmm* c, const fm_product* prods, int count) {
         }
     }
     std::stable_sort(L.kp.begin(), L.kp.end(), [](const KProduct& a, const KProduct& b) { return a.eval_index < b.eval_index; });
+    group_swaptions(L);
     return L;
 }
 

@@ -74,8 +74,13 @@ struct KProduct {
     double  exercise;
     double  weight;       // adjoint: objective weight of this (lowered) product per path
     int32_t param_off;    // offset into the per-period parameter table
-    int32_t pad2;
+    int32_t pad2;         // swaption: number of following swaptions valued with it (pricing);
+                          // adjoint Bermudan period: the Bermudan's index
 };
+
+// Swaptions valued together by one walk over their shared periods (KProduct.pad2 of the first
+// is the number of those that follow it)
+constexpr int kSwaptionGroup = 4;
 
 // Per-period parameters (strike, period length, discounting tau, adjustment, notional)
 struct KPeriod {

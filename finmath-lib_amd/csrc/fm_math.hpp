@@ -54,14 +54,21 @@ __host__ __device__ inline double fm_log(double x) {
 // within one unit in the last place of the correctly rounded quotient and almost always
 // equal to it; b = +-inf gives NaN instead of zero. Seven instructions against the eleven
 // of the scaled IEEE division sequence.
-__device__ __forceinline__ double fm_div_nr(double a, double b) {
+// fm_rcp_nr(b) is the refined reciprocal, fm_div_nr_r(a, b, r) the quotient from it: divisions of
+// several numerators by one b share the reciprocal and give the same results as fm_div_nr.
+__device__ __forceinline__ double fm_rcp_nr(double b) {
     double r = __builtin_amdgcn_rcp(b);
     double e = fma(-b, r, 1.0);
     r = fma(r, e, r);
     e = fma(-b, r, 1.0);
-    r = fma(r, e, r);
+    return fma(r, e, r);
+}
+
+__device__ __forceinline__ double fm_div_nr_r(double a, double b, double r) {
     const double q = a * r;
     return fma(r, fma(-b, q, a), q);
 }
+
+__device__ __forceinline__ double fm_div_nr(double a, double b) { return fm_div_nr_r(a, b, fm_rcp_nr(b)); }
 
 } // namespace fm

@@ -104,6 +104,13 @@ struct ChunkedForwards {
 template <typename real> __device__ __forceinline__ real spot_div(real a, real b);
 template <> __device__ __forceinline__ double spot_div<double>(double a, double b) { return fm_div_nr(a, b); }
 template <> __device__ __forceinline__ float spot_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+// spot_div split at the reciprocal of b, for numerators sharing one divisor (same results)
+template <typename real> __device__ __forceinline__ real spot_rcp(real b);
+template <> __device__ __forceinline__ double spot_rcp<double>(double b) { return fm_rcp_nr(b); }
+template <> __device__ __forceinline__ float spot_rcp<float>(float b) { return __builtin_amdgcn_rcpf(b); }
+template <typename real> __device__ __forceinline__ real spot_div_r(real a, real b, real r);
+template <> __device__ __forceinline__ double spot_div_r<double>(double a, double b, double r) { return fm_div_nr_r(a, b, r); }
+template <> __device__ __forceinline__ float spot_div_r<float>(float a, float, float r) { return a * r; }
 
 // The spot kernel carries log-forwards in FP64 and values in FP32 (see above).
 template <typename real> constexpr bool carries_log() { return std::is_same<real, double>::value; }
@@ -159,33 +166,65 @@ __device__ __forceinline__ void spot_step(ChunkedForwards<NCH, real>& Y, int n, 
 
 // Products evaluated at time index ti, in the order they were lowered (qp advances over them):
 // swaptions and caplets are reduced into their wave partials, swap periods and Bermudan
-// regression bases go to per-path contribution columns. value_of(j) gives L_j(t_ti); N and
-// Nprev are the numeraires N(T_knum) and the one before it.
-template <typename real, typename ValueOf>
+// regression bases go to per-path contribution columns. Y holds the path's states and val(y)
+// gives the forward L_j(t_ti) of state y; N and Nprev are the numeraires N(T_knum) and the one
+// before it. GROUP values a swaption group in one walk; without it every member walks its own
+// copy of the periods (the spot kernels: the shared walk measured slower there).
+template <bool GROUP, int NCH, typename real, typename Val>
 __device__ __forceinline__ void eval_products_at(int ti, int& qp, int num_products, const KProduct* __restrict__ products,
                                                  const KPeriod* __restrict__ periods, const double* __restrict__ delta,
-                                                 double N, double Nprev, int knum, ValueOf&& value_of, bool active,
-                                                 int cnt, int lane, int64_t wave, double* __restrict__ wave_sum,
-                                                 double* __restrict__ wave_m2, int stride, double* __restrict__ contrib,
-                                                 int64_t p, int P) {
+                                                 double N, double Nprev, int knum, ChunkedForwards<NCH, real>& Y,
+                                                 Val&& val, bool active, int cnt, int lane, int64_t wave,
+                                                 double* __restrict__ wave_sum, double* __restrict__ wave_m2,
+                                                 int stride, double* __restrict__ contrib, int64_t p, int P) {
+    constexpr int W = chunk_width<real>();
+    auto value_of = [&](int j) -> real { return val(Y.get(j)); };
     while (qp < num_products && products[qp].eval_index == ti) {
         const KProduct pr = products[qp];
         qp++;
         const double Nsel = (pr.num_k == knum) ? N : Nprev;
         if (pr.type == FM_PRODUCT_SWAPTION) {
-            // Swaption.java:150-190: the periods backwards, value = (value + payoff) / (1 + L tau) * adj
-            real V = (real)0;
+            // Swaption.java:150-190: the periods backwards, value = (value + payoff) / (1 + L tau) * adj.
+            // The periods are walked chunk by chunk, each element read with one indexed move. With
+            // GROUP, the pr.pad2 swaptions that follow differ from this one only in strike and
+            // notional: the walk values them too, sharing L_j and the reciprocal of 1 + L_j tau;
+            // member s's periods follow this one's at s * num_periods and carry its notional.
+            constexpr int G = GROUP ? kSwaptionGroup : 1;
+            const int m = GROUP ? 1 + pr.pad2 : 1;
+            real V[G];
+#pragma unroll
+            for (int g = 0; g < G; g++) V[g] = (real)0;
+            const int j0 = pr.first_j, j1 = pr.first_j + pr.num_periods;
+            static_for<0, NCH>([&](auto QR) {
+                constexpr int q = NCH - 1 - decltype(QR)::value;
+                const int ja = j0 > W * q ? j0 : W * q;
+                const int jb = j1 < W * q + W ? j1 : W * q + W;
 #pragma unroll 1
-            for (int j = pr.first_j + pr.num_periods - 1; j >= pr.first_j; j--) {
-                const KPeriod pe = periods[pr.param_off + (j - pr.first_j)];
-                const real Lj = value_of(j);
-                const real payoff = ((Lj - (real)pe.strike) * (real)pe.period_length) * (real)pr.notional;
-                V = V + payoff;
-                V = spot_div<real>(V, (real)1.0 + Lj * (real)pe.tau);
-                V = V * (real)pe.adj;
-            }
-            const double x = (double)jmax0(V) / Nsel;
-            reduce_slot(x, active, cnt, lane, wave, wave_sum, wave_m2, stride, pr.out_slot);
+                for (int j = jb - 1; j >= ja; j--) {
+                    const KPeriod pe = periods[pr.param_off + (j - j0)];
+                    const real Lj = val(Y.template ch<q>()[j - W * q]);
+                    const real b = (real)1.0 + Lj * (real)pe.tau;
+                    const real r = spot_rcp<real>(b);
+#pragma unroll
+                    for (int g = 0; g < G; g++) {
+                        if (g < m) {
+                            const KPeriod& pg = periods[pr.param_off + g * pr.num_periods + (j - j0)];
+                            // a single swaption's periods do not carry its notional
+                            const real notional = (real)(m == 1 ? pr.notional : pg.notional);
+                            const real payoff = ((Lj - (real)pg.strike) * (real)pe.period_length) * notional;
+                            V[g] = V[g] + payoff;
+                            V[g] = spot_div_r<real>(V[g], b, r);
+                            V[g] = V[g] * (real)pe.adj;
+                        }
+                    }
+                }
+            });
+#pragma unroll
+            for (int g = 0; g < G; g++)
+                if (g < m)
+                    reduce_slot((double)jmax0(V[g]) / Nsel, active, cnt, lane, wave, wave_sum, wave_m2, stride,
+                                products[qp - 1 + g].out_slot);
+            qp += m - 1;
         } else if (pr.type == FM_PRODUCT_CAPLET) {
             const KPeriod pe = periods[pr.param_off];
             const real Lj = value_of(pr.first_j);
@@ -311,9 +350,8 @@ lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __
             if (numeraire_out && active) numeraire_out[(size_t)kcur * P + p] = N;
         }
         // ---- products evaluated at this time index
-        eval_products_at<real>(ti, qp, a.num_products, products, periods, delta, N, Nprev, knum,
-                               [&](int j) { return val(Y.get(j)); }, active, cnt, lane, wave, wave_sum, wave_m2,
-                               stride, contrib, p, P);
+        eval_products_at<false, NCH, real>(ti, qp, a.num_products, products, periods, delta, N, Nprev, knum, Y, val,
+                                           active, cnt, lane, wave, wave_sum, wave_m2, stride, contrib, p, P);
 
         if (ti == nT) break;
 

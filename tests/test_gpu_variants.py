@@ -126,6 +126,38 @@ def test_swaps_and_caplets_without_discount_curve(measure):
 
 @pytest.mark.parametrize("measure,scheme", [("TERMINAL", "EULER_FUNCTIONAL"), ("SPOT", "PREDICTOR_CORRECTOR"),
                                             ("TERMINAL", "PREDICTOR_CORRECTOR")])
+def test_swaptions_sharing_their_periods_match_oracle(measure, scheme):
+    # the FP64 terminal and predictor-corrector kernels value swaptions over the same periods in
+    # one walk, at most four at a time: six such swaptions (flat and per-period strikes, other
+    # notionals) interleaved with a caplet and a swaption over fewer periods
+    wl = W.c0()
+    t = O.ModelTables(wl)
+    P = 6000
+    say(f"case swaptions sharing periods c0 P={P} {measure} {scheme}")
+    fix, pay = [2.0, 2.5, 3.0, 3.5], [2.5, 3.0, 3.5, 4.0]
+    sw = [dict(type="swaption", exercise=2.0, fixing=fix, payment=pay, strike=[0.03] * 4),
+          dict(type="swaption", exercise=2.0, fixing=fix, payment=pay, strike=[0.02, 0.025, 0.03, 0.035], notional=2.0),
+          dict(type="swaption", exercise=2.0, fixing=fix[:3], payment=pay[:3], strike=[0.03] * 3),
+          dict(type="swaption", exercise=2.0, fixing=fix, payment=pay, strike=[0.04] * 4, notional=0.25),
+          dict(type="swaption", exercise=2.0, fixing=fix, payment=pay, strike=[0.05] * 4),
+          dict(type="swaption", exercise=2.0, fixing=fix, payment=pay, strike=[0.01] * 4, notional=0.5),
+          dict(type="swaption", exercise=2.0, fixing=fix, payment=pay, strike=[0.035] * 4, notional=3.0)]
+    prods = [_abi.Product.swaption(s["exercise"], s["fixing"], s["payment"], s["strike"],
+                                   notional=s.get("notional", 1.0)) for s in sw]
+    prods.insert(2, _abi.Product.caplet(2.0, 0.5, 0.03))
+    ctx = variant_context(wl, t, P, measure, scheme)
+    vals, errs = ctx.value_products(prods)
+    om = oracle(t, P, measure, scheme)
+    ref = [om.swaption(s) for s in sw]
+    ref.insert(2, om.caplet(2.0, 0.5, 0.03))
+    for q, (rv, re_) in enumerate(ref):
+        say(f"  q={q}: gpu {vals[q]:.15e} oracle {rv:.15e}")
+        assert abs(vals[q] - rv) <= 1e-10 * abs(rv)
+        assert abs(errs[q] - re_) <= 1e-8 * abs(re_)
+
+
+@pytest.mark.parametrize("measure,scheme", [("TERMINAL", "EULER_FUNCTIONAL"), ("SPOT", "PREDICTOR_CORRECTOR"),
+                                            ("TERMINAL", "PREDICTOR_CORRECTOR")])
 @pytest.mark.parametrize("basis", [0, 1])
 def test_bermudan_matches_oracle(measure, scheme, basis):
     wl = W.c0()
