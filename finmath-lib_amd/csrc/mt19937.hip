@@ -98,27 +98,46 @@ jump_level_kernel(uint32_t* __restrict__ states, const uint64_t* __restrict__ po
 // ---------------------------------------------------------------------------
 constexpr int kGenThreads = 320;
 
+// Coefficient x as an R. An FP64 VALU operand cannot be a literal on this target, so a
+// double coefficient is built in an SGPR pair by two scalar moves at its point of use: the
+// compiler would otherwise hoist the 48 coefficients of the three rational functions out of
+// the generator's loops into 96 VGPRs, halving the resident wavefronts. The moves are
+// volatile so they stay where they are; the value is the same double either way.
+template <typename R, uint64_t B> struct Coef {
+    __device__ __forceinline__ static R get() { return (R)__builtin_bit_cast(double, B); }
+};
+template <uint64_t B> struct Coef<double, B> {
+    __device__ __forceinline__ static double get() {
+        uint32_t lo, hi;
+        asm volatile("s_mov_b32 %0, %2\n\ts_mov_b32 %1, %3"
+                     : "=s"(lo), "=s"(hi)
+                     : "i"((uint32_t)(B & 0xffffffffu)), "i"((uint32_t)(B >> 32)));
+        return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+    }
+};
+#define AS241_C(R, x) (Coef<R, __builtin_bit_cast(uint64_t, (double)(x))>::get())
+
 // Wichura AS241 (PPND16), as functions/NormalDistribution.java:67-162; the
 // published coefficient table. Split into the central branch |q| <= 0.425 and the
 // tail branch so the generator can run the tail densely (about 15 % of the draws).
 // p <= 0 (or >= 1) in the tail branch gives 0.
 template <typename R> __device__ __forceinline__ R as241_central(R q) {
-    const R r = (R)0.180625 - q * q;
-    R num = (R)2.5090809287301226727e+03;
-    num = num * r + (R)3.3430575583588128105e+04;
-    num = num * r + (R)6.7265770927008700853e+04;
-    num = num * r + (R)4.5921953931549871457e+04;
-    num = num * r + (R)1.3731693765509461125e+04;
-    num = num * r + (R)1.9715909503065514427e+03;
-    num = num * r + (R)1.3314166789178437745e+02;
-    num = num * r + (R)3.3871328727963666080e+00;
-    R den = (R)5.2264952788528545610e+03;
-    den = den * r + (R)2.8729085735721942674e+04;
-    den = den * r + (R)3.9307895800092710610e+04;
-    den = den * r + (R)2.1213794301586595867e+04;
-    den = den * r + (R)5.3941960214247511077e+03;
-    den = den * r + (R)6.8718700749205790830e+02;
-    den = den * r + (R)4.2313330701600911252e+01;
+    const R r = AS241_C(R, 0.180625) - q * q;
+    R num = AS241_C(R, 2.5090809287301226727e+03);
+    num = num * r + AS241_C(R, 3.3430575583588128105e+04);
+    num = num * r + AS241_C(R, 6.7265770927008700853e+04);
+    num = num * r + AS241_C(R, 4.5921953931549871457e+04);
+    num = num * r + AS241_C(R, 1.3731693765509461125e+04);
+    num = num * r + AS241_C(R, 1.9715909503065514427e+03);
+    num = num * r + AS241_C(R, 1.3314166789178437745e+02);
+    num = num * r + AS241_C(R, 3.3871328727963666080e+00);
+    R den = AS241_C(R, 5.2264952788528545610e+03);
+    den = den * r + AS241_C(R, 2.8729085735721942674e+04);
+    den = den * r + AS241_C(R, 3.9307895800092710610e+04);
+    den = den * r + AS241_C(R, 2.1213794301586595867e+04);
+    den = den * r + AS241_C(R, 5.3941960214247511077e+03);
+    den = den * r + AS241_C(R, 6.8718700749205790830e+02);
+    den = den * r + AS241_C(R, 4.2313330701600911252e+01);
     den = den * r + (R)1.0;
     return q * num / den;
 }
@@ -147,6 +166,14 @@ __device__ __forceinline__ float as241_central_f32(float q) {
     return q * num * __builtin_amdgcn_rcpf(den);
 }
 
+// (bits * 2^-52) - (c - 1) for a 52-bit integer bits and c in {1, 1.5}: the double with
+// exponent 0 and mantissa bits is 1 + bits * 2^-52, and subtracting c is exact (the result is a
+// multiple of 2^-52 below 1 in magnitude), so c = 1 gives the uniform u and c = 1.5 gives
+// u - 0.5, both as the conversion and subtraction in double give them.
+__device__ __forceinline__ double unit_offset(uint64_t bits, double c) {
+    return __builtin_bit_cast(double, 0x3ff0000000000000ull | bits) - c;
+}
+
 template <typename R> __device__ __forceinline__ R dev_sqrt_neg_log(double r);
 template <> __device__ __forceinline__ double dev_sqrt_neg_log<double>(double r) { return sqrt(-log(r)); }
 template <> __device__ __forceinline__ float dev_sqrt_neg_log<float>(double r) { return sqrtf(-logf((float)r)); }
@@ -157,41 +184,41 @@ template <typename R> __device__ __forceinline__ R as241_tail(double p, double q
     if (rp <= 0.0) return (R)0;
     R r = dev_sqrt_neg_log<R>(rp);
     R num, den;
-    if (r <= (R)5.0) {
-        r -= (R)1.6;
-        num = (R)7.74545014278341407640e-04;
-        num = num * r + (R)2.27238449892691845833e-02;
-        num = num * r + (R)2.41780725177450611770e-01;
-        num = num * r + (R)1.27045825245236838258e+00;
-        num = num * r + (R)3.64784832476320460504e+00;
-        num = num * r + (R)5.76949722146069140550e+00;
-        num = num * r + (R)4.63033784615654529590e+00;
-        num = num * r + (R)1.42343711074968357734e+00;
-        den = (R)1.05075007164441684324e-09;
-        den = den * r + (R)5.47593808499534494600e-04;
-        den = den * r + (R)1.51986665636164571966e-02;
-        den = den * r + (R)1.48103976427480074590e-01;
-        den = den * r + (R)6.89767334985100004550e-01;
-        den = den * r + (R)1.67638483018380384940e+00;
-        den = den * r + (R)2.05319162663775882187e+00;
+    if (r <= AS241_C(R, 5.0)) {
+        r -= AS241_C(R, 1.6);
+        num = AS241_C(R, 7.74545014278341407640e-04);
+        num = num * r + AS241_C(R, 2.27238449892691845833e-02);
+        num = num * r + AS241_C(R, 2.41780725177450611770e-01);
+        num = num * r + AS241_C(R, 1.27045825245236838258e+00);
+        num = num * r + AS241_C(R, 3.64784832476320460504e+00);
+        num = num * r + AS241_C(R, 5.76949722146069140550e+00);
+        num = num * r + AS241_C(R, 4.63033784615654529590e+00);
+        num = num * r + AS241_C(R, 1.42343711074968357734e+00);
+        den = AS241_C(R, 1.05075007164441684324e-09);
+        den = den * r + AS241_C(R, 5.47593808499534494600e-04);
+        den = den * r + AS241_C(R, 1.51986665636164571966e-02);
+        den = den * r + AS241_C(R, 1.48103976427480074590e-01);
+        den = den * r + AS241_C(R, 6.89767334985100004550e-01);
+        den = den * r + AS241_C(R, 1.67638483018380384940e+00);
+        den = den * r + AS241_C(R, 2.05319162663775882187e+00);
         den = den * r + (R)1.0;
     } else {
-        r -= (R)5.0;
-        num = (R)2.01033439929228813265e-07;
-        num = num * r + (R)2.71155556874348757815e-05;
-        num = num * r + (R)1.24266094738807843860e-03;
-        num = num * r + (R)2.65321895265761230930e-02;
-        num = num * r + (R)2.96560571828504891230e-01;
-        num = num * r + (R)1.78482653991729133580e+00;
-        num = num * r + (R)5.46378491116411436990e+00;
-        num = num * r + (R)6.65790464350110377720e+00;
-        den = (R)2.04426310338993978564e-15;
-        den = den * r + (R)1.42151175831644588870e-07;
-        den = den * r + (R)1.84631831751005468180e-05;
-        den = den * r + (R)7.86869131145613259100e-04;
-        den = den * r + (R)1.48753612908506148525e-02;
-        den = den * r + (R)1.36929880922735805310e-01;
-        den = den * r + (R)5.99832206555887937690e-01;
+        r -= AS241_C(R, 5.0);
+        num = AS241_C(R, 2.01033439929228813265e-07);
+        num = num * r + AS241_C(R, 2.71155556874348757815e-05);
+        num = num * r + AS241_C(R, 1.24266094738807843860e-03);
+        num = num * r + AS241_C(R, 2.65321895265761230930e-02);
+        num = num * r + AS241_C(R, 2.96560571828504891230e-01);
+        num = num * r + AS241_C(R, 1.78482653991729133580e+00);
+        num = num * r + AS241_C(R, 5.46378491116411436990e+00);
+        num = num * r + AS241_C(R, 6.65790464350110377720e+00);
+        den = AS241_C(R, 2.04426310338993978564e-15);
+        den = den * r + AS241_C(R, 1.42151175831644588870e-07);
+        den = den * r + AS241_C(R, 1.84631831751005468180e-05);
+        den = den * r + AS241_C(R, 7.86869131145613259100e-04);
+        den = den * r + AS241_C(R, 1.48753612908506148525e-02);
+        den = den * r + AS241_C(R, 1.36929880922735805310e-01);
+        den = den * r + AS241_C(R, 5.99832206555887937690e-01);
         den = den * r + (R)1.0;
     }
     const R v = num / den;
@@ -272,7 +299,7 @@ brownian_kernel(const uint32_t* __restrict__ states, int64_t num_sub, uint64_t t
     auto drain = [&](int nq) {
         wave_lds_order();
         for (int k = lane; k < nq; k += 64) {
-            const double u = (double)tq_bits[wv][k] * 0x1.0p-52;
+            const double u = unit_offset(tq_bits[wv][k], 1.0);
             const double sq = uniform_dt ? sq0 : sqrt_dt[tq_ti[wv][k]];
             out[d_begin + tq_off[wv][k]] = icdf_out<OutT>(as241_tail<Real<OutT>>(u, u - 0.5), sq);
         }
@@ -312,8 +339,7 @@ brownian_kernel(const uint32_t* __restrict__ states, int64_t num_sub, uint64_t t
                     if (fabsf(q) <= 0.425f) out[wd >> 1] = (OutT)(as241_central_f32(q) * (float)sq);
                     else tail = true;
                 } else {
-                    const double u = (double)bits * 0x1.0p-52;
-                    const double q = u - 0.5;
+                    const double q = unit_offset(bits, 1.5);
                     if (fabs(q) <= 0.425) out[wd >> 1] = icdf_out<OutT>(as241_central<double>(q), sq);
                     else tail = true;
                 }
