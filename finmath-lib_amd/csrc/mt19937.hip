@@ -296,18 +296,19 @@ brownian_kernel(const uint32_t* __restrict__ states, int64_t num_sub, uint64_t t
     uint32_t rem = uniform_dt ? 0u : (uint32_t)((d_begin + (uint64_t)lane) % per);
     const uint32_t step64 = 64u % per, step56 = (uint32_t)(N / 2 - 4 * 64) % per;
     int count = 0;                                     // queued tail draws (uniform)
+    // transform the first nq <= 64 queued draws, one per lane
     auto drain = [&](int nq) {
         wave_lds_order();
-        for (int k = lane; k < nq; k += 64) {
-            const double u = unit_offset(tq_bits[wv][k], 1.0);
-            const double sq = uniform_dt ? sq0 : sqrt_dt[tq_ti[wv][k]];
-            out[d_begin + tq_off[wv][k]] = icdf_out<OutT>(as241_tail<Real<OutT>>(u, u - 0.5), sq);
+        if (lane < nq) {
+            const double u = unit_offset(tq_bits[wv][lane], 1.0);
+            const double sq = uniform_dt ? sq0 : sqrt_dt[tq_ti[wv][lane]];
+            out[d_begin + tq_off[wv][lane]] = icdf_out<OutT>(as241_tail<Real<OutT>>(u, u - 0.5), sq);
         }
         wave_lds_order();
     };
     for (uint64_t w = w_begin; w < w_end; w += N) {
-        // in-place twist
-#pragma unroll 1
+        // in-place twist, rounds unrolled so the LDS indices fold into instruction offsets
+#pragma unroll
         for (int k0 = 0; k0 < N; k0 += 64) {
             const int k = k0 + lane;
             uint32_t a = 0, b = 0, c = 0;
@@ -352,7 +353,17 @@ brownian_kernel(const uint32_t* __restrict__ states, int64_t num_sub, uint64_t t
                 tq_ti[wv][pos] = (uint32_t)ti;
             }
             count += __popcll(m);
-            if (count >= 64) { drain(count); count = 0; }
+            if (count >= 64) {
+                // a full wavefront of tail draws; the rest (count - 64 < 64) moves to the front
+                drain(64);
+                count -= 64;
+                uint64_t b = 0;
+                uint32_t o = 0, t = 0;
+                if (lane < count) { b = tq_bits[wv][64 + lane]; o = tq_off[wv][64 + lane]; t = tq_ti[wv][64 + lane]; }
+                wave_lds_order();
+                if (lane < count) { tq_bits[wv][lane] = b; tq_off[wv][lane] = o; tq_ti[wv][lane] = t; }
+                wave_lds_order();
+            }
             if (!uniform_dt) {
                 rem += (r < 4) ? step64 : step56;
                 if (rem >= per) rem -= per;
