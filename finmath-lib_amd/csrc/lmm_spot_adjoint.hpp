@@ -129,7 +129,7 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
                 });
                 // pass B (descending): the drift chain back from S^(n-1), S^(j-1) = S^(j) - w_j lambda_j;
                 // the per-path dV/dsigma_j(t) of groups of four j are summed over the wavefront together
-                double gq0 = 0.0, gq1 = 0.0, gq2 = 0.0, gq3 = 0.0;
+                real gq0 = (real)0, gq1 = (real)0, gq2 = (real)0, gq3 = (real)0;
                 static_for<0, NCH>([&](auto QR) {
                     constexpr int q = NCH - 1 - decltype(QR)::value;
                     const int j0 = first > W * q ? first : W * q;
@@ -143,34 +143,39 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
                         const real Yb = Lb.template ch<q>()[j - W * q];     // adjoint of y_j(t+1)
                         const real mb = Yb * dt;
                         real wb = (real)0;
-                        real sgb = (real)0;             // dV/dsigma_j(t) on this path
+                        // dV/dsigma_j(t) on this path = sum_k lambda-bar_jk f_jk with lambda-bar_jk =
+                        // Yb dW_k + mb S_k + w Sb_k, gathered as Yb (dW.f) + mb (S.f) + w (Sb.f)
+                        real dwf = (real)0, sf = (real)0, sbf = (real)0;
 #pragma unroll
                         for (int k = 0; k < FCAP; k++) {
+                            const real f = fsig[j * FCAP + k];
                             Sb[k] = Sb[k] + mb * R.lam(k);
                             wb = wb + Sb[k] * R.lam(k);
-                            const real lamb = Yb * dW[k] + mb * S[k] + Sb[k] * w;
-                            sgb = sgb + lamb * fsig[j * FCAP + k];
+                            dwf = dwf + dW[k] * f;
+                            sf = sf + S[k] * f;
+                            sbf = sbf + Sb[k] * f;
                             S[k] = S[k] - w * R.lam(k);    // S^(j) -> S^(j-1)
                         }
+                        real sgb = Yb * dwf + mb * sf + w * sbf;
                         // variance sigma^2 c in the drift: d(-var/2)/dsigma = -sigma c
                         if (lognormal) sgb = sgb + mb * vterm[(size_t)t * n + j];
                         // y_j(t+1) = log L_j(t) + ... (or L_j(t) + ...), and w_j depends on L_j(t)
                         const real dwdL = lognormal ? R.c2() * r * r : -R.c1() * r * r;
                         const real direct = lognormal ? Yb * adj_rcp<real>(Lj) : Yb;
                         Lb.template ch<q>()[j - W * q] = direct + wb * dwdL;
-                        const double gs = active ? (double)sgb : 0.0;
+                        const real gs = active ? sgb : (real)0;
                         const int m = j & 3;
                         gq0 = (m == 0) ? gs : gq0;
                         gq1 = (m == 1) ? gs : gq1;
                         gq2 = (m == 2) ? gs : gq2;
                         gq3 = (m == 3) ? gs : gq3;
                         if (m == 0 || j == first) {     // uniform: the group of four is complete
-                            const double v4[4] = {gq0, gq1, gq2, gq3};
+                            const double v4[4] = {(double)gq0, (double)gq1, (double)gq2, (double)gq3};
                             const double s = wave_sum4(v4, lane);
                             const int jw = (j & ~3) + 2 * (lane & 1) + ((lane >> 1) & 1);
                             if (lane < 4 && cnt > 0 && jw < n && jw >= first)
                                 wave_grad[(size_t)wave * gstride + (size_t)t * n + jw] = s;
-                            gq0 = gq1 = gq2 = gq3 = 0.0;
+                            gq0 = gq1 = gq2 = gq3 = (real)0;
                         }
                     }
                 });
