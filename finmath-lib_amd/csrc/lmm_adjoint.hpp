@@ -7,8 +7,9 @@
 namespace fm {
 
 // Wavefront sums of four per-lane values at once (a transposed butterfly): two exchange
-// steps halve the values a lane carries, four more sum across the wavefront. Lane l < 4
-// returns the total of value 2*(l & 1) + ((l >> 1) & 1); 7 lane exchanges for 4 sums.
+// steps within each quad (DPP quad permutes) halve the values a lane carries, two DPP row
+// rotations sum the four lanes of each row that carry the same value, and two lane permutes
+// sum across the rows. Lane l < 4 returns the total of value 2*(l & 1) + ((l >> 1) & 1).
 __device__ __forceinline__ double wave_sum4(const double (&v)[4], int lane) {
     const bool b0 = (lane & 1) != 0, b1 = (lane & 2) != 0;
     double a2[2];
@@ -16,13 +17,13 @@ __device__ __forceinline__ double wave_sum4(const double (&v)[4], int lane) {
     for (int i = 0; i < 2; i++) {
         const double send = b0 ? v[i] : v[i + 2];
         const double keep = b0 ? v[i + 2] : v[i];
-        a2[i] = keep + __shfl_xor(send, 1, 64);
+        a2[i] = keep + dpp_move<0xb1>(send);      // quad_perm [1,0,3,2]: lane ^ 1
     }
     const double send = b1 ? a2[0] : a2[1];
     const double keep = b1 ? a2[1] : a2[0];
-    double a = keep + __shfl_xor(send, 2, 64);
-    a = a + __shfl_xor(a, 4, 64);
-    a = a + __shfl_xor(a, 8, 64);
+    double a = keep + dpp_move<0x4e>(send);       // quad_perm [2,3,0,1]: lane ^ 2
+    a = a + dpp_move<0x124>(a);                   // row_ror:4
+    a = a + dpp_move<0x128>(a);                   // row_ror:8
     a = a + __shfl_xor(a, 16, 64);
     a = a + __shfl_xor(a, 32, 64);
     return a;
