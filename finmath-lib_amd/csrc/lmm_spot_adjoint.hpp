@@ -112,7 +112,9 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
                 load_increments<FCAP, real>(dW, dw, pc, nT, t, F);
 #pragma unroll
                 for (int k = 0; k < FCAP; k++) { S[k] = (real)0; Sb[k] = (real)0; }
-                // pass A (ascending): the drift sums S_k = sum_j w_j lambda_jk of the step
+                // pass A: Sb_k = sum_j mb_j lambda_jk with mb_j = Yb_j dt (Yb_j the adjoint of
+                // y_j(t+1)), the adjoint of the step's drift sums before any component's share is
+                // taken off; it reads only Lb and the factor loadings
                 static_for<0, NCH>([&](auto Q) {
                     constexpr int q = decltype(Q)::value;
                     const int j0 = first > W * q ? first : W * q;
@@ -120,22 +122,22 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
 #pragma unroll 1
                     for (int j = j0; j < j1; j++) {
                         const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, t, j);
-                        const real Lj = val(X.template ch<q>()[j - W * q]);
-                        real w = spot_div<real>((real)1.0, Lj * R.c1() + R.c2());
-                        if (lognormal) w = w * Lj;
+                        const real mb = Lb.template ch<q>()[j - W * q] * dt;
 #pragma unroll
-                        for (int k = 0; k < FCAP; k++) S[k] = S[k] + w * R.lam(k);
+                        for (int k = 0; k < FCAP; k++) Sb[k] = Sb[k] + mb * R.lam(k);
                     }
                 });
-                // pass B (descending): the drift chain back from S^(n-1), S^(j-1) = S^(j) - w_j lambda_j;
-                // the per-path dV/dsigma_j(t) of groups of four j are summed over the wavefront together
+                // pass B (ascending): the drift sums S^(j) = S^(j-1) + w_j lambda_j as the step built
+                // them, and Sb^(j) = sum_{i >= j} mb_i lambda_i, pass A's total with the shares of the
+                // components below j taken off; the per-path dV/dsigma_j(t) of groups of four j are
+                // summed over the wavefront together
                 real gq0 = (real)0, gq1 = (real)0, gq2 = (real)0, gq3 = (real)0;
-                static_for<0, NCH>([&](auto QR) {
-                    constexpr int q = NCH - 1 - decltype(QR)::value;
+                static_for<0, NCH>([&](auto Q) {
+                    constexpr int q = decltype(Q)::value;
                     const int j0 = first > W * q ? first : W * q;
                     const int j1 = n < W * q + W ? n : W * q + W;
 #pragma unroll 1
-                    for (int j = j1 - 1; j >= j0; j--) {
+                    for (int j = j0; j < j1; j++) {
                         const Rec<FCAP, real> R = rec_at<FCAP, real>(rec, n, t, j);
                         const real Lj = val(X.template ch<q>()[j - W * q]);
                         const real r = spot_div<real>((real)1.0, Lj * R.c1() + R.c2());
@@ -149,12 +151,12 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
 #pragma unroll
                         for (int k = 0; k < FCAP; k++) {
                             const real f = fsig[j * FCAP + k];
-                            Sb[k] = Sb[k] + mb * R.lam(k);
+                            S[k] = S[k] + w * R.lam(k);    // S^(j-1) -> S^(j)
                             wb = wb + Sb[k] * R.lam(k);
                             dwf = dwf + dW[k] * f;
                             sf = sf + S[k] * f;
                             sbf = sbf + Sb[k] * f;
-                            S[k] = S[k] - w * R.lam(k);    // S^(j) -> S^(j-1)
+                            Sb[k] = Sb[k] - mb * R.lam(k); // Sb^(j) -> Sb^(j+1)
                         }
                         real sgb = Yb * dwf + mb * sf + w * sbf;
                         // variance sigma^2 c in the drift: d(-var/2)/dsigma = -sigma c
@@ -169,7 +171,7 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
                         gq1 = (m == 1) ? gs : gq1;
                         gq2 = (m == 2) ? gs : gq2;
                         gq3 = (m == 3) ? gs : gq3;
-                        if (m == 0 || j == first) {     // uniform: the group of four is complete
+                        if (m == 3 || j == n - 1) {     // uniform: the group of four is complete
                             const double v4[4] = {(double)gq0, (double)gq1, (double)gq2, (double)gq3};
                             const double s = wave_sum4(v4, lane);
                             const int jw = (j & ~3) + 2 * (lane & 1) + ((lane >> 1) & 1);
