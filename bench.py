@@ -43,10 +43,10 @@ HBM_PEAK_GBPS = 8000.0       # MI355X published HBM3E bandwidth
 # HBM bytes per launch of lmm_spot_kernel<5,5,double,true,0> on the 40 x 5 x 1,000,000-path strip:
 # FETCH_SIZE and WRITE_SIZE counter passes scaled by a plain 1 GiB copy
 # (profiles/lmm_spot_fp64_c1_hbm_traffic.md)
-PATH_KERNEL_TRAFFIC_BYTES = 2.672e9
+PATH_KERNEL_TRAFFIC_BYTES = 2.730e9
 # VALU-busy fraction of the same kernel from its SQ counter pass: 4 SQ_ACTIVE_INST_VALU over
 # 1,024 SIMDs x GRBM_GUI_ACTIVE / 8 (profiles/lmm_spot_c1_sq_counters.md)
-PATH_KERNEL_VALU_BUSY = 0.746
+PATH_KERNEL_VALU_BUSY = 0.758
 FP32_PEAK_TFLOPS = 157.3     # MI355X published FP32 peak
 
 
@@ -380,8 +380,8 @@ def main():
             "fp64_ms_per_step": ms4d, "fp64_price": float(v4d[0]),
             "err_bp_vs_fp64": float(abs(v4[0] - v4d[0]) * 1e4), "mc_stderr_bp": float(e4d[0] * 1e4)}
         if not args.no_greeks:
-            pm4, gm4, g4, gst4 = run_greeks(wl4, 2, 4)
-            pm4z, gm4z, g4z, _ = run_greeks(wl4, 2, 4, dirac="ZERO")
+            pm4, gm4, g4, gst4 = run_greeks(wl4, 5, 4)
+            pm4z, gm4z, g4z, _ = run_greeks(wl4, 3, 4, dirac="ZERO")
             line["bermudan"]["greeks"] = {
                 "outputs": "80 deltas, 80 forward vegas, 4 vol-parameter vegas, dV/dsigma table; exercise under the "
                            "reference's default DISCRETE_DELTA (trigger term regressed on each date's basis, width "
