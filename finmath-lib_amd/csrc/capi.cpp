@@ -1606,8 +1606,8 @@ int fm_lmm_adjoint(fm_lmm* c, const fm_product* products, int32_t count, const d
     }
     const int64_t num_waves = ((int64_t)P + 63) / 64;
     const int gstride = nT * n + n;
+    // every wavefront's row (nT x n dV/dsigma entries, then n dV/dL(0)) is written in full by the sweep
     ga.wave_grad = (double*)c->d_wave_grad.get((size_t)num_waves * gstride * sizeof(double));
-    HIP_CHECK(hipMemsetAsync(ga.wave_grad, 0, (size_t)num_waves * gstride * sizeof(double), s));
     HIP_CHECK(hipEventRecord(c->ev[0], s));
     if (c->fp32()) launch_lmm_adjoint_fp32(a, ga, s); else launch_lmm_adjoint_fp64(a, ga, s);
     HIP_CHECK(hipEventRecord(c->ev[1], s));

@@ -326,13 +326,13 @@ lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __
 
     for (int ti = 0; ti <= nT; ti++) {
         // ---- the carried states every ckpt_stride steps, for the adjoint's reverse sweep:
-        // [ti / stride][j][P]
+        // [ti / stride][j][P], stored non-temporal (read back only by the sweep, after this pass)
         if (ckpt && active && (ti % a.ckpt_stride) == 0) {
             const size_t base = (size_t)(ti / a.ckpt_stride) * n;
             const int cfirst = a.ckpt_first[ti / a.ckpt_stride];
             static_for<0, W * NCH>([&](auto J) {
                 constexpr int j = decltype(J)::value;
-                if (j < n && j >= cfirst) ckpt[(base + j) * P + p] = Y.template at<j>();
+                if (j < n && j >= cfirst) __builtin_nontemporal_store(Y.template at<j>(), &ckpt[(base + j) * P + p]);
             });
         }
         // ---- N(T_{k+1}) = N(T_k).accrue(L_k(T_k), d_k) once L_k is final (LMM:1050-1066)

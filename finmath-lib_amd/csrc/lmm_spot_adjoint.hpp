@@ -181,6 +181,10 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
                         }
                     }
                 });
+                // the components the step does not move have dV/dsigma_j(t) = 0: this row of the
+                // wavefront's gradient is then written in full and needs no clearing beforehand
+                if (cnt > 0)
+                    for (int j = lane; j < first; j += 64) wave_grad[(size_t)wave * gstride + (size_t)t * n + j] = 0.0;
             }
 
             // ---- 2. products evaluated at t (reverse order of evaluation). The per-path conditions
