@@ -80,11 +80,13 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
 #if defined(__HIP_DEVICE_COMPILE__)
             asm volatile("" : "+s"(row));
 #endif
-            // components below cfirst are dead and read by nothing in this segment
+            // components below cfirst are dead and read by nothing in this segment. Checkpoints and
+            // segment states are read and written non-temporal: a wavefront's segment (two states of
+            // n components at stride 4) outgrows its share of the L2 before it is read back.
             const int cfirst = a.ckpt_first[c / stride];
             static_for<0, W * NCH>([&](auto J) {
                 constexpr int j = decltype(J)::value;
-                X.template put<j>((j < n && j >= cfirst) ? src[(row + j) * P + pc] : (real)0);
+                X.template put<j>((j < n && j >= cfirst) ? __builtin_nontemporal_load(&src[(row + j) * P + pc]) : (real)0);
             });
             if (t == hi) {
                 for (int s = c; s < t; s++) {
@@ -97,7 +99,7 @@ lmm_spot_adjoint_kernel(const StepArgs a, const AdjArgs g, const real* __restric
                         const size_t srow = (size_t)(s - c) * n;
                         static_for<0, W * NCH>([&](auto J) {
                             constexpr int j = decltype(J)::value;
-                            if (j < n && j >= cfirst) seg[(srow + j) * P + p] = X.template at<j>();
+                            if (j < n && j >= cfirst) __builtin_nontemporal_store(X.template at<j>(), &seg[(srow + j) * P + p]);
                         });
                     }
                 }
