@@ -169,7 +169,7 @@ __device__ __forceinline__ void spot_step(ChunkedForwards<NCH, real>& Y, int n, 
 // regression bases go to per-path contribution columns. Y holds the path's states and val(y)
 // gives the forward L_j(t_ti) of state y; N and Nprev are the numeraires N(T_knum) and the one
 // before it. GROUP values a swaption group in one walk; without it every member walks its own
-// copy of the periods (the spot kernels: the shared walk measured slower there).
+// copy of the periods (the FP32 kernels: the shared walk measured slower there).
 template <bool GROUP, int NCH, typename real, typename Val>
 __device__ __forceinline__ void eval_products_at(int ti, int& qp, int num_products, const KProduct* __restrict__ products,
                                                  const KPeriod* __restrict__ periods, const double* __restrict__ delta,
@@ -205,13 +205,19 @@ __device__ __forceinline__ void eval_products_at(int ti, int& qp, int num_produc
                     const real Lj = val(Y.template ch<q>()[j - W * q]);
                     const real b = (real)1.0 + Lj * (real)pe.tau;
                     const real r = spot_rcp<real>(b);
+                    // every member's strike and notional are read up front (indices clamped to the
+                    // group) so their scalar loads issue together
+                    real Kg[G], Ng[G];
+#pragma unroll
+                    for (int g = 0; g < G; g++) {
+                        const KPeriod& pg = periods[pr.param_off + (g < m ? g : 0) * pr.num_periods + (j - j0)];
+                        Kg[g] = (real)pg.strike;
+                        Ng[g] = GROUP ? (real)pg.notional : (real)pr.notional;
+                    }
 #pragma unroll
                     for (int g = 0; g < G; g++) {
                         if (g < m) {
-                            const KPeriod& pg = periods[pr.param_off + g * pr.num_periods + (j - j0)];
-                            // a single swaption's periods do not carry its notional
-                            const real notional = (real)(m == 1 ? pr.notional : pg.notional);
-                            const real payoff = ((Lj - (real)pg.strike) * (real)pe.period_length) * notional;
+                            const real payoff = ((Lj - Kg[g]) * (real)pe.period_length) * Ng[g];
                             V[g] = V[g] + payoff;
                             V[g] = spot_div_r<real>(V[g], b, r);
                             V[g] = V[g] * (real)pe.adj;
@@ -350,7 +356,8 @@ lmm_spot_kernel(const StepArgs a, const real* __restrict__ rec, const double* __
             if (numeraire_out && active) numeraire_out[(size_t)kcur * P + p] = N;
         }
         // ---- products evaluated at this time index
-        eval_products_at<false, NCH, real>(ti, qp, a.num_products, products, periods, delta, N, Nprev, knum, Y, val,
+        constexpr bool group = sizeof(real) == 8;
+        eval_products_at<group, NCH, real>(ti, qp, a.num_products, products, periods, delta, N, Nprev, knum, Y, val,
                                            active, cnt, lane, wave, wave_sum, wave_m2, stride, contrib, p, P);
 
         if (ti == nT) break;
