@@ -43,10 +43,10 @@ HBM_PEAK_GBPS = 8000.0       # MI355X published HBM3E bandwidth
 # HBM bytes per launch of lmm_spot_kernel<5,5,double,true,0> on the 40 x 5 x 1,000,000-path strip:
 # FETCH_SIZE and WRITE_SIZE counter passes scaled by a plain 1 GiB copy
 # (profiles/lmm_spot_fp64_c1_hbm_traffic.md)
-PATH_KERNEL_TRAFFIC_BYTES = 2.730e9
+PATH_KERNEL_TRAFFIC_BYTES = 2.933e9
 # VALU-busy fraction of the same kernel from its SQ counter pass: 4 SQ_ACTIVE_INST_VALU over
 # 1,024 SIMDs x GRBM_GUI_ACTIVE / 8 (profiles/lmm_spot_c1_sq_counters.md)
-PATH_KERNEL_VALU_BUSY = 0.758
+PATH_KERNEL_VALU_BUSY = 0.673
 FP32_PEAK_TFLOPS = 157.3     # MI355X published FP32 peak
 
 
